@@ -1,0 +1,324 @@
+"""Deterministic synthetic mesh generators and an OFF reader.
+
+Every generator returns ``(V, F, S)``: ``V`` float64 ``(nv, 3)``, ``F`` uint32
+flat face-index array, ``S`` uint32 face sizes or ``None`` when every face is a
+triangle (the convention of ``mcDispatch``'s ``pFaceSizes == NULL``,
+mcut.h:1008-1020, preproc.cpp:206).
+
+Randomness comes from a counter-based SplitMix64 stream (``u01``), so a value
+depends only on ``(seed, stream, index)`` and never on call order or platform.
+"""
+from __future__ import annotations
+
+import math
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    z = z.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def u64(seed: int, stream: int, n: int, start: int = 0) -> np.ndarray:
+    """n 64-bit words of the SplitMix64 stream (seed, stream), from index start."""
+    base = np.uint64((seed * 0x100000001B3 + stream * 0x9E3779B1) & 0xFFFFFFFFFFFFFFFF)
+    idx = np.arange(start + 1, start + n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = base + idx * _GOLD
+    return _mix64(z)
+
+
+def u01(seed: int, stream: int, n: int, start: int = 0) -> np.ndarray:
+    """n doubles uniform on [0,1) with 53 random bits each."""
+    return (u64(seed, stream, n, start) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+# --------------------------------------------------------------------------
+# primitive shapes
+# --------------------------------------------------------------------------
+
+def uv_sphere(slices: int = 32, stacks: int = 16, r: float = 1.0, phi0: float = 0.0):
+    """UV sphere, poles on +-z, ring k at theta=k*pi/stacks, vertex j at
+    phi=phi0+2*pi*j/slices; caps are fans, bands are quads split into two
+    triangles; outward (counter-clockwise seen from outside) winding."""
+    verts = [(0.0, 0.0, r)]
+    for k in range(1, stacks):
+        th = k * math.pi / stacks
+        for j in range(slices):
+            ph = phi0 + 2.0 * math.pi * j / slices
+            verts.append((r * math.sin(th) * math.cos(ph), r * math.sin(th) * math.sin(ph), r * math.cos(th)))
+    verts.append((0.0, 0.0, -r))
+    south = len(verts) - 1
+
+    def ring(k, j):
+        return 1 + (k - 1) * slices + (j % slices)
+
+    faces = []
+    for j in range(slices):
+        faces.append((0, ring(1, j), ring(1, j + 1)))
+    for k in range(1, stacks - 1):
+        for j in range(slices):
+            a, b = ring(k, j), ring(k, j + 1)
+            c, d = ring(k + 1, j), ring(k + 1, j + 1)
+            faces.append((a, c, d))
+            faces.append((a, d, b))
+    for j in range(slices):
+        faces.append((ring(stacks - 1, j), south, ring(stacks - 1, j + 1)))
+    return np.asarray(verts, dtype=np.float64), np.asarray(faces, dtype=np.uint32).ravel(), None
+
+
+def tessellated_box(center=(0.0, 0.0, 0.0), edge=1.0, m: int = 4, rotation=None, triangulate=True):
+    """Axis-aligned cube of side ``edge`` (scalar or 3-vector) centred at
+    ``center``; each face split into m x m quads (then into two triangles when
+    ``triangulate``).  Grid vertices are shared between faces; winding is
+    outward.  ``rotation`` (3x3) is applied about the centre."""
+    edge = np.broadcast_to(np.asarray(edge, dtype=np.float64), (3,))
+    half = edge / 2.0
+    key_to_id = {}
+    verts = []
+
+    def vid(ix, iy, iz):
+        key = (ix, iy, iz)
+        if key not in key_to_id:
+            key_to_id[key] = len(verts)
+            verts.append((-half[0] + edge[0] * ix / m, -half[1] + edge[1] * iy / m, -half[2] + edge[2] * iz / m))
+        return key_to_id[key]
+
+    faces = []
+    sizes = []
+    # each face: (fixed axis, fixed value index, u axis, v axis) with u x v = outward normal
+    for axis in range(3):
+        for side in (0, m):
+            u_ax, v_ax = [(1, 2), (2, 0), (0, 1)][axis]
+            if side == 0:
+                u_ax, v_ax = v_ax, u_ax
+            for a in range(m):
+                for b in range(m):
+                    def p(du, dv):
+                        c = [0, 0, 0]
+                        c[axis] = side
+                        c[u_ax] = a + du
+                        c[v_ax] = b + dv
+                        return vid(*c)
+                    q = (p(0, 0), p(1, 0), p(1, 1), p(0, 1))
+                    if triangulate:
+                        faces.append((q[0], q[1], q[2]))
+                        faces.append((q[0], q[2], q[3]))
+                        sizes += [3, 3]
+                    else:
+                        faces.append(q)
+                        sizes.append(4)
+    V = np.asarray(verts, dtype=np.float64)
+    if rotation is not None:
+        V = V @ np.asarray(rotation, dtype=np.float64).T
+    V = V + np.asarray(center, dtype=np.float64)
+    F = np.asarray([i for f in faces for i in f], dtype=np.uint32)
+    return V, F, (None if triangulate else np.asarray(sizes, dtype=np.uint32))
+
+
+_ICO_V = None
+
+
+def _icosahedron():
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0),
+                  (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+                  (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    f = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11),
+                  (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8),
+                  (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9),
+                  (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], dtype=np.int64)
+    return v, f
+
+
+def icosphere(level: int):
+    """Icosphere: icosahedron, ``level`` rounds of 1->4 midpoint subdivision
+    with re-projection to the unit sphere.  20*4^level faces."""
+    V, F = _icosahedron()
+    for _ in range(level):
+        nv = V.shape[0]
+        e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]], axis=0)
+        lo = np.minimum(e[:, 0], e[:, 1])
+        hi = np.maximum(e[:, 0], e[:, 1])
+        key = lo * (1 << 32) + hi
+        uk, inv = np.unique(key, return_inverse=True)
+        ulo = uk >> 32
+        uhi = uk & 0xFFFFFFFF
+        mid = V[ulo] + V[uhi]
+        mid /= np.linalg.norm(mid, axis=1)[:, None]
+        V = np.concatenate([V, mid], axis=0)
+        m = (inv + nv).reshape(3, -1)
+        m01, m12, m20 = m[0], m[1], m[2]
+        a, b, c = F[:, 0], F[:, 1], F[:, 2]
+        F = np.concatenate([np.stack([a, m01, m20], 1), np.stack([b, m12, m01], 1),
+                            np.stack([c, m20, m12], 1), np.stack([m01, m12, m20], 1)], axis=0)
+    return V, F.astype(np.uint32).ravel(), None
+
+
+def geodesic_sphere(n: int):
+    """Class-I geodesic sphere of frequency n: each icosahedron face is split
+    into n^2 triangles on a barycentric grid, points projected to the unit
+    sphere.  20*n^2 faces, 10*n^2+2 vertices (grid points shared)."""
+    V0, F0 = _icosahedron()
+    # vertex ids: global dedup through integer barycentric keys on shared edges
+    key_to_id = {}
+    pts = []
+    faces = []
+
+    def gid(face, i, j):
+        # point a + (b-a)*i/n + (c-a)*j/n; canonical key from the (vertex-id, weight) set
+        a, b, c = F0[face]
+        w = {}
+        for vv, ww in ((a, n - i - j), (b, i), (c, j)):
+            if ww:
+                w[vv] = w.get(vv, 0) + ww
+        key = tuple(sorted(w.items()))
+        if key not in key_to_id:
+            key_to_id[key] = len(pts)
+            p = sum(V0[vv] * ww for vv, ww in key) / n
+            pts.append(p / np.linalg.norm(p))
+        return key_to_id[key]
+
+    for f in range(20):
+        for i in range(n):
+            for j in range(n - i):
+                p0 = gid(f, i, j)
+                p1 = gid(f, i + 1, j)
+                p2 = gid(f, i, j + 1)
+                faces.append((p0, p1, p2))
+                if i + j < n - 1:
+                    p3 = gid(f, i + 1, j + 1)
+                    faces.append((p1, p3, p2))
+    return np.asarray(pts, dtype=np.float64), np.asarray(faces, dtype=np.uint32).ravel(), None
+
+
+def random_rotation(seed: int, stream: int = 0) -> np.ndarray:
+    """Uniform random rotation (Shoemake's subgroup algorithm for a unit quaternion)."""
+    u1, u2, u3 = u01(seed, stream, 3)
+    a = math.sqrt(1.0 - u1)
+    b = math.sqrt(u1)
+    w = a * math.sin(2 * math.pi * u2)
+    x = a * math.cos(2 * math.pi * u2)
+    y = b * math.sin(2 * math.pi * u3)
+    z = b * math.cos(2 * math.pi * u3)
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]], dtype=np.float64)
+
+
+def radial_displace(V: np.ndarray, amp: float, seed: int, stream: int = 1) -> np.ndarray:
+    """Scale each vertex radially by 1 + U(-amp, amp)."""
+    s = 1.0 + amp * (2.0 * u01(seed, stream, V.shape[0]) - 1.0)
+    return V * s[:, None]
+
+
+def grid_patch(m: int, side: float = 1.0):
+    """Open planar m x m quad grid on [-side/2, side/2]^2 at z=0, triangulated."""
+    g = np.linspace(-side / 2.0, side / 2.0, m + 1)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    V = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], axis=1)
+    i, j = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    a = (i * (m + 1) + j).ravel()
+    b = a + (m + 1)
+    c = b + 1
+    d = a + 1
+    F = np.stack([np.stack([a, b, c], 1), np.stack([a, c, d], 1)], 1).reshape(-1, 3)
+    return V, F.astype(np.uint32).ravel(), None
+
+
+def value_noise_fbm(x: np.ndarray, y: np.ndarray, octaves: int, seed: int) -> np.ndarray:
+    """2-D fBm of lattice value noise (smoothstep interpolation, lacunarity 2, gain 0.5)."""
+    out = np.zeros_like(x)
+    amp, freq = 1.0, 4.0
+    for o in range(octaves):
+        xs, ys = x * freq, y * freq
+        xi, yi = np.floor(xs).astype(np.int64), np.floor(ys).astype(np.int64)
+        fx, fy = xs - xi, ys - yi
+        sx, sy = fx * fx * (3 - 2 * fx), fy * fy * (3 - 2 * fy)
+
+        def lat(ix, iy):
+            k = (ix.astype(np.uint64) * np.uint64(73856093)) ^ (iy.astype(np.uint64) * np.uint64(19349663))
+            with np.errstate(over="ignore"):
+                z = _mix64(k + np.uint64((seed * 1000003 + o) & 0xFFFFFFFF) * _GOLD)
+            return (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+        n00, n10, n01, n11 = lat(xi, yi), lat(xi + 1, yi), lat(xi, yi + 1), lat(xi + 1, yi + 1)
+        nx0 = n00 + (n10 - n00) * sx
+        nx1 = n01 + (n11 - n01) * sx
+        out += amp * (nx0 + (nx1 - nx0) * sy)
+        amp *= 0.5
+        freq *= 2.0
+    return out
+
+
+# --------------------------------------------------------------------------
+# benchmark workloads
+# --------------------------------------------------------------------------
+
+def config1(enforce_variant: bool = False, phi0: float = 0.0):
+    """UV sphere r=1 (32 slices x 16 stacks, 960 tris) and an axis-aligned cube
+    of edge 1.2 centred at (0.4, 0.3, 0.2), 4x4 quads per face triangulated
+    (192 tris)."""
+    src = uv_sphere(32, 16, 1.0, phi0)
+    cut = tessellated_box((0.4, 0.3, 0.2), 1.2, 4)
+    return src, cut
+
+
+def config2(level: int = 7, amp: float = 0.02):
+    """Two icospheres (20*4^level tris each), radial noise U(-amp, amp) from
+    seeds 1 and 2; the second is rotated by a random rotation (seed 3) about
+    its centre and translated by +0.5 in x."""
+    V, F, _ = icosphere(level)
+    A = radial_displace(V, amp, seed=1)
+    B = radial_displace(V, amp, seed=2)
+    R = random_rotation(seed=3)
+    B = B @ R.T
+    B[:, 0] += 0.5
+    return (A, F, None), (B.copy(), F.copy(), None)
+
+
+def read_off(path: str):
+    """Read an OFF file; returns (V, F, S) with S None when all faces are triangles."""
+    with open(path, "r") as fh:
+        toks = []
+        for line in fh:
+            line = line.split("#", 1)[0].strip()
+            if line:
+                toks.extend(line.split())
+    assert toks[0] == "OFF", path
+    nv, nf = int(toks[1]), int(toks[2])
+    p = 4
+    V = np.asarray(toks[p:p + 3 * nv], dtype=np.float64).reshape(nv, 3)
+    p += 3 * nv
+    F, S = [], []
+    for _ in range(nf):
+        k = int(toks[p])
+        F.extend(int(t) for t in toks[p + 1:p + 1 + k])
+        S.append(k)
+        p += 1 + k
+    S = np.asarray(S, dtype=np.uint32)
+    F = np.asarray(F, dtype=np.uint32)
+    return V, F, (None if np.all(S == 3) else S)
+
+
+def newell_min_sq(V: np.ndarray, F: np.ndarray, S=None) -> float:
+    """Smallest squared Newell-normal length over the faces (plain float64)."""
+    if S is None:
+        T = V[F.reshape(-1, 3).astype(np.int64)]
+        n = np.zeros((T.shape[0], 3))
+        for i in range(3):
+            c, d = T[:, i], T[:, (i + 1) % 3]
+            n[:, 0] += (c[:, 1] - d[:, 1]) * (c[:, 2] + d[:, 2])
+            n[:, 1] += (c[:, 2] - d[:, 2]) * (c[:, 0] + d[:, 0])
+            n[:, 2] += (c[:, 0] - d[:, 0]) * (c[:, 1] + d[:, 1])
+        return float(np.min(np.sum(n * n, axis=1)))
+    raise NotImplementedError

@@ -1,0 +1,192 @@
+"""Write the golden fixtures in tests/golden/ from the reference MCUT.
+
+The reference library is compiled from the MCUT sources by ``make -C oracle
+ref`` (outputs in oracle/_ref/); this script calls it through the test probe
+(oracle/ref_probe.cpp) and stores inputs and the reference's outputs as .npz
+files.  Run from the repository root:
+
+    python tests/golden/make_golden.py
+
+Fixtures:
+  ref_functions.npz  per-call vectors of the geometry core (plane coefficients,
+                     orient2d values, orient3d signs, noncollinear triplets,
+                     segment/plane types, intersection points, point-in-polygon)
+  ref_cases.npz      end-to-end cases through mcDispatch: status, shift
+                     parameters, candidate face pairs, intersection points
+                     (seam vertices), perturbation vector, CC count
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from mcut_amd import meshgen as mg  # noqa: E402
+from oracle import refprobe as rp  # noqa: E402
+import meshes  # noqa: E402  (tests/meshes.py)
+
+FLAGS = rp.MC_DISPATCH_VERTEX_ARRAY_DOUBLE | rp.MC_DISPATCH_FILTER_ALL
+
+
+def random_polygon(rng, n, scale=1.0, base=2.0):
+    """Convex planar-ish polygon in the positive quadrant (shifted space)."""
+    c = base + rng.uniform(-0.5, 0.5, 3)
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    v = np.cross(u, rng.normal(size=3))
+    v /= np.linalg.norm(v)
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    rad = scale * rng.uniform(0.5, 1.0, n)
+    P = c + rad[:, None] * (np.cos(ang)[:, None] * u + np.sin(ang)[:, None] * v)
+    if rng.uniform() < 0.5:  # small off-plane noise: the polygon is no longer exactly planar
+        P += rng.normal(scale=1e-9, size=P.shape)
+    return P
+
+
+def function_vectors(seed=7, count=3000):
+    rng = np.random.default_rng(seed)
+    out = {k: [] for k in ("poly", "poly_n", "plane_n", "plane_d", "plane_k", "triple", "triple_ok", "q", "r",
+                           "seg_type", "seg_point", "pip_p", "pip")}
+    o2 = {"a": [], "b": [], "c": [], "v": []}
+    o3 = {"a": [], "b": [], "c": [], "d": [], "v": []}
+    for it in range(count):
+        n = int(rng.choice([3, 3, 3, 4, 5, 6, 7, 8]))
+        P = random_polygon(rng, n, scale=rng.choice([1e-3, 1e-1, 1.0]))
+        nrm, d, k = rp.plane(P)
+        if not np.any(nrm):
+            continue
+        ok, ijk = rp.best_triple(P, nrm, k) if n > 3 else (1, (0, 1, 2))
+        # segment endpoints: random, on the plane (vertex / edge point), or straddling
+        mode = it % 5
+        cen = P.mean(axis=0)
+        if mode == 0:
+            q, r = cen + nrm * 0.3, cen - nrm * 0.2
+        elif mode == 1:
+            q, r = P[0].copy(), cen + nrm * 0.5
+        elif mode == 2:
+            q = P[1] + nrm * 1e-14
+            r = P[2] - nrm * 1e-14
+        elif mode == 3:
+            w = rng.uniform(size=n)
+            w /= w.sum()
+            q = (w[:, None] * P).sum(0) + nrm * rng.uniform(0.1, 1)
+            r = (w[:, None] * P).sum(0) - nrm * rng.uniform(0.1, 1)
+        else:
+            q = cen + rng.normal(size=3) * 0.5
+            r = cen + rng.normal(size=3) * 0.5
+        ty = rp.segment_type(q, r, P, nrm, k)
+        _, p = rp.segment_point(nrm, d, q, r)
+        pip = rp.point_in_polygon(p, P, nrm, k)
+        pad = np.zeros((8, 3))
+        pad[:n] = P
+        out["poly"].append(pad)
+        out["poly_n"].append(n)
+        out["plane_n"].append(nrm)
+        out["plane_d"].append(d)
+        out["plane_k"].append(k)
+        out["triple"].append(ijk)
+        out["triple_ok"].append(ok)
+        out["q"].append(q)
+        out["r"].append(r)
+        out["seg_type"].append(ord(ty))
+        out["seg_point"].append(p)
+        out["pip_p"].append(p)
+        out["pip"].append(ord(pip))
+        # a vertex of the polygon and a point on one of its edges are PIP special cases
+        if it % 7 == 0:
+            for pt in (P[n - 1], 0.5 * P[0] + 0.5 * P[1]):
+                out["poly"].append(pad)
+                out["poly_n"].append(n)
+                out["plane_n"].append(nrm)
+                out["plane_d"].append(d)
+                out["plane_k"].append(k)
+                out["triple"].append(ijk)
+                out["triple_ok"].append(ok)
+                out["q"].append(q)
+                out["r"].append(r)
+                out["seg_type"].append(ord(ty))
+                out["seg_point"].append(p)
+                out["pip_p"].append(pt)
+                out["pip"].append(ord(rp.point_in_polygon(pt, P, nrm, k)))
+    # orient2d near-collinear triples exercise every adaptive stage
+    for it in range(4000):
+        a = rng.uniform(0, 4, 2)
+        b = rng.uniform(0, 4, 2)
+        t = rng.uniform(-1, 2)
+        c = a + t * (b - a)
+        if it % 4:
+            c = c + rng.normal(size=2) * 10.0 ** rng.uniform(-18, -8)
+        o2["a"].append(a)
+        o2["b"].append(b)
+        o2["c"].append(c)
+        o2["v"].append(rp.orient2d(a, b, c))
+    # orient3d near-coplanar points: signs
+    for it in range(4000):
+        a, b, c = (rng.uniform(1, 3, 3) for _ in range(3))
+        w = rng.uniform(-0.5, 1.5, 2)
+        d = a + w[0] * (b - a) + w[1] * (c - a)
+        if it % 3:
+            d = d + rng.normal(size=3) * 10.0 ** rng.uniform(-18, -10)
+        for key, val in zip("abcd", (a, b, c, d)):
+            o3[key].append(val)
+        o3["v"].append(np.sign(rp.orient3d(a, b, c, d)))
+    res = {k: np.asarray(v) for k, v in out.items()}
+    res.update({"o2_" + k: np.asarray(v) for k, v in o2.items()})
+    res.update({"o3_" + k: np.asarray(v) for k, v in o3.items()})
+    return res
+
+
+def case(name, src, cut, flags=FLAGS, store_pairs=True):
+    st, ms, ccs = rp.dispatch(src, cut, flags)
+    b = rp.broad_phase(src, cut)
+    pts = np.array(sorted(rp.seam_points(ccs)), dtype=np.float64).reshape(-1, 3)
+    nin = sorted(cc["face_sizes"].size for cc in ccs if cc["type"] == rp.CC_INPUT)
+    partitioned = bool(st == 0 and nin != sorted([meshes.nfaces(src), meshes.nfaces(cut)]))
+    pert = ccs[0]["perturbation"] if ccs else np.zeros(3)
+    d = {
+        f"{name}/status": np.int32(st),
+        f"{name}/com": b["com"],
+        f"{name}/t": b["t"],
+        f"{name}/eps": np.float64(b["eps"]),
+        f"{name}/points": pts,
+        f"{name}/partitioned": np.bool_(partitioned),
+        f"{name}/n_cc": np.int32(len(ccs)),
+        f"{name}/perturbation": pert,
+    }
+    if store_pairs:
+        d[f"{name}/pairs"] = b["pairs"].astype(np.uint32)
+    print(f"{name}: status={st} pairs={b['pairs'].shape[0]} points={pts.shape[0]} partitioned={partitioned}")
+    return d
+
+
+def main():
+    if not rp.available():
+        sys.exit("build the reference first: make -C oracle ref")
+    fv = function_vectors()
+    np.savez_compressed(os.path.join(HERE, "ref_functions.npz"), **fv)
+    cases = {}
+    src, cut = mg.config1()
+    cases.update(case("config1", src, cut))
+    cases.update(case("config1_enforce", src, cut, FLAGS | rp.MC_DISPATCH_ENFORCE_GENERAL_POSITION))
+    s2, c2 = meshes.config1_rotated()
+    cases.update(case("config1_rot", s2, c2))
+    for i in range(64):
+        s, c = meshes.bench_pair(i)
+        cases.update(case(f"bench{i:03d}", s, c))
+    # displaced icosphere pairs (the +-2% noise of the large benchmark makes the
+    # reference's own cutting stage partition or throw from level 4 up, so the
+    # point-set cases use gentler noise)
+    for lvl, amp in ((3, 0.02), (4, 0.02), (5, 0.02), (4, 0.001), (5, 0.005), (6, 0.005)):
+        s, c = meshes.icosphere_pair(lvl, amp=amp)
+        cases.update(case(f"ico{lvl}_{int(amp * 1000):03d}", s, c))
+    np.savez_compressed(os.path.join(HERE, "ref_cases.npz"), **cases)
+
+
+if __name__ == "__main__":
+    main()

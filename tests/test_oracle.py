@@ -1,0 +1,247 @@
+"""The CPU oracle pinned against the reference (golden fixtures written from the
+reference MCUT by tests/golden/make_golden.py) and against exact arithmetic.
+
+These tests run without a GPU and without the reference sources.
+"""
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import oracle as om
+from mcut_amd import meshgen as mg
+import meshes
+
+GOLD = meshes.GOLDEN
+FN = np.load(os.path.join(GOLD, "ref_functions.npz"))
+CASES = np.load(os.path.join(GOLD, "ref_cases.npz"))
+CASE_NAMES = sorted({k.split("/")[0] for k in CASES.files})
+
+
+def case(name, key):
+    return CASES[f"{name}/{key}"]
+
+
+def case_meshes(name):
+    if name.startswith("bench"):
+        return meshes.bench_pair(int(name[5:]))
+    if name == "config1_rot":
+        return meshes.config1_rotated()
+    if name.startswith("config1"):
+        return mg.config1()
+    if name.startswith("ico"):
+        lvl, amp = name[3:].split("_")
+        return meshes.icosphere_pair(int(lvl), amp=int(amp) / 1000.0)
+    raise KeyError(name)
+
+
+# ---------------------------------------------------------------- geometry core
+
+def test_plane_coefficients_bit_exact(oracle_lib):
+    for i in range(len(FN["poly_n"])):
+        n = int(FN["poly_n"][i])
+        P = FN["poly"][i][:n]
+        nrm, d, k = om.plane(P)
+        assert np.array_equal(nrm, FN["plane_n"][i]), i
+        assert d == FN["plane_d"][i], i
+        assert k == FN["plane_k"][i], i
+
+
+def test_noncollinear_triplet_choice(oracle_lib):
+    L = oracle_lib.lib()
+    checked = 0
+    for i in range(len(FN["poly_n"])):
+        n = int(FN["poly_n"][i])
+        if n == 3:
+            continue
+        P = np.ascontiguousarray(FN["poly"][i][:n])
+        nrm = np.ascontiguousarray(FN["plane_n"][i])
+        ijk = np.zeros(3, np.int32)
+        ok = L.om_best_triple(P.ctypes.data, n, nrm.ctypes.data, int(FN["plane_k"][i]), ijk.ctypes.data)
+        assert ok == FN["triple_ok"][i]
+        assert tuple(ijk) == tuple(FN["triple"][i]), (i, n)
+        checked += 1
+    assert checked > 500
+
+
+def test_segment_type_point_pip_bit_exact(oracle_lib):
+    L = oracle_lib.lib()
+    for i in range(len(FN["poly_n"])):
+        n = int(FN["poly_n"][i])
+        P = np.ascontiguousarray(FN["poly"][i][:n])
+        nrm = FN["plane_n"][i]
+        d = float(FN["plane_d"][i])
+        k = int(FN["plane_k"][i])
+        q, r = FN["q"][i], FN["r"][i]
+        # type: orient3d signs against the reference's triple
+        ijk = np.ascontiguousarray(FN["triple"][i], dtype=np.int32)
+        a, b, c = P[ijk[0]], P[ijk[1]], P[ijk[2]]
+        sq, sr = om.orient3d_sign(a, b, c, q), om.orient3d_sign(a, b, c, r)
+        ty = "p" if sq == 0 and sr == 0 else "q" if sq == 0 else "r" if sr == 0 else "0" if sq == sr else "1"
+        assert ord(ty) == FN["seg_type"][i], i
+        p = om.segment_point(nrm, d, q, r)
+        assert np.array_equal(p, FN["seg_point"][i]), i
+        assert ord(om.point_in_polygon(FN["pip_p"][i], P, nrm, k)) == FN["pip"][i], i
+
+
+def test_orient2d_values_bit_exact(oracle_lib):
+    for a, b, c, v in zip(FN["o2_a"], FN["o2_b"], FN["o2_c"], FN["o2_v"]):
+        got = om.orient2d(a, b, c)
+        assert got == v or (np.isnan(got) and np.isnan(v))
+
+
+def test_orient3d_signs_match_reference(oracle_lib):
+    for a, b, c, d, v in zip(FN["o3_a"], FN["o3_b"], FN["o3_c"], FN["o3_d"], FN["o3_v"]):
+        assert om.orient3d_sign(a, b, c, d) == v
+
+
+def _exact_det3(a, b, c, d):
+    A = [Fraction(x) - Fraction(y) for x, y in zip(a, d)]
+    B = [Fraction(x) - Fraction(y) for x, y in zip(b, d)]
+    C = [Fraction(x) - Fraction(y) for x, y in zip(c, d)]
+    return (A[0] * (B[1] * C[2] - B[2] * C[1]) - A[1] * (B[0] * C[2] - B[2] * C[0]) + A[2] * (B[0] * C[1] - B[1] * C[0]))
+
+
+def test_orient3d_sign_exact_rational(oracle_lib):
+    """Exact sign against rational arithmetic, with coplanar, near-coplanar,
+    tiny-difference and wide-magnitude inputs."""
+    rng = np.random.default_rng(11)
+    zero = 0
+    for it in range(3000):
+        a, b, c = (rng.uniform(-1, 1, 3) * 10.0 ** rng.integers(-3, 4) for _ in range(3))
+        mode = it % 4
+        if mode == 0:
+            d = rng.uniform(-1, 1, 3)
+        else:
+            # integer-valued points on a lattice plane are exactly coplanar in binary64
+            if mode == 1:
+                a, b, c = (np.round(x * 64) for x in (a, b, c))
+                d = a + 3 * (b - a) - 2 * (c - a)
+            else:
+                w = rng.uniform(-1, 2, 2)
+                d = a + w[0] * (b - a) + w[1] * (c - a)
+                d = d + rng.normal(size=3) * 10.0 ** rng.uniform(-19, -12) * (mode == 3)
+        s = om.orient3d_sign(a, b, c, d)
+        e = _exact_det3(a, b, c, d)
+        want = (e > 0) - (e < 0)
+        # orient3d(a,b,c,d) is positive when d lies below the plane (a,b,c): the
+        # determinant of (a-d, b-d, c-d)
+        assert s == want, (it, a, b, c, d)
+        zero += want == 0
+    assert zero > 500  # the exactly-coplanar branch is exercised
+
+
+def test_orient2d_sign_exact_rational(oracle_lib):
+    rng = np.random.default_rng(12)
+    for it in range(3000):
+        a, b = rng.uniform(-2, 2, 2), rng.uniform(-2, 2, 2)
+        c = a + rng.uniform(-1, 2) * (b - a)
+        if it % 3:
+            c = c + rng.normal(size=2) * 10.0 ** rng.uniform(-19, -12)
+        v = om.orient2d(a, b, c)
+        e = (Fraction(a[0]) - Fraction(c[0])) * (Fraction(b[1]) - Fraction(c[1])) - \
+            (Fraction(a[1]) - Fraction(c[1])) * (Fraction(b[0]) - Fraction(c[0]))
+        assert (v > 0) - (v < 0) == (e > 0) - (e < 0)
+
+
+def test_point_in_polygon_special_cases(oracle_lib):
+    P = np.array([[1.0, 1.0, 2.0], [3.0, 1.0, 2.0], [3.0, 3.0, 2.0], [1.0, 3.0, 2.0]])
+    nrm, d, k = om.plane(P)
+    assert k == 2 and d == 2.0
+    assert om.point_in_polygon(np.array([2.0, 2.0, 2.0]), P, nrm, k) == "i"
+    assert om.point_in_polygon(np.array([5.0, 2.0, 2.0]), P, nrm, k) == "o"
+    assert om.point_in_polygon(P[2], P, nrm, k) == "v"
+    assert om.point_in_polygon(np.array([2.0, 1.0, 2.0]), P, nrm, k) == "e"
+
+
+# ---------------------------------------------------------------- end to end
+
+def _point_set(a):
+    return set(map(tuple, np.asarray(a).reshape(-1, 3).tolist()))
+
+
+@pytest.mark.parametrize("name", [n for n in CASE_NAMES if not n.startswith("ico6")])
+def test_cases_against_reference(oracle_lib, name):
+    src, cut = case_meshes(name)
+    enforce = name == "config1_enforce"
+    pert = case(name, "perturbation") if enforce else None
+    R = om.frontend(src, cut, enforce_gp=enforce, enforcement_count=1 if enforce else 0, perturbation=pert,
+                    detail=False)
+    assert np.array_equal(R["com"], case(name, "com"))
+    assert np.array_equal(R["t"], case(name, "t"))
+    assert R["eps"] == float(case(name, "eps"))
+    if not enforce:
+        # the reference lists each face's candidates in traversal order; compare as sorted pairs
+        ref_pairs = case(name, "pairs")
+        ref_pairs = ref_pairs[np.lexsort((ref_pairs[:, 1], ref_pairs[:, 0]))]
+        assert np.array_equal(R["pairs"], ref_pairs)
+    st = int(case(name, "status"))
+    if st == 0:
+        assert R["status"] == om.SUCCESS
+        if not bool(case(name, "partitioned")):
+            assert _point_set(R["reg_point"]) == _point_set(case(name, "points"))
+    elif st == -2 and name in ("config1", "bench061", "bench062"):
+        assert R["status"] != om.SUCCESS  # rejected by the front end itself
+
+
+@pytest.mark.slow
+def test_ico6_case_against_reference(oracle_lib):
+    test_cases_against_reference(oracle_lib, "ico6_005")
+
+
+def test_candidate_pairs_brute_force(oracle_lib):
+    """Candidate pairs are every (src, cut) face pair with overlapping AABBs."""
+    for src, cut in (meshes.config1_rotated(), meshes.icosphere_pair(3), meshes.bench_pair(30)):
+        R = om.frontend(src, cut)
+        V = R["ps_vertices"]
+        ns, nc = meshes.nfaces(src), meshes.nfaces(cut)
+
+        def boxes(m, off, enlarge):
+            Vv, F, S = m
+            sizes = np.full(meshes.nfaces(m), 3) if S is None else S
+            starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+            lo = np.array([V[off + F[s:s + n]].min(0) for s, n in zip(starts, sizes)])
+            hi = np.array([V[off + F[s:s + n]].max(0) for s, n in zip(starts, sizes)])
+            return lo - enlarge, hi + enlarge
+
+        slo, shi = boxes(src, 0, 0.0)
+        clo, chi = boxes(cut, src[0].shape[0], R["eps"])
+        ov = np.all((slo[:, None, :] <= chi[None]) & (shi[:, None, :] >= clo[None]), axis=2)
+        want = np.argwhere(ov)
+        assert np.array_equal(R["pairs"], want.astype(np.uint32))
+
+
+def test_registry_points_against_exact_rationals(oracle_lib):
+    """Each intersection point is within 1e-12 (relative) of the exact
+    intersection of the segment line with the exact plane through the
+    triangle; the exact signs of the two endpoints straddle the plane."""
+    src, cut = meshes.config1_rotated()
+    R = om.frontend(src, cut)
+    V = R["ps_vertices"]
+    fv = {}
+    Fs = meshes.nfaces(src)
+    Vs = src[0].shape[0]
+    for f in range(Fs):
+        a, b, c = src[1][3 * f:3 * f + 3]
+        fv[f] = [b, c, a]
+    for f in range(meshes.nfaces(cut)):
+        u0, u1, u2 = cut[1][3 * f:3 * f + 3] + Vs
+        fv[Fs + f] = [u2, u0, u1]
+    worst = 0.0
+    for e, g, p in zip(R["reg_edge"], R["reg_face"], R["reg_point"]):
+        q = [Fraction(x) for x in V[R["edge_v0"][e]]]
+        r = [Fraction(x) for x in V[R["edge_v1"][e]]]
+        A, B, Cc = ([Fraction(x) for x in V[i]] for i in fv[int(g)])
+
+        def o3(dd):
+            return _exact_det3(A, B, Cc, dd)
+
+        oq, orr = o3(q), o3(r)
+        assert (oq > 0) != (orr > 0) and oq != 0 and orr != 0
+        t = oq / (oq - orr)
+        exact = [qi + t * (ri - qi) for qi, ri in zip(q, r)]
+        for pi, xi in zip(p, exact):
+            err = abs(Fraction(pi) - xi) / max(abs(xi), Fraction(1))
+            worst = max(worst, float(err))
+    assert worst <= 1e-12, worst
