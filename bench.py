@@ -1,0 +1,155 @@
+"""Front-end benchmark: intersecting edge-face pairs per second on one MI355X.
+
+Workload: two icospheres at subdivision 7 (327,680 triangles each), every
+vertex scaled radially by 1 + U(-0.02, 0.02) (seeds 1 and 2), the second
+rotated by a random rotation (seed 3) and moved +0.5 in x; float64
+coordinates, synthetic data.  A step is one full front end on the GPU
+(shift, polygon-soup numbering, both LBVHs, traversal, edge-face tests, face
+planes, narrow phase, sorted registry, status) with the input meshes already
+resident in HBM.
+
+Prints one JSON line.  The CPU baseline is the single-threaded C restatement
+of the same front end (oracle/) timed on this host on the same input.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(name, n):
+    """Algorithmic HBM bytes of one front-end stage (each datum moved once)."""
+    V, F, K, E = n["V"], n["F"], n["K"], n["E"]
+    C, T, I, FC, ESC = n["pairs"], n["tests"], n["registry"], n["cand"], n["escalated"]
+    table = {
+        # sequential coordinate sums + bounding boxes: every coordinate read once
+        "shift_params": 24 * V,
+        # edge numbering: face lists read, corner->edge written, edge table written
+        "topology": 4 * K + 4 * K + 16 * E,
+        # shifted coordinates written, face boxes written (vertex reads counted once)
+        "shift_boxes": 24 * V + 24 * V + 4 * K + 48 * F,
+        # Morton keys, one sort pass moving (key, face), node links and node boxes
+        "bvh_build": 48 * F + 8 * F + 16 * F + 12 * F + 48 * F,
+        # query boxes read once, tree read once, pairs written
+        "traversal": 48 * F + 56 * F + 8 * C,
+        # candidate flags, candidate-face vertices (3 x 24 B) and planes (40 B)
+        "face_planes": F + FC * (72 + 40),
+        # pairs read, tests written, per test: edge endpoints and face box
+        "tests": 8 * C + 8 * T + T * (16 + 48 + 48),
+        # per test: (edge, face), endpoint ids and coordinates, plane, face
+        # vertices; registry entries written (key 8 B + point 24 B)
+        "narrow": T * (8 + 8 + 48 + 40 + 12 + 72) + 32 * I,
+        # registry keys sorted once, entries gathered, status keys
+        "registry": 32 * I + 40 * I + 16 * I,
+    }
+    return table.get(name, 0)
+
+
+def cpu_baseline(src, cut, registry):
+    from oracle import oracle as om
+    t0 = time.perf_counter()
+    O = om.frontend(src, cut, detail=False)
+    dt = time.perf_counter() - t0
+    assert O["counts"]["registry"] == registry, (O["counts"], registry)
+    return {"value": registry / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": f"the whole config front end once on the host ({dt:.2f} s, single thread)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--level", type=int, default=7)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mcut_amd import frontend, meshgen as mg
+
+    src, cut = mg.config2(level=args.level)
+    dev = torch.device("cuda:0")
+    keep = []
+
+    def to_dev(m):
+        V, F, _ = m
+        tv = torch.from_numpy(np.ascontiguousarray(V)).to(dev)
+        tf = torch.from_numpy(np.ascontiguousarray(F).view(np.int32)).to(dev)
+        keep.extend([tv, tf])
+        return frontend.device_mesh(tv.data_ptr(), tf.data_ptr(), None, V.shape[0], F.size // 3)
+
+    ms, mc = to_dev(src), to_dev(cut)
+    torch.cuda.synchronize()
+    ctx = frontend.Context(0)
+    for _ in range(args.warmup):
+        R = ctx.dispatch_device(ms, mc)
+    torch.cuda.synchronize()
+    lat = []
+    stage_tot = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        R = ctx.dispatch_device(ms, mc)
+        lat.append((time.perf_counter() - s0) * 1e3)
+        for k, v in R.stage_ms().items():
+            stage_tot[k] = stage_tot.get(k, 0.0) + v
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed * 1e3 / args.steps
+    I = R.counts["registry"]
+    assert R.status == 0, R.status_name
+
+    stages = {k: v / args.steps for k, v in stage_tot.items()}
+    counts = dict(R.counts)
+    n = dict(V=src[0].shape[0] + cut[0].shape[0], F=(src[1].size + cut[1].size) // 3, K=src[1].size + cut[1].size,
+             E=counts["edges"], pairs=counts["pairs"], tests=counts["tests"], registry=I, cand=counts["cand"],
+             escalated=counts["escalated"])
+    dom = max(stages, key=stages.get)
+    dom_bytes = stage_bytes(dom, n)
+    achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9 if stages[dom] > 0 else 0.0
+    lat_sorted = sorted(lat)
+
+    line = {
+        "metric": "intersecting edge-face pairs/sec (front end, one mcDispatch pair)",
+        "value": I / (ms_step * 1e-3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded displaced icospheres)",
+        "config": {"workload": f"icosphere-{args.level} vs rotated icosphere-{args.level} "
+                               f"({n['F'] // 2} tris each, +-2% radial noise, offset +0.5x)",
+                   "parallelism": "single GPU"},
+        "frontend_device_ms": R.device_ms,
+        "latency_ms": {"p50": lat_sorted[len(lat) // 2], "p99": lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))]},
+        "counts": counts,
+        "stage_ms": stages,
+        "roofline": {"bound": "hbm", "kernel_stage": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frontend_alg_bytes": sum(stage_bytes(k, n) for k in stages)},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(src, cut, I)
+        line["speedup_vs_cpu_baseline"] = line["value"] / line["cpu_baseline"]["value"]
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

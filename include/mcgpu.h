@@ -1,0 +1,168 @@
+/*
+ * mcgpu -- MI355X intersection front end for MCUT (C ABI).
+ *
+ * This library replaces MCUT's intersection front end: the broad phase of
+ * source/bvh.cpp and the narrow phase at the head of dispatch() in
+ * source/kernel.cpp.  Given a source mesh and a cut mesh exactly as
+ * mcDispatch receives them (include/mcut/mcut.h:1008-1020), it computes on the
+ * GPU:
+ *   - the coordinate shift of preproc.cpp:2124-2290 (vertices moved into the
+ *     positive quadrant; all outputs are in that shifted space, as in MCUT);
+ *   - the polygon-soup numbering of edges and halfedges (hmesh.cpp:406-733,
+ *     kernel.cpp:1593-1732);
+ *   - the candidate face pairs whose AABBs overlap (bvh.cpp:219-783);
+ *   - the edge-face tests after expansion and culling (kernel.cpp:1779-2177);
+ *   - the plane of every candidate face (kernel.cpp:2184-2356);
+ *   - the edge-face narrow phase with exact orient3d signs and the
+ *     intersection registry with its points (kernel.cpp:2415-2878), sorted by
+ *     (edge, face);
+ *   - the dispatch status the reference derives from these
+ *     (kernel.cpp:2237-2331, 2532-2604, 3211-3229, 3439-3452).
+ *
+ * Plain C types only.  Every function returns an mcgpu_result; no exception
+ * crosses this boundary.  A context owns one HIP stream and its device
+ * buffers; it must not be used from two threads at once.
+ */
+#ifndef MCGPU_H_
+#define MCGPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define MCGPU_API __attribute__((visibility("default")))
+#else
+#define MCGPU_API
+#endif
+
+typedef struct mcgpu_context_t* mcgpu_context;
+
+/* Return codes of every entry point (the ABI's McResult, mcut.h:202-208). */
+typedef enum {
+    MCGPU_SUCCESS = 0,
+    MCGPU_ERROR_INVALID_VALUE = -1, /* bad argument: null pointer, size 0, face size < 3, index out of range */
+    MCGPU_ERROR_DEVICE = -2,        /* HIP runtime error */
+    MCGPU_ERROR_OUT_OF_MEMORY = -3,
+    MCGPU_ERROR_UNSUPPORTED = -4,   /* candidate polygon with more than 16 vertices */
+    MCGPU_ERROR_NO_RESULT = -5      /* query before a successful dispatch */
+} mcgpu_result;
+
+/* Front-end status: the values of MCUT's status_t (kernel.h:51-86). */
+typedef enum {
+    MCGPU_STATUS_SUCCESS = 0,
+    MCGPU_STATUS_INVALID_SRC_MESH = -1,
+    MCGPU_STATUS_INVALID_CUT_MESH = -2,
+    MCGPU_STATUS_INVALID_MESH_INTERSECTION = -3,
+    MCGPU_STATUS_GENERAL_POSITION_VIOLATION = -4
+} mcgpu_status;
+
+/* MCUT dispatch flags the front end reads (mcut.h:379-441). */
+#define MCGPU_DISPATCH_REQUIRE_THROUGH_CUTS (1u << 2)
+#define MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION (1u << 15)
+#define MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE (1u << 16)
+
+/* A mesh as mcDispatch takes it: xyz doubles per vertex, concatenated face
+ * vertex indices, and face sizes (NULL = every face is a triangle,
+ * preproc.cpp:206).  Pointers are host memory unless
+ * mcgpu_dispatch_desc.input_on_device is set. */
+typedef struct {
+    const double* vertices;
+    const uint32_t* face_indices;
+    const uint32_t* face_sizes;
+    uint32_t num_vertices;
+    uint32_t num_faces;
+} mcgpu_mesh;
+
+typedef struct {
+    uint32_t flags;              /* MCGPU_DISPATCH_* bits, same values as MC_DISPATCH_* */
+    double gp_constant;          /* MC_CONTEXT_GENERAL_POSITION_ENFORCEMENT_CONSTANT (default 1e-4) */
+    uint32_t enforcement_count;  /* perturbation attempts made so far (input_t, kernel.h:125-172) */
+    uint32_t input_on_device;    /* 1: mesh pointers are device pointers already resident in HBM */
+    uint32_t keep_tests;         /* 1: keep per-test signs/types for mcgpu_get_tests */
+    uint32_t reserved;
+} mcgpu_dispatch_desc;
+
+typedef struct {
+    int32_t status;              /* mcgpu_status */
+    uint32_t gp_violation;       /* narrow phase saw a touching or edge/vertex hit */
+    uint64_t num_candidate_pairs;
+    uint64_t num_candidate_faces;
+    uint64_t num_edges;          /* polygon-soup edges */
+    uint64_t num_tests;          /* edge-face tests after expansion, dedup and culling */
+    uint64_t num_escalated;      /* tests whose orient3d signs needed exact arithmetic */
+    uint64_t num_registry;       /* intersecting (edge, face) pairs */
+    double com[3];               /* shift: x' = (x - com) + translation */
+    double translation[3];
+    double eps;                  /* cut-face AABB enlargement */
+    float device_ms;             /* GPU time of the whole front end */
+    float reserved;
+} mcgpu_frontend_info;
+
+/* --- context (one per MCUT context / API thread) ---------------------------- */
+
+/* mcCreateContext (mcut.h:567) counterpart: binds a device and creates the stream. */
+MCGPU_API mcgpu_result mcgpu_create_context(int device, mcgpu_context* out);
+/* mcReleaseContext (mcut.h:1377) counterpart. */
+MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx);
+MCGPU_API const char* mcgpu_result_string(mcgpu_result r);
+
+/* --- front end ---------------------------------------------------------------
+ * Replaces, in one call, preproc.cpp:2304-2902 (shift, BVH builds
+ * bvh.cpp:219, traversal bvh.cpp:638) and kernel.cpp:1779-3229 (edge-face
+ * pairs, cull, face planes, narrow phase, registry).  Returns after the
+ * front end has finished; results stay on the device for the getters below. */
+MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
+                                               const mcgpu_dispatch_desc* desc, mcgpu_frontend_info* info);
+
+/* Re-runs the narrow phase with the cut mesh moved by `perturbation` (3
+ * doubles), keeping the candidate pairs and cut-face AABBs of the last
+ * mcgpu_dispatch_frontend, as the reference's general-position retry does
+ * (preproc.cpp:2590-2698, 2826). */
+MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const double* perturbation,
+                                                  uint32_t enforcement_count, mcgpu_frontend_info* info);
+
+/* --- results of the last dispatch (host copies; capacity in elements) ----- */
+
+/* Intersection registry sorted by (edge, face): m0_ivtx_to_intersection_registry_entry
+ * and the intersection points (kernel.cpp:2603-2615).  points: 3 doubles per entry. */
+MCGPU_API mcgpu_result mcgpu_get_registry(mcgpu_context ctx, uint32_t* edges, uint32_t* faces, double* points,
+                                          uint64_t capacity);
+/* Candidate (src face, cut face) pairs, sorted: the contents of
+ * ps_face_to_potentially_intersecting_others (bvh.cpp:709-716) without the mirror. */
+MCGPU_API mcgpu_result mcgpu_get_candidate_pairs(mcgpu_context ctx, uint32_t* pairs, uint64_t capacity);
+/* Polygon-soup edges: h0 = v0 -> v1, face of h0 / h1 (-1 = border). */
+MCGPU_API mcgpu_result mcgpu_get_edges(mcgpu_context ctx, uint32_t* v0, uint32_t* v1, int32_t* f0, int32_t* f1,
+                                       uint64_t capacity);
+/* Edge-face tests sorted by (edge, face) with orient3d signs of q and r, the
+ * segment/plane type ('0','1','p','q','r') and the last point-in-polygon
+ * result ('-' when none).  Needs desc.keep_tests. */
+MCGPU_API mcgpu_result mcgpu_get_tests(mcgpu_context ctx, uint32_t* edges, uint32_t* faces, int8_t* qsign,
+                                       int8_t* rsign, char* type, char* pip, uint64_t capacity);
+/* Planes of the candidate faces sorted by face: unit normal, d, largest
+ * component, the three vertices used by orient3d (kernel.cpp:2188-2191). */
+MCGPU_API mcgpu_result mcgpu_get_face_planes(mcgpu_context ctx, uint32_t* faces, double* normals, double* d,
+                                             int32_t* max_comp, int32_t* triple, uint64_t capacity);
+/* Shifted polygon-soup vertex coordinates (source vertices, then cut vertices). */
+MCGPU_API mcgpu_result mcgpu_get_ps_vertices(mcgpu_context ctx, double* xyz, uint64_t capacity);
+
+/* Device pointers to the sorted registry (valid until the next dispatch on ctx),
+ * for consumers that stay on the GPU. */
+MCGPU_API mcgpu_result mcgpu_registry_device(mcgpu_context ctx, const uint32_t** edges, const uint32_t** faces,
+                                             const double** points, uint64_t* count);
+
+/* --- measurement -------------------------------------------------------------- */
+
+/* Per-stage device times of the last dispatch in ms (stage names via
+ * mcgpu_stage_name); returns the number of stages written. */
+MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity);
+MCGPU_API const char* mcgpu_stage_name(int stage);
+/* Average device time of the narrow-phase kernel over the last dispatch. */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCGPU_H_ */

@@ -1,0 +1,153 @@
+// Front-end kernels: shift parameters, polygon-soup topology, LBVH broad
+// phase, edge-face test expansion, face planes and the narrow phase.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "geom.hpp"
+
+namespace mcgpu {
+
+constexpr uint64_t kEmpty64 = ~0ull;
+constexpr int kBlock = 256;
+constexpr int kMaxPoly = 16; // largest candidate polygon whose triplet choice is replayed
+
+struct Params {
+    double com[3];      // shift x' = ((x - com) + t) + pert
+    double t[3];
+    double eps;         // cut-face AABB enlargement
+    double diag;
+    double pert[3];
+    int has_pert;
+    int eps_absolute;
+    double gp_constant;
+    double sum[2][3];   // per-mesh sequential coordinate sums
+    double bmin[2][3];  // per-mesh raw bounding boxes
+    double bmax[2][3];
+};
+
+struct Counters {
+    unsigned long long n_pairs, n_tests, n_esc, n_reg, n_big, n_cand;
+    unsigned long long n_edges_src, n_edges_cut;
+    unsigned int ov_pairs, ov_tests, ov_esc, ov_reg, ov_big;
+    unsigned int gp, bad_src, bad_cut, unsupported, nonmanifold, bad_input, pair_fail, partial;
+    unsigned int pad[2];
+};
+
+// polygon soup view: source faces [0,Fs), cut faces [Fs, Fs+Fc); vertices likewise
+struct Soup {
+    const uint32_t* idx[2];  // face vertex indices (mesh-local)
+    const uint32_t* off[2];  // face offsets (nf+1)
+    uint32_t nv[2], nf[2];
+    const double* ps_v;      // shifted coordinates, xyz per ps vertex
+    const double* box;       // per ps face: min xyz, max xyz (cut faces enlarged)
+    const uint32_t* cedge[2];// per corner (mesh corner order): ps edge id
+    const uint32_t* ev0;     // per ps edge: h0 = v0 -> v1
+    const uint32_t* ev1;
+    const int32_t* ef0;      // face of h0, h1 (-1 none)
+    const int32_t* ef1;
+};
+
+__device__ __forceinline__ void load3(const double* p, double* o)
+{
+    o[0] = p[0];
+    o[1] = p[1];
+    o[2] = p[2];
+}
+
+// ps vertex list of ps face g (targets of its halfedges): source faces are
+// stored [v1..v_{n-1}, v0], cut faces [u2..u0, u1] (hmesh.cpp:705-733,
+// kernel.cpp:1717-1727)
+struct FaceView {
+    const uint32_t* idx;
+    uint32_t base, n, rot, vofs;
+    const double* ps_v;
+    __device__ __forceinline__ uint32_t vid(int i) const
+    {
+        uint32_t j = (uint32_t)i + rot;
+        if (j >= n) j -= n;
+        if (j >= n) j -= n;
+        return vofs + idx[base + j];
+    }
+    __device__ __forceinline__ void operator()(int i, double* o) const { load3(ps_v + 3 * (uint64_t)vid(i), o); }
+};
+
+__device__ __forceinline__ FaceView face_view(const Soup& S, uint32_t g)
+{
+    FaceView v;
+    const int m = g >= S.nf[0];
+    const uint32_t f = m ? g - S.nf[0] : g;
+    v.idx = S.idx[m];
+    v.base = S.off[m][f];
+    v.n = S.off[m][f + 1] - v.base;
+    v.rot = m ? 2 : 1;
+    v.vofs = m ? S.nv[0] : 0;
+    v.ps_v = S.ps_v;
+    return v;
+}
+
+struct Plane {
+    double n[3];
+    double d;
+    int32_t k;      // largest normal component; -1 = degenerate
+    uint8_t tri[3]; // vertex triple for orient3d; tri[0] = 0xff: all collinear
+    uint8_t big;    // triple decided by the large-polygon pass
+};
+
+__device__ __forceinline__ bool boxes_overlap(const double* a, const double* b)
+{
+    // intersect_bounding_boxes, math.h:931-941 (inclusive)
+    return (a[0] <= b[3] && a[3] >= b[0]) && (a[1] <= b[4] && a[4] >= b[1]) && (a[2] <= b[5] && a[5] >= b[2]);
+}
+
+__device__ __forceinline__ uint64_t hash64(uint64_t k)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// insert-or-find; returns slot
+// Tables are sized to at least twice the keys they receive, so probing ends at
+// an empty slot; the probe count is still bounded by the table size.
+__device__ __forceinline__ uint64_t hash_insert(unsigned long long* keys, uint64_t mask, uint64_t key)
+{
+    uint64_t h = hash64(key) & mask;
+    for (uint64_t probe = 0; probe <= mask; ++probe) {
+        unsigned long long prev = atomicCAS(&keys[h], (unsigned long long)kEmpty64, (unsigned long long)key);
+        if (prev == kEmpty64 || prev == key) return h;
+        h = (h + 1) & mask;
+    }
+    return h; // unreachable while the table is at most half full
+}
+
+__device__ __forceinline__ int64_t hash_find(const unsigned long long* keys, uint64_t mask, uint64_t key)
+{
+    uint64_t h = hash64(key) & mask;
+    for (uint64_t probe = 0; probe <= mask; ++probe) {
+        unsigned long long k = keys[h];
+        if (k == key) return (int64_t)h;
+        if (k == kEmpty64) return -1;
+        h = (h + 1) & mask;
+    }
+    return -1;
+}
+
+// wavefront-aggregated append: every active lane gets a slot; returns its index
+__device__ __forceinline__ unsigned long long wave_append(unsigned long long* counter)
+{
+    const uint64_t active = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)active) - 1;
+    const uint32_t rank = __popcll(active & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(active));
+    base = __shfl(base, leader, 64);
+    return base + rank;
+}
+
+} // namespace mcgpu

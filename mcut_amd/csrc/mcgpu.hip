@@ -1,0 +1,835 @@
+// Host side of libmcgpu: the C ABI of include/mcgpu.h, device buffers, and
+// the enqueue order of the front-end pipeline.
+#include "mcgpu.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "frontend_kernels.hpp"
+#include "sort.hpp"
+
+using namespace mcgpu;
+
+namespace {
+
+struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+enum Stage {
+    ST_PARAMS,
+    ST_TOPOLOGY,
+    ST_SHIFT,
+    ST_BVH,
+    ST_TRAVERSE,
+    ST_PLANES,
+    ST_TESTS,
+    ST_NARROW,
+    ST_REGISTRY,
+    ST_COUNT
+};
+
+const char* kStageNames[ST_COUNT] = {"shift_params", "topology", "shift_boxes", "bvh_build", "traversal",
+                                     "face_planes",  "tests",    "narrow",      "registry"};
+
+uint64_t next_pow2(uint64_t x)
+{
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+int bits_for(uint64_t n)
+{
+    int b = 1;
+    while ((1ull << b) < n) ++b;
+    return b;
+}
+
+} // namespace
+
+struct mcgpu_context_t {
+    int device = 0;
+    hipStream_t s = nullptr;
+    hipEvent_t ev[ST_COUNT + 1];
+    hipError_t last_error = hipSuccess;
+
+    Buf in_v[2], in_idx[2], in_sz[2];
+    Buf off[2], params, counters, ps_v, box;
+    Buf hkeys[2], hfirst[2], hcount[2], eflag[2], eid[2], cedge[2], etotal;
+    Buf ev0, ev1, ef0, ef1;
+    Buf mcode[2], mface[2], mcode2[2], mface2[2];
+    Buf child[2], nparent[2], lparent[2], visits[2], nodebox[2];
+    Buf pairs, phash, cand, planes, big_list, big_scratch;
+    Buf te, tg, tqs, trs, ttype, tpip, esc;
+    Buf rkey, rval, rkey2, rval2, rp, re, rg, rpts, rcount;
+    Buf shkeys, shcnt;
+    Buf scan_tmp, hist;
+
+    uint64_t cap_pairs = 0, cap_tests = 0, cap_esc = 0, cap_reg = 0, cap_big = 0;
+
+    // state of the last dispatch
+    bool have = false;
+    uint32_t nv[2] = {0, 0}, nf[2] = {0, 0};
+    uint64_t ncorner[2] = {0, 0};
+    const double* dv[2] = {nullptr, nullptr};
+    const uint32_t* didx[2] = {nullptr, nullptr};
+    const uint32_t* dsz[2] = {nullptr, nullptr};
+    int gbits = 1, kbits = 2;
+    uint32_t flags = 0, enforcement_count = 0;
+    double gp_constant = 1e-4;
+    int keep_tests = 0;
+    bool mface_alt[2] = {false, false};
+    bool reg_alt = false;
+    Counters C;
+    Params P;
+    float stage_ms[ST_COUNT];
+    float total_ms = 0.f;
+};
+
+#define CK(x)                                                                                                          \
+    do {                                                                                                               \
+        hipError_t e_ = (x);                                                                                           \
+        if (e_ != hipSuccess) {                                                                                        \
+            ctx->last_error = e_;                                                                                      \
+            return e_ == hipErrorOutOfMemory ? MCGPU_ERROR_OUT_OF_MEMORY : MCGPU_ERROR_DEVICE;                         \
+        }                                                                                                              \
+    } while (0)
+
+namespace {
+
+template <typename T>
+hipError_t ensure(Buf& b, uint64_t count, T** out)
+{
+    size_t bytes = (size_t)std::max<uint64_t>(count, 1) * sizeof(T);
+    if (b.cap < bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        size_t want = bytes + bytes / 4;
+        hipError_t e = hipMalloc(&b.p, want);
+        if (e != hipSuccess) return e;
+        b.cap = want;
+    }
+    *out = (T*)b.p;
+    return hipSuccess;
+}
+
+template <typename T>
+T* P_(Buf& b)
+{
+    return (T*)b.p;
+}
+
+inline unsigned grid_for(uint64_t n, unsigned block = kBlock, unsigned maxg = 4096)
+{
+    uint64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > maxg) g = maxg;
+    return (unsigned)g;
+}
+
+__global__ void k_reset_narrow(Counters* C)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        C->n_tests = C->n_esc = C->n_reg = C->n_big = C->n_cand = 0;
+        C->ov_tests = C->ov_esc = C->ov_reg = C->ov_big = 0;
+        C->gp = C->bad_src = C->bad_cut = C->unsupported = C->pair_fail = C->partial = 0;
+    }
+}
+
+void fill_u32(void* p, uint32_t v, uint64_t n, hipStream_t s)
+{
+    k_fill_u32<<<grid_for(n), kBlock, 0, s>>>((uint32_t*)p, v, n);
+}
+
+void fill_u64(void* p, uint64_t v, uint64_t n, hipStream_t s)
+{
+    k_fill_u64<<<grid_for(n), kBlock, 0, s>>>((unsigned long long*)p, (unsigned long long)v, n);
+}
+
+mcgpu_result check_mesh(const mcgpu_mesh* m)
+{
+    if (!m || !m->vertices || !m->face_indices || m->num_vertices < 3 || m->num_faces < 1)
+        return MCGPU_ERROR_INVALID_VALUE;
+    return MCGPU_SUCCESS;
+}
+
+int32_t decide_status(const Counters& C, uint32_t flags, uint32_t enforcement_count)
+{
+    const bool enforce =
+        (flags & (MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION | MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE)) != 0;
+    // kernel.cpp:2320-2331 (zero-area candidate face), 2532-2604 (narrow-phase
+    // general-position violation), 3211-3229 (empty registry), 3269-3278
+    // (partial cut with through-cuts required), 3439-3452 (fewer than two
+    // intersection points on a face pair)
+    if (C.bad_cut) return MCGPU_STATUS_INVALID_CUT_MESH;
+    if (C.bad_src) return MCGPU_STATUS_INVALID_SRC_MESH;
+    if (C.gp) return MCGPU_STATUS_GENERAL_POSITION_VIOLATION;
+    if (C.n_reg == 0)
+        return (enforce && enforcement_count > 0) ? MCGPU_STATUS_GENERAL_POSITION_VIOLATION : MCGPU_STATUS_SUCCESS;
+    if (C.partial && (flags & MCGPU_DISPATCH_REQUIRE_THROUGH_CUTS)) return MCGPU_STATUS_SUCCESS;
+    if (C.pair_fail)
+        return enforce ? MCGPU_STATUS_GENERAL_POSITION_VIOLATION : MCGPU_STATUS_INVALID_MESH_INTERSECTION;
+    return MCGPU_STATUS_SUCCESS;
+}
+
+Soup make_soup(mcgpu_context ctx)
+{
+    Soup S;
+    for (int m = 0; m < 2; ++m) {
+        S.idx[m] = ctx->didx[m];
+        S.off[m] = P_<uint32_t>(ctx->off[m]);
+        S.nv[m] = ctx->nv[m];
+        S.nf[m] = ctx->nf[m];
+        S.cedge[m] = P_<uint32_t>(ctx->cedge[m]);
+    }
+    S.ps_v = P_<double>(ctx->ps_v);
+    S.box = P_<double>(ctx->box);
+    S.ev0 = P_<uint32_t>(ctx->ev0);
+    S.ev1 = P_<uint32_t>(ctx->ev1);
+    S.ef0 = P_<int32_t>(ctx->ef0);
+    S.ef1 = P_<int32_t>(ctx->ef1);
+    return S;
+}
+
+// buffers whose size depends only on the meshes
+mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
+{
+    const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
+    const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1];
+    const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
+    void* p;
+    Params* pp;
+    Counters* cc;
+    CK(ensure(ctx->params, 1, &pp));
+    CK(ensure(ctx->counters, 1, &cc));
+    double* dp;
+    CK(ensure(ctx->ps_v, 3 * V, &dp));
+    CK(ensure(ctx->box, 6 * F, &dp));
+    uint32_t* u;
+    int32_t* i32;
+    unsigned long long* ull;
+    CK(ensure(ctx->ev0, K, &u));
+    CK(ensure(ctx->ev1, K, &u));
+    CK(ensure(ctx->ef0, K, &i32));
+    CK(ensure(ctx->ef1, K, &i32));
+    CK(ensure(ctx->etotal, 2, &u));
+    uint64_t scan_need = scan_tmp_elems(K + 8);
+    for (int m = 0; m < 2; ++m) {
+        const uint64_t nk = ctx->ncorner[m], nf = ctx->nf[m];
+        const uint64_t hs = next_pow2(2 * nk + 2);
+        CK(ensure(ctx->off[m], nf + 1, &u));
+        CK(ensure(ctx->hkeys[m], hs, &ull));
+        CK(ensure(ctx->hfirst[m], hs, &ull));
+        CK(ensure(ctx->hcount[m], hs, &u));
+        CK(ensure(ctx->eflag[m], nk, &u));
+        CK(ensure(ctx->eid[m], nk, &u));
+        CK(ensure(ctx->cedge[m], nk, &u));
+        CK(ensure(ctx->mcode[m], nf, &u));
+        CK(ensure(ctx->mface[m], nf, &u));
+        CK(ensure(ctx->mcode2[m], nf, &u));
+        CK(ensure(ctx->mface2[m], nf, &u));
+        CK(ensure(ctx->child[m], 2 * nf, &u));
+        CK(ensure(ctx->nparent[m], nf, &i32));
+        CK(ensure(ctx->lparent[m], nf, &i32));
+        CK(ensure(ctx->visits[m], nf, &u));
+        CK(ensure(ctx->nodebox[m], 6 * nf, &dp));
+        scan_need = std::max<uint64_t>(scan_need, scan_tmp_elems(nf + 8));
+    }
+    uint8_t* b8;
+    CK(ensure(ctx->cand, F + 8, &b8));
+    Plane* pl;
+    CK(ensure(ctx->planes, F, &pl));
+    (void)p;
+    CK(ensure(ctx->scan_tmp, scan_need + 1024, &u));
+    return MCGPU_SUCCESS;
+}
+
+mcgpu_result alloc_list_buffers(mcgpu_context ctx)
+{
+    uint32_t* u;
+    unsigned long long* ull;
+    double* dp;
+    CK(ensure(ctx->pairs, ctx->cap_pairs, &ull));
+    CK(ensure(ctx->phash, next_pow2(2 * ctx->cap_pairs), &ull));
+    CK(ensure(ctx->te, ctx->cap_tests, &u));
+    CK(ensure(ctx->tg, ctx->cap_tests, &u));
+    if (ctx->keep_tests) {
+        int8_t* i8;
+        char* c8;
+        CK(ensure(ctx->tqs, ctx->cap_tests, &i8));
+        CK(ensure(ctx->trs, ctx->cap_tests, &i8));
+        CK(ensure(ctx->ttype, ctx->cap_tests, &c8));
+        CK(ensure(ctx->tpip, ctx->cap_tests, &c8));
+    }
+    CK(ensure(ctx->esc, ctx->cap_esc, &u));
+    CK(ensure(ctx->rkey, ctx->cap_reg, &ull));
+    CK(ensure(ctx->rkey2, ctx->cap_reg, &ull));
+    CK(ensure(ctx->rval, ctx->cap_reg, &u));
+    CK(ensure(ctx->rval2, ctx->cap_reg, &u));
+    CK(ensure(ctx->rp, 3 * ctx->cap_reg, &dp));
+    CK(ensure(ctx->re, ctx->cap_reg, &u));
+    CK(ensure(ctx->rg, ctx->cap_reg, &u));
+    CK(ensure(ctx->rpts, 3 * ctx->cap_reg, &dp));
+    CK(ensure(ctx->rcount, 1, &ull));
+    CK(ensure(ctx->shkeys, next_pow2(4 * ctx->cap_reg), &ull));
+    CK(ensure(ctx->shcnt, next_pow2(4 * ctx->cap_reg), &u));
+    CK(ensure(ctx->big_list, ctx->cap_big, &u));
+    Trip* tr;
+    CK(ensure(ctx->big_scratch, 256ull * 560, &tr));
+    const uint64_t sort_cap = std::max<uint64_t>({ctx->cap_reg, (uint64_t)ctx->nf[0], (uint64_t)ctx->nf[1]});
+    CK(ensure(ctx->hist, radix_hist_elems(sort_cap), &u));
+    // the scan scratch also serves the mesh-sized scans (sized in alloc_mesh_buffers)
+    const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
+    const uint64_t need = std::max<uint64_t>(scan_tmp_elems(radix_hist_elems(sort_cap)), scan_tmp_elems(K + 8));
+    CK(ensure(ctx->scan_tmp, need + 1024, &u));
+    return MCGPU_SUCCESS;
+}
+
+// the part of the pipeline that runs again for a perturbed cut mesh:
+// candidate-face planes, tests, narrow phase, registry, status
+void enqueue_narrow(mcgpu_context ctx, const Soup& S)
+{
+    hipStream_t s = ctx->s;
+    const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
+    Counters* C = P_<Counters>(ctx->counters);
+    (void)hipEventRecord(ctx->ev[ST_PLANES], s);
+    fill_u32(ctx->cand.p, 0u, (F + 8) / 4 + 1, s);
+    k_mark_candidates<<<grid_for(ctx->cap_pairs), kBlock, 0, s>>>(P_<unsigned long long>(ctx->pairs), C, ctx->cap_pairs,
+                                                                   ctx->nf[0], P_<uint8_t>(ctx->cand));
+    k_planes<<<grid_for(F), kBlock, 0, s>>>(S, P_<uint8_t>(ctx->cand), P_<Plane>(ctx->planes), P_<uint32_t>(ctx->big_list),
+                                             ctx->cap_big, C);
+    k_big_polygons<<<4, 64, 0, s>>>(S, P_<uint32_t>(ctx->big_list), C, ctx->cap_big, P_<Plane>(ctx->planes),
+                                     P_<Trip>(ctx->big_scratch), C);
+    (void)hipEventRecord(ctx->ev[ST_TESTS], s);
+    const uint64_t pmask = next_pow2(2 * ctx->cap_pairs) - 1;
+    k_tests<<<grid_for(ctx->cap_pairs), kBlock, 0, s>>>(S, P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
+                                                        P_<unsigned long long>(ctx->phash), pmask, P_<uint32_t>(ctx->te),
+                                                        P_<uint32_t>(ctx->tg), ctx->cap_tests, C);
+    (void)hipEventRecord(ctx->ev[ST_NARROW], s);
+    TestOut TO{nullptr, nullptr, nullptr, nullptr};
+    if (ctx->keep_tests)
+        TO = TestOut{P_<int8_t>(ctx->tqs), P_<int8_t>(ctx->trs), P_<char>(ctx->ttype), P_<char>(ctx->tpip)};
+    RegOut R{P_<unsigned long long>(ctx->rkey), P_<double>(ctx->rp), ctx->cap_reg, ctx->gbits};
+    k_narrow<<<grid_for(ctx->cap_tests), kBlock, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
+                                                         P_<uint32_t>(ctx->tg), ctx->cap_tests, P_<uint32_t>(ctx->esc),
+                                                         ctx->cap_esc, TO, R, C);
+    k_narrow_exact<<<grid_for(ctx->cap_esc, 64, 2048), 64, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
+                                                                   P_<uint32_t>(ctx->tg), P_<uint32_t>(ctx->esc),
+                                                                   ctx->cap_esc, TO, R, C);
+    (void)hipEventRecord(ctx->ev[ST_REGISTRY], s);
+    unsigned long long* rcount = P_<unsigned long long>(ctx->rcount);
+    k_clamp_count<<<1, 1, 0, s>>>(&C->n_reg, ctx->cap_reg, rcount);
+    k_iota<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(P_<uint32_t>(ctx->rval), rcount, ctx->cap_reg);
+    RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
+    const bool alt = radix_sort<unsigned long long>(P_<unsigned long long>(ctx->rkey), P_<uint32_t>(ctx->rval),
+                                                    P_<unsigned long long>(ctx->rkey2), P_<uint32_t>(ctx->rval2),
+                                                    (const uint64_t*)rcount, 0, ctx->cap_reg, ctx->kbits, rt, s);
+    const unsigned long long* sk = P_<unsigned long long>(alt ? ctx->rkey2 : ctx->rkey);
+    const uint32_t* sv = P_<uint32_t>(alt ? ctx->rval2 : ctx->rval);
+    k_registry_gather<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(sk, sv, P_<double>(ctx->rp), rcount, ctx->cap_reg,
+                                                                 ctx->gbits, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg),
+                                                                 P_<double>(ctx->rpts));
+    const uint64_t hs = next_pow2(4 * ctx->cap_reg);
+    fill_u64(ctx->shkeys.p, kEmpty64, hs, s);
+    fill_u32(ctx->shcnt.p, 0u, hs, s);
+    k_status_pairs<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg), rcount,
+                                                             ctx->cap_reg, P_<unsigned long long>(ctx->shkeys),
+                                                             P_<uint32_t>(ctx->shcnt), hs - 1, C);
+    k_status_check<<<grid_for(hs), kBlock, 0, s>>>(P_<unsigned long long>(ctx->shkeys), P_<uint32_t>(ctx->shcnt), hs, C);
+    (void)hipEventRecord(ctx->ev[ST_COUNT], s);
+}
+
+// face offsets and input validation; the caller syncs and rejects bad input
+// before any kernel indexes vertices through the face lists
+void enqueue_validate(mcgpu_context ctx)
+{
+    hipStream_t s = ctx->s;
+    Counters* C = P_<Counters>(ctx->counters);
+    uint32_t* etotal = P_<uint32_t>(ctx->etotal);
+    fill_u32(C, 0u, sizeof(Counters) / 4, s);
+    for (int m = 0; m < 2; ++m) {
+        uint32_t* off = P_<uint32_t>(ctx->off[m]);
+        if (ctx->dsz[m]) {
+            scan_exclusive(ctx->dsz[m], off, ctx->nf[m], P_<uint32_t>(ctx->scan_tmp), etotal + m, s);
+            k_offsets_tail<<<1, 1, 0, s>>>(off, ctx->nf[m], etotal + m);
+        } else {
+            k_triangle_offsets<<<grid_for((uint64_t)ctx->nf[m] + 1), kBlock, 0, s>>>(off, ctx->nf[m]);
+        }
+        k_validate<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(ctx->didx[m], off, ctx->dsz[m], ctx->nf[m], ctx->nv[m], C);
+    }
+}
+
+void enqueue_frontend(mcgpu_context ctx)
+{
+    hipStream_t s = ctx->s;
+    const uint32_t Fs = ctx->nf[0], Fc = ctx->nf[1];
+    const uint64_t F = (uint64_t)Fs + Fc, V = (uint64_t)ctx->nv[0] + ctx->nv[1];
+    Counters* C = P_<Counters>(ctx->counters);
+    Params* Pd = P_<Params>(ctx->params);
+
+    (void)hipEventRecord(ctx->ev[ST_PARAMS], s);
+    fill_u32(C, 0u, sizeof(Counters) / 4, s);
+    k_mesh_sums<<<2, 64, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd);
+    const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
+    k_shift_params<<<1, 1, 0, s>>>(Pd, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
+
+    // topology: polygon-soup edge numbering
+    (void)hipEventRecord(ctx->ev[ST_TOPOLOGY], s);
+    uint32_t* etotal = P_<uint32_t>(ctx->etotal);
+    const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
+    fill_u32(ctx->ef1.p, 0xffffffffu, K, s);
+    for (int m = 0; m < 2; ++m) {
+        const uint64_t nk = ctx->ncorner[m];
+        const uint64_t hs = next_pow2(2 * nk + 2);
+        fill_u64(ctx->hkeys[m].p, kEmpty64, hs, s);
+        fill_u64(ctx->hfirst[m].p, kEmpty64, hs, s);
+        fill_u32(ctx->hcount[m].p, 0u, hs, s);
+        const uint32_t* off = P_<uint32_t>(ctx->off[m]);
+        k_edge_insert<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(ctx->didx[m], off, ctx->nf[m], m,
+                                                               P_<unsigned long long>(ctx->hkeys[m]),
+                                                               P_<unsigned long long>(ctx->hfirst[m]),
+                                                               P_<uint32_t>(ctx->hcount[m]), hs - 1);
+        k_edge_flag<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(ctx->didx[m], off, ctx->nf[m], m,
+                                                             P_<unsigned long long>(ctx->hkeys[m]),
+                                                             P_<unsigned long long>(ctx->hfirst[m]),
+                                                             P_<uint32_t>(ctx->hcount[m]), hs - 1,
+                                                             P_<uint32_t>(ctx->eflag[m]), C);
+        scan_exclusive(P_<uint32_t>(ctx->eflag[m]), P_<uint32_t>(ctx->eid[m]), nk, P_<uint32_t>(ctx->scan_tmp),
+                       etotal + m, s);
+        k_store_edge_count<<<1, 1, 0, s>>>(etotal + m, C, m);
+    }
+    for (int m = 0; m < 2; ++m) {
+        const uint64_t hs = next_pow2(2 * ctx->ncorner[m] + 2);
+        k_edge_assign<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(
+            ctx->didx[m], P_<uint32_t>(ctx->off[m]), ctx->nf[m], m, P_<unsigned long long>(ctx->hkeys[m]),
+            P_<unsigned long long>(ctx->hfirst[m]), hs - 1, P_<uint32_t>(ctx->eid[m]), C, m ? ctx->nv[0] : 0,
+            m ? ctx->nf[0] : 0, P_<uint32_t>(ctx->cedge[m]), P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1),
+            P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
+    }
+
+    // shifted coordinates and face boxes
+    (void)hipEventRecord(ctx->ev[ST_SHIFT], s);
+    k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
+                                                        P_<double>(ctx->ps_v), 0);
+    const Soup S = make_soup(ctx);
+    k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, Pd, P_<double>(ctx->box), 0);
+
+    // LBVH per mesh
+    (void)hipEventRecord(ctx->ev[ST_BVH], s);
+    Tree T[2];
+    RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
+    for (int m = 0; m < 2; ++m) {
+        const uint32_t nf = ctx->nf[m];
+        const double* fbox = P_<double>(ctx->box) + 6 * (uint64_t)(m ? Fs : 0);
+        k_morton<<<grid_for(nf), kBlock, 0, s>>>(fbox, Pd, m, nf, P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]));
+        const bool alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]),
+                                              P_<uint32_t>(ctx->mcode2[m]), P_<uint32_t>(ctx->mface2[m]), nullptr, nf, nf,
+                                              30, rt, s);
+        ctx->mface_alt[m] = alt;
+        const uint32_t* code = P_<uint32_t>(alt ? ctx->mcode2[m] : ctx->mcode[m]);
+        const uint32_t* sface = P_<uint32_t>(alt ? ctx->mface2[m] : ctx->mface[m]);
+        fill_u32(ctx->lparent[m].p, 0xffffffffu, nf, s);
+        fill_u32(ctx->visits[m].p, 0u, nf, s);
+        k_lbvh_nodes<<<grid_for(nf > 1 ? nf - 1 : 1), kBlock, 0, s>>>(code, (int)nf, P_<uint32_t>(ctx->child[m]),
+                                                                      P_<int32_t>(ctx->nparent[m]),
+                                                                      P_<int32_t>(ctx->lparent[m]));
+        k_lbvh_refit<<<grid_for(nf), kBlock, 0, s>>>((int)nf, sface, fbox, P_<uint32_t>(ctx->child[m]),
+                                                     P_<int32_t>(ctx->nparent[m]), P_<int32_t>(ctx->lparent[m]),
+                                                     P_<uint32_t>(ctx->visits[m]), P_<double>(ctx->nodebox[m]));
+        T[m].child = P_<uint32_t>(ctx->child[m]);
+        T[m].sface = sface;
+        T[m].nodebox = P_<double>(ctx->nodebox[m]);
+        T[m].facebox = fbox;
+        T[m].n = (int)nf;
+    }
+
+    // broad phase
+    (void)hipEventRecord(ctx->ev[ST_TRAVERSE], s);
+    const uint64_t ps = next_pow2(2 * ctx->cap_pairs);
+    fill_u64(ctx->phash.p, kEmpty64, ps, s);
+    k_traverse<<<grid_for(Fs), kBlock, 0, s>>>(T[0], T[1], P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
+                                               P_<unsigned long long>(ctx->phash), ps - 1, C);
+    (void)Fc;
+    enqueue_narrow(ctx, S);
+}
+
+mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow)
+{
+    CK(hipStreamSynchronize(ctx->s));
+    CK(hipGetLastError());
+    CK(hipMemcpy(&ctx->C, ctx->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(&ctx->P, ctx->params.p, sizeof(Params), hipMemcpyDeviceToHost));
+    const Counters& C = ctx->C;
+    *overflow = false;
+    if (getenv("MCGPU_DEBUG"))
+        fprintf(stderr,
+                "[mcgpu] pairs=%llu tests=%llu esc=%llu reg=%llu big=%llu cand=%llu edges=%llu+%llu ov=%u/%u/%u/%u/%u "
+                "gp=%u bad=%u/%u unsup=%u nonman=%u badin=%u pairfail=%u partial=%u caps=%llu/%llu/%llu/%llu\n",
+                C.n_pairs, C.n_tests, C.n_esc, C.n_reg, C.n_big, C.n_cand, C.n_edges_src, C.n_edges_cut, C.ov_pairs,
+                C.ov_tests, C.ov_esc, C.ov_reg, C.ov_big, C.gp, C.bad_src, C.bad_cut, C.unsupported, C.nonmanifold,
+                C.bad_input, C.pair_fail, C.partial, (unsigned long long)ctx->cap_pairs,
+                (unsigned long long)ctx->cap_tests, (unsigned long long)ctx->cap_esc, (unsigned long long)ctx->cap_reg);
+    if (C.ov_pairs & 2u) return MCGPU_ERROR_UNSUPPORTED; // tree deeper than the traversal stack
+    if (C.ov_pairs || C.ov_tests || C.ov_esc || C.ov_reg || C.ov_big) {
+        *overflow = true;
+        if (C.ov_pairs) ctx->cap_pairs = std::max<uint64_t>(2 * ctx->cap_pairs, C.n_pairs + C.n_pairs / 4);
+        if (C.ov_tests) ctx->cap_tests = std::max<uint64_t>(2 * ctx->cap_tests, C.n_tests + C.n_tests / 4);
+        if (C.ov_esc) ctx->cap_esc = std::max<uint64_t>(2 * ctx->cap_esc, C.n_esc + C.n_esc / 4);
+        if (C.ov_reg) ctx->cap_reg = std::max<uint64_t>(2 * ctx->cap_reg, C.n_reg + C.n_reg / 4);
+        if (C.ov_big) ctx->cap_big = std::max<uint64_t>(2 * ctx->cap_big, C.n_big + C.n_big / 4);
+        return MCGPU_SUCCESS;
+    }
+    if (C.bad_input || C.nonmanifold) return MCGPU_ERROR_INVALID_VALUE;
+    if (C.unsupported) return MCGPU_ERROR_UNSUPPORTED;
+    for (int i = 0; i < ST_COUNT; ++i) {
+        float ms = 0.f;
+        hipEvent_t a = ctx->ev[i], b = ctx->ev[i + 1];
+        if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 0.f;
+        ctx->stage_ms[i] = ms;
+    }
+    float tot = 0.f;
+    if (hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[ST_COUNT]) != hipSuccess) tot = 0.f;
+    ctx->total_ms = tot;
+    ctx->have = true;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->status = decide_status(C, ctx->flags, ctx->enforcement_count);
+        info->gp_violation = C.gp;
+        info->num_candidate_pairs = C.n_pairs;
+        info->num_candidate_faces = C.n_cand;
+        info->num_edges = C.n_edges_src + C.n_edges_cut;
+        info->num_tests = C.n_tests;
+        info->num_escalated = C.n_esc;
+        info->num_registry = C.n_reg;
+        for (int j = 0; j < 3; ++j) {
+            info->com[j] = ctx->P.com[j];
+            info->translation[j] = ctx->P.t[j];
+        }
+        info->eps = ctx->P.eps;
+        info->device_ms = tot;
+    }
+    return MCGPU_SUCCESS;
+}
+
+mcgpu_result upload_mesh(mcgpu_context ctx, int m, const mcgpu_mesh* mesh, bool on_device)
+{
+    ctx->nv[m] = mesh->num_vertices;
+    ctx->nf[m] = mesh->num_faces;
+    if (on_device) {
+        ctx->dv[m] = mesh->vertices;
+        ctx->didx[m] = mesh->face_indices;
+        ctx->dsz[m] = mesh->face_sizes;
+        if (mesh->face_sizes) {
+            std::vector<uint32_t> h(mesh->num_faces);
+            CK(hipMemcpy(h.data(), mesh->face_sizes, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            uint64_t k = 0;
+            for (uint32_t x : h) k += x;
+            ctx->ncorner[m] = k;
+        } else {
+            ctx->ncorner[m] = 3ull * mesh->num_faces;
+        }
+        return MCGPU_SUCCESS;
+    }
+    uint64_t k = 3ull * mesh->num_faces;
+    if (mesh->face_sizes) {
+        k = 0;
+        for (uint32_t f = 0; f < mesh->num_faces; ++f) {
+            if (mesh->face_sizes[f] < 3) return MCGPU_ERROR_INVALID_VALUE;
+            k += mesh->face_sizes[f];
+        }
+    }
+    ctx->ncorner[m] = k;
+    double* v;
+    uint32_t* ix;
+    CK(ensure(ctx->in_v[m], 3ull * mesh->num_vertices, &v));
+    CK(ensure(ctx->in_idx[m], k, &ix));
+    CK(hipMemcpyAsync(v, mesh->vertices, 24ull * mesh->num_vertices, hipMemcpyHostToDevice, ctx->s));
+    CK(hipMemcpyAsync(ix, mesh->face_indices, 4ull * k, hipMemcpyHostToDevice, ctx->s));
+    ctx->dv[m] = v;
+    ctx->didx[m] = ix;
+    ctx->dsz[m] = nullptr;
+    if (mesh->face_sizes) {
+        uint32_t* sz;
+        CK(ensure(ctx->in_sz[m], mesh->num_faces, &sz));
+        CK(hipMemcpyAsync(sz, mesh->face_sizes, 4ull * mesh->num_faces, hipMemcpyHostToDevice, ctx->s));
+        ctx->dsz[m] = sz;
+    }
+    return MCGPU_SUCCESS;
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+MCGPU_API const char* mcgpu_result_string(mcgpu_result r)
+{
+    switch (r) {
+    case MCGPU_SUCCESS: return "success";
+    case MCGPU_ERROR_INVALID_VALUE: return "invalid value";
+    case MCGPU_ERROR_DEVICE: return "HIP runtime error";
+    case MCGPU_ERROR_OUT_OF_MEMORY: return "out of device memory";
+    case MCGPU_ERROR_UNSUPPORTED: return "unsupported input";
+    case MCGPU_ERROR_NO_RESULT: return "no dispatch result";
+    }
+    return "unknown";
+}
+
+MCGPU_API mcgpu_result mcgpu_create_context(int device, mcgpu_context* out)
+{
+    if (!out) return MCGPU_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MCGPU_ERROR_DEVICE;
+    mcgpu_context ctx = new mcgpu_context_t();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MCGPU_ERROR_DEVICE;
+    }
+    for (int i = 0; i <= ST_COUNT; ++i) (void)hipEventCreate(&ctx->ev[i]);
+    *out = ctx;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->s);
+    Buf* all[] = {&ctx->in_v[0], &ctx->in_v[1], &ctx->in_idx[0], &ctx->in_idx[1], &ctx->in_sz[0], &ctx->in_sz[1],
+                  &ctx->off[0], &ctx->off[1], &ctx->params, &ctx->counters, &ctx->ps_v, &ctx->box, &ctx->hkeys[0],
+                  &ctx->hkeys[1], &ctx->hfirst[0], &ctx->hfirst[1], &ctx->hcount[0], &ctx->hcount[1], &ctx->eflag[0],
+                  &ctx->eflag[1], &ctx->eid[0], &ctx->eid[1], &ctx->cedge[0], &ctx->cedge[1], &ctx->etotal, &ctx->ev0,
+                  &ctx->ev1, &ctx->ef0, &ctx->ef1, &ctx->mcode[0], &ctx->mcode[1], &ctx->mface[0], &ctx->mface[1],
+                  &ctx->mcode2[0], &ctx->mcode2[1], &ctx->mface2[0], &ctx->mface2[1], &ctx->child[0], &ctx->child[1],
+                  &ctx->nparent[0], &ctx->nparent[1], &ctx->lparent[0], &ctx->lparent[1], &ctx->visits[0],
+                  &ctx->visits[1], &ctx->nodebox[0], &ctx->nodebox[1], &ctx->pairs, &ctx->phash, &ctx->cand,
+                  &ctx->planes, &ctx->big_list, &ctx->big_scratch, &ctx->te, &ctx->tg, &ctx->tqs, &ctx->trs,
+                  &ctx->ttype, &ctx->tpip, &ctx->esc, &ctx->rkey, &ctx->rval, &ctx->rkey2, &ctx->rval2, &ctx->rp,
+                  &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->hist};
+    for (Buf* b : all)
+        if (b->p) (void)hipFree(b->p);
+    for (int i = 0; i <= ST_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->s);
+    delete ctx;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
+                                               const mcgpu_dispatch_desc* desc, mcgpu_frontend_info* info)
+{
+    if (!ctx || !desc) return MCGPU_ERROR_INVALID_VALUE;
+    mcgpu_result r;
+    if ((r = check_mesh(src)) != MCGPU_SUCCESS || (r = check_mesh(cut)) != MCGPU_SUCCESS) return r;
+    if (!(desc->gp_constant > 0.0)) return MCGPU_ERROR_INVALID_VALUE; // frontend.cpp:391-397
+    CK(hipSetDevice(ctx->device));
+    ctx->have = false;
+    ctx->flags = desc->flags;
+    ctx->gp_constant = desc->gp_constant;
+    ctx->enforcement_count = desc->enforcement_count;
+    ctx->keep_tests = desc->keep_tests ? 1 : 0;
+    if ((r = upload_mesh(ctx, 0, src, desc->input_on_device != 0)) != MCGPU_SUCCESS) return r;
+    if ((r = upload_mesh(ctx, 1, cut, desc->input_on_device != 0)) != MCGPU_SUCCESS) return r;
+    const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
+    ctx->gbits = bits_for(F + 1);
+    ctx->kbits = ctx->gbits + bits_for(ctx->ncorner[0] + ctx->ncorner[1] + 1);
+    if (ctx->kbits > 64) return MCGPU_ERROR_UNSUPPORTED;
+    if ((r = alloc_mesh_buffers(ctx)) != MCGPU_SUCCESS) return r;
+    if (ctx->cap_pairs < 4 * F) ctx->cap_pairs = std::max<uint64_t>(1 << 14, 4 * F);
+    if (ctx->cap_tests < 2 * ctx->cap_pairs) ctx->cap_tests = 2 * ctx->cap_pairs;
+    if (ctx->cap_esc < (1u << 14)) ctx->cap_esc = std::max<uint64_t>(1 << 14, ctx->cap_tests / 16);
+    if (ctx->cap_reg < (1u << 14)) ctx->cap_reg = std::max<uint64_t>(1 << 14, ctx->cap_tests / 8);
+    if (ctx->cap_big < 1024) ctx->cap_big = 1024;
+    // validate the face lists before anything indexes vertices through them
+    enqueue_validate(ctx);
+    CK(hipStreamSynchronize(ctx->s));
+    CK(hipGetLastError());
+    {
+        Counters c0;
+        CK(hipMemcpy(&c0, ctx->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
+        if (c0.bad_input) return MCGPU_ERROR_INVALID_VALUE;
+    }
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
+        enqueue_frontend(ctx);
+        bool overflow = false;
+        if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
+        if (!overflow) return MCGPU_SUCCESS;
+    }
+    return MCGPU_ERROR_OUT_OF_MEMORY;
+}
+
+MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const double* perturbation,
+                                                  uint32_t enforcement_count, mcgpu_frontend_info* info)
+{
+    if (!ctx || !perturbation) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    CK(hipSetDevice(ctx->device));
+    ctx->enforcement_count = enforcement_count;
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        mcgpu_result r;
+        if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
+        hipStream_t s = ctx->s;
+        const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1];
+        Counters* C = P_<Counters>(ctx->counters);
+        (void)hipEventRecord(ctx->ev[ST_PARAMS], s);
+        k_reset_narrow<<<1, 1, 0, s>>>(C);
+        k_set_perturbation<<<1, 1, 0, s>>>(P_<Params>(ctx->params), 1, perturbation[0], perturbation[1], perturbation[2]);
+        for (int i = ST_TOPOLOGY; i <= ST_TRAVERSE; ++i) (void)hipEventRecord(ctx->ev[i], s);
+        k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1],
+                                                            P_<Params>(ctx->params), P_<double>(ctx->ps_v), 1);
+        enqueue_narrow(ctx, make_soup(ctx));
+        bool overflow = false;
+        if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
+        if (!overflow) return MCGPU_SUCCESS;
+        // a list overflowed: the pairs are intact, so only the narrow part reruns
+    }
+    return MCGPU_ERROR_OUT_OF_MEMORY;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_registry(mcgpu_context ctx, uint32_t* edges, uint32_t* faces, double* points,
+                                          uint64_t capacity)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    const uint64_t n = std::min<uint64_t>(ctx->C.n_reg, capacity);
+    if (edges) CK(hipMemcpy(edges, ctx->re.p, 4 * n, hipMemcpyDeviceToHost));
+    if (faces) CK(hipMemcpy(faces, ctx->rg.p, 4 * n, hipMemcpyDeviceToHost));
+    if (points) CK(hipMemcpy(points, ctx->rpts.p, 24 * n, hipMemcpyDeviceToHost));
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_registry_device(mcgpu_context ctx, const uint32_t** edges, const uint32_t** faces,
+                                             const double** points, uint64_t* count)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    if (edges) *edges = P_<uint32_t>(ctx->re);
+    if (faces) *faces = P_<uint32_t>(ctx->rg);
+    if (points) *points = P_<double>(ctx->rpts);
+    if (count) *count = ctx->C.n_reg;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_candidate_pairs(mcgpu_context ctx, uint32_t* pairs, uint64_t capacity)
+{
+    if (!ctx || !pairs) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    const uint64_t n = ctx->C.n_pairs;
+    std::vector<unsigned long long> h(n);
+    if (n) CK(hipMemcpy(h.data(), ctx->pairs.p, 8 * n, hipMemcpyDeviceToHost));
+    std::sort(h.begin(), h.end()); // listing order only; the set is the device's
+    const uint64_t m = std::min<uint64_t>(n, capacity);
+    for (uint64_t i = 0; i < m; ++i) {
+        pairs[2 * i] = (uint32_t)(h[i] >> 32);
+        pairs[2 * i + 1] = (uint32_t)h[i];
+    }
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_edges(mcgpu_context ctx, uint32_t* v0, uint32_t* v1, int32_t* f0, int32_t* f1,
+                                       uint64_t capacity)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    const uint64_t n = std::min<uint64_t>(ctx->C.n_edges_src + ctx->C.n_edges_cut, capacity);
+    if (v0) CK(hipMemcpy(v0, ctx->ev0.p, 4 * n, hipMemcpyDeviceToHost));
+    if (v1) CK(hipMemcpy(v1, ctx->ev1.p, 4 * n, hipMemcpyDeviceToHost));
+    if (f0) CK(hipMemcpy(f0, ctx->ef0.p, 4 * n, hipMemcpyDeviceToHost));
+    if (f1) CK(hipMemcpy(f1, ctx->ef1.p, 4 * n, hipMemcpyDeviceToHost));
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_tests(mcgpu_context ctx, uint32_t* edges, uint32_t* faces, int8_t* qsign,
+                                       int8_t* rsign, char* type, char* pip, uint64_t capacity)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have || !ctx->keep_tests) return MCGPU_ERROR_NO_RESULT;
+    const uint64_t n = ctx->C.n_tests;
+    std::vector<uint32_t> e(n), g(n);
+    std::vector<int8_t> qs(n), rs(n);
+    std::vector<char> ty(n), pp(n);
+    if (n) {
+        CK(hipMemcpy(e.data(), ctx->te.p, 4 * n, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(g.data(), ctx->tg.p, 4 * n, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(qs.data(), ctx->tqs.p, n, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(rs.data(), ctx->trs.p, n, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(ty.data(), ctx->ttype.p, n, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(pp.data(), ctx->tpip.p, n, hipMemcpyDeviceToHost));
+    }
+    std::vector<uint64_t> order(n);
+    for (uint64_t i = 0; i < n; ++i) order[i] = ((uint64_t)e[i] << 32 | g[i]) << 0;
+    std::vector<uint32_t> perm(n);
+    for (uint64_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+    std::sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return order[a] < order[b]; });
+    const uint64_t m = std::min<uint64_t>(n, capacity);
+    for (uint64_t i = 0; i < m; ++i) {
+        const uint32_t j = perm[i];
+        if (edges) edges[i] = e[j];
+        if (faces) faces[i] = g[j];
+        if (qsign) qsign[i] = qs[j];
+        if (rsign) rsign[i] = rs[j];
+        if (type) type[i] = ty[j];
+        if (pip) pip[i] = pp[j];
+    }
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_face_planes(mcgpu_context ctx, uint32_t* faces, double* normals, double* d,
+                                             int32_t* max_comp, int32_t* triple, uint64_t capacity)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
+    std::vector<uint8_t> cand(F);
+    std::vector<Plane> pl(F);
+    CK(hipMemcpy(cand.data(), ctx->cand.p, F, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(pl.data(), ctx->planes.p, F * sizeof(Plane), hipMemcpyDeviceToHost));
+    uint64_t k = 0;
+    for (uint64_t f = 0; f < F && k < capacity; ++f) {
+        if (!cand[f]) continue;
+        if (faces) faces[k] = (uint32_t)f;
+        for (int j = 0; j < 3; ++j) {
+            if (normals) normals[3 * k + j] = pl[f].n[j];
+            if (triple) triple[3 * k + j] = pl[f].tri[0] == 0xff ? -1 : pl[f].tri[j];
+        }
+        if (d) d[k] = pl[f].d;
+        if (max_comp) max_comp[k] = pl[f].k;
+        ++k;
+    }
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_ps_vertices(mcgpu_context ctx, double* xyz, uint64_t capacity)
+{
+    if (!ctx || !xyz) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    const uint64_t n = std::min<uint64_t>((uint64_t)ctx->nv[0] + ctx->nv[1], capacity);
+    CK(hipMemcpy(xyz, ctx->ps_v.p, 24 * n, hipMemcpyDeviceToHost));
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity)
+{
+    if (!ctx || !ms || !ctx->have) return 0;
+    int n = std::min(capacity, (int)ST_COUNT);
+    for (int i = 0; i < n; ++i) ms[i] = ctx->stage_ms[i];
+    return n;
+}
+
+MCGPU_API const char* mcgpu_stage_name(int stage)
+{
+    return (stage >= 0 && stage < ST_COUNT) ? kStageNames[stage] : "";
+}
+
+} // extern "C"

@@ -1,0 +1,244 @@
+// Device-wide exclusive scan and LSD radix sort for the front end.
+//
+// Element counts may live on the device (the producer kernel's append
+// counter): kernels are launched for a host-known capacity and every tile
+// checks the device count, so a pipeline never waits on the host between
+// stages.  The radix sort is stable; a tile is 256 threads x 8 keys staged
+// through LDS, ranked per 8-bit digit with wavefront ballots.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcgpu {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16; // elements per thread in the scan tiles
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortBlock * kSortItems;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+__device__ __forceinline__ uint64_t read_count(const uint64_t* n_dev, uint64_t n_host)
+{
+    return n_dev ? *n_dev : n_host;
+}
+
+// block-wide exclusive scan of one value per thread (256 threads); returns the total in *total
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
+{
+    __shared__ uint32_t wsum[kScanBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+        uint32_t s = wsum[w];
+        if (w < wid) off += s;
+        tot += s;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return off + x - v;
+}
+
+// pass 1: per-tile sums
+__global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const uint32_t* __restrict__ in, uint32_t* __restrict__ tile_sums,
+                                                            uint64_t n)
+{
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint32_t s = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        uint64_t idx = base + (uint64_t)i * kScanBlock + threadIdx.x;
+        if (idx < n) s += in[idx];
+    }
+    uint32_t tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// pass 2: exclusive scan of the tile sums by one block (any count), total into *total_out
+__global__ void __launch_bounds__(kScanBlock) k_scan_tiles(uint32_t* __restrict__ tile_sums, uint32_t ntiles,
+                                                           uint32_t* __restrict__ total_out)
+{
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += kScanBlock) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < ntiles ? tile_sums[i] : 0;
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan(v, &tot);
+        if (i < ntiles) tile_sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+// pass 3: scan inside each tile, offset by the tile's prefix
+__global__ void __launch_bounds__(kScanBlock) k_scan_apply(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                           const uint32_t* __restrict__ tile_sums, uint64_t n)
+{
+    // each thread owns kScanItems consecutive elements
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        s += v[i];
+    }
+    uint32_t run = block_exclusive_scan(s, nullptr) + tile_sums[blockIdx.x];
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+}
+
+// Host launcher: exclusive scan of n (host-known) u32 into out (may alias in);
+// tmp needs ceil(n / kScanTile) entries; total (device) optional.
+inline void scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, uint32_t* total,
+                           hipStream_t s)
+{
+    uint32_t ntiles = (uint32_t)((n + kScanTile - 1) / kScanTile);
+    if (ntiles == 0) ntiles = 1;
+    k_scan_reduce<<<ntiles, kScanBlock, 0, s>>>(in, tmp, n);
+    k_scan_tiles<<<1, kScanBlock, 0, s>>>(tmp, ntiles, total);
+    k_scan_apply<<<ntiles, kScanBlock, 0, s>>>(in, out, tmp, n);
+}
+
+inline uint64_t scan_tmp_elems(uint64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+// ---------------------------------------------------------------------------
+// Radix sort
+// ---------------------------------------------------------------------------
+
+template <typename K>
+__device__ __forceinline__ uint32_t digit_of(K key, int shift)
+{
+    return (uint32_t)(key >> shift) & (kRadix - 1);
+}
+
+// histogram of one digit per tile, stored digit-major: hist[d * ntiles + t]
+template <typename K>
+__global__ void __launch_bounds__(kSortBlock) k_radix_hist(const K* __restrict__ keys, const uint64_t* n_dev,
+                                                           uint64_t n_host, int shift, uint32_t* __restrict__ hist,
+                                                           uint32_t ntiles)
+{
+    __shared__ uint32_t h[kRadix];
+    const uint64_t n = read_count(n_dev, n_host);
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        h[threadIdx.x] = 0;
+        __syncthreads();
+        const uint64_t base = (uint64_t)t * kSortTile;
+        if (base < n) {
+            for (int j = 0; j < kSortItems; ++j) {
+                uint64_t idx = base + (uint64_t)j * kSortBlock + threadIdx.x;
+                if (idx < n) atomicAdd(&h[digit_of(keys[idx], shift)], 1u);
+            }
+        }
+        __syncthreads();
+        hist[(uint64_t)threadIdx.x * ntiles + t] = h[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// stable scatter by digit; offs = exclusive scan of hist (digit-major)
+template <typename K, bool HasVal>
+__global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+                                                              K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+                                                              const uint64_t* n_dev, uint64_t n_host, int shift,
+                                                              const uint32_t* __restrict__ offs, uint32_t ntiles)
+{
+    __shared__ uint32_t wcount[kSortBlock / 64][kRadix];
+    __shared__ uint32_t running[kRadix];
+    __shared__ uint32_t gbase[kRadix];
+    const uint64_t n = read_count(n_dev, n_host);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t base = (uint64_t)t * kSortTile;
+        if (base >= n) break; // tiles are visited in increasing order
+        running[threadIdx.x] = 0;
+        gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * ntiles + t];
+        for (int w = 0; w < kSortBlock / 64; ++w) wcount[w][threadIdx.x] = 0;
+        __syncthreads();
+        for (int j = 0; j < kSortItems; ++j) {
+            const uint64_t idx = base + (uint64_t)j * kSortBlock + threadIdx.x;
+            const bool valid = idx < n;
+            K key = valid ? keys_in[idx] : (K)0;
+            uint32_t val = 0;
+            if (HasVal && valid) val = vals_in[idx];
+            const uint32_t d = digit_of(key, shift);
+            // lanes of this wave holding the same digit
+            uint64_t peers = __ballot(valid);
+            for (int b = 0; b < kRadixBits; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            const uint32_t rank = __popcll(peers & lt_mask);
+            if (valid && rank == 0) wcount[wid][d] = __popcll(peers);
+            __syncthreads();
+            if (valid) {
+                uint32_t pos = running[d] + rank;
+                for (int w = 0; w < wid; ++w) pos += wcount[w][d];
+                const uint32_t dst = gbase[d] + pos;
+                keys_out[dst] = key;
+                if (HasVal) vals_out[dst] = val;
+            }
+            __syncthreads();
+            {
+                uint32_t add = 0;
+                for (int w = 0; w < kSortBlock / 64; ++w) {
+                    add += wcount[w][threadIdx.x];
+                    wcount[w][threadIdx.x] = 0;
+                }
+                running[threadIdx.x] += add;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+struct RadixTemp {
+    uint32_t* hist;      // kRadix * ntiles
+    uint32_t* scan_tmp;  // scan_tmp_elems(kRadix * ntiles)
+};
+
+inline uint32_t radix_tiles(uint64_t cap) { return (uint32_t)((cap + kSortTile - 1) / kSortTile) + 1; }
+inline uint64_t radix_hist_elems(uint64_t cap) { return (uint64_t)kRadix * radix_tiles(cap); }
+
+// Sorts keys[0..n) (n on device or host) by bits [0, bits) with an optional
+// u32 payload.  Ping-pongs between (keys, vals) and (keys_alt, vals_alt);
+// returns true when the sorted result ends in the *_alt buffers.
+template <typename K>
+inline bool radix_sort(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, const uint64_t* n_dev,
+                       uint64_t n_host, uint64_t cap, int bits, const RadixTemp& tmp, hipStream_t s)
+{
+    const uint32_t ntiles = radix_tiles(cap);
+    const uint32_t grid = ntiles < 2048 ? ntiles : 2048;
+    bool alt = false;
+    for (int shift = 0; shift < bits; shift += kRadixBits) {
+        K* kin = alt ? keys_alt : keys;
+        K* kout = alt ? keys : keys_alt;
+        uint32_t* vin = alt ? vals_alt : vals;
+        uint32_t* vout = alt ? vals : vals_alt;
+        k_radix_hist<K><<<grid, kSortBlock, 0, s>>>(kin, n_dev, n_host, shift, tmp.hist, ntiles);
+        scan_exclusive(tmp.hist, tmp.hist, (uint64_t)kRadix * ntiles, tmp.scan_tmp, nullptr, s);
+        if (vals)
+            k_radix_scatter<K, true><<<grid, kSortBlock, 0, s>>>(kin, vin, kout, vout, n_dev, n_host, shift, tmp.hist, ntiles);
+        else
+            k_radix_scatter<K, false><<<grid, kSortBlock, 0, s>>>(kin, nullptr, kout, nullptr, n_dev, n_host, shift, tmp.hist,
+                                                                  ntiles);
+        alt = !alt;
+    }
+    return alt;
+}
+
+} // namespace mcgpu

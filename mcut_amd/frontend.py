@@ -1,0 +1,223 @@
+"""Python binding of libmcgpu (include/mcgpu.h) through its C ABI.
+
+``Context.dispatch(src, cut, flags)`` runs MCUT's intersection front end on
+the GPU for one source/cut mesh pair given as ``(V, F, S)`` arrays
+(``S`` = face sizes or ``None`` for triangles), the same inputs mcDispatch
+takes (include/mcut/mcut.h:1008-1020).  Results stay on the device; the
+``Result`` object copies them out on request.
+
+The library must be built (``make -C mcut_amd/csrc``); loading fails loudly
+when it is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmcgpu.so")
+
+# MC_DISPATCH_* bits the front end reads (mcut.h:379-441)
+REQUIRE_THROUGH_CUTS = 1 << 2
+ENFORCE_GENERAL_POSITION = 1 << 15
+ENFORCE_GENERAL_POSITION_ABSOLUTE = 1 << 16
+
+STATUS = {0: "SUCCESS", -1: "INVALID_SRC_MESH", -2: "INVALID_CUT_MESH", -3: "INVALID_MESH_INTERSECTION",
+          -4: "GENERAL_POSITION_VIOLATION"}
+RESULT = {0: "success", -1: "invalid value", -2: "HIP runtime error", -3: "out of device memory",
+          -4: "unsupported input", -5: "no dispatch result"}
+
+
+class McgpuError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {RESULT.get(code, code)} ({code})")
+        self.code = code
+
+
+class Mesh(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("face_indices", C.c_void_p), ("face_sizes", C.c_void_p),
+                ("num_vertices", C.c_uint32), ("num_faces", C.c_uint32)]
+
+
+class DispatchDesc(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("gp_constant", C.c_double), ("enforcement_count", C.c_uint32),
+                ("input_on_device", C.c_uint32), ("keep_tests", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class FrontendInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("gp_violation", C.c_uint32), ("num_candidate_pairs", C.c_uint64),
+                ("num_candidate_faces", C.c_uint64), ("num_edges", C.c_uint64), ("num_tests", C.c_uint64),
+                ("num_escalated", C.c_uint64), ("num_registry", C.c_uint64), ("com", C.c_double * 3),
+                ("translation", C.c_double * 3), ("eps", C.c_double), ("device_ms", C.c_float),
+                ("reserved", C.c_float)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise McgpuError(-2, f"libmcgpu is not built ({LIB_PATH}); run make -C mcut_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        vp, u64 = C.c_void_p, C.c_uint64
+        L.mcgpu_create_context.argtypes = [C.c_int, C.POINTER(vp)]
+        L.mcgpu_release_context.argtypes = [vp]
+        L.mcgpu_result_string.argtypes = [C.c_int]
+        L.mcgpu_result_string.restype = C.c_char_p
+        L.mcgpu_dispatch_frontend.argtypes = [vp, C.POINTER(Mesh), C.POINTER(Mesh), C.POINTER(DispatchDesc),
+                                              C.POINTER(FrontendInfo)]
+        L.mcgpu_redispatch_perturbed.argtypes = [vp, vp, C.c_uint32, C.POINTER(FrontendInfo)]
+        L.mcgpu_get_registry.argtypes = [vp, vp, vp, vp, u64]
+        L.mcgpu_get_candidate_pairs.argtypes = [vp, vp, u64]
+        L.mcgpu_get_edges.argtypes = [vp, vp, vp, vp, vp, u64]
+        L.mcgpu_get_tests.argtypes = [vp, vp, vp, vp, vp, vp, vp, u64]
+        L.mcgpu_get_face_planes.argtypes = [vp, vp, vp, vp, vp, vp, u64]
+        L.mcgpu_get_ps_vertices.argtypes = [vp, vp, u64]
+        L.mcgpu_registry_device.argtypes = [vp, vp, vp, vp, vp]
+        L.mcgpu_stage_times.argtypes = [vp, vp, C.c_int]
+        L.mcgpu_stage_name.argtypes = [C.c_int]
+        L.mcgpu_stage_name.restype = C.c_char_p
+        for fn in ("mcgpu_create_context", "mcgpu_release_context", "mcgpu_dispatch_frontend",
+                   "mcgpu_redispatch_perturbed", "mcgpu_get_registry", "mcgpu_get_candidate_pairs", "mcgpu_get_edges",
+                   "mcgpu_get_tests", "mcgpu_get_face_planes", "mcgpu_get_ps_vertices", "mcgpu_registry_device",
+                   "mcgpu_stage_times"):
+            getattr(L, fn).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _check(code, what):
+    if code != 0:
+        raise McgpuError(code, what)
+
+
+def _host_mesh(m, keep):
+    V, F, S = m
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.uint32)
+    S = None if S is None else np.ascontiguousarray(S, dtype=np.uint32)
+    keep += [V, F, S]
+    nf = F.size // 3 if S is None else S.size
+    return Mesh(_ptr(V), _ptr(F), _ptr(S), V.shape[0], nf)
+
+
+def device_mesh(V_ptr, F_ptr, S_ptr, nv, nf):
+    """Mesh descriptor over device pointers (e.g. torch tensors' data_ptr())."""
+    return Mesh(V_ptr, F_ptr, S_ptr, nv, nf)
+
+
+class Result:
+    """Front-end outputs of one dispatch; arrays are copied from the device lazily."""
+
+    def __init__(self, ctx, info: FrontendInfo, nv: int):
+        self._ctx = ctx
+        self.info = info
+        self.status = info.status
+        self.status_name = STATUS.get(info.status, str(info.status))
+        self.counts = dict(pairs=info.num_candidate_pairs, cand=info.num_candidate_faces, edges=info.num_edges,
+                           tests=info.num_tests, escalated=info.num_escalated, registry=info.num_registry)
+        self.com = np.array(info.com[:])
+        self.t = np.array(info.translation[:])
+        self.eps = info.eps
+        self.device_ms = info.device_ms
+        self._nv = nv
+
+    def registry(self):
+        n = self.info.num_registry
+        e, g, p = np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros((n, 3), np.float64)
+        _check(lib().mcgpu_get_registry(self._ctx.h, _ptr(e), _ptr(g), _ptr(p), n), "mcgpu_get_registry")
+        return e, g, p
+
+    def pairs(self):
+        n = self.info.num_candidate_pairs
+        out = np.zeros((n, 2), np.uint32)
+        _check(lib().mcgpu_get_candidate_pairs(self._ctx.h, _ptr(out), n), "mcgpu_get_candidate_pairs")
+        return out
+
+    def edges(self):
+        n = self.info.num_edges
+        v0, v1 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        f0, f1 = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        _check(lib().mcgpu_get_edges(self._ctx.h, _ptr(v0), _ptr(v1), _ptr(f0), _ptr(f1), n), "mcgpu_get_edges")
+        return v0, v1, f0, f1
+
+    def tests(self):
+        n = self.info.num_tests
+        e, g = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        qs, rs = np.zeros(n, np.int8), np.zeros(n, np.int8)
+        ty, pp = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+        _check(lib().mcgpu_get_tests(self._ctx.h, _ptr(e), _ptr(g), _ptr(qs), _ptr(rs), _ptr(ty), _ptr(pp), n),
+               "mcgpu_get_tests")
+        return dict(edge=e, face=g, qsign=qs, rsign=rs, type=ty, pip=pp)
+
+    def planes(self):
+        n = self.info.num_candidate_faces
+        f = np.zeros(n, np.uint32)
+        nrm = np.zeros((n, 3))
+        d = np.zeros(n)
+        k = np.zeros(n, np.int32)
+        tri = np.zeros((n, 3), np.int32)
+        _check(lib().mcgpu_get_face_planes(self._ctx.h, _ptr(f), _ptr(nrm), _ptr(d), _ptr(k), _ptr(tri), n),
+               "mcgpu_get_face_planes")
+        return dict(face=f, normal=nrm, d=d, k=k, triple=tri)
+
+    def ps_vertices(self):
+        out = np.zeros((self._nv, 3))
+        _check(lib().mcgpu_get_ps_vertices(self._ctx.h, _ptr(out), self._nv), "mcgpu_get_ps_vertices")
+        return out
+
+    def stage_ms(self):
+        ms = np.zeros(16, np.float32)
+        n = lib().mcgpu_stage_times(self._ctx.h, _ptr(ms), 16)
+        return {lib().mcgpu_stage_name(i).decode(): float(ms[i]) for i in range(n)}
+
+
+class Context:
+    """One libmcgpu context: a HIP stream and its device buffers on one GPU."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().mcgpu_create_context(device, C.byref(h)), "mcgpu_create_context")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mcgpu_release_context(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def dispatch(self, src, cut, flags=0, gp_constant=1e-4, enforcement_count=0, keep_tests=False):
+        keep = []
+        ms, mc = _host_mesh(src, keep), _host_mesh(cut, keep)
+        d = DispatchDesc(flags, gp_constant, enforcement_count, 0, 1 if keep_tests else 0, 0)
+        info = FrontendInfo()
+        _check(lib().mcgpu_dispatch_frontend(self.h, C.byref(ms), C.byref(mc), C.byref(d), C.byref(info)),
+               "mcgpu_dispatch_frontend")
+        return Result(self, info, ms.num_vertices + mc.num_vertices)
+
+    def dispatch_device(self, src: Mesh, cut: Mesh, flags=0, gp_constant=1e-4, enforcement_count=0, keep_tests=False):
+        d = DispatchDesc(flags, gp_constant, enforcement_count, 1, 1 if keep_tests else 0, 0)
+        info = FrontendInfo()
+        _check(lib().mcgpu_dispatch_frontend(self.h, C.byref(src), C.byref(cut), C.byref(d), C.byref(info)),
+               "mcgpu_dispatch_frontend")
+        return Result(self, info, src.num_vertices + cut.num_vertices)
+
+    def redispatch_perturbed(self, perturbation, enforcement_count, nv):
+        p = np.ascontiguousarray(perturbation, dtype=np.float64)
+        info = FrontendInfo()
+        _check(lib().mcgpu_redispatch_perturbed(self.h, _ptr(p), enforcement_count, C.byref(info)),
+               "mcgpu_redispatch_perturbed")
+        return Result(self, info, nv)

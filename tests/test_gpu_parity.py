@@ -1,0 +1,143 @@
+"""HIP front end (through libmcgpu's C ABI) against the CPU oracle, bit for bit.
+
+Every intermediate the reference produces is compared: shift parameters,
+shifted polygon-soup vertices, the halfedge edge table, candidate face pairs,
+candidate-face planes and vertex triplets, every edge-face test with its two
+orient3d signs, segment type and point-in-polygon result, the intersection
+registry (edge, face, point bits) and the dispatch status.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as om
+import meshes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from mcut_amd import frontend
+    c = frontend.Context(0)
+    yield c
+    c.close()
+
+
+def _sorted_pairs(a):
+    a = np.asarray(a).reshape(-1, 2)
+    return a[np.lexsort((a[:, 1], a[:, 0]))]
+
+
+def compare(ctx, src, cut, detail=True, **kw):
+    print(f"compare: Fs={meshes.nfaces(src)} Fc={meshes.nfaces(cut)} {kw}", flush=True)
+    R = ctx.dispatch(src, cut, keep_tests=detail, **kw)
+    okw = {}
+    if kw.get("flags", 0) & (1 << 15):
+        okw["enforce_gp"] = True
+    O = om.frontend(src, cut, detail=detail, **okw)
+    assert np.array_equal(R.com, O["com"]) and np.array_equal(R.t, O["t"]) and R.eps == O["eps"]
+    assert R.status == O["status"], (R.status_name, O["status"], O["stage"])
+    assert R.counts["pairs"] == O["counts"]["pairs"]
+    assert np.array_equal(R.pairs(), O["pairs"])
+    if detail:
+        assert np.array_equal(R.ps_vertices(), O["ps_vertices"])
+        v0, v1, f0, f1 = R.edges()
+        assert np.array_equal(v0, O["edge_v0"]) and np.array_equal(v1, O["edge_v1"])
+        assert np.array_equal(f0, O["edge_f0"]) and np.array_equal(f1, O["edge_f1"])
+    if O["stage"] >= 1:  # the oracle got past the face-property stage
+        P = R.planes()
+        assert np.array_equal(P["face"], O["cand_face"])
+        assert np.array_equal(P["normal"], O["plane_n"]) and np.array_equal(P["d"], O["plane_d"])
+        assert np.array_equal(P["k"], O["plane_k"])
+        ot = O["plane_triple"].copy()
+        gt = P["triple"].copy()
+        coll = ot[:, 0] < 0
+        assert np.array_equal(gt[:, 0] < 0, coll)
+        assert np.array_equal(gt[~coll], ot[~coll])
+        assert R.counts["tests"] == O["counts"]["tests"]
+        if detail:
+            T = R.tests()
+            assert np.array_equal(T["edge"], O["test_edge"]) and np.array_equal(T["face"], O["test_face"])
+            assert np.array_equal(T["qsign"], O["qsign"]) and np.array_equal(T["rsign"], O["rsign"])
+            assert np.array_equal(T["type"], O["type"])
+            assert np.array_equal(T["pip"], O["pip"])
+        if O["status"] == om.SUCCESS or O["stage"] >= 2:
+            e, g, p = R.registry()
+            assert np.array_equal(e, O["reg_edge"]) and np.array_equal(g, O["reg_face"])
+            assert np.array_equal(p.view(np.uint64), O["reg_point"].view(np.uint64))
+    return R, O
+
+
+def test_config1_rotated(ctx):
+    R, O = compare(ctx, *meshes.config1_rotated())
+    assert R.status == 0 and R.counts["registry"] == 134
+
+
+def test_config1_general_position_violation(ctx):
+    from mcut_amd import meshgen as mg
+    R, O = compare(ctx, *mg.config1())
+    assert R.status_name == "GENERAL_POSITION_VIOLATION"
+
+
+def test_config1_perturbed_retry(ctx):
+    """Reference's enforce-general-position retry: same candidates, cut mesh
+    moved by the reference's perturbation vector (golden fixture)."""
+    import os
+    from mcut_amd import meshgen as mg
+    cases = np.load(os.path.join(meshes.GOLDEN, "ref_cases.npz"))
+    pert = cases["config1_enforce/perturbation"]
+    src, cut = mg.config1()
+    R = ctx.dispatch(src, cut, flags=1 << 15)
+    assert R.status_name == "GENERAL_POSITION_VIOLATION"
+    nv = src[0].shape[0] + cut[0].shape[0]
+    R2 = ctx.redispatch_perturbed(pert, 1, nv)
+    O = om.frontend(src, cut, enforce_gp=True, enforcement_count=1, perturbation=pert)
+    assert R2.status == O["status"] == 0
+    e, g, p = R2.registry()
+    assert np.array_equal(e, O["reg_edge"]) and np.array_equal(g, O["reg_face"])
+    assert np.array_equal(p.view(np.uint64), O["reg_point"].view(np.uint64))
+    want = {tuple(x) for x in cases["config1_enforce/points"].tolist()}
+    assert {tuple(x) for x in p.tolist()} == want
+
+
+@pytest.mark.parametrize("i", range(64))
+def test_benchmark_pairs(ctx, i):
+    compare(ctx, *meshes.bench_pair(i))
+
+
+@pytest.mark.parametrize("lvl,amp", [(3, 0.02), (4, 0.02), (5, 0.005), (6, 0.005), (6, 0.02)])
+def test_icosphere_pairs(ctx, lvl, amp):
+    compare(ctx, *meshes.icosphere_pair(lvl, amp=amp))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_box_cuts(ctx, seed):
+    """Tessellated boxes (triangles and quads) at random poses through an icosphere."""
+    from mcut_amd import meshgen as mg
+    rng = np.random.default_rng(100 + seed)
+    V, F, _ = mg.icosphere(4)
+    V = mg.radial_displace(V, 0.01, seed=seed)
+    Rm = mg.random_rotation(seed=200 + seed)
+    box = mg.tessellated_box(rng.uniform(-0.3, 0.3, 3), rng.uniform(0.6, 1.4, 3), int(rng.integers(2, 7)),
+                             rotation=Rm, triangulate=bool(seed % 2))
+    compare(ctx, (V, F, None), box)
+
+
+def test_empty_intersection(ctx):
+    from mcut_amd import meshgen as mg
+    src = mg.uv_sphere(16, 8)
+    cut = mg.tessellated_box((5.0, 0.0, 0.0), 1.0, 2)
+    R, O = compare(ctx, src, cut)
+    assert R.counts["pairs"] == 0 and R.counts["registry"] == 0 and R.status == 0
+
+
+def test_invalid_input_rejected(ctx):
+    from mcut_amd import frontend, meshgen as mg
+    V, F, S = mg.uv_sphere(8, 4)
+    bad = F.copy()
+    bad[5] = V.shape[0] + 7  # vertex index out of range
+    with pytest.raises(frontend.McgpuError) as ei:
+        ctx.dispatch((V, bad, None), mg.tessellated_box((0, 0, 0), 1.0, 2))
+    assert ei.value.code == -1
+    # the context stays usable
+    compare(ctx, *meshes.config1_rotated())
