@@ -12,62 +12,88 @@ namespace mcgpu {
 // 1. shift parameters (calculate_vertex_parameters, preproc.cpp:2124-2290)
 // ===========================================================================
 
-// One 64-lane block per mesh.  The coordinate sums must be the reference's
-// left-to-right sums (any other association changes the centre's bits and so
-// every shifted coordinate), so the lanes walk the vertex array in order with
-// uniform loads; the bounding box is an order-independent lane reduction.
+// One 64-lane wavefront per (mesh, coordinate): six independent chains.  Each
+// sum must be the reference's left-to-right sum (any other association
+// changes the centre's bits and so every shifted coordinate), so the chain is
+// inherently serial; what the kernel removes is everything around the adds.
+// The wave loads 512 coordinates at a time with coalesced vector loads into
+// one half of an LDS double buffer while the serial adds consume the other
+// half through broadcast LDS reads.  The bounding box is an order-independent
+// lane reduction over the same loads.
+constexpr int kSumChunk = 512;
+
 __global__ void __launch_bounds__(64) k_mesh_sums(const double* __restrict__ v0, uint32_t n0,
                                                   const double* __restrict__ v1, uint32_t n1, Params* P)
 {
-    const int m = blockIdx.x;
+    __shared__ double buf[2][kSumChunk];
+    const int m = blockIdx.x / 3, j = blockIdx.x % 3;
     const double* __restrict__ v = m ? v1 : v0;
     const uint32_t n = m ? n1 : n0;
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    uint32_t i = 0;
-    for (; i + 4 <= n; i += 4) {
-        const double* p = v + 3 * (uint64_t)i;
-        const double a0 = p[0], a1 = p[1], a2 = p[2], b0 = p[3], b1 = p[4], b2 = p[5];
-        const double c0 = p[6], c1 = p[7], c2 = p[8], d0 = p[9], d1 = p[10], d2 = p[11];
-        sx += a0;
-        sy += a1;
-        sz += a2;
-        sx += b0;
-        sy += b1;
-        sz += b2;
-        sx += c0;
-        sy += c1;
-        sz += c2;
-        sx += d0;
-        sy += d1;
-        sz += d2;
+    const int lane = threadIdx.x;
+    constexpr int U = kSumChunk / 64;
+    double lo = 1.7976931348623157e308, hi = -1.7976931348623157e308;
+    double r[U];
+    const uint32_t nchunks = (n + kSumChunk - 1) / kSumChunk;
+    // prologue: chunk 0
+    for (int u = 0; u < U; ++u) {
+        const uint32_t idx = (uint32_t)u * 64 + lane;
+        r[u] = idx < n ? v[3 * (uint64_t)idx + j] : 0.0;
+        if (idx < n) {
+            hi = (hi < r[u]) ? r[u] : hi;
+            lo = (r[u] < lo) ? r[u] : lo;
+        }
     }
-    for (; i < n; ++i) {
-        sx += v[3 * (uint64_t)i];
-        sy += v[3 * (uint64_t)i + 1];
-        sz += v[3 * (uint64_t)i + 2];
+    for (int u = 0; u < U; ++u) buf[0][u * 64 + lane] = r[u];
+    __syncthreads();
+    double sum = 0.0;
+    for (uint32_t c = 0; c < nchunks; ++c) {
+        const int cur = c & 1;
+        const uint32_t nb = (c + 1) * kSumChunk;
+        const bool more = c + 1 < nchunks;
+        // next chunk: unconditional (clamped) loads, so they stay in flight
+        // while the serial adds below consume the current chunk
+        if (more)
+            for (int u = 0; u < U; ++u) {
+                uint32_t idx = nb + (uint32_t)u * 64 + lane;
+                idx = idx < n ? idx : n - 1;
+                r[u] = v[3 * (uint64_t)idx + j];
+            }
+        const uint32_t cnt = (n - c * kSumChunk) < (uint32_t)kSumChunk ? (n - c * kSumChunk) : (uint32_t)kSumChunk;
+        const double* b = buf[cur];
+        if (cnt == (uint32_t)kSumChunk) {
+            // registers hold the next 32 values while the current 32 are added
+            double x[32], y[32];
+            for (int q = 0; q < 32; ++q) x[q] = b[q];
+            for (int k = 32; k < kSumChunk; k += 64) {
+                for (int q = 0; q < 32; ++q) y[q] = b[k + q];
+                for (int q = 0; q < 32; ++q) sum += x[q];
+                if (k + 32 < kSumChunk)
+                    for (int q = 0; q < 32; ++q) x[q] = b[k + 32 + q];
+                for (int q = 0; q < 32; ++q) sum += y[q];
+            }
+        } else {
+            for (uint32_t k = 0; k < cnt; ++k) sum += b[k];
+        }
+        if (more)
+            for (int u = 0; u < U; ++u) {
+                const uint32_t idx = nb + (uint32_t)u * 64 + lane;
+                if (idx < n) {
+                    hi = (hi < r[u]) ? r[u] : hi;
+                    lo = (r[u] < lo) ? r[u] : lo;
+                }
+                buf[cur ^ 1][u * 64 + lane] = r[u];
+            }
+        __syncthreads();
     }
-    double lo[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
-    double hi[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
-    for (uint32_t k = threadIdx.x; k < n; k += 64)
-        for (int j = 0; j < 3; ++j) {
-            const double x = v[3 * (uint64_t)k + j];
-            hi[j] = (hi[j] < x) ? x : hi[j];
-            lo[j] = (x < lo[j]) ? x : lo[j];
-        }
-    for (int o = 32; o > 0; o >>= 1)
-        for (int j = 0; j < 3; ++j) {
-            const double a = __shfl_xor(lo[j], o, 64), b = __shfl_xor(hi[j], o, 64);
-            lo[j] = (a < lo[j]) ? a : lo[j];
-            hi[j] = (hi[j] < b) ? b : hi[j];
-        }
-    if (threadIdx.x == 0) {
-        P->sum[m][0] = sx;
-        P->sum[m][1] = sy;
-        P->sum[m][2] = sz;
-        for (int j = 0; j < 3; ++j) {
-            P->bmin[m][j] = lo[j];
-            P->bmax[m][j] = hi[j];
-        }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double a = __shfl_xor(lo, o, 64), bb = __shfl_xor(hi, o, 64);
+        lo = (a < lo) ? a : lo;
+        hi = (hi < bb) ? bb : hi;
+    }
+    if (lane == 0) {
+        P->sum[m][j] = sum;
+        P->bmin[m][j] = lo;
+        P->bmax[m][j] = hi;
     }
 }
 
@@ -192,41 +218,61 @@ __device__ __forceinline__ uint32_t corner_rank(uint32_t b, uint32_t i, uint32_t
     return b + (is_cut ? (i == 0 ? n - 1 : i - 1) : i);
 }
 
-__global__ void k_edge_insert(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off, uint32_t nf,
-                              int is_cut, unsigned long long* __restrict__ hkeys, unsigned long long* __restrict__ hfirst,
-                              uint32_t* __restrict__ hcount, uint64_t mask)
-{
-    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t b = off[f], n = off[f + 1] - b;
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t a = idx[b + i], c = idx[b + (i + 1 == n ? 0 : i + 1)];
-            const uint64_t slot = hash_insert(hkeys, mask, edge_key(a, c));
-            atomicMin(&hfirst[slot], ((unsigned long long)corner_rank(b, i, n, is_cut) << 32) | a);
-            atomicAdd(&hcount[slot], 1u);
+// corner c -> (face, corner-in-face); triangle meshes need no table
+struct CornerMap {
+    const uint32_t* off;
+    const uint32_t* cface; // polygon meshes: face of each corner; null for triangles
+    __device__ __forceinline__ void operator()(uint64_t c, uint32_t& f, uint32_t& b, uint32_t& n, uint32_t& i) const
+    {
+        if (!cface) {
+            f = (uint32_t)(c / 3);
+            b = 3 * f;
+            n = 3;
+        } else {
+            f = cface[c];
+            b = off[f];
+            n = off[f + 1] - b;
         }
+        i = (uint32_t)c - b;
+    }
+};
+
+__global__ void k_corner_faces(const uint32_t* __restrict__ off, uint32_t nf, uint32_t* __restrict__ cface)
+{
+    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x)
+        for (uint32_t c = off[f]; c < off[f + 1]; ++c) cface[c] = (uint32_t)f;
+}
+
+__global__ void k_edge_insert(const uint32_t* __restrict__ idx, CornerMap cm, uint64_t nk, int is_cut,
+                              unsigned long long* __restrict__ hkeys, unsigned long long* __restrict__ hfirst, uint64_t mask)
+{
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nk; c += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t f, b, n, i;
+        cm(c, f, b, n, i);
+        const uint32_t a = idx[c], d = idx[b + (i + 1 == n ? 0 : i + 1)];
+        const uint64_t slot = hash_insert(hkeys, mask, edge_key(a, d));
+        atomicMin(&hfirst[slot], ((unsigned long long)corner_rank(b, i, n, is_cut) << 32) | a);
     }
 }
 
-__global__ void k_edge_flag(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off, uint32_t nf, int is_cut,
+__global__ void k_edge_flag(const uint32_t* __restrict__ idx, CornerMap cm, uint64_t nk, int is_cut,
                             const unsigned long long* __restrict__ hkeys, const unsigned long long* __restrict__ hfirst,
-                            const uint32_t* __restrict__ hcount, uint64_t mask, uint32_t* __restrict__ flag, Counters* C)
+                            uint64_t mask, uint32_t* __restrict__ flag, Counters* C)
 {
-    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t b = off[f], n = off[f + 1] - b;
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t a = idx[b + i], c = idx[b + (i + 1 == n ? 0 : i + 1)];
-            const uint32_t rank = corner_rank(b, i, n, is_cut);
-            const int64_t slot = hash_find(hkeys, mask, edge_key(a, c));
-            if (slot < 0) {
-                atomicOr(&C->nonmanifold, 2u);
-                continue;
-            }
-            const unsigned long long first = hfirst[slot];
-            const uint32_t r0 = (uint32_t)(first >> 32);
-            flag[rank] = (r0 == rank) ? 1u : 0u;
-            if (hcount[slot] > 2) atomicOr(&C->nonmanifold, 1u);
-            if (r0 != rank && (uint32_t)first == a) atomicOr(&C->nonmanifold, 1u); // used twice in one direction
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nk; c += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t f, b, n, i;
+        cm(c, f, b, n, i);
+        const uint32_t a = idx[c], d = idx[b + (i + 1 == n ? 0 : i + 1)];
+        const uint32_t rank = corner_rank(b, i, n, is_cut);
+        const int64_t slot = hash_find(hkeys, mask, edge_key(a, d));
+        if (slot < 0) {
+            atomicOr(&C->nonmanifold, 2u);
+            continue;
         }
+        const unsigned long long first = hfirst[slot];
+        const uint32_t r0 = (uint32_t)(first >> 32);
+        flag[rank] = (r0 == rank) ? 1u : 0u;
+        if (r0 != rank && (uint32_t)first == a) atomicOr(&C->nonmanifold, 1u); // used twice in one direction
     }
 }
 
@@ -238,31 +284,31 @@ __global__ void k_store_edge_count(const uint32_t* __restrict__ total, Counters*
     }
 }
 
-// corner -> ps edge; edge endpoints (h0 direction) and incident faces
-__global__ void k_edge_assign(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off, uint32_t nf, int is_cut,
+// corner -> ps edge; edge endpoints (h0 direction) and incident faces.  A
+// third corner on one edge (non-manifold) finds the h1 face already taken.
+__global__ void k_edge_assign(const uint32_t* __restrict__ idx, CornerMap cm, uint64_t nk, int is_cut,
                               const unsigned long long* __restrict__ hkeys, const unsigned long long* __restrict__ hfirst,
-                              uint64_t mask, const uint32_t* __restrict__ eid_by_rank, const Counters* __restrict__ C,
-                              uint32_t vofs, uint32_t fofs, uint32_t* __restrict__ cedge, uint32_t* __restrict__ ev0,
+                              uint64_t mask, const uint32_t* __restrict__ eid_by_rank, Counters* C, uint32_t vofs,
+                              uint32_t fofs, uint32_t* __restrict__ cedge, uint32_t* __restrict__ ev0,
                               uint32_t* __restrict__ ev1, int32_t* __restrict__ ef0, int32_t* __restrict__ ef1)
 {
     const uint32_t ebase = is_cut ? (uint32_t)C->n_edges_src : 0u;
-    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t b = off[f], n = off[f + 1] - b;
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t a = idx[b + i], c = idx[b + (i + 1 == n ? 0 : i + 1)];
-            const uint32_t rank = corner_rank(b, i, n, is_cut);
-            const int64_t slot = hash_find(hkeys, mask, edge_key(a, c));
-            if (slot < 0) continue; // cannot happen: every corner's key was inserted
-            const uint32_t r0 = (uint32_t)(hfirst[slot] >> 32);
-            const uint32_t e = ebase + eid_by_rank[r0];
-            cedge[b + i] = e;
-            if (r0 == rank) {
-                ev0[e] = vofs + a;
-                ev1[e] = vofs + c;
-                ef0[e] = (int32_t)(fofs + f);
-            } else {
-                ef1[e] = (int32_t)(fofs + f);
-            }
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nk; c += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t f, b, n, i;
+        cm(c, f, b, n, i);
+        const uint32_t a = idx[c], d = idx[b + (i + 1 == n ? 0 : i + 1)];
+        const uint32_t rank = corner_rank(b, i, n, is_cut);
+        const int64_t slot = hash_find(hkeys, mask, edge_key(a, d));
+        if (slot < 0) continue; // flagged by k_edge_flag
+        const uint32_t r0 = (uint32_t)(hfirst[slot] >> 32);
+        const uint32_t e = ebase + eid_by_rank[r0];
+        cedge[c] = e;
+        if (r0 == rank) {
+            ev0[e] = vofs + a;
+            ev1[e] = vofs + d;
+            ef0[e] = (int32_t)(fofs + f);
+        } else if (atomicExch(&ef1[e], (int32_t)(fofs + f)) != -1) {
+            atomicOr(&C->nonmanifold, 4u);
         }
     }
 }
@@ -307,23 +353,85 @@ __device__ __forceinline__ uint32_t expand_bits10(uint32_t v)
     return v;
 }
 
-// 30-bit Morton codes of face-box centres within the mesh's shifted box.  The
-// candidate set does not depend on the tree (every overlapping leaf pair is
-// found), so the codes only shape the tree.
-__global__ void k_morton(const double* __restrict__ box, const Params* __restrict__ P, int mesh, uint32_t nf,
-                         uint32_t* __restrict__ code, uint32_t* __restrict__ face)
+// order-preserving map of doubles to unsigned integers (for atomic min/max)
+__device__ __forceinline__ unsigned long long ord_key(double x)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double ord_value(unsigned long long k)
+{
+    const unsigned long long u = (k & 0x8000000000000000ull) ? (k & ~0x8000000000000000ull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+// raw (unshifted) bounding box of a mesh: ord keys lo[3], hi[3]; one atomic
+// per block and coordinate after an LDS reduction
+__global__ void __launch_bounds__(256) k_raw_bbox(const double* __restrict__ v, uint32_t n, unsigned long long* __restrict__ box)
+{
+    __shared__ double red[4][6];
+    double lo[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+    double hi[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        for (int j = 0; j < 3; ++j) {
+            const double x = v[3 * i + j];
+            lo[j] = x < lo[j] ? x : lo[j];
+            hi[j] = x > hi[j] ? x : hi[j];
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int j = 0; j < 3; ++j) {
+            const double a = __shfl_xor(lo[j], o, 64), b = __shfl_xor(hi[j], o, 64);
+            lo[j] = a < lo[j] ? a : lo[j];
+            hi[j] = b > hi[j] ? b : hi[j];
+        }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int j = 0; j < 3; ++j) {
+            red[w][j] = lo[j];
+            red[w][3 + j] = hi[j];
+        }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int j = threadIdx.x;
+        double a = red[0][j], b = red[0][3 + j];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            a = red[q][j] < a ? red[q][j] : a;
+            b = red[q][3 + j] > b ? red[q][3 + j] : b;
+        }
+        atomicMin(&box[j], ord_key(a));
+        atomicMax(&box[3 + j], ord_key(b));
+    }
+}
+
+// 30-bit Morton codes of raw face-box centres.  The candidate set does not
+// depend on the tree (every overlapping leaf pair is found whatever the leaf
+// order), so the tree is built from unshifted coordinates and runs while the
+// serial coordinate sums that fix the shift are still in flight.
+__global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __restrict__ idx,
+                             const uint32_t* __restrict__ off, uint32_t nf, const unsigned long long* __restrict__ rbox,
+                             uint32_t* __restrict__ code, uint32_t* __restrict__ face)
 {
     double lo[3], ext[3];
     for (int j = 0; j < 3; ++j) {
-        lo[j] = (P->bmin[mesh][j] - P->com[j]) + P->t[j] - P->eps;
-        const double hi = (P->bmax[mesh][j] - P->com[j]) + P->t[j] + P->eps;
-        ext[j] = hi > lo[j] ? hi - lo[j] : 1.0;
+        lo[j] = ord_value(rbox[j]);
+        const double h = ord_value(rbox[3 + j]);
+        ext[j] = h > lo[j] ? h - lo[j] : 1.0;
     }
     for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
-        const double* b = box + 6 * f;
+        const uint32_t b = off[f], n = off[f + 1] - b;
+        double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+        double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+        for (uint32_t i = 0; i < n; ++i) {
+            const double* p = v + 3 * (uint64_t)idx[b + i];
+            for (int j = 0; j < 3; ++j) {
+                mn[j] = p[j] < mn[j] ? p[j] : mn[j];
+                mx[j] = p[j] > mx[j] ? p[j] : mx[j];
+            }
+        }
         uint32_t c[3];
         for (int j = 0; j < 3; ++j) {
-            float x = (float)((((b[j] + b[3 + j]) * 0.5) - lo[j]) / ext[j]);
+            float x = (float)((((mn[j] + mx[j]) * 0.5) - lo[j]) / ext[j]);
             x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
             c[j] = expand_bits10((uint32_t)x);
         }
@@ -332,145 +440,228 @@ __global__ void k_morton(const double* __restrict__ box, const Params* __restric
     }
 }
 
-constexpr uint32_t kLeaf = 0x80000000u;
+// ---------------------------------------------------------------------------
+// Implicit complete binary tree over the Morton-ordered leaves (the layout
+// family of the reference's OIBVH, bvh.cpp:81-193): leaf level D holds 2^D
+// slots, node (l, k) lives at (2^l - 1) + k and has children (l+1, 2k) and
+// (l+1, 2k+1); slots past the last leaf hold empty boxes, which overlap
+// nothing.  The refit needs no atomics and no parent links: a 256-thread block
+// reduces 512 nodes of one level through LDS into the 9 levels above them.
+// ---------------------------------------------------------------------------
+constexpr int kRefitWidth = 512;
+constexpr int kRefitLevels = 9;
 
-__device__ __forceinline__ int lbvh_delta(const uint32_t* __restrict__ code, int n, int i, int j)
+__device__ __forceinline__ void empty_box(double* b)
 {
-    if (j < 0 || j >= n) return -1;
-    const uint32_t a = code[i], b = code[j];
-    if (a == b) return 32 + __clz((uint32_t)(i ^ j));
-    return __clz(a ^ b);
+    b[0] = b[1] = b[2] = 1.7976931348623157e308;
+    b[3] = b[4] = b[5] = -1.7976931348623157e308;
 }
 
-// binary radix tree over sorted codes (Karras 2012); duplicates ordered by index
-__global__ void k_lbvh_nodes(const uint32_t* __restrict__ code, int n, uint32_t* __restrict__ child,
-                             int32_t* __restrict__ nparent, int32_t* __restrict__ lparent)
+__global__ void __launch_bounds__(256) k_refit_levels(const double* __restrict__ facebox, const uint32_t* __restrict__ sface,
+                                                      int n, int D, int lin, double* __restrict__ nodes)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n - 1; i += gridDim.x * blockDim.x) {
-        const int d = (lbvh_delta(code, n, i, i + 1) - lbvh_delta(code, n, i, i - 1)) >= 0 ? 1 : -1;
-        const int dmin = lbvh_delta(code, n, i, i - d);
-        int lmax = 2;
-        while (lbvh_delta(code, n, i, i + lmax * d) > dmin) lmax *= 2;
-        int l = 0;
-        for (int t = lmax / 2; t >= 1; t /= 2)
-            if (lbvh_delta(code, n, i, i + (l + t) * d) > dmin) l += t;
-        const int j = i + l * d;
-        const int dnode = lbvh_delta(code, n, i, j);
-        int s = 0, t = l;
-        do {
-            t = (t + 1) >> 1;
-            if (lbvh_delta(code, n, i, i + (s + t) * d) > dnode) s += t;
-        } while (t > 1);
-        const int gamma = i + s * d + (d < 0 ? d : 0);
-        const int lo = i < j ? i : j, hi = i < j ? j : i;
-        const uint32_t left = (lo == gamma) ? (kLeaf | (uint32_t)gamma) : (uint32_t)gamma;
-        const uint32_t right = (hi == gamma + 1) ? (kLeaf | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
-        child[2 * i] = left;
-        child[2 * i + 1] = right;
-        if (left & kLeaf) lparent[gamma] = i;
-        else nparent[gamma] = i;
-        if (right & kLeaf) lparent[gamma + 1] = i;
-        else nparent[gamma + 1] = i;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) nparent[0] = -1;
-}
-
-__device__ __forceinline__ void load_box_coherent(const double* p, double* o)
-{
-    const volatile double* q = p;
-    for (int j = 0; j < 6; ++j) o[j] = q[j];
-}
-
-// bottom-up refit: the second child to arrive at a node writes the union
-__global__ void k_lbvh_refit(int n, const uint32_t* __restrict__ sface, const double* __restrict__ facebox,
-                             const uint32_t* __restrict__ child, const int32_t* __restrict__ nparent,
-                             const int32_t* __restrict__ lparent, uint32_t* __restrict__ visits, double* nodebox)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int p = lparent[i];
-        for (int level = 0; p >= 0 && level < 128; ++level) {
-            __threadfence();
-            if (atomicAdd(&visits[p], 1u) == 0) break;
-            __threadfence();
-            double b[6], c[6];
-            for (int side = 0; side < 2; ++side) {
-                const uint32_t ch = child[2 * p + side];
-                if (ch & kLeaf) {
-                    const double* fb = facebox + 6 * (uint64_t)sface[ch & ~kLeaf];
-                    for (int j = 0; j < 6; ++j) c[j] = fb[j];
-                } else {
-                    load_box_coherent(nodebox + 6 * (uint64_t)ch, c);
-                }
-                if (side == 0) {
-                    for (int j = 0; j < 6; ++j) b[j] = c[j];
-                } else {
-                    for (int j = 0; j < 3; ++j) {
-                        b[j] = (c[j] < b[j]) ? c[j] : b[j];
-                        b[3 + j] = (b[3 + j] < c[3 + j]) ? c[3 + j] : b[3 + j];
-                    }
-                }
+    __shared__ double sb[kRefitWidth * 6];
+    const int width = 1 << lin;
+    const int per = width < kRefitWidth ? width : kRefitWidth;
+    const int steps = lin < kRefitLevels ? lin : kRefitLevels;
+    const int base = blockIdx.x * per;
+    for (int t = threadIdx.x; t < per; t += blockDim.x) {
+        const int k = base + t;
+        double b[6];
+        if (lin == D) {
+            if (k < n) {
+                const double* fb = facebox + 6 * (uint64_t)sface[k];
+                for (int q = 0; q < 6; ++q) b[q] = fb[q];
+            } else {
+                empty_box(b);
             }
-            for (int j = 0; j < 6; ++j) nodebox[6 * (uint64_t)p + j] = b[j];
-            p = nparent[p];
+            double* o = nodes + 6 * ((uint64_t)(1u << D) - 1 + k);
+            for (int q = 0; q < 6; ++q) o[q] = b[q];
+        } else {
+            const double* src = nodes + 6 * ((uint64_t)(1u << lin) - 1 + k);
+            for (int q = 0; q < 6; ++q) b[q] = src[q];
         }
+        for (int q = 0; q < 6; ++q) sb[6 * t + q] = b[q];
+    }
+    __syncthreads();
+    int cur = per;
+    for (int st = 1; st <= steps; ++st) {
+        const int half = cur >> 1;
+        double u[6];
+        const int t = threadIdx.x;
+        if (t < half) {
+            for (int q = 0; q < 3; ++q) {
+                const double a0 = sb[6 * (2 * t) + q], a1 = sb[6 * (2 * t + 1) + q];
+                const double b0 = sb[6 * (2 * t) + 3 + q], b1 = sb[6 * (2 * t + 1) + 3 + q];
+                u[q] = a1 < a0 ? a1 : a0;
+                u[3 + q] = b0 < b1 ? b1 : b0;
+            }
+        }
+        __syncthreads();
+        if (t < half) {
+            for (int q = 0; q < 6; ++q) sb[6 * t + q] = u[q];
+            const int l = lin - st;
+            double* o = nodes + 6 * ((uint64_t)(1u << l) - 1 + (base >> st) + t);
+            for (int q = 0; q < 6; ++q) o[q] = u[q];
+        }
+        __syncthreads();
+        cur = half;
     }
 }
 
 // ===========================================================================
-// 5. broad phase traversal: every source face box against the cut tree.
-// Emits each (src, cut) face pair whose boxes overlap (bvh.cpp:661-781 finds
-// exactly these) and records it in the pair hash used for test dedup.
+// 5. broad phase traversal.  Every (src, cut) face pair whose boxes overlap is
+// emitted -- exactly the pairs bvh.cpp:661-781 finds -- and recorded in the
+// pair hash used for test dedup.  Walks are stackless: the node after a
+// subtree follows from index arithmetic on the implicit tree.
 // ===========================================================================
 
 struct Tree {
-    const uint32_t* child;
-    const uint32_t* sface; // leaf -> mesh-local face
-    const double* nodebox;
-    const double* facebox; // ps face boxes of this mesh
-    int n;
+    const uint32_t* sface;  // leaf -> mesh-local face
+    const double* nodes;    // implicit tree boxes
+    const double* facebox;  // ps face boxes of this mesh
+    int n, D;
 };
 
-__global__ void __launch_bounds__(kBlock) k_traverse(Tree src, Tree cut, unsigned long long* __restrict__ pairs,
-                                                     uint64_t cap, unsigned long long* __restrict__ phash, uint64_t pmask,
-                                                     Counters* C)
+// Wavefront-cooperative BVH x BVH traversal.  A 64-lane wave owns 64
+// Morton-consecutive source leaves; their union is the source-tree node at
+// depth D_s - 6.  The wave walks the cut tree breadth first with that box,
+// all lanes testing children in parallel and compacting the survivors into
+// an LDS frontier with ballots; at the leaf level every lane tests its own
+// source face against the surviving cut leaves.  A frontier that outgrows
+// LDS sends the wave to the per-lane stackless walk instead (same result).
+constexpr int kFrontierCap = 2048;
+
+__device__ __forceinline__ void per_lane_walk(const double* q, uint32_t s, const Tree& cut, unsigned long long* pairs,
+                                              uint64_t cap, unsigned long long* phash, uint64_t pmask, Counters* C)
 {
-    uint32_t stack[64];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < src.n; i += gridDim.x * blockDim.x) {
-        const uint32_t s = src.sface[i];
-        double q[6];
-        for (int j = 0; j < 6; ++j) q[j] = src.facebox[6 * (uint64_t)s + j];
-        int sp = 0;
-        if (cut.n == 1) {
-            stack[sp++] = kLeaf | 0u;
-        } else {
-            stack[sp++] = 0u;
-        }
-        // each node is entered at most once per query: the visit count is bounded by the tree size
-        for (int visits = 0; sp > 0 && visits < 2 * cut.n + 2; ++visits) {
-            const uint32_t node = stack[--sp];
-            uint32_t kids[2];
-            int nk;
-            if (node & kLeaf) {
-                kids[0] = node;
-                nk = 1;
-            } else {
-                if (sp == 0 && node == 0u) {
-                    // root: test it before descending
-                    double r[6];
-                    for (int j = 0; j < 6; ++j) r[j] = cut.nodebox[j];
-                    if (!boxes_overlap(q, r)) break;
+    const int D = cut.D;
+    const long long budget = 2ll << D;
+    int l = 0;
+    uint32_t k = 0;
+    for (long long it = 0; it < budget; ++it) {
+        const double* nb = cut.nodes + 6 * ((uint64_t)(1u << l) - 1 + k);
+        double b[6];
+        for (int j = 0; j < 6; ++j) b[j] = nb[j];
+        if (boxes_overlap(q, b)) {
+            if (l == D) {
+                const uint32_t c = cut.sface[k];
+                const uint64_t key = ((uint64_t)s << 32) | c;
+                const unsigned long long at = wave_append(&C->n_pairs);
+                if (at < cap) {
+                    pairs[at] = key;
+                    hash_insert(phash, pmask, key);
+                } else {
+                    atomicOr(&C->ov_pairs, 1u);
                 }
-                kids[0] = cut.child[2 * node];
-                kids[1] = cut.child[2 * node + 1];
-                nk = 2;
+            } else {
+                ++l;
+                k <<= 1;
+                continue;
             }
-            for (int k = 0; k < nk; ++k) {
-                const uint32_t ch = kids[k];
-                double b[6];
-                if (ch & kLeaf) {
-                    const uint32_t c = cut.sface[ch & ~kLeaf];
-                    for (int j = 0; j < 6; ++j) b[j] = cut.facebox[6 * (uint64_t)c + j];
-                    if (boxes_overlap(q, b)) {
+        }
+        while (l > 0 && (k & 1u)) {
+            k >>= 1;
+            --l;
+        }
+        if (l == 0) break;
+        ++k;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsigned long long* __restrict__ pairs,
+                                                      uint64_t cap, unsigned long long* __restrict__ phash,
+                                                      uint64_t pmask, Counters* C)
+{
+    __shared__ uint32_t front[2][kFrontierCap];
+    __shared__ double lbox[64][6];
+    __shared__ uint32_t lface[64];
+    const int lane = threadIdx.x;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int nwaves = (src.n + 63) / 64;
+    for (int w = blockIdx.x; w < nwaves; w += gridDim.x) {
+        const int i = w * 64 + lane;
+        const bool valid = i < src.n;
+        double q[6];
+        uint32_t s = 0;
+        if (valid) {
+            s = src.sface[i];
+            for (int j = 0; j < 6; ++j) q[j] = src.facebox[6 * (uint64_t)s + j];
+        } else {
+            empty_box(q);
+        }
+        double wb[6];
+        {
+            const int lw = src.D >= 6 ? src.D - 6 : 0;
+            const uint32_t kw = src.D >= 6 ? (uint32_t)w : 0u;
+            const double* nb = src.nodes + 6 * ((uint64_t)(1u << lw) - 1 + kw);
+            for (int j = 0; j < 6; ++j) wb[j] = nb[j];
+        }
+        // breadth-first walk of the cut tree with the wave box
+        int cur = 0, nf = 0, level = 0;
+        bool overflow = false;
+        {
+            double rb[6];
+            for (int j = 0; j < 6; ++j) rb[j] = cut.nodes[j];
+            if (boxes_overlap(wb, rb)) {
+                if (lane == 0) front[0][0] = 0u;
+                nf = 1;
+            }
+        }
+        __syncthreads();
+        while (nf > 0 && level < cut.D && !overflow) {
+            int nn = 0;
+            const int nchild = 2 * nf;
+            const uint32_t base_next = (uint32_t)(1u << (level + 1)) - 1;
+            for (int c0 = 0; c0 < nchild; c0 += 64) {
+                const int c = c0 + lane;
+                bool keep = false;
+                uint32_t child = 0;
+                if (c < nchild) {
+                    child = 2u * front[cur][c >> 1] + (uint32_t)(c & 1);
+                    const double* nb = cut.nodes + 6 * ((uint64_t)base_next + child);
+                    double b[6];
+                    for (int j = 0; j < 6; ++j) b[j] = nb[j];
+                    keep = boxes_overlap(wb, b);
+                }
+                const uint64_t m = __ballot(keep);
+                const int pos = nn + __popcll(m & lt);
+                if (keep && pos < kFrontierCap) front[cur ^ 1][pos] = child;
+                nn += __popcll(m);
+            }
+            __syncthreads();
+            if (nn > kFrontierCap) overflow = true;
+            nf = nn;
+            cur ^= 1;
+            ++level;
+        }
+        if (lane == 0) {
+            atomicAdd(&C->n_visits, (unsigned long long)level);
+            if (overflow) atomicAdd(&C->n_fallback, 1ull);
+            else atomicAdd(&C->n_leaf_tests, (unsigned long long)nf);
+        }
+        if (overflow) {
+            if (valid) per_lane_walk(q, s, cut, pairs, cap, phash, pmask, C);
+        } else if (nf > 0) {
+            // surviving cut leaves are staged through LDS 64 at a time (one
+            // coalesced load round per 64 leaves), then every lane tests its
+            // own source face against each staged box
+            const uint32_t leaf_base = (uint32_t)(1u << cut.D) - 1;
+            for (int t0 = 0; t0 < nf; t0 += 64) {
+                const int t = t0 + lane;
+                if (t < nf) {
+                    const uint32_t k = front[cur][t];
+                    const double* nb = cut.nodes + 6 * ((uint64_t)leaf_base + k);
+                    for (int j = 0; j < 6; ++j) lbox[lane][j] = nb[j];
+                    lface[lane] = cut.sface[k];
+                }
+                __syncthreads();
+                const int cnt = (nf - t0) < 64 ? (nf - t0) : 64;
+                for (int u = 0; u < cnt; ++u) {
+                    double b[6];
+                    for (int j = 0; j < 6; ++j) b[j] = lbox[u][j];
+                    if (valid && boxes_overlap(q, b)) {
+                        const uint32_t c = lface[u];
                         const uint64_t key = ((uint64_t)s << 32) | c;
                         const unsigned long long at = wave_append(&C->n_pairs);
                         if (at < cap) {
@@ -480,15 +671,11 @@ __global__ void __launch_bounds__(kBlock) k_traverse(Tree src, Tree cut, unsigne
                             atomicOr(&C->ov_pairs, 1u);
                         }
                     }
-                } else {
-                    for (int j = 0; j < 6; ++j) b[j] = cut.nodebox[6 * (uint64_t)ch + j];
-                    if (boxes_overlap(q, b)) {
-                        if (sp < 64) stack[sp++] = ch;
-                        else atomicOr(&C->ov_pairs, 2u);
-                    }
                 }
+                __syncthreads();
             }
         }
+        __syncthreads();
     }
 }
 
@@ -496,14 +683,20 @@ __global__ void __launch_bounds__(kBlock) k_traverse(Tree src, Tree cut, unsigne
 // 6. candidate faces and their planes (kernel.cpp:2184-2356)
 // ===========================================================================
 
-__global__ void k_mark_candidates(const unsigned long long* __restrict__ pairs, const Counters* __restrict__ C,
-                                  uint64_t cap, uint32_t Fs, uint8_t* __restrict__ cand)
+// flag each face of each pair once, appending it to the candidate list
+__global__ void k_mark_candidates(const unsigned long long* __restrict__ pairs, Counters* C, uint64_t cap, uint32_t Fs,
+                                  uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_list)
 {
     const uint64_t n = C->n_pairs < cap ? C->n_pairs : cap;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t k = pairs[i];
-        cand[k >> 32] = 1;
-        cand[Fs + (uint32_t)k] = 1;
+        const uint32_t f[2] = {(uint32_t)(k >> 32), Fs + (uint32_t)k};
+        for (int side = 0; side < 2; ++side) {
+            if (cand[f[side]] == 0u && atomicExch(&cand[f[side]], 1u) == 0u) {
+                const unsigned long long at = wave_append(&C->n_cand);
+                cand_list[at] = f[side]; // at most one entry per face
+            }
+        }
     }
 }
 
@@ -535,29 +728,28 @@ __device__ inline bool best_triple_small(int n, VertexAt vat, const double* norm
     return found;
 }
 
-__global__ void k_planes(Soup S, const uint8_t* __restrict__ cand, Plane* __restrict__ planes,
+__global__ void k_planes(Soup S, const uint32_t* __restrict__ cand_list, Plane* __restrict__ planes,
                          uint32_t* __restrict__ big_list, uint64_t cap_big, Counters* C)
 {
-    const uint32_t F = S.nf[0] + S.nf[1];
-    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < F; g += (uint64_t)gridDim.x * blockDim.x) {
-        if (!cand[g]) continue;
-        const FaceView fv = face_view(S, (uint32_t)g);
+    const uint64_t nc = C->n_cand;
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nc; w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t g = cand_list[w];
+        const FaceView fv = face_view(S, g);
         Plane pl;
         const int k = plane_coefficients((int)fv.n, fv, pl.n, &pl.d);
         pl.k = k;
         pl.tri[0] = 0, pl.tri[1] = 1, pl.tri[2] = 2;
         pl.big = 0;
-        atomicAdd(&C->n_cand, 1ull);
         if (k < 0) {
             // kernel.cpp:2320-2331 names the mesh with "face id > source face count"
-            if ((uint32_t)g > S.nf[0]) atomicOr(&C->bad_cut, 1u);
+            if (g > S.nf[0]) atomicOr(&C->bad_cut, 1u);
             else atomicOr(&C->bad_src, 1u);
         } else if (fv.n > 3 && fv.n <= 5) {
             if (!best_triple_small((int)fv.n, fv, pl.n, k, pl.tri)) pl.tri[0] = 0xff;
         } else if (fv.n > 5 && fv.n <= kMaxPoly) {
             pl.big = 1;
             const unsigned long long at = atomicAdd(&C->n_big, 1ull);
-            if (at < cap_big) big_list[at] = (uint32_t)g;
+            if (at < cap_big) big_list[at] = g;
             else atomicOr(&C->ov_big, 1u);
         } else if (fv.n > kMaxPoly) {
             atomicOr(&C->unsupported, 1u);
@@ -749,6 +941,8 @@ __device__ __forceinline__ bool edge_face_cull(const Soup& S, uint32_t e, uint32
     return boxes_overlap(eb, fb);
 }
 
+// six work items per pair: (side, corner slot); a polygon's corners beyond
+// the third are taken by the same slot in steps of three
 __global__ void __launch_bounds__(kBlock) k_tests(Soup S, const unsigned long long* __restrict__ pairs, uint64_t cap_pairs,
                                                   const unsigned long long* __restrict__ phash, uint64_t pmask,
                                                   uint32_t* __restrict__ te, uint32_t* __restrict__ tg, uint64_t cap_tests,
@@ -756,44 +950,34 @@ __global__ void __launch_bounds__(kBlock) k_tests(Soup S, const unsigned long lo
 {
     const uint64_t n = C->n_pairs < cap_pairs ? C->n_pairs : cap_pairs;
     const uint32_t Fs = S.nf[0];
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < 6 * n; w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = w / 6;
+        const uint32_t slot = (uint32_t)(w - 6 * i);
+        const int side = slot >= 3;
+        const uint32_t j0 = side ? slot - 3 : slot;
         const uint64_t key = pairs[i];
         const uint32_t s = (uint32_t)(key >> 32), c = (uint32_t)key;
-        // edges of the source face against the cut face
-        {
-            const uint32_t b = S.off[0][s], nn = S.off[0][s + 1] - b;
-            for (uint32_t k = 0; k < nn; ++k) {
-                const uint32_t e = S.cedge[0][b + k];
-                const int32_t f0 = S.ef0[e], f1 = S.ef1[e];
-                const int32_t fo = (f0 == (int32_t)s) ? f1 : f0;
-                if (fo >= 0 && (uint32_t)fo < s && pair_exists(phash, pmask, (uint32_t)fo, c)) continue;
-                if (!edge_face_cull(S, e, Fs + c)) continue;
-                const unsigned long long at = wave_append(&C->n_tests);
-                if (at < cap_tests) {
-                    te[at] = e;
-                    tg[at] = Fs + c;
-                } else {
-                    atomicOr(&C->ov_tests, 1u);
-                }
+        const uint32_t f = side ? c : s;                  // face whose edges are tested
+        const uint32_t self = side ? Fs + c : s;          // its ps id
+        const uint32_t g = side ? s : Fs + c;             // the other face (ps id)
+        const uint32_t b = S.off[side][f], nn = S.off[side][f + 1] - b;
+        for (uint32_t k = j0; k < nn; k += 3) {
+            const uint32_t e = S.cedge[side][b + k];
+            const int32_t f0 = S.ef0[e], f1 = S.ef1[e];
+            const int32_t fo = (f0 == (int32_t)self) ? f1 : f0;
+            // the lowest-numbered face of e paired with g emits the test
+            if (fo >= 0 && (uint32_t)fo < self) {
+                const bool paired = side ? pair_exists(phash, pmask, s, (uint32_t)fo - Fs)
+                                         : pair_exists(phash, pmask, (uint32_t)fo, c);
+                if (paired) continue;
             }
-        }
-        // edges of the cut face against the source face
-        {
-            const uint32_t b = S.off[1][c], nn = S.off[1][c + 1] - b;
-            const int32_t self = (int32_t)(Fs + c);
-            for (uint32_t k = 0; k < nn; ++k) {
-                const uint32_t e = S.cedge[1][b + k];
-                const int32_t f0 = S.ef0[e], f1 = S.ef1[e];
-                const int32_t fo = (f0 == self) ? f1 : f0;
-                if (fo >= 0 && fo < self && pair_exists(phash, pmask, s, (uint32_t)fo - Fs)) continue;
-                if (!edge_face_cull(S, e, s)) continue;
-                const unsigned long long at = wave_append(&C->n_tests);
-                if (at < cap_tests) {
-                    te[at] = e;
-                    tg[at] = s;
-                } else {
-                    atomicOr(&C->ov_tests, 1u);
-                }
+            if (!edge_face_cull(S, e, g)) continue;
+            const unsigned long long at = wave_append(&C->n_tests);
+            if (at < cap_tests) {
+                te[at] = e;
+                tg[at] = g;
+            } else {
+                atomicOr(&C->ov_tests, 1u);
             }
         }
     }
@@ -980,12 +1164,33 @@ __global__ void k_registry_gather(const unsigned long long* __restrict__ skey, c
 // cut face) pair joined by an intersection point needs at least two points
 // (kernel.cpp:3439-3452); a cut-mesh border edge hit marks a partial cut
 // (kernel.cpp:2646-2650)
-__global__ void k_status_pairs(Soup S, const uint32_t* __restrict__ re, const uint32_t* __restrict__ rg,
-                               const unsigned long long* __restrict__ n_dev, uint64_t cap,
-                               unsigned long long* __restrict__ hkeys, uint32_t* __restrict__ hcnt, uint64_t mask,
-                               Counters* C)
+// status hash: 4 slots per registry entry, the power of two above 4 n
+__device__ __forceinline__ uint64_t status_slots(uint64_t n)
 {
-    const uint64_t n = *n_dev < cap ? *n_dev : cap;
+    uint64_t p = 64;
+    while (p < 4 * n) p <<= 1;
+    return p;
+}
+
+__global__ void k_status_clear(const unsigned long long* __restrict__ n_dev, unsigned long long* __restrict__ hkeys,
+                               uint32_t* __restrict__ hcnt, uint64_t max_slots)
+{
+    uint64_t slots = status_slots(*n_dev);
+    if (slots > max_slots) slots = max_slots;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slots; i += (uint64_t)gridDim.x * blockDim.x) {
+        hkeys[i] = kEmpty64;
+        hcnt[i] = 0u;
+    }
+}
+
+__global__ void k_status_pairs(Soup S, const uint32_t* __restrict__ re, const uint32_t* __restrict__ rg,
+                               const unsigned long long* __restrict__ n_dev, uint64_t max_slots,
+                               unsigned long long* __restrict__ hkeys, uint32_t* __restrict__ hcnt, Counters* C)
+{
+    const uint64_t n = *n_dev;
+    uint64_t slots = status_slots(n);
+    if (slots > max_slots) slots = max_slots;
+    const uint64_t mask = slots - 1;
     const uint32_t Fs = S.nf[0], Vs = S.nv[0];
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t e = re[i], g = rg[i];
@@ -1005,9 +1210,11 @@ __global__ void k_status_pairs(Soup S, const uint32_t* __restrict__ re, const ui
     }
 }
 
-__global__ void k_status_check(const unsigned long long* __restrict__ hkeys, const uint32_t* __restrict__ hcnt,
-                               uint64_t slots, Counters* C)
+__global__ void k_status_check(const unsigned long long* __restrict__ n_dev, const unsigned long long* __restrict__ hkeys,
+                               const uint32_t* __restrict__ hcnt, uint64_t max_slots, Counters* C)
 {
+    uint64_t slots = status_slots(*n_dev);
+    if (slots > max_slots) slots = max_slots;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slots; i += (uint64_t)gridDim.x * blockDim.x)
         if (hkeys[i] != kEmpty64 && hcnt[i] < 2) atomicOr(&C->pair_fail, 1u);
 }

@@ -30,6 +30,7 @@ struct Params {
 struct Counters {
     unsigned long long n_pairs, n_tests, n_esc, n_reg, n_big, n_cand;
     unsigned long long n_edges_src, n_edges_cut;
+    unsigned long long n_visits, n_fallback, n_leaf_tests; // traversal statistics
     unsigned int ov_pairs, ov_tests, ov_esc, ov_reg, ov_big;
     unsigned int gp, bad_src, bad_cut, unsupported, nonmanifold, bad_input, pair_fail, partial;
     unsigned int pad[2];

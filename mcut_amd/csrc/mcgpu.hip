@@ -20,21 +20,29 @@ struct Buf {
     size_t cap = 0;
 };
 
+// Stages of one dispatch.  Stream A carries the chain that depends on the
+// shift (coordinate sums first); stream B runs the shift-independent topology
+// and leaf ordering alongside it.
 enum Stage {
-    ST_PARAMS,
-    ST_TOPOLOGY,
-    ST_SHIFT,
-    ST_BVH,
-    ST_TRAVERSE,
-    ST_PLANES,
-    ST_TESTS,
-    ST_NARROW,
-    ST_REGISTRY,
+    ST_PARAMS,    // A: serial coordinate sums, shift parameters
+    ST_TOPOLOGY,  // B: polygon-soup edge numbering
+    ST_BVH_SORT,  // B: raw boxes, Morton codes, leaf sort
+    ST_SHIFT,     // A: shifted vertices, face boxes (after joining B)
+    ST_REFIT,     // A: implicit-tree boxes
+    ST_TRAVERSE,  // A: broad phase
+    ST_PLANES,    // A: candidate-face planes
+    ST_TESTS,     // A: edge-face tests
+    ST_NARROW,    // A: narrow phase
+    ST_REGISTRY,  // A: registry sort, status
     ST_COUNT
 };
 
-const char* kStageNames[ST_COUNT] = {"shift_params", "topology", "shift_boxes", "bvh_build", "traversal",
-                                     "face_planes",  "tests",    "narrow",      "registry"};
+const char* kStageNames[ST_COUNT] = {"shift_params", "topology", "bvh_sort", "shift_boxes", "bvh_refit",
+                                     "traversal",    "face_planes", "tests", "narrow",      "registry"};
+
+// event slots: one per stage boundary on each stream
+enum Ev { EV_A0, EV_A_PARAMS, EV_B0, EV_B_TOPO, EV_B_SORT, EV_A_SHIFT, EV_A_REFIT, EV_A_TRAV, EV_A_PLANES,
+          EV_A_TESTS, EV_A_NARROW, EV_A_END, EV_FORK, EV_COUNT };
 
 uint64_t next_pow2(uint64_t x)
 {
@@ -54,17 +62,17 @@ int bits_for(uint64_t n)
 
 struct mcgpu_context_t {
     int device = 0;
-    hipStream_t s = nullptr;
-    hipEvent_t ev[ST_COUNT + 1];
+    hipStream_t s = nullptr, s2 = nullptr;
+    hipEvent_t ev[EV_COUNT];
     hipError_t last_error = hipSuccess;
 
     Buf in_v[2], in_idx[2], in_sz[2];
     Buf off[2], params, counters, ps_v, box;
-    Buf hkeys[2], hfirst[2], hcount[2], eflag[2], eid[2], cedge[2], etotal;
+    Buf hkeys[2], hfirst[2], eflag[2], eid[2], cedge[2], cface[2], etotal;
     Buf ev0, ev1, ef0, ef1;
     Buf mcode[2], mface[2], mcode2[2], mface2[2];
-    Buf child[2], nparent[2], lparent[2], visits[2], nodebox[2];
-    Buf pairs, phash, cand, planes, big_list, big_scratch;
+    Buf nodes[2], rbox[2];
+    Buf pairs, phash, cand, cand_list, planes, big_list, big_scratch;
     Buf te, tg, tqs, trs, ttype, tpip, esc;
     Buf rkey, rval, rkey2, rval2, rp, re, rg, rpts, rcount;
     Buf shkeys, shcnt;
@@ -84,10 +92,12 @@ struct mcgpu_context_t {
     double gp_constant = 1e-4;
     int keep_tests = 0;
     bool mface_alt[2] = {false, false};
+    int depth[2] = {0, 0};
     bool reg_alt = false;
     Counters C;
     Params P;
     float stage_ms[ST_COUNT];
+    int stage_mode = 0; // 0: full dispatch (two streams), 1: perturbed narrow-phase rerun
     float total_ms = 0.f;
 };
 
@@ -226,7 +236,7 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
         CK(ensure(ctx->off[m], nf + 1, &u));
         CK(ensure(ctx->hkeys[m], hs, &ull));
         CK(ensure(ctx->hfirst[m], hs, &ull));
-        CK(ensure(ctx->hcount[m], hs, &u));
+        CK(ensure(ctx->cface[m], nk, &u));
         CK(ensure(ctx->eflag[m], nk, &u));
         CK(ensure(ctx->eid[m], nk, &u));
         CK(ensure(ctx->cedge[m], nk, &u));
@@ -234,15 +244,15 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
         CK(ensure(ctx->mface[m], nf, &u));
         CK(ensure(ctx->mcode2[m], nf, &u));
         CK(ensure(ctx->mface2[m], nf, &u));
-        CK(ensure(ctx->child[m], 2 * nf, &u));
-        CK(ensure(ctx->nparent[m], nf, &i32));
-        CK(ensure(ctx->lparent[m], nf, &i32));
-        CK(ensure(ctx->visits[m], nf, &u));
-        CK(ensure(ctx->nodebox[m], 6 * nf, &dp));
+        int D = 0;
+        while ((1ull << D) < nf) ++D;
+        ctx->depth[m] = D;
+        CK(ensure(ctx->nodes[m], 6 * ((2ull << D) - 1), &dp));
+        CK(ensure(ctx->rbox[m], 6, &ull));
         scan_need = std::max<uint64_t>(scan_need, scan_tmp_elems(nf + 8));
     }
-    uint8_t* b8;
-    CK(ensure(ctx->cand, F + 8, &b8));
+    CK(ensure(ctx->cand, F + 8, &u));
+    CK(ensure(ctx->cand_list, F + 8, &u));
     Plane* pl;
     CK(ensure(ctx->planes, F, &pl));
     (void)p;
@@ -298,20 +308,20 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S)
     hipStream_t s = ctx->s;
     const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
     Counters* C = P_<Counters>(ctx->counters);
-    (void)hipEventRecord(ctx->ev[ST_PLANES], s);
-    fill_u32(ctx->cand.p, 0u, (F + 8) / 4 + 1, s);
+    fill_u32(ctx->cand.p, 0u, F, s);
     k_mark_candidates<<<grid_for(ctx->cap_pairs), kBlock, 0, s>>>(P_<unsigned long long>(ctx->pairs), C, ctx->cap_pairs,
-                                                                   ctx->nf[0], P_<uint8_t>(ctx->cand));
-    k_planes<<<grid_for(F), kBlock, 0, s>>>(S, P_<uint8_t>(ctx->cand), P_<Plane>(ctx->planes), P_<uint32_t>(ctx->big_list),
-                                             ctx->cap_big, C);
+                                                                   ctx->nf[0], P_<uint32_t>(ctx->cand),
+                                                                   P_<uint32_t>(ctx->cand_list));
+    k_planes<<<grid_for(F), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->cand_list), P_<Plane>(ctx->planes),
+                                             P_<uint32_t>(ctx->big_list), ctx->cap_big, C);
     k_big_polygons<<<4, 64, 0, s>>>(S, P_<uint32_t>(ctx->big_list), C, ctx->cap_big, P_<Plane>(ctx->planes),
                                      P_<Trip>(ctx->big_scratch), C);
-    (void)hipEventRecord(ctx->ev[ST_TESTS], s);
+    (void)hipEventRecord(ctx->ev[EV_A_PLANES], s);
     const uint64_t pmask = next_pow2(2 * ctx->cap_pairs) - 1;
-    k_tests<<<grid_for(ctx->cap_pairs), kBlock, 0, s>>>(S, P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
+    k_tests<<<grid_for(6 * ctx->cap_pairs), kBlock, 0, s>>>(S, P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
                                                         P_<unsigned long long>(ctx->phash), pmask, P_<uint32_t>(ctx->te),
                                                         P_<uint32_t>(ctx->tg), ctx->cap_tests, C);
-    (void)hipEventRecord(ctx->ev[ST_NARROW], s);
+    (void)hipEventRecord(ctx->ev[EV_A_TESTS], s);
     TestOut TO{nullptr, nullptr, nullptr, nullptr};
     if (ctx->keep_tests)
         TO = TestOut{P_<int8_t>(ctx->tqs), P_<int8_t>(ctx->trs), P_<char>(ctx->ttype), P_<char>(ctx->tpip)};
@@ -322,7 +332,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S)
     k_narrow_exact<<<grid_for(ctx->cap_esc, 64, 2048), 64, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
                                                                    P_<uint32_t>(ctx->tg), P_<uint32_t>(ctx->esc),
                                                                    ctx->cap_esc, TO, R, C);
-    (void)hipEventRecord(ctx->ev[ST_REGISTRY], s);
+    (void)hipEventRecord(ctx->ev[EV_A_NARROW], s);
     unsigned long long* rcount = P_<unsigned long long>(ctx->rcount);
     k_clamp_count<<<1, 1, 0, s>>>(&C->n_reg, ctx->cap_reg, rcount);
     k_iota<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(P_<uint32_t>(ctx->rval), rcount, ctx->cap_reg);
@@ -336,13 +346,13 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S)
                                                                  ctx->gbits, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg),
                                                                  P_<double>(ctx->rpts));
     const uint64_t hs = next_pow2(4 * ctx->cap_reg);
-    fill_u64(ctx->shkeys.p, kEmpty64, hs, s);
-    fill_u32(ctx->shcnt.p, 0u, hs, s);
-    k_status_pairs<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg), rcount,
-                                                             ctx->cap_reg, P_<unsigned long long>(ctx->shkeys),
-                                                             P_<uint32_t>(ctx->shcnt), hs - 1, C);
-    k_status_check<<<grid_for(hs), kBlock, 0, s>>>(P_<unsigned long long>(ctx->shkeys), P_<uint32_t>(ctx->shcnt), hs, C);
-    (void)hipEventRecord(ctx->ev[ST_COUNT], s);
+    k_status_clear<<<grid_for(hs), kBlock, 0, s>>>(rcount, P_<unsigned long long>(ctx->shkeys), P_<uint32_t>(ctx->shcnt), hs);
+    k_status_pairs<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg), rcount, hs,
+                                                             P_<unsigned long long>(ctx->shkeys), P_<uint32_t>(ctx->shcnt),
+                                                             C);
+    k_status_check<<<grid_for(hs), kBlock, 0, s>>>(rcount, P_<unsigned long long>(ctx->shkeys), P_<uint32_t>(ctx->shcnt),
+                                                   hs, C);
+    (void)hipEventRecord(ctx->ev[EV_A_END], s);
 }
 
 // face offsets and input validation; the caller syncs and rejects bad input
@@ -367,101 +377,115 @@ void enqueue_validate(mcgpu_context ctx)
 
 void enqueue_frontend(mcgpu_context ctx)
 {
-    hipStream_t s = ctx->s;
-    const uint32_t Fs = ctx->nf[0], Fc = ctx->nf[1];
-    const uint64_t F = (uint64_t)Fs + Fc, V = (uint64_t)ctx->nv[0] + ctx->nv[1];
+    hipStream_t s = ctx->s, s2 = ctx->s2;
+    const uint32_t Fs = ctx->nf[0];
+    const uint64_t F = (uint64_t)Fs + ctx->nf[1], V = (uint64_t)ctx->nv[0] + ctx->nv[1];
     Counters* C = P_<Counters>(ctx->counters);
     Params* Pd = P_<Params>(ctx->params);
 
-    (void)hipEventRecord(ctx->ev[ST_PARAMS], s);
+    (void)hipEventRecord(ctx->ev[EV_A0], s);
     fill_u32(C, 0u, sizeof(Counters) / 4, s);
-    k_mesh_sums<<<2, 64, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd);
-    const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
-    k_shift_params<<<1, 1, 0, s>>>(Pd, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
+    (void)hipEventRecord(ctx->ev[EV_FORK], s);
+    (void)hipStreamWaitEvent(s2, ctx->ev[EV_FORK], 0);
 
-    // topology: polygon-soup edge numbering
-    (void)hipEventRecord(ctx->ev[ST_TOPOLOGY], s);
+    // ---- stream B: shift-independent topology and leaf order ----
+    (void)hipEventRecord(ctx->ev[EV_B0], s2);
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
     const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
-    fill_u32(ctx->ef1.p, 0xffffffffu, K, s);
+    fill_u32(ctx->ef1.p, 0xffffffffu, K, s2);
+    CornerMap cmap[2];
+    for (int m = 0; m < 2; ++m) {
+        cmap[m].off = P_<uint32_t>(ctx->off[m]);
+        cmap[m].cface = ctx->dsz[m] ? P_<uint32_t>(ctx->cface[m]) : nullptr;
+        if (ctx->dsz[m])
+            k_corner_faces<<<grid_for(ctx->nf[m]), kBlock, 0, s2>>>(cmap[m].off, ctx->nf[m], P_<uint32_t>(ctx->cface[m]));
+    }
     for (int m = 0; m < 2; ++m) {
         const uint64_t nk = ctx->ncorner[m];
         const uint64_t hs = next_pow2(2 * nk + 2);
-        fill_u64(ctx->hkeys[m].p, kEmpty64, hs, s);
-        fill_u64(ctx->hfirst[m].p, kEmpty64, hs, s);
-        fill_u32(ctx->hcount[m].p, 0u, hs, s);
-        const uint32_t* off = P_<uint32_t>(ctx->off[m]);
-        k_edge_insert<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(ctx->didx[m], off, ctx->nf[m], m,
-                                                               P_<unsigned long long>(ctx->hkeys[m]),
-                                                               P_<unsigned long long>(ctx->hfirst[m]),
-                                                               P_<uint32_t>(ctx->hcount[m]), hs - 1);
-        k_edge_flag<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(ctx->didx[m], off, ctx->nf[m], m,
-                                                             P_<unsigned long long>(ctx->hkeys[m]),
-                                                             P_<unsigned long long>(ctx->hfirst[m]),
-                                                             P_<uint32_t>(ctx->hcount[m]), hs - 1,
-                                                             P_<uint32_t>(ctx->eflag[m]), C);
+        fill_u64(ctx->hkeys[m].p, kEmpty64, hs, s2);
+        fill_u64(ctx->hfirst[m].p, kEmpty64, hs, s2);
+        k_edge_insert<<<grid_for(nk), kBlock, 0, s2>>>(ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]),
+                                                       P_<unsigned long long>(ctx->hfirst[m]), hs - 1);
+        k_edge_flag<<<grid_for(nk), kBlock, 0, s2>>>(ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]),
+                                                     P_<unsigned long long>(ctx->hfirst[m]), hs - 1,
+                                                     P_<uint32_t>(ctx->eflag[m]), C);
         scan_exclusive(P_<uint32_t>(ctx->eflag[m]), P_<uint32_t>(ctx->eid[m]), nk, P_<uint32_t>(ctx->scan_tmp),
-                       etotal + m, s);
-        k_store_edge_count<<<1, 1, 0, s>>>(etotal + m, C, m);
+                       etotal + m, s2);
+        k_store_edge_count<<<1, 1, 0, s2>>>(etotal + m, C, m);
     }
     for (int m = 0; m < 2; ++m) {
-        const uint64_t hs = next_pow2(2 * ctx->ncorner[m] + 2);
-        k_edge_assign<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(
-            ctx->didx[m], P_<uint32_t>(ctx->off[m]), ctx->nf[m], m, P_<unsigned long long>(ctx->hkeys[m]),
-            P_<unsigned long long>(ctx->hfirst[m]), hs - 1, P_<uint32_t>(ctx->eid[m]), C, m ? ctx->nv[0] : 0,
-            m ? ctx->nf[0] : 0, P_<uint32_t>(ctx->cedge[m]), P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1),
-            P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
+        const uint64_t nk = ctx->ncorner[m];
+        const uint64_t hs = next_pow2(2 * nk + 2);
+        k_edge_assign<<<grid_for(nk), kBlock, 0, s2>>>(
+            ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]), P_<unsigned long long>(ctx->hfirst[m]),
+            hs - 1, P_<uint32_t>(ctx->eid[m]), C, m ? ctx->nv[0] : 0, m ? ctx->nf[0] : 0, P_<uint32_t>(ctx->cedge[m]),
+            P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
     }
-
-    // shifted coordinates and face boxes
-    (void)hipEventRecord(ctx->ev[ST_SHIFT], s);
-    k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
-                                                        P_<double>(ctx->ps_v), 0);
-    const Soup S = make_soup(ctx);
-    k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, Pd, P_<double>(ctx->box), 0);
-
-    // LBVH per mesh
-    (void)hipEventRecord(ctx->ev[ST_BVH], s);
-    Tree T[2];
+    (void)hipEventRecord(ctx->ev[EV_B_TOPO], s2);
     RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
     for (int m = 0; m < 2; ++m) {
         const uint32_t nf = ctx->nf[m];
+        unsigned long long* rb = P_<unsigned long long>(ctx->rbox[m]);
+        fill_u64(rb, ~0ull, 3, s2);
+        fill_u64(rb + 3, 0ull, 3, s2);
+        k_raw_bbox<<<grid_for(ctx->nv[m], kBlock, 64), kBlock, 0, s2>>>(ctx->dv[m], ctx->nv[m], rb);
+        k_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf, rb,
+                                                      P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]));
+        ctx->mface_alt[m] = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]),
+                                                 P_<uint32_t>(ctx->mcode2[m]), P_<uint32_t>(ctx->mface2[m]), nullptr,
+                                                 nf, nf, 30, rt, s2);
+    }
+    (void)hipEventRecord(ctx->ev[EV_B_SORT], s2);
+
+    // ---- stream A: the shift chain ----
+    k_mesh_sums<<<6, 64, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd);
+    const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
+    k_shift_params<<<1, 1, 0, s>>>(Pd, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
+    (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
+    k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
+                                                        P_<double>(ctx->ps_v), 0);
+    (void)hipStreamWaitEvent(s, ctx->ev[EV_B_SORT], 0);
+    const Soup S = make_soup(ctx);
+    k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, Pd, P_<double>(ctx->box), 0);
+    (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
+
+    Tree T[2];
+    for (int m = 0; m < 2; ++m) {
+        const uint32_t nf = ctx->nf[m];
+        const int D = ctx->depth[m];
         const double* fbox = P_<double>(ctx->box) + 6 * (uint64_t)(m ? Fs : 0);
-        k_morton<<<grid_for(nf), kBlock, 0, s>>>(fbox, Pd, m, nf, P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]));
-        const bool alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]),
-                                              P_<uint32_t>(ctx->mcode2[m]), P_<uint32_t>(ctx->mface2[m]), nullptr, nf, nf,
-                                              30, rt, s);
-        ctx->mface_alt[m] = alt;
-        const uint32_t* code = P_<uint32_t>(alt ? ctx->mcode2[m] : ctx->mcode[m]);
-        const uint32_t* sface = P_<uint32_t>(alt ? ctx->mface2[m] : ctx->mface[m]);
-        fill_u32(ctx->lparent[m].p, 0xffffffffu, nf, s);
-        fill_u32(ctx->visits[m].p, 0u, nf, s);
-        k_lbvh_nodes<<<grid_for(nf > 1 ? nf - 1 : 1), kBlock, 0, s>>>(code, (int)nf, P_<uint32_t>(ctx->child[m]),
-                                                                      P_<int32_t>(ctx->nparent[m]),
-                                                                      P_<int32_t>(ctx->lparent[m]));
-        k_lbvh_refit<<<grid_for(nf), kBlock, 0, s>>>((int)nf, sface, fbox, P_<uint32_t>(ctx->child[m]),
-                                                     P_<int32_t>(ctx->nparent[m]), P_<int32_t>(ctx->lparent[m]),
-                                                     P_<uint32_t>(ctx->visits[m]), P_<double>(ctx->nodebox[m]));
-        T[m].child = P_<uint32_t>(ctx->child[m]);
+        const uint32_t* sface = P_<uint32_t>(ctx->mface_alt[m] ? ctx->mface2[m] : ctx->mface[m]);
+        double* nodes = P_<double>(ctx->nodes[m]);
+        int lin = D;
+        do {
+            const int width = 1 << lin;
+            const int per = width < kRefitWidth ? width : kRefitWidth;
+            k_refit_levels<<<width / per, 256, 0, s>>>(fbox, sface, (int)nf, D, lin, nodes);
+            lin -= (lin < kRefitLevels ? lin : kRefitLevels);
+        } while (lin > 0);
         T[m].sface = sface;
-        T[m].nodebox = P_<double>(ctx->nodebox[m]);
+        T[m].nodes = nodes;
         T[m].facebox = fbox;
         T[m].n = (int)nf;
+        T[m].D = D;
     }
+    (void)hipEventRecord(ctx->ev[EV_A_REFIT], s);
 
-    // broad phase
-    (void)hipEventRecord(ctx->ev[ST_TRAVERSE], s);
     const uint64_t ps = next_pow2(2 * ctx->cap_pairs);
     fill_u64(ctx->phash.p, kEmpty64, ps, s);
-    k_traverse<<<grid_for(Fs), kBlock, 0, s>>>(T[0], T[1], P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
-                                               P_<unsigned long long>(ctx->phash), ps - 1, C);
-    (void)Fc;
+    const unsigned nwaves = (Fs + 63) / 64;
+    k_traverse_wave<<<nwaves < 65536 ? nwaves : 65536, 64, 0, s>>>(T[0], T[1], P_<unsigned long long>(ctx->pairs),
+                                                                   ctx->cap_pairs, P_<unsigned long long>(ctx->phash),
+                                                                   ps - 1, C);
+    (void)hipEventRecord(ctx->ev[EV_A_TRAV], s);
     enqueue_narrow(ctx, S);
 }
 
 mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow)
 {
     CK(hipStreamSynchronize(ctx->s));
+    CK(hipStreamSynchronize(ctx->s2));
     CK(hipGetLastError());
     CK(hipMemcpy(&ctx->C, ctx->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
     CK(hipMemcpy(&ctx->P, ctx->params.p, sizeof(Params), hipMemcpyDeviceToHost));
@@ -470,11 +494,13 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
     if (getenv("MCGPU_DEBUG"))
         fprintf(stderr,
                 "[mcgpu] pairs=%llu tests=%llu esc=%llu reg=%llu big=%llu cand=%llu edges=%llu+%llu ov=%u/%u/%u/%u/%u "
-                "gp=%u bad=%u/%u unsup=%u nonman=%u badin=%u pairfail=%u partial=%u caps=%llu/%llu/%llu/%llu\n",
+                "gp=%u bad=%u/%u unsup=%u nonman=%u badin=%u pairfail=%u partial=%u caps=%llu/%llu/%llu/%llu "
+                "trav levels=%llu fallback_waves=%llu leaf_tests=%llu\n",
                 C.n_pairs, C.n_tests, C.n_esc, C.n_reg, C.n_big, C.n_cand, C.n_edges_src, C.n_edges_cut, C.ov_pairs,
                 C.ov_tests, C.ov_esc, C.ov_reg, C.ov_big, C.gp, C.bad_src, C.bad_cut, C.unsupported, C.nonmanifold,
                 C.bad_input, C.pair_fail, C.partial, (unsigned long long)ctx->cap_pairs,
-                (unsigned long long)ctx->cap_tests, (unsigned long long)ctx->cap_esc, (unsigned long long)ctx->cap_reg);
+                (unsigned long long)ctx->cap_tests, (unsigned long long)ctx->cap_esc, (unsigned long long)ctx->cap_reg,
+                C.n_visits, C.n_fallback, C.n_leaf_tests);
     if (C.ov_pairs & 2u) return MCGPU_ERROR_UNSUPPORTED; // tree deeper than the traversal stack
     if (C.ov_pairs || C.ov_tests || C.ov_esc || C.ov_reg || C.ov_big) {
         *overflow = true;
@@ -487,14 +513,27 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
     }
     if (C.bad_input || C.nonmanifold) return MCGPU_ERROR_INVALID_VALUE;
     if (C.unsupported) return MCGPU_ERROR_UNSUPPORTED;
-    for (int i = 0; i < ST_COUNT; ++i) {
+    auto el = [&](int a, int b) {
         float ms = 0.f;
-        hipEvent_t a = ctx->ev[i], b = ctx->ev[i + 1];
-        if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 0.f;
-        ctx->stage_ms[i] = ms;
-    }
-    float tot = 0.f;
-    if (hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[ST_COUNT]) != hipSuccess) tot = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) ms = 0.f;
+        return ms;
+    };
+    const bool twostream = ctx->stage_mode == 0;
+    ctx->stage_ms[ST_PARAMS] = el(EV_A0, EV_A_PARAMS);
+    ctx->stage_ms[ST_TOPOLOGY] = twostream ? el(EV_B0, EV_B_TOPO) : 0.f;
+    ctx->stage_ms[ST_BVH_SORT] = twostream ? el(EV_B_TOPO, EV_B_SORT) : 0.f;
+    // time stream A spent waiting for stream B is not part of the shift stage
+    float wait = twostream ? el(EV_A_PARAMS, EV_B_SORT) : 0.f;
+    if (wait < 0.f) wait = 0.f;
+    float sh = el(EV_A_PARAMS, EV_A_SHIFT) - wait;
+    ctx->stage_ms[ST_SHIFT] = sh > 0.f ? sh : 0.f;
+    ctx->stage_ms[ST_REFIT] = el(EV_A_SHIFT, EV_A_REFIT);
+    ctx->stage_ms[ST_TRAVERSE] = el(EV_A_REFIT, EV_A_TRAV);
+    ctx->stage_ms[ST_PLANES] = el(EV_A_TRAV, EV_A_PLANES);
+    ctx->stage_ms[ST_TESTS] = el(EV_A_PLANES, EV_A_TESTS);
+    ctx->stage_ms[ST_NARROW] = el(EV_A_TESTS, EV_A_NARROW);
+    ctx->stage_ms[ST_REGISTRY] = el(EV_A_NARROW, EV_A_END);
+    float tot = el(EV_A0, EV_A_END);
     ctx->total_ms = tot;
     ctx->have = true;
     if (info) {
@@ -592,11 +631,12 @@ MCGPU_API mcgpu_result mcgpu_create_context(int device, mcgpu_context* out)
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MCGPU_ERROR_DEVICE;
     mcgpu_context ctx = new mcgpu_context_t();
     ctx->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->s2, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return MCGPU_ERROR_DEVICE;
     }
-    for (int i = 0; i <= ST_COUNT; ++i) (void)hipEventCreate(&ctx->ev[i]);
+    for (int i = 0; i < EV_COUNT; ++i) (void)hipEventCreate(&ctx->ev[i]);
     *out = ctx;
     return MCGPU_SUCCESS;
 }
@@ -606,20 +646,21 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->s);
+    (void)hipStreamSynchronize(ctx->s2);
     Buf* all[] = {&ctx->in_v[0], &ctx->in_v[1], &ctx->in_idx[0], &ctx->in_idx[1], &ctx->in_sz[0], &ctx->in_sz[1],
                   &ctx->off[0], &ctx->off[1], &ctx->params, &ctx->counters, &ctx->ps_v, &ctx->box, &ctx->hkeys[0],
-                  &ctx->hkeys[1], &ctx->hfirst[0], &ctx->hfirst[1], &ctx->hcount[0], &ctx->hcount[1], &ctx->eflag[0],
+                  &ctx->hkeys[1], &ctx->hfirst[0], &ctx->hfirst[1], &ctx->cface[0], &ctx->cface[1], &ctx->eflag[0],
                   &ctx->eflag[1], &ctx->eid[0], &ctx->eid[1], &ctx->cedge[0], &ctx->cedge[1], &ctx->etotal, &ctx->ev0,
                   &ctx->ev1, &ctx->ef0, &ctx->ef1, &ctx->mcode[0], &ctx->mcode[1], &ctx->mface[0], &ctx->mface[1],
-                  &ctx->mcode2[0], &ctx->mcode2[1], &ctx->mface2[0], &ctx->mface2[1], &ctx->child[0], &ctx->child[1],
-                  &ctx->nparent[0], &ctx->nparent[1], &ctx->lparent[0], &ctx->lparent[1], &ctx->visits[0],
-                  &ctx->visits[1], &ctx->nodebox[0], &ctx->nodebox[1], &ctx->pairs, &ctx->phash, &ctx->cand,
+                  &ctx->mcode2[0], &ctx->mcode2[1], &ctx->mface2[0], &ctx->mface2[1], &ctx->nodes[0], &ctx->nodes[1],
+                  &ctx->rbox[0], &ctx->rbox[1], &ctx->pairs, &ctx->phash, &ctx->cand,
                   &ctx->planes, &ctx->big_list, &ctx->big_scratch, &ctx->te, &ctx->tg, &ctx->tqs, &ctx->trs,
                   &ctx->ttype, &ctx->tpip, &ctx->esc, &ctx->rkey, &ctx->rval, &ctx->rkey2, &ctx->rval2, &ctx->rp,
                   &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->hist};
     for (Buf* b : all)
         if (b->p) (void)hipFree(b->p);
-    for (int i = 0; i <= ST_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
+    for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->s2);
     (void)hipStreamDestroy(ctx->s);
     delete ctx;
     return MCGPU_SUCCESS;
@@ -661,6 +702,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     }
     for (int attempt = 0; attempt < 6; ++attempt) {
         if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
+        ctx->stage_mode = 0;
         enqueue_frontend(ctx);
         bool overflow = false;
         if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
@@ -682,12 +724,16 @@ MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const doubl
         hipStream_t s = ctx->s;
         const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1];
         Counters* C = P_<Counters>(ctx->counters);
-        (void)hipEventRecord(ctx->ev[ST_PARAMS], s);
+        ctx->stage_mode = 1;
+        (void)hipEventRecord(ctx->ev[EV_A0], s);
         k_reset_narrow<<<1, 1, 0, s>>>(C);
         k_set_perturbation<<<1, 1, 0, s>>>(P_<Params>(ctx->params), 1, perturbation[0], perturbation[1], perturbation[2]);
-        for (int i = ST_TOPOLOGY; i <= ST_TRAVERSE; ++i) (void)hipEventRecord(ctx->ev[i], s);
+        (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
         k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1],
                                                             P_<Params>(ctx->params), P_<double>(ctx->ps_v), 1);
+        (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
+        (void)hipEventRecord(ctx->ev[EV_A_REFIT], s);
+        (void)hipEventRecord(ctx->ev[EV_A_TRAV], s);
         enqueue_narrow(ctx, make_soup(ctx));
         bool overflow = false;
         if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
@@ -791,9 +837,9 @@ MCGPU_API mcgpu_result mcgpu_get_face_planes(mcgpu_context ctx, uint32_t* faces,
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
     if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
     const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
-    std::vector<uint8_t> cand(F);
+    std::vector<uint32_t> cand(F);
     std::vector<Plane> pl(F);
-    CK(hipMemcpy(cand.data(), ctx->cand.p, F, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(cand.data(), ctx->cand.p, 4 * F, hipMemcpyDeviceToHost));
     CK(hipMemcpy(pl.data(), ctx->planes.p, F * sizeof(Plane), hipMemcpyDeviceToHost));
     uint64_t k = 0;
     for (uint64_t f = 0; f < F && k < capacity; ++f) {

@@ -100,6 +100,52 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const uint32_t* __res
     }
 }
 
+// One-block exclusive scan for radix histograms.  The histogram holds
+// kRadix x (tiles in use) entries, where the tile count follows the element
+// count (device or host); 1024 threads walk it in coalesced chunks of 4096
+// with a carried prefix.  In place.
+constexpr int kScan1Block = 1024;
+constexpr int kScan1Items = 4;
+
+__global__ void __launch_bounds__(kScan1Block) k_scan_hist(uint32_t* __restrict__ data, const uint64_t* n_dev,
+                                                           uint64_t n_host, uint32_t tile)
+{
+    __shared__ uint32_t wsum[kScan1Block / 64];
+    const uint64_t n = n_dev ? *n_dev : n_host;
+    const uint64_t entries = (uint64_t)kRadix * ((n + tile - 1) / tile);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (uint64_t base = 0; base < entries; base += (uint64_t)kScan1Block * kScan1Items) {
+        const uint64_t b = base + (uint64_t)threadIdx.x * kScan1Items;
+        uint32_t v[kScan1Items];
+        uint32_t tot = 0;
+        for (int q = 0; q < kScan1Items; ++q) {
+            v[q] = (b + q < entries) ? data[b + q] : 0u;
+            tot += v[q];
+        }
+        uint32_t x = tot;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        uint32_t off = 0, all = 0;
+        for (int w = 0; w < kScan1Block / 64; ++w) {
+            const uint32_t t = wsum[w];
+            if (w < wid) off += t;
+            all += t;
+        }
+        uint32_t run = carry + off + x - tot;
+        for (int q = 0; q < kScan1Items; ++q) {
+            if (b + q < entries) data[b + q] = run;
+            run += v[q];
+        }
+        carry += all;
+        __syncthreads();
+    }
+}
+
 // Host launcher: exclusive scan of n (host-known) u32 into out (may alias in);
 // tmp needs ceil(n / kScanTile) entries; total (device) optional.
 inline void scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, uint32_t* total,
@@ -132,7 +178,9 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_hist(const K* __restrict__
 {
     __shared__ uint32_t h[kRadix];
     const uint64_t n = read_count(n_dev, n_host);
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t nt = (uint32_t)((n + kSortTile - 1) / kSortTile); // tiles in use (<= ntiles)
+    (void)ntiles;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         h[threadIdx.x] = 0;
         __syncthreads();
         const uint64_t base = (uint64_t)t * kSortTile;
@@ -143,7 +191,7 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_hist(const K* __restrict__
             }
         }
         __syncthreads();
-        hist[(uint64_t)threadIdx.x * ntiles + t] = h[threadIdx.x];
+        hist[(uint64_t)threadIdx.x * nt + t] = h[threadIdx.x];
         __syncthreads();
     }
 }
@@ -159,13 +207,15 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restric
     __shared__ uint32_t running[kRadix];
     __shared__ uint32_t gbase[kRadix];
     const uint64_t n = read_count(n_dev, n_host);
+    const uint32_t nt = (uint32_t)((n + kSortTile - 1) / kSortTile);
+    (void)ntiles;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint64_t base = (uint64_t)t * kSortTile;
         if (base >= n) break; // tiles are visited in increasing order
         running[threadIdx.x] = 0;
-        gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * ntiles + t];
+        gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * nt + t];
         for (int w = 0; w < kSortBlock / 64; ++w) wcount[w][threadIdx.x] = 0;
         __syncthreads();
         for (int j = 0; j < kSortItems; ++j) {
@@ -230,7 +280,11 @@ inline bool radix_sort(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
         uint32_t* vin = alt ? vals_alt : vals;
         uint32_t* vout = alt ? vals : vals_alt;
         k_radix_hist<K><<<grid, kSortBlock, 0, s>>>(kin, n_dev, n_host, shift, tmp.hist, ntiles);
-        scan_exclusive(tmp.hist, tmp.hist, (uint64_t)kRadix * ntiles, tmp.scan_tmp, nullptr, s);
+        const uint64_t nh_host = (uint64_t)kRadix * ((n_host + kSortTile - 1) / kSortTile);
+        if (!n_dev && nh_host > 16384)
+            scan_exclusive(tmp.hist, tmp.hist, nh_host, tmp.scan_tmp, nullptr, s);
+        else
+            k_scan_hist<<<1, kScan1Block, 0, s>>>(tmp.hist, n_dev, n_host, kSortTile);
         if (vals)
             k_radix_scatter<K, true><<<grid, kSortBlock, 0, s>>>(kin, vin, kout, vout, n_dev, n_host, shift, tmp.hist, ntiles);
         else

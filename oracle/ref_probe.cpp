@@ -304,6 +304,41 @@ double ref_orient3d(const double* a, const double* b, const double* c, const dou
 
 double ref_orient2d(const double* a, const double* b, const double* c) { return ::orient2d(a, b, c); }
 
+// Polygon soup as dispatch() assembles it (kernel.cpp:1593-1732): a copy of
+// the source hmesh, then the cut mesh's vertices and, face by face, the cut
+// hmesh's vertex lists offset by the source vertex count.  Writes per edge
+// the endpoints of halfedge 0 and the faces of halfedges 0 and 1 (-1 = none);
+// returns the edge count (-1 if buffers are too small).
+static std::vector<int32_t> g_ps_edges;
+
+int64_t ref_ps_edges(const double* sv, const uint32_t* sf, const uint32_t* ssz, uint32_t nsv, uint32_t nsf,
+                     const double* cv, const uint32_t* cf, const uint32_t* csz, uint32_t ncv, uint32_t ncf)
+{
+    hmesh_t sm, cs;
+    const vec3_<double> zero(0.0);
+    build_hmesh(sm, sv, sf, ssz, nsv, nsf, zero, zero);
+    build_hmesh(cs, cv, cf, csz, ncv, ncf, zero, zero);
+    hmesh_t ps = sm;
+    for (auto i = cs.vertices_begin(); i != cs.vertices_end(); ++i) ps.add_vertex(cs.vertex(*i));
+    std::vector<vd_t> fv;
+    for (auto i = cs.faces_begin(); i != cs.faces_end(); ++i) {
+        cs.get_vertices_around_face(fv, *i, nsv);
+        ps.add_face(fv);
+    }
+    g_ps_edges.clear();
+    for (auto e = ps.edges_begin(); e != ps.edges_end(); ++e) {
+        const hd_t h0 = ps.halfedge(*e, 0), h1 = ps.halfedge(*e, 1);
+        const fd_t f0 = ps.face(h0), f1 = ps.face(h1);
+        g_ps_edges.push_back((int32_t)ps.source(h0));
+        g_ps_edges.push_back((int32_t)ps.target(h0));
+        g_ps_edges.push_back(f0 == hmesh_t::null_face() ? -1 : (int32_t)f0);
+        g_ps_edges.push_back(f1 == hmesh_t::null_face() ? -1 : (int32_t)f1);
+    }
+    return (int64_t)(g_ps_edges.size() / 4);
+}
+
+void ref_ps_edges_copy(int32_t* out) { std::memcpy(out, g_ps_edges.data(), g_ps_edges.size() * sizeof(int32_t)); }
+
 // determine_three_noncollinear_vertices, math.cpp:289-389
 int ref_best_triple(const double* verts, int n, const double* normal, int k, int* ijk)
 {

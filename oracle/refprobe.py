@@ -62,6 +62,9 @@ def lib():
         L.ref_orient2d.restype = C.c_double
         L.ref_best_triple.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.ref_best_triple.restype = C.c_int
+        L.ref_ps_edges.argtypes = mesh + mesh
+        L.ref_ps_edges.restype = C.c_int64
+        L.ref_ps_edges_copy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -185,3 +188,14 @@ def best_triple(verts, normal, k):
     ijk = np.zeros(3, np.int32)
     ok = lib().ref_best_triple(_ptr(v), v.shape[0], _ptr(n), k, _ptr(ijk))
     return ok, tuple(ijk.tolist())
+
+
+def ps_edges(src, cut):
+    """Polygon-soup edge table built with the reference's hmesh_t: (v0, v1, f0, f1) per edge."""
+    L = lib()
+    keep_s, sa = _mesh_args(src)
+    keep_c, ca = _mesh_args(cut)
+    n = L.ref_ps_edges(*sa, *ca)
+    out = np.zeros((n, 4), np.int32)
+    L.ref_ps_edges_copy(_ptr(out))
+    return out

@@ -14,6 +14,8 @@ Fixtures:
   ref_cases.npz      end-to-end cases through mcDispatch: status, shift
                      parameters, candidate face pairs, intersection points
                      (seam vertices), perturbation vector, CC count
+  ref_ps_edges.npz   polygon-soup edge tables (halfedge-0 endpoints and the
+                     faces of both halfedges) built with the reference's hmesh_t
 """
 from __future__ import annotations
 
@@ -186,6 +188,11 @@ def main():
         s, c = meshes.icosphere_pair(lvl, amp=amp)
         cases.update(case(f"ico{lvl}_{int(amp * 1000):03d}", s, c))
     np.savez_compressed(os.path.join(HERE, "ref_cases.npz"), **cases)
+    edges = {}
+    for name, (s, c) in [("config1_rot", meshes.config1_rotated()), ("ico3_020", meshes.icosphere_pair(3))] + \
+            [(f"bench{i:03d}", meshes.bench_pair(i)) for i in range(64)]:
+        edges[name] = rp.ps_edges(s, c)
+    np.savez_compressed(os.path.join(HERE, "ref_ps_edges.npz"), **edges)
 
 
 if __name__ == "__main__":

@@ -245,3 +245,15 @@ def test_registry_points_against_exact_rationals(oracle_lib):
             err = abs(Fraction(pi) - xi) / max(abs(xi), Fraction(1))
             worst = max(worst, float(err))
     assert worst <= 1e-12, worst
+
+
+def test_polygon_soup_edge_numbering(oracle_lib):
+    """Edge ids, halfedge-0 direction and both incident faces equal the
+    reference hmesh_t's for config 1, an icosphere pair and all 64 benchmark
+    pairs (triangles, quads and larger polygons)."""
+    E = np.load(os.path.join(GOLD, "ref_ps_edges.npz"))
+    for name in E.files:
+        src, cut = case_meshes(name)
+        R = om.frontend(src, cut)
+        mine = np.stack([R["edge_v0"], R["edge_v1"], R["edge_f0"], R["edge_f1"]], 1).astype(np.int32)
+        assert np.array_equal(mine, E[name]), name
