@@ -25,23 +25,45 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+# stage -> the kernel that carries it (stages of one kernel are timed exactly
+# by the events around that kernel on its stream)
+STAGE_KERNEL = {
+    "coordinate_sums": "k_mesh_sums",
+    "traversal": "k_traverse_wave",
+    "tests": "k_tests",
+    "pair_filter": "k_filter_pairs",
+    "face_planes": "k_planes + k_big_polygons",
+    "narrow": "k_narrow + k_narrow_exact",
+}
+
+
 def stage_bytes(name, n):
-    """Algorithmic HBM bytes of one front-end stage (each datum moved once)."""
+    """Algorithmic HBM bytes of one front-end stage: every datum it must read
+    or write, moved once."""
     V, F, K, E = n["V"], n["F"], n["K"], n["E"]
-    C, T, I, FC, ESC = n["pairs"], n["tests"], n["registry"], n["cand"], n["escalated"]
+    C, CR, T, I, FC = n["pairs"], n["raw_pairs"], n["tests"], n["registry"], n["cand"]
+    NODES = n["nodes"]
     table = {
-        # sequential coordinate sums + bounding boxes: every coordinate read once
-        "shift_params": 24 * V,
-        # edge numbering: face lists read, corner->edge written, edge table written
-        "topology": 4 * K + 4 * K + 16 * E,
-        # shifted coordinates written, face boxes written (vertex reads counted once)
+        # every coordinate read once by the serial sums
+        "coordinate_sums": 24 * V,
+        "shift_params": 0,
+        # coordinates read and shifted coordinates written; face lists read,
+        # shifted face boxes written
         "shift_boxes": 24 * V + 24 * V + 4 * K + 48 * F,
-        # Morton keys, one sort pass moving (key, face), node links and node boxes
-        "bvh_build": 48 * F + 8 * F + 16 * F + 12 * F + 48 * F,
-        # query boxes read once, tree read once, pairs written
-        "traversal": 48 * F + 56 * F + 8 * C,
-        # candidate flags, candidate-face vertices (3 x 24 B) and planes (40 B)
-        "face_planes": F + FC * (72 + 40),
+        # face lists read, corner -> edge written, edge table written
+        "topology": 4 * K + 4 * K + 16 * E,
+        # face lists and coordinates read; Morton keys, face ids and raw boxes
+        # written; four 8-bit radix passes over (key, face)
+        "bvh_sort": 4 * K + 24 * V + 56 * F + 4 * 16 * F,
+        # leaf boxes and leaf order read, node boxes written
+        "bvh_refit": 52 * F + 48 * NODES,
+        # source leaf boxes and the cut tree read once, raw pairs written
+        "traversal": 48 * F + 24 * NODES + 8 * CR,
+        # raw pairs and both shifted boxes read; pairs, pair hash and
+        # candidate faces written
+        "pair_filter": CR * (8 + 96) + 16 * C + 4 * FC,
+        # candidate faces: face list and vertices read, plane written
+        "face_planes": FC * (72 + 40),
         # pairs read, tests written, per test: edge endpoints and face box
         "tests": 8 * C + 8 * T + T * (16 + 48 + 48),
         # per test: (edge, face), endpoint ids and coordinates, plane, face
@@ -112,10 +134,13 @@ def main():
 
     stages = {k: v / args.steps for k, v in stage_tot.items()}
     counts = dict(R.counts)
-    n = dict(V=src[0].shape[0] + cut[0].shape[0], F=(src[1].size + cut[1].size) // 3, K=src[1].size + cut[1].size,
-             E=counts["edges"], pairs=counts["pairs"], tests=counts["tests"], registry=I, cand=counts["cand"],
-             escalated=counts["escalated"])
-    dom = max(stages, key=stages.get)
+    Fs, Fc = src[1].size // 3, cut[1].size // 3
+    nodes = sum((2 << max(0, int(np.ceil(np.log2(f))))) - 1 for f in (Fs, Fc))
+    n = dict(V=src[0].shape[0] + cut[0].shape[0], F=Fs + Fc, K=src[1].size + cut[1].size, E=counts["edges"],
+             pairs=counts["pairs"], raw_pairs=counts["raw_pairs"], tests=counts["tests"], registry=I,
+             cand=counts["cand"], nodes=nodes)
+    # the roofline kernel: the longest stage that one kernel carries alone
+    dom = max((k for k in stages if k in STAGE_KERNEL), key=stages.get)
     dom_bytes = stage_bytes(dom, n)
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9 if stages[dom] > 0 else 0.0
     lat_sorted = sorted(lat)
@@ -140,8 +165,9 @@ def main():
         "latency_ms": {"p50": lat_sorted[len(lat) // 2], "p99": lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))]},
         "counts": counts,
         "stage_ms": stages,
-        "roofline": {"bound": "hbm", "kernel_stage": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "stage": dom, "kernel_ms": stages[dom],
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes": dom_bytes,
                      "frontend_alg_bytes": sum(stage_bytes(k, n) for k in stages)},
     }
     if not args.no_cpu_baseline:

@@ -22,6 +22,12 @@
  * Plain C types only.  Every function returns an mcgpu_result; no exception
  * crosses this boundary.  A context owns one HIP stream and its device
  * buffers; it must not be used from two threads at once.
+ *
+ * The front end is a superset of what MCUT computes in one place: its broad
+ * phase runs on the unshifted coordinates with a rounding bound, overlapped
+ * with the coordinate sums, and an exact test on the shifted boxes then leaves
+ * exactly MCUT's candidate pairs.  Inputs whose bound is not finite
+ * (coordinates near the double range) return MCGPU_ERROR_UNSUPPORTED.
  */
 #ifndef MCGPU_H_
 #define MCGPU_H_
@@ -47,7 +53,7 @@ typedef enum {
     MCGPU_ERROR_INVALID_VALUE = -1, /* bad argument: null pointer, size 0, face size < 3, index out of range */
     MCGPU_ERROR_DEVICE = -2,        /* HIP runtime error */
     MCGPU_ERROR_OUT_OF_MEMORY = -3,
-    MCGPU_ERROR_UNSUPPORTED = -4,   /* candidate polygon with more than 16 vertices */
+    MCGPU_ERROR_UNSUPPORTED = -4,   /* candidate polygon with more than 16 vertices; coordinates too large to shift */
     MCGPU_ERROR_NO_RESULT = -5      /* query before a successful dispatch */
 } mcgpu_result;
 
@@ -95,6 +101,7 @@ typedef struct {
     uint64_t num_tests;          /* edge-face tests after expansion, dedup and culling */
     uint64_t num_escalated;      /* tests whose orient3d signs needed exact arithmetic */
     uint64_t num_registry;       /* intersecting (edge, face) pairs */
+    uint64_t num_raw_pairs;      /* pairs of the raw-coordinate broad phase before the exact shifted-box test */
     double com[3];               /* shift: x' = (x - com) + translation */
     double translation[3];
     double eps;                  /* cut-face AABB enlargement */
@@ -160,7 +167,6 @@ MCGPU_API mcgpu_result mcgpu_registry_device(mcgpu_context ctx, const uint32_t**
  * mcgpu_stage_name); returns the number of stages written. */
 MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity);
 MCGPU_API const char* mcgpu_stage_name(int stage);
-/* Average device time of the narrow-phase kernel over the last dispatch. */
 
 #ifdef __cplusplus
 }

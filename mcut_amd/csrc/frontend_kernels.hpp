@@ -404,14 +404,22 @@ __global__ void __launch_bounds__(256) k_raw_bbox(const double* __restrict__ v, 
     }
 }
 
-// 30-bit Morton codes of raw face-box centres.  The candidate set does not
-// depend on the tree (every overlapping leaf pair is found whatever the leaf
-// order), so the tree is built from unshifted coordinates and runs while the
-// serial coordinate sums that fix the shift are still in flight.
+// 30-bit Morton codes of raw face-box centres, and the raw face boxes the
+// broad phase runs on.
+//
+// The broad phase runs in unshifted coordinates so that it overlaps the serial
+// coordinate sums that fix the shift.  The shift s(x) = ((x - com) + t) + 0 is
+// a composition of correctly rounded additions, hence non-decreasing in x, so
+// a shifted face box is s() of the raw box, and raw boxes whose cut side is
+// widened by a tolerance tol covering eps plus every rounding error of the
+// shift find a superset of the pairs whose shifted boxes overlap.  The exact
+// shifted-box test (k_filter_pairs) then leaves exactly the reference's set.
 __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __restrict__ idx,
                              const uint32_t* __restrict__ off, uint32_t nf, const unsigned long long* __restrict__ rbox,
-                             uint32_t* __restrict__ code, uint32_t* __restrict__ face)
+                             uint32_t* __restrict__ code, uint32_t* __restrict__ face, double* __restrict__ rawbox,
+                             double tol0, double tol1, double tol2)
 {
+    const double tol[3] = {tol0, tol1, tol2};
     double lo[3], ext[3];
     for (int j = 0; j < 3; ++j) {
         lo[j] = ord_value(rbox[j]);
@@ -437,6 +445,11 @@ __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __res
         }
         code[f] = c[0] * 4 + c[1] * 2 + c[2];
         face[f] = (uint32_t)f;
+        double* o = rawbox + 6 * f;
+        for (int j = 0; j < 3; ++j) {
+            o[j] = mn[j] - tol[j];
+            o[3 + j] = mx[j] + tol[j];
+        }
     }
 }
 
@@ -510,10 +523,11 @@ __global__ void __launch_bounds__(256) k_refit_levels(const double* __restrict__
 }
 
 // ===========================================================================
-// 5. broad phase traversal.  Every (src, cut) face pair whose boxes overlap is
-// emitted -- exactly the pairs bvh.cpp:661-781 finds -- and recorded in the
-// pair hash used for test dedup.  Walks are stackless: the node after a
-// subtree follows from index arithmetic on the implicit tree.
+// 5. broad phase traversal over the raw-space trees.  Every (src, cut) face
+// pair whose raw boxes overlap is emitted: a superset of the pairs
+// bvh.cpp:661-781 finds, which k_filter_pairs reduces to exactly that set.
+// Walks are stackless: the node after a subtree follows from index arithmetic
+// on the implicit tree.
 // ===========================================================================
 
 struct Tree {
@@ -533,7 +547,7 @@ struct Tree {
 constexpr int kFrontierCap = 2048;
 
 __device__ __forceinline__ void per_lane_walk(const double* q, uint32_t s, const Tree& cut, unsigned long long* pairs,
-                                              uint64_t cap, unsigned long long* phash, uint64_t pmask, Counters* C)
+                                              uint64_t cap, Counters* C)
 {
     const int D = cut.D;
     const long long budget = 2ll << D;
@@ -547,13 +561,9 @@ __device__ __forceinline__ void per_lane_walk(const double* q, uint32_t s, const
             if (l == D) {
                 const uint32_t c = cut.sface[k];
                 const uint64_t key = ((uint64_t)s << 32) | c;
-                const unsigned long long at = wave_append(&C->n_pairs);
-                if (at < cap) {
-                    pairs[at] = key;
-                    hash_insert(phash, pmask, key);
-                } else {
-                    atomicOr(&C->ov_pairs, 1u);
-                }
+                const unsigned long long at = wave_append(&C->n_raw);
+                if (at < cap) pairs[at] = key;
+                else atomicOr(&C->ov_pairs, 1u);
             } else {
                 ++l;
                 k <<= 1;
@@ -570,8 +580,7 @@ __device__ __forceinline__ void per_lane_walk(const double* q, uint32_t s, const
 }
 
 __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsigned long long* __restrict__ pairs,
-                                                      uint64_t cap, unsigned long long* __restrict__ phash,
-                                                      uint64_t pmask, Counters* C)
+                                                      uint64_t cap, Counters* C)
 {
     __shared__ uint32_t front[2][kFrontierCap];
     __shared__ double lbox[64][6];
@@ -641,7 +650,7 @@ __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsign
             else atomicAdd(&C->n_leaf_tests, (unsigned long long)nf);
         }
         if (overflow) {
-            if (valid) per_lane_walk(q, s, cut, pairs, cap, phash, pmask, C);
+            if (valid) per_lane_walk(q, s, cut, pairs, cap, C);
         } else if (nf > 0) {
             // surviving cut leaves are staged through LDS 64 at a time (one
             // coalesced load round per 64 leaves), then every lane tests its
@@ -663,13 +672,9 @@ __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsign
                     if (valid && boxes_overlap(q, b)) {
                         const uint32_t c = lface[u];
                         const uint64_t key = ((uint64_t)s << 32) | c;
-                        const unsigned long long at = wave_append(&C->n_pairs);
-                        if (at < cap) {
-                            pairs[at] = key;
-                            hash_insert(phash, pmask, key);
-                        } else {
-                            atomicOr(&C->ov_pairs, 1u);
-                        }
+                        const unsigned long long at = wave_append(&C->n_raw);
+                        if (at < cap) pairs[at] = key;
+                        else atomicOr(&C->ov_pairs, 1u);
                     }
                 }
                 __syncthreads();
@@ -683,18 +688,32 @@ __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsign
 // 6. candidate faces and their planes (kernel.cpp:2184-2356)
 // ===========================================================================
 
-// flag each face of each pair once, appending it to the candidate list
-__global__ void k_mark_candidates(const unsigned long long* __restrict__ pairs, Counters* C, uint64_t cap, uint32_t Fs,
-                                  uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_list)
+// The exact broad-phase test on the shifted boxes (bvh.cpp:661-781 with the
+// boxes of bvh.cpp:242-313) applied to the raw-space superset.  A surviving
+// pair is appended to the candidate-pair list and the pair hash used for test
+// dedup, and each of its faces is flagged once and appended to the candidate
+// face list.
+__global__ void k_filter_pairs(const double* __restrict__ box, const unsigned long long* __restrict__ raw, uint64_t cap,
+                               uint32_t Fs, unsigned long long* __restrict__ pairs, unsigned long long* __restrict__ phash,
+                               uint64_t pmask, uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_list, Counters* C)
 {
-    const uint64_t n = C->n_pairs < cap ? C->n_pairs : cap;
+    const uint64_t n = C->n_raw < cap ? C->n_raw : cap;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = pairs[i];
+        const uint64_t k = raw[i];
         const uint32_t f[2] = {(uint32_t)(k >> 32), Fs + (uint32_t)k};
+        double a[6], b[6];
+        for (int j = 0; j < 6; ++j) {
+            a[j] = box[6 * (uint64_t)f[0] + j];
+            b[j] = box[6 * (uint64_t)f[1] + j];
+        }
+        if (!boxes_overlap(a, b)) continue;
+        const unsigned long long at = wave_append(&C->n_pairs);
+        pairs[at] = k; // at < n <= cap
+        hash_insert(phash, pmask, k);
         for (int side = 0; side < 2; ++side) {
             if (cand[f[side]] == 0u && atomicExch(&cand[f[side]], 1u) == 0u) {
-                const unsigned long long at = wave_append(&C->n_cand);
-                cand_list[at] = f[side]; // at most one entry per face
+                const unsigned long long c = wave_append(&C->n_cand);
+                cand_list[c] = f[side]; // at most one entry per face
             }
         }
     }

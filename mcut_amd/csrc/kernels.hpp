@@ -30,10 +30,14 @@ struct Params {
 struct Counters {
     unsigned long long n_pairs, n_tests, n_esc, n_reg, n_big, n_cand;
     unsigned long long n_edges_src, n_edges_cut;
+    unsigned long long n_raw;                              // raw-space (superset) candidate pairs
     unsigned long long n_visits, n_fallback, n_leaf_tests; // traversal statistics
     unsigned int ov_pairs, ov_tests, ov_esc, ov_reg, ov_big;
     unsigned int gp, bad_src, bad_cut, unsupported, nonmanifold, bad_input, pair_fail, partial;
     unsigned int pad[2];
+    // raw bounding box of each mesh as order keys (lo xyz, hi xyz); written
+    // while the input is validated and kept by the per-dispatch counter reset
+    unsigned long long rbox[2][6];
 };
 
 // polygon soup view: source faces [0,Fs), cut faces [Fs, Fs+Fc); vertices likewise

@@ -3,6 +3,8 @@
 #include "mcgpu.h"
 
 #include <algorithm>
+#include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -21,15 +23,19 @@ struct Buf {
 };
 
 // Stages of one dispatch.  Stream A carries the chain that depends on the
-// shift (coordinate sums first); stream B runs the shift-independent topology
-// and leaf ordering alongside it.
+// shift (serial coordinate sums first); stream B builds both trees and runs
+// the broad phase in raw coordinates alongside it; stream C numbers the
+// polygon-soup edges.  A joins B before the exact pair filter and C before
+// the edge-face tests.
 enum Stage {
-    ST_PARAMS,    // A: serial coordinate sums, shift parameters
-    ST_TOPOLOGY,  // B: polygon-soup edge numbering
-    ST_BVH_SORT,  // B: raw boxes, Morton codes, leaf sort
-    ST_SHIFT,     // A: shifted vertices, face boxes (after joining B)
-    ST_REFIT,     // A: implicit-tree boxes
-    ST_TRAVERSE,  // A: broad phase
+    ST_SUMS,      // A: serial coordinate sums and raw boxes (k_mesh_sums alone)
+    ST_PARAMS,    // A: shift parameters
+    ST_SHIFT,     // A: shifted vertices, shifted face boxes
+    ST_TOPOLOGY,  // C: polygon-soup edge numbering
+    ST_BVH_SORT,  // B: raw face boxes, Morton codes, leaf sort
+    ST_REFIT,     // B: implicit-tree boxes
+    ST_TRAVERSE,  // B: raw-space broad phase (superset of the pairs)
+    ST_FILTER,    // A: exact shifted-box test, pair hash, candidate faces
     ST_PLANES,    // A: candidate-face planes
     ST_TESTS,     // A: edge-face tests
     ST_NARROW,    // A: narrow phase
@@ -37,12 +43,13 @@ enum Stage {
     ST_COUNT
 };
 
-const char* kStageNames[ST_COUNT] = {"shift_params", "topology", "bvh_sort", "shift_boxes", "bvh_refit",
-                                     "traversal",    "face_planes", "tests", "narrow",      "registry"};
+const char* kStageNames[ST_COUNT] = {"coordinate_sums", "shift_params", "shift_boxes", "topology",
+                                     "bvh_sort",        "bvh_refit",    "traversal",   "pair_filter",
+                                     "face_planes",     "tests",        "narrow",      "registry"};
 
-// event slots: one per stage boundary on each stream
-enum Ev { EV_A0, EV_A_PARAMS, EV_B0, EV_B_TOPO, EV_B_SORT, EV_A_SHIFT, EV_A_REFIT, EV_A_TRAV, EV_A_PLANES,
-          EV_A_TESTS, EV_A_NARROW, EV_A_END, EV_FORK, EV_COUNT };
+// event slots: stage boundaries on the three streams
+enum Ev { EV_A0, EV_A_SUMS, EV_A_PARAMS, EV_A_SHIFT, EV_A_JOIN, EV_A_FILTER, EV_A_PLANES, EV_A_JOIN2, EV_A_TESTS, EV_A_NARROW,
+          EV_A_END, EV_FORK, EV_B0, EV_B_SORT, EV_B_REFIT, EV_B_TRAV, EV_C0, EV_C_FILL, EV_C_TOPO, EV_COUNT };
 
 uint64_t next_pow2(uint64_t x)
 {
@@ -62,7 +69,7 @@ int bits_for(uint64_t n)
 
 struct mcgpu_context_t {
     int device = 0;
-    hipStream_t s = nullptr, s2 = nullptr;
+    hipStream_t s = nullptr, s2 = nullptr, s3 = nullptr;
     hipEvent_t ev[EV_COUNT];
     hipError_t last_error = hipSuccess;
 
@@ -71,12 +78,12 @@ struct mcgpu_context_t {
     Buf hkeys[2], hfirst[2], eflag[2], eid[2], cedge[2], cface[2], etotal;
     Buf ev0, ev1, ef0, ef1;
     Buf mcode[2], mface[2], mcode2[2], mface2[2];
-    Buf nodes[2], rbox[2];
-    Buf pairs, phash, cand, cand_list, planes, big_list, big_scratch;
+    Buf nodes[2], rawbox;
+    Buf raw_pairs, pairs, phash, cand, cand_list, planes, big_list, big_scratch;
     Buf te, tg, tqs, trs, ttype, tpip, esc;
     Buf rkey, rval, rkey2, rval2, rp, re, rg, rpts, rcount;
     Buf shkeys, shcnt;
-    Buf scan_tmp, hist;
+    Buf scan_tmp, scan_tmp_c, hist;
 
     uint64_t cap_pairs = 0, cap_tests = 0, cap_esc = 0, cap_reg = 0, cap_big = 0;
 
@@ -93,11 +100,12 @@ struct mcgpu_context_t {
     int keep_tests = 0;
     bool mface_alt[2] = {false, false};
     int depth[2] = {0, 0};
+    double tol[3] = {0, 0, 0}; // raw-space broad-phase tolerance per axis
     bool reg_alt = false;
     Counters C;
     Params P;
     float stage_ms[ST_COUNT];
-    int stage_mode = 0; // 0: full dispatch (two streams), 1: perturbed narrow-phase rerun
+    int stage_mode = 0; // 0: full dispatch (three streams), 1: perturbed narrow-phase rerun
     float total_ms = 0.f;
 };
 
@@ -146,7 +154,7 @@ inline unsigned grid_for(uint64_t n, unsigned block = kBlock, unsigned maxg = 40
 __global__ void k_reset_narrow(Counters* C)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        C->n_tests = C->n_esc = C->n_reg = C->n_big = C->n_cand = 0;
+        C->n_tests = C->n_esc = C->n_reg = C->n_big = 0; // candidate faces are kept
         C->ov_tests = C->ov_esc = C->ov_reg = C->ov_big = 0;
         C->gp = C->bad_src = C->bad_cut = C->unsupported = C->pair_fail = C->partial = 0;
     }
@@ -221,6 +229,7 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
     double* dp;
     CK(ensure(ctx->ps_v, 3 * V, &dp));
     CK(ensure(ctx->box, 6 * F, &dp));
+    CK(ensure(ctx->rawbox, 6 * F, &dp));
     uint32_t* u;
     int32_t* i32;
     unsigned long long* ull;
@@ -248,7 +257,6 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
         while ((1ull << D) < nf) ++D;
         ctx->depth[m] = D;
         CK(ensure(ctx->nodes[m], 6 * ((2ull << D) - 1), &dp));
-        CK(ensure(ctx->rbox[m], 6, &ull));
         scan_need = std::max<uint64_t>(scan_need, scan_tmp_elems(nf + 8));
     }
     CK(ensure(ctx->cand, F + 8, &u));
@@ -257,6 +265,7 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
     CK(ensure(ctx->planes, F, &pl));
     (void)p;
     CK(ensure(ctx->scan_tmp, scan_need + 1024, &u));
+    CK(ensure(ctx->scan_tmp_c, scan_tmp_elems(std::max(ctx->ncorner[0], ctx->ncorner[1]) + 8) + 1024, &u));
     return MCGPU_SUCCESS;
 }
 
@@ -265,6 +274,7 @@ mcgpu_result alloc_list_buffers(mcgpu_context ctx)
     uint32_t* u;
     unsigned long long* ull;
     double* dp;
+    CK(ensure(ctx->raw_pairs, ctx->cap_pairs, &ull));
     CK(ensure(ctx->pairs, ctx->cap_pairs, &ull));
     CK(ensure(ctx->phash, next_pow2(2 * ctx->cap_pairs), &ull));
     CK(ensure(ctx->te, ctx->cap_tests, &u));
@@ -301,22 +311,21 @@ mcgpu_result alloc_list_buffers(mcgpu_context ctx)
     return MCGPU_SUCCESS;
 }
 
-// the part of the pipeline that runs again for a perturbed cut mesh:
+// the part of the pipeline that runs again for a perturbed cut mesh (whose
+// candidate pairs and faces are those of the unperturbed dispatch):
 // candidate-face planes, tests, narrow phase, registry, status
-void enqueue_narrow(mcgpu_context ctx, const Soup& S)
+void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
 {
     hipStream_t s = ctx->s;
     const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
     Counters* C = P_<Counters>(ctx->counters);
-    fill_u32(ctx->cand.p, 0u, F, s);
-    k_mark_candidates<<<grid_for(ctx->cap_pairs), kBlock, 0, s>>>(P_<unsigned long long>(ctx->pairs), C, ctx->cap_pairs,
-                                                                   ctx->nf[0], P_<uint32_t>(ctx->cand),
-                                                                   P_<uint32_t>(ctx->cand_list));
     k_planes<<<grid_for(F), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->cand_list), P_<Plane>(ctx->planes),
                                              P_<uint32_t>(ctx->big_list), ctx->cap_big, C);
     k_big_polygons<<<4, 64, 0, s>>>(S, P_<uint32_t>(ctx->big_list), C, ctx->cap_big, P_<Plane>(ctx->planes),
                                      P_<Trip>(ctx->big_scratch), C);
     (void)hipEventRecord(ctx->ev[EV_A_PLANES], s);
+    if (join_topology) (void)hipStreamWaitEvent(s, ctx->ev[EV_C_TOPO], 0);
+    (void)hipEventRecord(ctx->ev[EV_A_JOIN2], s);
     const uint64_t pmask = next_pow2(2 * ctx->cap_pairs) - 1;
     k_tests<<<grid_for(6 * ctx->cap_pairs), kBlock, 0, s>>>(S, P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
                                                         P_<unsigned long long>(ctx->phash), pmask, P_<uint32_t>(ctx->te),
@@ -364,6 +373,8 @@ void enqueue_validate(mcgpu_context ctx)
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
     fill_u32(C, 0u, sizeof(Counters) / 4, s);
     for (int m = 0; m < 2; ++m) {
+        fill_u64(C->rbox[m], ~0ull, 3, s);
+        k_raw_bbox<<<grid_for(ctx->nv[m], kBlock, 64), kBlock, 0, s>>>(ctx->dv[m], ctx->nv[m], C->rbox[m]);
         uint32_t* off = P_<uint32_t>(ctx->off[m]);
         if (ctx->dsz[m]) {
             scan_exclusive(ctx->dsz[m], off, ctx->nf[m], P_<uint32_t>(ctx->scan_tmp), etotal + m, s);
@@ -377,91 +388,56 @@ void enqueue_validate(mcgpu_context ctx)
 
 void enqueue_frontend(mcgpu_context ctx)
 {
-    hipStream_t s = ctx->s, s2 = ctx->s2;
+    hipStream_t s = ctx->s, s2 = ctx->s2, s3 = ctx->s3;
     const uint32_t Fs = ctx->nf[0];
     const uint64_t F = (uint64_t)Fs + ctx->nf[1], V = (uint64_t)ctx->nv[0] + ctx->nv[1];
     Counters* C = P_<Counters>(ctx->counters);
     Params* Pd = P_<Params>(ctx->params);
 
+    // ---- stream A, enqueued first: the serial sums start at once ----
     (void)hipEventRecord(ctx->ev[EV_A0], s);
-    fill_u32(C, 0u, sizeof(Counters) / 4, s);
+    fill_u32(C, 0u, offsetof(Counters, rbox) / 4, s); // keeps the raw boxes of the validation pass
     (void)hipEventRecord(ctx->ev[EV_FORK], s);
-    (void)hipStreamWaitEvent(s2, ctx->ev[EV_FORK], 0);
-
-    // ---- stream B: shift-independent topology and leaf order ----
-    (void)hipEventRecord(ctx->ev[EV_B0], s2);
-    uint32_t* etotal = P_<uint32_t>(ctx->etotal);
-    const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
-    fill_u32(ctx->ef1.p, 0xffffffffu, K, s2);
-    CornerMap cmap[2];
-    for (int m = 0; m < 2; ++m) {
-        cmap[m].off = P_<uint32_t>(ctx->off[m]);
-        cmap[m].cface = ctx->dsz[m] ? P_<uint32_t>(ctx->cface[m]) : nullptr;
-        if (ctx->dsz[m])
-            k_corner_faces<<<grid_for(ctx->nf[m]), kBlock, 0, s2>>>(cmap[m].off, ctx->nf[m], P_<uint32_t>(ctx->cface[m]));
-    }
-    for (int m = 0; m < 2; ++m) {
-        const uint64_t nk = ctx->ncorner[m];
-        const uint64_t hs = next_pow2(2 * nk + 2);
-        fill_u64(ctx->hkeys[m].p, kEmpty64, hs, s2);
-        fill_u64(ctx->hfirst[m].p, kEmpty64, hs, s2);
-        k_edge_insert<<<grid_for(nk), kBlock, 0, s2>>>(ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]),
-                                                       P_<unsigned long long>(ctx->hfirst[m]), hs - 1);
-        k_edge_flag<<<grid_for(nk), kBlock, 0, s2>>>(ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]),
-                                                     P_<unsigned long long>(ctx->hfirst[m]), hs - 1,
-                                                     P_<uint32_t>(ctx->eflag[m]), C);
-        scan_exclusive(P_<uint32_t>(ctx->eflag[m]), P_<uint32_t>(ctx->eid[m]), nk, P_<uint32_t>(ctx->scan_tmp),
-                       etotal + m, s2);
-        k_store_edge_count<<<1, 1, 0, s2>>>(etotal + m, C, m);
-    }
-    for (int m = 0; m < 2; ++m) {
-        const uint64_t nk = ctx->ncorner[m];
-        const uint64_t hs = next_pow2(2 * nk + 2);
-        k_edge_assign<<<grid_for(nk), kBlock, 0, s2>>>(
-            ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]), P_<unsigned long long>(ctx->hfirst[m]),
-            hs - 1, P_<uint32_t>(ctx->eid[m]), C, m ? ctx->nv[0] : 0, m ? ctx->nf[0] : 0, P_<uint32_t>(ctx->cedge[m]),
-            P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
-    }
-    (void)hipEventRecord(ctx->ev[EV_B_TOPO], s2);
-    RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
-    for (int m = 0; m < 2; ++m) {
-        const uint32_t nf = ctx->nf[m];
-        unsigned long long* rb = P_<unsigned long long>(ctx->rbox[m]);
-        fill_u64(rb, ~0ull, 3, s2);
-        fill_u64(rb + 3, 0ull, 3, s2);
-        k_raw_bbox<<<grid_for(ctx->nv[m], kBlock, 64), kBlock, 0, s2>>>(ctx->dv[m], ctx->nv[m], rb);
-        k_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf, rb,
-                                                      P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]));
-        ctx->mface_alt[m] = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]),
-                                                 P_<uint32_t>(ctx->mcode2[m]), P_<uint32_t>(ctx->mface2[m]), nullptr,
-                                                 nf, nf, 30, rt, s2);
-    }
-    (void)hipEventRecord(ctx->ev[EV_B_SORT], s2);
-
-    // ---- stream A: the shift chain ----
     k_mesh_sums<<<6, 64, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd);
+    (void)hipEventRecord(ctx->ev[EV_A_SUMS], s);
     const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
     k_shift_params<<<1, 1, 0, s>>>(Pd, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
     (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
     k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
                                                         P_<double>(ctx->ps_v), 0);
-    (void)hipStreamWaitEvent(s, ctx->ev[EV_B_SORT], 0);
     const Soup S = make_soup(ctx);
     k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, Pd, P_<double>(ctx->box), 0);
     (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
 
+    // ---- stream B: raw-space trees and broad phase ----
+    (void)hipStreamWaitEvent(s2, ctx->ev[EV_FORK], 0);
+    (void)hipEventRecord(ctx->ev[EV_B0], s2);
+    RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
+    double* rawbox = P_<double>(ctx->rawbox);
+    for (int m = 0; m < 2; ++m) {
+        const uint32_t nf = ctx->nf[m];
+        const double* tl = m ? ctx->tol : nullptr;
+        k_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf,
+                                                      C->rbox[m], P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]),
+                                                      rawbox + 6 * (uint64_t)(m ? Fs : 0), tl ? tl[0] : 0.0,
+                                                      tl ? tl[1] : 0.0, tl ? tl[2] : 0.0);
+        ctx->mface_alt[m] = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]),
+                                                 P_<uint32_t>(ctx->mcode2[m]), P_<uint32_t>(ctx->mface2[m]), nullptr,
+                                                 nf, nf, 30, rt, s2);
+    }
+    (void)hipEventRecord(ctx->ev[EV_B_SORT], s2);
     Tree T[2];
     for (int m = 0; m < 2; ++m) {
         const uint32_t nf = ctx->nf[m];
         const int D = ctx->depth[m];
-        const double* fbox = P_<double>(ctx->box) + 6 * (uint64_t)(m ? Fs : 0);
+        const double* fbox = rawbox + 6 * (uint64_t)(m ? Fs : 0);
         const uint32_t* sface = P_<uint32_t>(ctx->mface_alt[m] ? ctx->mface2[m] : ctx->mface[m]);
         double* nodes = P_<double>(ctx->nodes[m]);
         int lin = D;
         do {
             const int width = 1 << lin;
             const int per = width < kRefitWidth ? width : kRefitWidth;
-            k_refit_levels<<<width / per, 256, 0, s>>>(fbox, sface, (int)nf, D, lin, nodes);
+            k_refit_levels<<<width / per, 256, 0, s2>>>(fbox, sface, (int)nf, D, lin, nodes);
             lin -= (lin < kRefitLevels ? lin : kRefitLevels);
         } while (lin > 0);
         T[m].sface = sface;
@@ -470,22 +446,71 @@ void enqueue_frontend(mcgpu_context ctx)
         T[m].n = (int)nf;
         T[m].D = D;
     }
-    (void)hipEventRecord(ctx->ev[EV_A_REFIT], s);
-
-    const uint64_t ps = next_pow2(2 * ctx->cap_pairs);
-    fill_u64(ctx->phash.p, kEmpty64, ps, s);
+    (void)hipEventRecord(ctx->ev[EV_B_REFIT], s2);
     const unsigned nwaves = (Fs + 63) / 64;
-    k_traverse_wave<<<nwaves < 65536 ? nwaves : 65536, 64, 0, s>>>(T[0], T[1], P_<unsigned long long>(ctx->pairs),
-                                                                   ctx->cap_pairs, P_<unsigned long long>(ctx->phash),
-                                                                   ps - 1, C);
-    (void)hipEventRecord(ctx->ev[EV_A_TRAV], s);
-    enqueue_narrow(ctx, S);
+    k_traverse_wave<<<nwaves < 65536 ? nwaves : 65536, 64, 0, s2>>>(T[0], T[1], P_<unsigned long long>(ctx->raw_pairs),
+                                                                    ctx->cap_pairs, C);
+    (void)hipEventRecord(ctx->ev[EV_B_TRAV], s2);
+
+    // ---- stream C: polygon-soup edge numbering, and the tables the filter fills ----
+    (void)hipStreamWaitEvent(s3, ctx->ev[EV_FORK], 0);
+    (void)hipEventRecord(ctx->ev[EV_C0], s3);
+    uint32_t* etotal = P_<uint32_t>(ctx->etotal);
+    const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
+    const uint64_t ps = next_pow2(2 * ctx->cap_pairs);
+    fill_u64(ctx->phash.p, kEmpty64, ps, s3);
+    fill_u32(ctx->cand.p, 0u, F, s3);
+    (void)hipEventRecord(ctx->ev[EV_C_FILL], s3);
+    fill_u32(ctx->ef1.p, 0xffffffffu, K, s3);
+    CornerMap cmap[2];
+    for (int m = 0; m < 2; ++m) {
+        cmap[m].off = P_<uint32_t>(ctx->off[m]);
+        cmap[m].cface = ctx->dsz[m] ? P_<uint32_t>(ctx->cface[m]) : nullptr;
+        if (ctx->dsz[m])
+            k_corner_faces<<<grid_for(ctx->nf[m]), kBlock, 0, s3>>>(cmap[m].off, ctx->nf[m], P_<uint32_t>(ctx->cface[m]));
+    }
+    // the scans of this stream use their own scratch: stream B's sort uses scan_tmp
+    uint32_t* ctmp = P_<uint32_t>(ctx->scan_tmp_c);
+    for (int m = 0; m < 2; ++m) {
+        const uint64_t nk = ctx->ncorner[m];
+        const uint64_t hs = next_pow2(2 * nk + 2);
+        fill_u64(ctx->hkeys[m].p, kEmpty64, hs, s3);
+        fill_u64(ctx->hfirst[m].p, kEmpty64, hs, s3);
+        k_edge_insert<<<grid_for(nk), kBlock, 0, s3>>>(ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]),
+                                                       P_<unsigned long long>(ctx->hfirst[m]), hs - 1);
+        k_edge_flag<<<grid_for(nk), kBlock, 0, s3>>>(ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]),
+                                                     P_<unsigned long long>(ctx->hfirst[m]), hs - 1,
+                                                     P_<uint32_t>(ctx->eflag[m]), C);
+        scan_exclusive(P_<uint32_t>(ctx->eflag[m]), P_<uint32_t>(ctx->eid[m]), nk, ctmp, etotal + m, s3);
+        k_store_edge_count<<<1, 1, 0, s3>>>(etotal + m, C, m);
+    }
+    for (int m = 0; m < 2; ++m) {
+        const uint64_t nk = ctx->ncorner[m];
+        const uint64_t hs = next_pow2(2 * nk + 2);
+        k_edge_assign<<<grid_for(nk), kBlock, 0, s3>>>(
+            ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]), P_<unsigned long long>(ctx->hfirst[m]),
+            hs - 1, P_<uint32_t>(ctx->eid[m]), C, m ? ctx->nv[0] : 0, m ? ctx->nf[0] : 0, P_<uint32_t>(ctx->cedge[m]),
+            P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
+    }
+    (void)hipEventRecord(ctx->ev[EV_C_TOPO], s3);
+
+    // ---- stream A: join the broad phase, exact filter, narrow phase ----
+    (void)hipStreamWaitEvent(s, ctx->ev[EV_B_TRAV], 0);
+    (void)hipStreamWaitEvent(s, ctx->ev[EV_C_FILL], 0);
+    (void)hipEventRecord(ctx->ev[EV_A_JOIN], s);
+    k_filter_pairs<<<grid_for(ctx->cap_pairs), kBlock, 0, s>>>(P_<double>(ctx->box), P_<unsigned long long>(ctx->raw_pairs),
+                                                               ctx->cap_pairs, Fs, P_<unsigned long long>(ctx->pairs),
+                                                               P_<unsigned long long>(ctx->phash), ps - 1,
+                                                               P_<uint32_t>(ctx->cand), P_<uint32_t>(ctx->cand_list), C);
+    (void)hipEventRecord(ctx->ev[EV_A_FILTER], s);
+    enqueue_narrow(ctx, S, true);
 }
 
 mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow)
 {
     CK(hipStreamSynchronize(ctx->s));
     CK(hipStreamSynchronize(ctx->s2));
+    CK(hipStreamSynchronize(ctx->s3));
     CK(hipGetLastError());
     CK(hipMemcpy(&ctx->C, ctx->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
     CK(hipMemcpy(&ctx->P, ctx->params.p, sizeof(Params), hipMemcpyDeviceToHost));
@@ -493,18 +518,17 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
     *overflow = false;
     if (getenv("MCGPU_DEBUG"))
         fprintf(stderr,
-                "[mcgpu] pairs=%llu tests=%llu esc=%llu reg=%llu big=%llu cand=%llu edges=%llu+%llu ov=%u/%u/%u/%u/%u "
+                "[mcgpu] raw=%llu pairs=%llu tests=%llu esc=%llu reg=%llu big=%llu cand=%llu edges=%llu+%llu ov=%u/%u/%u/%u/%u "
                 "gp=%u bad=%u/%u unsup=%u nonman=%u badin=%u pairfail=%u partial=%u caps=%llu/%llu/%llu/%llu "
                 "trav levels=%llu fallback_waves=%llu leaf_tests=%llu\n",
-                C.n_pairs, C.n_tests, C.n_esc, C.n_reg, C.n_big, C.n_cand, C.n_edges_src, C.n_edges_cut, C.ov_pairs,
+                C.n_raw, C.n_pairs, C.n_tests, C.n_esc, C.n_reg, C.n_big, C.n_cand, C.n_edges_src, C.n_edges_cut, C.ov_pairs,
                 C.ov_tests, C.ov_esc, C.ov_reg, C.ov_big, C.gp, C.bad_src, C.bad_cut, C.unsupported, C.nonmanifold,
                 C.bad_input, C.pair_fail, C.partial, (unsigned long long)ctx->cap_pairs,
                 (unsigned long long)ctx->cap_tests, (unsigned long long)ctx->cap_esc, (unsigned long long)ctx->cap_reg,
                 C.n_visits, C.n_fallback, C.n_leaf_tests);
-    if (C.ov_pairs & 2u) return MCGPU_ERROR_UNSUPPORTED; // tree deeper than the traversal stack
     if (C.ov_pairs || C.ov_tests || C.ov_esc || C.ov_reg || C.ov_big) {
         *overflow = true;
-        if (C.ov_pairs) ctx->cap_pairs = std::max<uint64_t>(2 * ctx->cap_pairs, C.n_pairs + C.n_pairs / 4);
+        if (C.ov_pairs) ctx->cap_pairs = std::max<uint64_t>(2 * ctx->cap_pairs, C.n_raw + C.n_raw / 4);
         if (C.ov_tests) ctx->cap_tests = std::max<uint64_t>(2 * ctx->cap_tests, C.n_tests + C.n_tests / 4);
         if (C.ov_esc) ctx->cap_esc = std::max<uint64_t>(2 * ctx->cap_esc, C.n_esc + C.n_esc / 4);
         if (C.ov_reg) ctx->cap_reg = std::max<uint64_t>(2 * ctx->cap_reg, C.n_reg + C.n_reg / 4);
@@ -518,19 +542,19 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) ms = 0.f;
         return ms;
     };
-    const bool twostream = ctx->stage_mode == 0;
-    ctx->stage_ms[ST_PARAMS] = el(EV_A0, EV_A_PARAMS);
-    ctx->stage_ms[ST_TOPOLOGY] = twostream ? el(EV_B0, EV_B_TOPO) : 0.f;
-    ctx->stage_ms[ST_BVH_SORT] = twostream ? el(EV_B_TOPO, EV_B_SORT) : 0.f;
-    // time stream A spent waiting for stream B is not part of the shift stage
-    float wait = twostream ? el(EV_A_PARAMS, EV_B_SORT) : 0.f;
-    if (wait < 0.f) wait = 0.f;
-    float sh = el(EV_A_PARAMS, EV_A_SHIFT) - wait;
-    ctx->stage_ms[ST_SHIFT] = sh > 0.f ? sh : 0.f;
-    ctx->stage_ms[ST_REFIT] = el(EV_A_SHIFT, EV_A_REFIT);
-    ctx->stage_ms[ST_TRAVERSE] = el(EV_A_REFIT, EV_A_TRAV);
-    ctx->stage_ms[ST_PLANES] = el(EV_A_TRAV, EV_A_PLANES);
-    ctx->stage_ms[ST_TESTS] = el(EV_A_PLANES, EV_A_TESTS);
+    // stages of streams B and C run only in a full dispatch; A's stages after a
+    // join start at the event recorded once the join completed
+    const bool full = ctx->stage_mode == 0;
+    ctx->stage_ms[ST_SUMS] = full ? el(EV_FORK, EV_A_SUMS) : 0.f;
+    ctx->stage_ms[ST_PARAMS] = full ? el(EV_A_SUMS, EV_A_PARAMS) : el(EV_A0, EV_A_PARAMS);
+    ctx->stage_ms[ST_SHIFT] = el(EV_A_PARAMS, EV_A_SHIFT);
+    ctx->stage_ms[ST_TOPOLOGY] = full ? el(EV_C0, EV_C_TOPO) : 0.f;
+    ctx->stage_ms[ST_BVH_SORT] = full ? el(EV_B0, EV_B_SORT) : 0.f;
+    ctx->stage_ms[ST_REFIT] = full ? el(EV_B_SORT, EV_B_REFIT) : 0.f;
+    ctx->stage_ms[ST_TRAVERSE] = full ? el(EV_B_REFIT, EV_B_TRAV) : 0.f;
+    ctx->stage_ms[ST_FILTER] = full ? el(EV_A_JOIN, EV_A_FILTER) : 0.f;
+    ctx->stage_ms[ST_PLANES] = el(EV_A_FILTER, EV_A_PLANES);
+    ctx->stage_ms[ST_TESTS] = el(EV_A_JOIN2, EV_A_TESTS);
     ctx->stage_ms[ST_NARROW] = el(EV_A_TESTS, EV_A_NARROW);
     ctx->stage_ms[ST_REGISTRY] = el(EV_A_NARROW, EV_A_END);
     float tot = el(EV_A0, EV_A_END);
@@ -546,6 +570,7 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         info->num_tests = C.n_tests;
         info->num_escalated = C.n_esc;
         info->num_registry = C.n_reg;
+        info->num_raw_pairs = C.n_raw;
         for (int j = 0; j < 3; ++j) {
             info->com[j] = ctx->P.com[j];
             info->translation[j] = ctx->P.t[j];
@@ -554,6 +579,53 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         info->device_ms = tot;
     }
     return MCGPU_SUCCESS;
+}
+
+double host_ord_value(unsigned long long k)
+{
+    const unsigned long long u = (k & 0x8000000000000000ull) ? (k & ~0x8000000000000000ull) : ~k;
+    double x;
+    std::memcpy(&x, &u, sizeof x);
+    return x;
+}
+
+// Per-axis tolerance that makes the raw-space broad phase find every pair
+// whose shifted boxes overlap.  With u = 2^-53 and, per axis, E the extent of
+// both meshes' raw box, W its largest magnitude and n the larger vertex count:
+//   |x - com| <= R = E + (n + 2) 2^-52 W   (com is a mean of sequentially
+//                                           summed coordinates),
+//   |t| <= (R + 1)(1 + 4u),
+//   s(x) = x - com + t + d(x) with |d(x)| <= u (3R + 1)(1 + 4u),
+// and the shifted test s(a_lo) <= fl(s(b_hi) + eps) implies
+//   a_lo <= b_hi + eps (1 + u) + u (8R + 4)(1 + 4u),
+// which the raw test a_lo <= fl(b_hi + tol) accepts once tol covers that
+// bound plus the rounding of b_hi + tol (u (W + tol)).  eps is bounded from
+// the cut mesh's raw extent.  tol below keeps a factor of 32 over every
+// rounding term.  Returns false when the bound is not finite.
+bool broad_phase_tolerance(const Counters& c0, const uint32_t* nv, double gp_constant, bool absolute, double* tol)
+{
+    const double u32 = std::ldexp(1.0, -48); // 32 u
+    const double n = (double)std::max(nv[0], nv[1]);
+    double lo[3], hi[3], cext2 = 0.0, wmax = 0.0, rmax = 0.0, R[3], W[3];
+    for (int j = 0; j < 3; ++j) {
+        lo[j] = std::min(host_ord_value(c0.rbox[0][j]), host_ord_value(c0.rbox[1][j]));
+        hi[j] = std::max(host_ord_value(c0.rbox[0][3 + j]), host_ord_value(c0.rbox[1][3 + j]));
+        W[j] = std::max(std::fabs(lo[j]), std::fabs(hi[j]));
+        R[j] = (hi[j] - lo[j]) + (n + 2.0) * std::ldexp(1.0, -52) * W[j];
+        const double ce = host_ord_value(c0.rbox[1][3 + j]) - host_ord_value(c0.rbox[1][j]);
+        cext2 += ce * ce;
+        wmax = std::max(wmax, W[j]);
+        rmax = std::max(rmax, R[j]);
+    }
+    // diag = |(bmax + t) - (bmin + t)| of the cut mesh, each term within
+    // u (2W + 2|t| + E) of the raw extent
+    const double diag_hi = (std::sqrt(cext2) + u32 * 2.0 * (wmax + 2.0 * rmax + 2.0)) * (1.0 + u32);
+    const double eps_hi = (absolute ? 1.0 : diag_hi) * gp_constant * (1.0 + u32);
+    for (int j = 0; j < 3; ++j) {
+        tol[j] = eps_hi * (1.0 + u32) + u32 * (8.0 * R[j] + W[j] + 4.0);
+        if (!std::isfinite(tol[j]) || !std::isfinite(lo[j]) || !std::isfinite(hi[j])) return false;
+    }
+    return true;
 }
 
 mcgpu_result upload_mesh(mcgpu_context ctx, int m, const mcgpu_mesh* mesh, bool on_device)
@@ -632,7 +704,8 @@ MCGPU_API mcgpu_result mcgpu_create_context(int device, mcgpu_context* out)
     mcgpu_context ctx = new mcgpu_context_t();
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->s2, hipStreamNonBlocking) != hipSuccess) {
+        hipStreamCreateWithFlags(&ctx->s2, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->s3, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return MCGPU_ERROR_DEVICE;
     }
@@ -647,19 +720,21 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->s);
     (void)hipStreamSynchronize(ctx->s2);
+    (void)hipStreamSynchronize(ctx->s3);
     Buf* all[] = {&ctx->in_v[0], &ctx->in_v[1], &ctx->in_idx[0], &ctx->in_idx[1], &ctx->in_sz[0], &ctx->in_sz[1],
                   &ctx->off[0], &ctx->off[1], &ctx->params, &ctx->counters, &ctx->ps_v, &ctx->box, &ctx->hkeys[0],
                   &ctx->hkeys[1], &ctx->hfirst[0], &ctx->hfirst[1], &ctx->cface[0], &ctx->cface[1], &ctx->eflag[0],
                   &ctx->eflag[1], &ctx->eid[0], &ctx->eid[1], &ctx->cedge[0], &ctx->cedge[1], &ctx->etotal, &ctx->ev0,
                   &ctx->ev1, &ctx->ef0, &ctx->ef1, &ctx->mcode[0], &ctx->mcode[1], &ctx->mface[0], &ctx->mface[1],
                   &ctx->mcode2[0], &ctx->mcode2[1], &ctx->mface2[0], &ctx->mface2[1], &ctx->nodes[0], &ctx->nodes[1],
-                  &ctx->rbox[0], &ctx->rbox[1], &ctx->pairs, &ctx->phash, &ctx->cand,
+                  &ctx->rawbox, &ctx->raw_pairs, &ctx->pairs, &ctx->phash, &ctx->cand, &ctx->cand_list,
                   &ctx->planes, &ctx->big_list, &ctx->big_scratch, &ctx->te, &ctx->tg, &ctx->tqs, &ctx->trs,
                   &ctx->ttype, &ctx->tpip, &ctx->esc, &ctx->rkey, &ctx->rval, &ctx->rkey2, &ctx->rval2, &ctx->rp,
-                  &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->hist};
+                  &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->scan_tmp_c, &ctx->hist};
     for (Buf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->s3);
     (void)hipStreamDestroy(ctx->s2);
     (void)hipStreamDestroy(ctx->s);
     delete ctx;
@@ -699,6 +774,9 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
         Counters c0;
         CK(hipMemcpy(&c0, ctx->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
         if (c0.bad_input) return MCGPU_ERROR_INVALID_VALUE;
+        if (!broad_phase_tolerance(c0, ctx->nv, ctx->gp_constant,
+                                   (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0, ctx->tol))
+            return MCGPU_ERROR_UNSUPPORTED;
     }
     for (int attempt = 0; attempt < 6; ++attempt) {
         if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
@@ -732,9 +810,9 @@ MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const doubl
         k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1],
                                                             P_<Params>(ctx->params), P_<double>(ctx->ps_v), 1);
         (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
-        (void)hipEventRecord(ctx->ev[EV_A_REFIT], s);
-        (void)hipEventRecord(ctx->ev[EV_A_TRAV], s);
-        enqueue_narrow(ctx, make_soup(ctx));
+        (void)hipEventRecord(ctx->ev[EV_A_JOIN], s);
+        (void)hipEventRecord(ctx->ev[EV_A_FILTER], s);
+        enqueue_narrow(ctx, make_soup(ctx), false);
         bool overflow = false;
         if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
         if (!overflow) return MCGPU_SUCCESS;

@@ -49,7 +49,8 @@ class DispatchDesc(C.Structure):
 class FrontendInfo(C.Structure):
     _fields_ = [("status", C.c_int32), ("gp_violation", C.c_uint32), ("num_candidate_pairs", C.c_uint64),
                 ("num_candidate_faces", C.c_uint64), ("num_edges", C.c_uint64), ("num_tests", C.c_uint64),
-                ("num_escalated", C.c_uint64), ("num_registry", C.c_uint64), ("com", C.c_double * 3),
+                ("num_escalated", C.c_uint64), ("num_registry", C.c_uint64), ("num_raw_pairs", C.c_uint64),
+                ("com", C.c_double * 3),
                 ("translation", C.c_double * 3), ("eps", C.c_double), ("device_ms", C.c_float),
                 ("reserved", C.c_float)]
 
@@ -123,7 +124,8 @@ class Result:
         self.status = info.status
         self.status_name = STATUS.get(info.status, str(info.status))
         self.counts = dict(pairs=info.num_candidate_pairs, cand=info.num_candidate_faces, edges=info.num_edges,
-                           tests=info.num_tests, escalated=info.num_escalated, registry=info.num_registry)
+                           tests=info.num_tests, escalated=info.num_escalated, registry=info.num_registry,
+                           raw_pairs=info.num_raw_pairs)
         self.com = np.array(info.com[:])
         self.t = np.array(info.translation[:])
         self.eps = info.eps

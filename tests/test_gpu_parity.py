@@ -123,6 +123,25 @@ def test_random_box_cuts(ctx, seed):
     compare(ctx, (V, F, None), box)
 
 
+@pytest.mark.parametrize("axis,sign", [(2, 1.0), (2, -1.0), (0, 1.0)])
+def test_boxes_touching_at_eps(ctx, axis, sign):
+    """A cut patch offset from a source patch by eps + k * 5e-17, so that the
+    shifted boxes go from overlapping to disjoint inside the sweep.  The GPU's
+    broad phase runs on raw boxes widened by a rounding bound and then applies
+    the exact shifted test: its pairs must equal the oracle's at every k."""
+    from mcut_amd import meshgen as mg
+    V, F, _ = mg.grid_patch(6)
+    src = (V.copy(), F, None)
+    base = 1.0 if axis == 0 else 0.0  # lateral case: next patch over, touching one column
+    counts = set()
+    for k in range(-12, 13):
+        Vc = V.copy()
+        Vc[:, axis] += sign * (base + 0.0001414213562373095 + k * 5e-17)
+        R, O = compare(ctx, src, (Vc, F, None))
+        counts.add(R.counts["pairs"])
+    assert 0 in counts and len(counts) > 1  # the sweep crosses the boundary
+
+
 def test_empty_intersection(ctx):
     from mcut_amd import meshgen as mg
     src = mg.uv_sphere(16, 8)
