@@ -15,17 +15,26 @@ namespace mcgpu {
 // One 64-lane wavefront per (mesh, coordinate): six independent chains.  Each
 // sum must be the reference's left-to-right sum (any other association
 // changes the centre's bits and so every shifted coordinate), so the chain is
-// inherently serial; what the kernel removes is everything around the adds.
-// The wave loads 512 coordinates at a time with coalesced vector loads into
-// one half of an LDS double buffer while the serial adds consume the other
-// half through broadcast LDS reads.  The bounding box is an order-independent
-// lane reduction over the same loads.
-constexpr int kSumChunk = 512;
+// inherently serial: one dependent v_add_f64 (6.6 cycles on gfx950) per
+// vertex.  What the kernel removes is everything around the adds:
+//  - coordinates arrive 2048 at a time through an LDS double buffer, loaded
+//    one chunk (about 5.7 us of adds) ahead, which covers HBM latency even
+//    while other streams load the memory system;
+//  - the adds read the buffer through broadcast LDS reads into registers,
+//    half a group (32 values) ahead of the adds that consume them;
+//  - the last chunk is padded with +0.0 to a whole group: the running sum
+//    starts at +0.0 and round-to-nearest never produces -0.0 from it, so
+//    adding +0.0 leaves every bit of it unchanged;
+//  - the wave raises its issue priority, and the host runs the kernel on a
+//    stream of its own whose CU mask keeps other kernels off its CUs.
+// The bounding box is an order-independent lane reduction over the same loads.
+constexpr int kSumChunk = 2048;
 
 __global__ void __launch_bounds__(64) k_mesh_sums(const double* __restrict__ v0, uint32_t n0,
                                                   const double* __restrict__ v1, uint32_t n1, Params* P)
 {
-    __shared__ double buf[2][kSumChunk];
+    __shared__ double buf[2 * kSumChunk + 32]; // + one half group read past the last group
+    __builtin_amdgcn_s_setprio(3);
     const int m = blockIdx.x / 3, j = blockIdx.x % 3;
     const double* __restrict__ v = m ? v1 : v0;
     const uint32_t n = m ? n1 : n0;
@@ -34,54 +43,73 @@ __global__ void __launch_bounds__(64) k_mesh_sums(const double* __restrict__ v0,
     double lo = 1.7976931348623157e308, hi = -1.7976931348623157e308;
     double r[U];
     const uint32_t nchunks = (n + kSumChunk - 1) / kSumChunk;
-    // prologue: chunk 0
+    // prologue: chunk 0 (clamped loads; values past the end are stored as +0.0)
     for (int u = 0; u < U; ++u) {
         const uint32_t idx = (uint32_t)u * 64 + lane;
-        r[u] = idx < n ? v[3 * (uint64_t)idx + j] : 0.0;
-        if (idx < n) {
+        r[u] = v[3 * (uint64_t)(idx < n ? idx : n - 1) + j];
+    }
+    for (int u = 0; u < U; ++u) {
+        const uint32_t idx = (uint32_t)u * 64 + lane;
+        const bool ok = idx < n;
+        if (ok) {
             hi = (hi < r[u]) ? r[u] : hi;
             lo = (r[u] < lo) ? r[u] : lo;
         }
+        buf[u * 64 + lane] = ok ? r[u] : 0.0;
     }
-    for (int u = 0; u < U; ++u) buf[0][u * 64 + lane] = r[u];
+    if (lane < 32) buf[2 * kSumChunk + lane] = 0.0;
     __syncthreads();
     double sum = 0.0;
     for (uint32_t c = 0; c < nchunks; ++c) {
         const int cur = c & 1;
         const uint32_t nb = (c + 1) * kSumChunk;
         const bool more = c + 1 < nchunks;
-        // next chunk: unconditional (clamped) loads, so they stay in flight
-        // while the serial adds below consume the current chunk
+        // next chunk: unconditional (clamped) loads stay in flight while the
+        // serial adds below consume the current chunk
         if (more)
             for (int u = 0; u < U; ++u) {
-                uint32_t idx = nb + (uint32_t)u * 64 + lane;
-                idx = idx < n ? idx : n - 1;
-                r[u] = v[3 * (uint64_t)idx + j];
+                const uint32_t idx = nb + (uint32_t)u * 64 + lane;
+                r[u] = v[3 * (uint64_t)(idx < n ? idx : n - 1) + j];
             }
-        const uint32_t cnt = (n - c * kSumChunk) < (uint32_t)kSumChunk ? (n - c * kSumChunk) : (uint32_t)kSumChunk;
-        const double* b = buf[cur];
-        if (cnt == (uint32_t)kSumChunk) {
-            // registers hold the next 32 values while the current 32 are added
-            double x[32], y[32];
-            for (int q = 0; q < 32; ++q) x[q] = b[q];
-            for (int k = 32; k < kSumChunk; k += 64) {
-                for (int q = 0; q < 32; ++q) y[q] = b[k + q];
-                for (int q = 0; q < 32; ++q) sum += x[q];
-                if (k + 32 < kSumChunk)
-                    for (int q = 0; q < 32; ++q) x[q] = b[k + 32 + q];
-                for (int q = 0; q < 32; ++q) sum += y[q];
+        const uint32_t rem = n - c * kSumChunk;
+        const uint32_t cnt = rem < (uint32_t)kSumChunk ? rem : (uint32_t)kSumChunk;
+        const int ngroups = (int)((cnt + 63) / 64);
+        const double* b = buf + cur * kSumChunk;
+        // x holds the first half of the group being added while the second
+        // half loads into y, then y is added while the next group's first half
+        // loads into x; the scheduling barriers keep one LDS read between
+        // every two adds, so the reads hide under the chain instead of being
+        // issued (and waited for) in one batch ahead of it
+        double x[32], y[32];
+        for (int q = 0; q < 32; ++q) x[q] = b[q];
+        for (int g = 0; g < ngroups; ++g) {
+            const double* bg = b + g * 64;
+#pragma unroll
+            for (int h = 0; h < 16; ++h) {
+                y[2 * h] = bg[32 + 2 * h];
+                y[2 * h + 1] = bg[33 + 2 * h];
+                sum += x[2 * h];
+                sum += x[2 * h + 1];
+                __builtin_amdgcn_sched_barrier(0);
             }
-        } else {
-            for (uint32_t k = 0; k < cnt; ++k) sum += b[k];
+#pragma unroll
+            for (int h = 0; h < 16; ++h) {
+                x[2 * h] = bg[64 + 2 * h]; // next group (unused after the last)
+                x[2 * h + 1] = bg[65 + 2 * h];
+                sum += y[2 * h];
+                sum += y[2 * h + 1];
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         if (more)
             for (int u = 0; u < U; ++u) {
                 const uint32_t idx = nb + (uint32_t)u * 64 + lane;
-                if (idx < n) {
+                const bool ok = idx < n;
+                if (ok) {
                     hi = (hi < r[u]) ? r[u] : hi;
                     lo = (r[u] < lo) ? r[u] : lo;
                 }
-                buf[cur ^ 1][u * 64 + lane] = r[u];
+                buf[(cur ^ 1) * kSumChunk + u * 64 + lane] = ok ? r[u] : 0.0;
             }
         __syncthreads();
     }
@@ -404,8 +432,9 @@ __global__ void __launch_bounds__(256) k_raw_bbox(const double* __restrict__ v, 
     }
 }
 
-// 30-bit Morton codes of raw face-box centres, and the raw face boxes the
-// broad phase runs on.
+// 21-bit Morton codes (128 cells per axis) of raw face-box centres with the
+// mesh as bit 21, so that one sort orders the leaves of both trees, and the
+// raw face boxes the broad phase runs on.
 //
 // The broad phase runs in unshifted coordinates so that it overlaps the serial
 // coordinate sums that fix the shift.  The shift s(x) = ((x - com) + t) + 0 is
@@ -417,7 +446,7 @@ __global__ void __launch_bounds__(256) k_raw_bbox(const double* __restrict__ v, 
 __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __restrict__ idx,
                              const uint32_t* __restrict__ off, uint32_t nf, const unsigned long long* __restrict__ rbox,
                              uint32_t* __restrict__ code, uint32_t* __restrict__ face, double* __restrict__ rawbox,
-                             double tol0, double tol1, double tol2)
+                             double tol0, double tol1, double tol2, uint32_t mesh_bit)
 {
     const double tol[3] = {tol0, tol1, tol2};
     double lo[3], ext[3];
@@ -440,10 +469,10 @@ __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __res
         uint32_t c[3];
         for (int j = 0; j < 3; ++j) {
             float x = (float)((((mn[j] + mx[j]) * 0.5) - lo[j]) / ext[j]);
-            x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+            x = fminf(fmaxf(x * 128.0f, 0.0f), 127.0f);
             c[j] = expand_bits10((uint32_t)x);
         }
-        code[f] = c[0] * 4 + c[1] * 2 + c[2];
+        code[f] = (c[0] * 4 + c[1] * 2 + c[2]) | mesh_bit;
         face[f] = (uint32_t)f;
         double* o = rawbox + 6 * f;
         for (int j = 0; j < 3; ++j) {
@@ -539,12 +568,34 @@ struct Tree {
 
 // Wavefront-cooperative BVH x BVH traversal.  A 64-lane wave owns 64
 // Morton-consecutive source leaves; their union is the source-tree node at
-// depth D_s - 6.  The wave walks the cut tree breadth first with that box,
-// all lanes testing children in parallel and compacting the survivors into
-// an LDS frontier with ballots; at the leaf level every lane tests its own
-// source face against the surviving cut leaves.  A frontier that outgrows
-// LDS sends the wave to the per-lane stackless walk instead (same result).
-constexpr int kFrontierCap = 2048;
+// depth D_s - 6.  The wave walks the cut tree breadth first with that node's
+// box, all lanes testing children in parallel and compacting the survivors
+// into an LDS frontier with ballots; at the leaf level every lane tests its
+// own source face against the surviving cut leaves, staged through LDS.
+// When the frontier outgrows LDS (a node whose leaves are spread out), the
+// wave splits the source node and walks its two children in turn, depth first
+// and without a stack (the next node follows from index arithmetic); only a
+// single source leaf whose frontier still overflows takes a per-lane walk.
+// Found pairs are staged in LDS and flushed with one global atomic per
+// kTravStage - 64 pairs or so.
+constexpr int kFrontierCap = 1024;
+constexpr int kTravStage = 512;
+
+// writes the wave's staged pairs to the global list (all lanes call it)
+__device__ __forceinline__ void flush_pairs(const unsigned long long* stage, int sn, unsigned long long* pairs,
+                                            uint64_t cap, Counters* C)
+{
+    __syncthreads();
+    unsigned long long base = 0;
+    if (__lane_id() == 0) base = atomicAdd(&C->n_raw, (unsigned long long)sn);
+    base = __shfl(base, 0, 64);
+    for (int q = __lane_id(); q < sn; q += 64) {
+        const unsigned long long at = base + q;
+        if (at < cap) pairs[at] = stage[q];
+        else atomicOr(&C->ov_pairs, 1u);
+    }
+    __syncthreads();
+}
 
 __device__ __forceinline__ void per_lane_walk(const double* q, uint32_t s, const Tree& cut, unsigned long long* pairs,
                                               uint64_t cap, Counters* C)
@@ -585,103 +636,122 @@ __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsign
     __shared__ uint32_t front[2][kFrontierCap];
     __shared__ double lbox[64][6];
     __shared__ uint32_t lface[64];
+    __shared__ unsigned long long stage[kTravStage];
+    int sn = 0; // staged pairs (wave-uniform)
     const int lane = threadIdx.x;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int nwaves = (src.n + 63) / 64;
+    const int Ds = src.D;
+    const int lw = Ds >= 6 ? Ds - 6 : 0; // level of a wave's source node
+    const int nwaves = Ds >= 6 ? (src.n + 63) / 64 : 1;
+    const uint32_t leaf_base = (uint32_t)(1u << cut.D) - 1;
+    unsigned long long splits = 0, lane_walks = 0;
     for (int w = blockIdx.x; w < nwaves; w += gridDim.x) {
-        const int i = w * 64 + lane;
-        const bool valid = i < src.n;
-        double q[6];
-        uint32_t s = 0;
-        if (valid) {
-            s = src.sface[i];
-            for (int j = 0; j < 6; ++j) q[j] = src.facebox[6 * (uint64_t)s + j];
-        } else {
-            empty_box(q);
-        }
-        double wb[6];
-        {
-            const int lw = src.D >= 6 ? src.D - 6 : 0;
-            const uint32_t kw = src.D >= 6 ? (uint32_t)w : 0u;
-            const double* nb = src.nodes + 6 * ((uint64_t)(1u << lw) - 1 + kw);
-            for (int j = 0; j < 6; ++j) wb[j] = nb[j];
-        }
-        // breadth-first walk of the cut tree with the wave box
-        int cur = 0, nf = 0, level = 0;
-        bool overflow = false;
-        {
-            double rb[6];
-            for (int j = 0; j < 6; ++j) rb[j] = cut.nodes[j];
-            if (boxes_overlap(wb, rb)) {
-                if (lane == 0) front[0][0] = 0u;
-                nf = 1;
+        int l = lw;
+        uint32_t k = (uint32_t)w;
+        for (;;) {
+            // source node (l, k): leaves [k << span, (k + 1) << span), one per lane
+            const int span = Ds - l;
+            const uint32_t i = (k << span) + (uint32_t)lane;
+            const bool valid = lane < (1 << span) && i < (uint32_t)src.n;
+            double q[6];
+            uint32_t s = 0;
+            if (valid) {
+                s = src.sface[i];
+                for (int j = 0; j < 6; ++j) q[j] = src.facebox[6 * (uint64_t)s + j];
+            } else {
+                empty_box(q);
             }
-        }
-        __syncthreads();
-        while (nf > 0 && level < cut.D && !overflow) {
-            int nn = 0;
-            const int nchild = 2 * nf;
-            const uint32_t base_next = (uint32_t)(1u << (level + 1)) - 1;
-            for (int c0 = 0; c0 < nchild; c0 += 64) {
-                const int c = c0 + lane;
-                bool keep = false;
-                uint32_t child = 0;
-                if (c < nchild) {
-                    child = 2u * front[cur][c >> 1] + (uint32_t)(c & 1);
-                    const double* nb = cut.nodes + 6 * ((uint64_t)base_next + child);
-                    double b[6];
-                    for (int j = 0; j < 6; ++j) b[j] = nb[j];
-                    keep = boxes_overlap(wb, b);
+            double wb[6];
+            {
+                const double* nb = src.nodes + 6 * ((uint64_t)(1u << l) - 1 + k);
+                for (int j = 0; j < 6; ++j) wb[j] = nb[j];
+            }
+            // breadth-first walk of the cut tree with the node box
+            int cur = 0, nf = 0, level = 0;
+            bool overflow = false;
+            {
+                double rb[6];
+                for (int j = 0; j < 6; ++j) rb[j] = cut.nodes[j];
+                if (boxes_overlap(wb, rb)) {
+                    if (lane == 0) front[0][0] = 0u;
+                    nf = 1;
                 }
-                const uint64_t m = __ballot(keep);
-                const int pos = nn + __popcll(m & lt);
-                if (keep && pos < kFrontierCap) front[cur ^ 1][pos] = child;
-                nn += __popcll(m);
             }
             __syncthreads();
-            if (nn > kFrontierCap) overflow = true;
-            nf = nn;
-            cur ^= 1;
-            ++level;
-        }
-        if (lane == 0) {
-            atomicAdd(&C->n_visits, (unsigned long long)level);
-            if (overflow) atomicAdd(&C->n_fallback, 1ull);
-            else atomicAdd(&C->n_leaf_tests, (unsigned long long)nf);
-        }
-        if (overflow) {
-            if (valid) per_lane_walk(q, s, cut, pairs, cap, C);
-        } else if (nf > 0) {
-            // surviving cut leaves are staged through LDS 64 at a time (one
-            // coalesced load round per 64 leaves), then every lane tests its
-            // own source face against each staged box
-            const uint32_t leaf_base = (uint32_t)(1u << cut.D) - 1;
-            for (int t0 = 0; t0 < nf; t0 += 64) {
-                const int t = t0 + lane;
-                if (t < nf) {
-                    const uint32_t k = front[cur][t];
-                    const double* nb = cut.nodes + 6 * ((uint64_t)leaf_base + k);
-                    for (int j = 0; j < 6; ++j) lbox[lane][j] = nb[j];
-                    lface[lane] = cut.sface[k];
-                }
-                __syncthreads();
-                const int cnt = (nf - t0) < 64 ? (nf - t0) : 64;
-                for (int u = 0; u < cnt; ++u) {
-                    double b[6];
-                    for (int j = 0; j < 6; ++j) b[j] = lbox[u][j];
-                    if (valid && boxes_overlap(q, b)) {
-                        const uint32_t c = lface[u];
-                        const uint64_t key = ((uint64_t)s << 32) | c;
-                        const unsigned long long at = wave_append(&C->n_raw);
-                        if (at < cap) pairs[at] = key;
-                        else atomicOr(&C->ov_pairs, 1u);
+            while (nf > 0 && level < cut.D && !overflow) {
+                int nn = 0;
+                const int nchild = 2 * nf;
+                const uint32_t base_next = (uint32_t)(1u << (level + 1)) - 1;
+                for (int c0 = 0; c0 < nchild; c0 += 64) {
+                    const int c = c0 + lane;
+                    bool keep = false;
+                    uint32_t child = 0;
+                    if (c < nchild) {
+                        child = 2u * front[cur][c >> 1] + (uint32_t)(c & 1);
+                        const double* nb = cut.nodes + 6 * ((uint64_t)base_next + child);
+                        double b[6];
+                        for (int j = 0; j < 6; ++j) b[j] = nb[j];
+                        keep = boxes_overlap(wb, b);
                     }
+                    const uint64_t m = __ballot(keep);
+                    const int pos = nn + __popcll(m & lt);
+                    if (keep && pos < kFrontierCap) front[cur ^ 1][pos] = child;
+                    nn += __popcll(m);
                 }
                 __syncthreads();
+                if (nn > kFrontierCap) overflow = true;
+                nf = nn;
+                cur ^= 1;
+                ++level;
             }
+            if (overflow && span > 0) {
+                ++splits; // walk the two halves of this source node instead
+                ++l;
+                k <<= 1;
+                continue;
+            }
+            if (overflow) {
+                ++lane_walks;
+                if (valid) per_lane_walk(q, s, cut, pairs, cap, C);
+            } else if (nf > 0) {
+                for (int t0 = 0; t0 < nf; t0 += 64) {
+                    const int t = t0 + lane;
+                    if (t < nf) {
+                        const uint32_t kk = front[cur][t];
+                        const double* nb = cut.nodes + 6 * ((uint64_t)leaf_base + kk);
+                        for (int j = 0; j < 6; ++j) lbox[lane][j] = nb[j];
+                        lface[lane] = cut.sface[kk];
+                    }
+                    __syncthreads();
+                    const int cnt = (nf - t0) < 64 ? (nf - t0) : 64;
+                    for (int u = 0; u < cnt; ++u) {
+                        if (sn > kTravStage - 64) {
+                            flush_pairs(stage, sn, pairs, cap, C);
+                            sn = 0;
+                        }
+                        double b[6];
+                        for (int j = 0; j < 6; ++j) b[j] = lbox[u][j];
+                        const bool hit = valid && boxes_overlap(q, b);
+                        const uint64_t m = __ballot(hit);
+                        if (hit) stage[sn + __popcll(m & lt)] = ((uint64_t)s << 32) | lface[u];
+                        sn += __popcll(m);
+                    }
+                    __syncthreads();
+                }
+            }
+            __syncthreads();
+            // next source node: climb past right children, then the sibling
+            while (l > lw && (k & 1u)) {
+                k >>= 1;
+                --l;
+            }
+            if (l == lw) break;
+            ++k;
         }
-        __syncthreads();
     }
+    if (sn > 0) flush_pairs(stage, sn, pairs, cap, C);
+    if (lane == 0 && splits) atomicAdd(&C->n_fallback, splits); // rare: statistics only
+    if (lane == 0 && lane_walks) atomicAdd(&C->n_lane_walks, lane_walks);
 }
 
 // ===========================================================================
@@ -693,29 +763,68 @@ __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsign
 // pair is appended to the candidate-pair list and the pair hash used for test
 // dedup, and each of its faces is flagged once and appended to the candidate
 // face list.
-__global__ void k_filter_pairs(const double* __restrict__ box, const unsigned long long* __restrict__ raw, uint64_t cap,
-                               uint32_t Fs, unsigned long long* __restrict__ pairs, unsigned long long* __restrict__ phash,
-                               uint64_t pmask, uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_list, Counters* C)
+// The pair hash is sized from the raw pair count: twice the pairs it can
+// receive, rounded up to a power of two, within the allocated table.
+__device__ __forceinline__ uint64_t pair_hash_slots(const Counters* C, uint64_t cap, uint64_t table)
 {
     const uint64_t n = C->n_raw < cap ? C->n_raw : cap;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = raw[i];
-        const uint32_t f[2] = {(uint32_t)(k >> 32), Fs + (uint32_t)k};
-        double a[6], b[6];
-        for (int j = 0; j < 6; ++j) {
-            a[j] = box[6 * (uint64_t)f[0] + j];
-            b[j] = box[6 * (uint64_t)f[1] + j];
-        }
-        if (!boxes_overlap(a, b)) continue;
-        const unsigned long long at = wave_append(&C->n_pairs);
-        pairs[at] = k; // at < n <= cap
-        hash_insert(phash, pmask, k);
-        for (int side = 0; side < 2; ++side) {
-            if (cand[f[side]] == 0u && atomicExch(&cand[f[side]], 1u) == 0u) {
-                const unsigned long long c = wave_append(&C->n_cand);
-                cand_list[c] = f[side]; // at most one entry per face
+    uint64_t m = 1024;
+    while (m < 2 * n) m <<= 1;
+    return m < table ? m : table;
+}
+
+__global__ void k_pair_hash_init(unsigned long long* __restrict__ phash, uint64_t cap, uint64_t table, Counters* C)
+{
+    const uint64_t m = pair_hash_slots(C, cap, table);
+    if (blockIdx.x == 0 && threadIdx.x == 0) C->pmask = m - 1;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        phash[i] = kEmpty64;
+}
+
+constexpr int kFilterItems = 4;
+constexpr int kFilterTile = kBlock * kFilterItems;
+
+__global__ void __launch_bounds__(kBlock) k_filter_pairs(const double* __restrict__ box,
+                                                         const unsigned long long* __restrict__ raw, uint64_t cap,
+                                                         uint32_t Fs, unsigned long long* __restrict__ pairs,
+                                                         unsigned long long* __restrict__ phash,
+                                                         uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_list,
+                                                         Counters* C)
+{
+    __shared__ unsigned long long st_pair[kFilterTile];
+    __shared__ uint32_t st_face[2 * kFilterTile];
+    __shared__ uint32_t st_np, st_nf;
+    __shared__ unsigned long long st_base;
+    const uint64_t n = C->n_raw < cap ? C->n_raw : cap;
+    const uint64_t pmask = C->pmask;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * kFilterTile; t0 < n; t0 += (uint64_t)gridDim.x * kFilterTile) {
+        if (threadIdx.x == 0) st_np = st_nf = 0;
+        __syncthreads();
+        for (int rr = 0; rr < kFilterItems; ++rr) {
+            const uint64_t i = t0 + (uint64_t)rr * kBlock + threadIdx.x;
+            if (i >= n) continue;
+            const uint64_t k = raw[i];
+            const uint32_t f[2] = {(uint32_t)(k >> 32), Fs + (uint32_t)k};
+            double a[6], b[6];
+            for (int j = 0; j < 6; ++j) {
+                a[j] = box[6 * (uint64_t)f[0] + j];
+                b[j] = box[6 * (uint64_t)f[1] + j];
             }
+            if (!boxes_overlap(a, b)) continue;
+            st_pair[wave_reserve(&st_np)] = k; // one per item: the stage cannot overflow
+            hash_insert(phash, pmask, k);
+            for (int side = 0; side < 2; ++side)
+                if (cand[f[side]] == 0u && atomicExch(&cand[f[side]], 1u) == 0u)
+                    st_face[wave_reserve(&st_nf)] = f[side]; // at most two per item
         }
+        __syncthreads();
+        const uint32_t np = st_np, nfc = st_nf;
+        const unsigned long long pb = block_reserve(&C->n_pairs, np, &st_base);
+        for (uint32_t q = threadIdx.x; q < np; q += kBlock) pairs[pb + q] = st_pair[q]; // pb + np <= n <= cap
+        __syncthreads();
+        const unsigned long long fb = block_reserve(&C->n_cand, nfc, &st_base);
+        for (uint32_t q = threadIdx.x; q < nfc; q += kBlock) cand_list[fb + q] = st_face[q]; // one entry per face
+        __syncthreads();
     }
 }
 
@@ -961,44 +1070,79 @@ __device__ __forceinline__ bool edge_face_cull(const Soup& S, uint32_t e, uint32
 }
 
 // six work items per pair: (side, corner slot); a polygon's corners beyond
-// the third are taken by the same slot in steps of three
+// the third are taken by the same slot in steps of three.  A block takes a
+// tile of kTestTile items and stages the tests it emits in LDS.
+constexpr int kTestItems = 4;
+constexpr int kTestTile = kBlock * kTestItems;
+constexpr int kTestStage = 2048;
+
 __global__ void __launch_bounds__(kBlock) k_tests(Soup S, const unsigned long long* __restrict__ pairs, uint64_t cap_pairs,
-                                                  const unsigned long long* __restrict__ phash, uint64_t pmask,
+                                                  const unsigned long long* __restrict__ phash,
                                                   uint32_t* __restrict__ te, uint32_t* __restrict__ tg, uint64_t cap_tests,
                                                   Counters* C)
 {
+    __shared__ uint32_t st_e[kTestStage], st_g[kTestStage];
+    __shared__ uint32_t st_n;
+    __shared__ unsigned long long st_base;
     const uint64_t n = C->n_pairs < cap_pairs ? C->n_pairs : cap_pairs;
+    const uint64_t pmask = C->pmask;
     const uint32_t Fs = S.nf[0];
-    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < 6 * n; w += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t i = w / 6;
-        const uint32_t slot = (uint32_t)(w - 6 * i);
-        const int side = slot >= 3;
-        const uint32_t j0 = side ? slot - 3 : slot;
-        const uint64_t key = pairs[i];
-        const uint32_t s = (uint32_t)(key >> 32), c = (uint32_t)key;
-        const uint32_t f = side ? c : s;                  // face whose edges are tested
-        const uint32_t self = side ? Fs + c : s;          // its ps id
-        const uint32_t g = side ? s : Fs + c;             // the other face (ps id)
-        const uint32_t b = S.off[side][f], nn = S.off[side][f + 1] - b;
-        for (uint32_t k = j0; k < nn; k += 3) {
-            const uint32_t e = S.cedge[side][b + k];
-            const int32_t f0 = S.ef0[e], f1 = S.ef1[e];
-            const int32_t fo = (f0 == (int32_t)self) ? f1 : f0;
-            // the lowest-numbered face of e paired with g emits the test
-            if (fo >= 0 && (uint32_t)fo < self) {
-                const bool paired = side ? pair_exists(phash, pmask, s, (uint32_t)fo - Fs)
-                                         : pair_exists(phash, pmask, (uint32_t)fo, c);
-                if (paired) continue;
+    const uint64_t items = 6 * n;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * kTestTile; t0 < items; t0 += (uint64_t)gridDim.x * kTestTile) {
+        if (threadIdx.x == 0) st_n = 0;
+        __syncthreads();
+        for (int rr = 0; rr < kTestItems; ++rr) {
+            const uint64_t w = t0 + (uint64_t)rr * kBlock + threadIdx.x;
+            if (w >= items) continue;
+            const uint64_t i = w / 6;
+            const uint32_t slot = (uint32_t)(w - 6 * i);
+            const int side = slot >= 3;
+            const uint32_t j0 = side ? slot - 3 : slot;
+            const uint64_t key = pairs[i];
+            const uint32_t s = (uint32_t)(key >> 32), c = (uint32_t)key;
+            const uint32_t f = side ? c : s;         // face whose edges are tested
+            const uint32_t self = side ? Fs + c : s; // its ps id
+            const uint32_t g = side ? s : Fs + c;    // the other face (ps id)
+            const uint32_t b = S.off[side][f], nn = S.off[side][f + 1] - b;
+            for (uint32_t k = j0; k < nn; k += 3) {
+                const uint32_t e = S.cedge[side][b + k];
+                const int32_t f0 = S.ef0[e], f1 = S.ef1[e];
+                const int32_t fo = (f0 == (int32_t)self) ? f1 : f0;
+                // the lowest-numbered face of e paired with g emits the test
+                if (fo >= 0 && (uint32_t)fo < self) {
+                    const bool paired = side ? pair_exists(phash, pmask, s, (uint32_t)fo - Fs)
+                                             : pair_exists(phash, pmask, (uint32_t)fo, c);
+                    if (paired) continue;
+                }
+                if (!edge_face_cull(S, e, g)) continue;
+                const uint32_t at = wave_reserve(&st_n);
+                if (at < (uint32_t)kTestStage) {
+                    st_e[at] = e;
+                    st_g[at] = g;
+                } else { // more than the stage holds (polygons): append directly
+                    const unsigned long long ga = wave_append(&C->n_tests);
+                    if (ga < cap_tests) {
+                        te[ga] = e;
+                        tg[ga] = g;
+                    } else {
+                        atomicOr(&C->ov_tests, 1u);
+                    }
+                }
             }
-            if (!edge_face_cull(S, e, g)) continue;
-            const unsigned long long at = wave_append(&C->n_tests);
+        }
+        __syncthreads();
+        const uint32_t m = st_n < (uint32_t)kTestStage ? st_n : (uint32_t)kTestStage;
+        const unsigned long long base = block_reserve(&C->n_tests, m, &st_base);
+        for (uint32_t q = threadIdx.x; q < m; q += kBlock) {
+            const unsigned long long at = base + q;
             if (at < cap_tests) {
-                te[at] = e;
-                tg[at] = g;
+                te[at] = st_e[q];
+                tg[at] = st_g[q];
             } else {
                 atomicOr(&C->ov_tests, 1u);
             }
         }
+        __syncthreads();
     }
 }
 
@@ -1014,15 +1158,31 @@ struct TestOut {
 };
 
 struct RegOut {
-    unsigned long long* key;
+    unsigned long long* key; // (edge << gbits) | face
     double* p;
+    uint32_t* val;           // sort payload: the entry's own index
     uint64_t cap;
     int gbits;
 };
 
-// the decision after the two orient3d signs are known
-__device__ inline void resolve_test(const Soup& S, const Plane* planes, uint32_t e, uint32_t g, int sq, int sr,
-                                    uint64_t ti, const TestOut& TO, const RegOut& R, Counters* C)
+__device__ __forceinline__ void store_registry(const RegOut& R, unsigned long long at, unsigned long long key,
+                                               const double* p, Counters* C)
+{
+    if (at < R.cap) {
+        R.key[at] = key;
+        R.p[3 * at] = p[0];
+        R.p[3 * at + 1] = p[1];
+        R.p[3 * at + 2] = p[2];
+        R.val[at] = (uint32_t)at;
+    } else {
+        atomicOr(&C->ov_reg, 1u);
+    }
+}
+
+// the decision after the two orient3d signs are known; true when (e, g)
+// enters the registry with intersection point p
+__device__ inline bool resolve_test(const Soup& S, const Plane* planes, uint32_t e, uint32_t g, int sq, int sr,
+                                    uint64_t ti, const TestOut& TO, Counters* C, double* p)
 {
     const char ty = segment_type(sq, sr);
     if (TO.qs) {
@@ -1031,7 +1191,7 @@ __device__ inline void resolve_test(const Soup& S, const Plane* planes, uint32_t
         TO.type[ti] = ty;
         TO.pip[ti] = '-';
     }
-    if (ty == '0') return;
+    if (ty == '0') return false;
     const Plane pl = planes[g];
     const FaceView fv = face_view(S, g);
     const double* qv = S.ps_v + 3 * (uint64_t)S.ev0[e]; // source(h0)
@@ -1057,64 +1217,86 @@ __device__ inline void resolve_test(const Soup& S, const Plane* planes, uint32_t
         }
         if (TO.qs) TO.pip[ti] = res;
         if (violated) atomicOr(&C->gp, 1u);
-        return;
+        return false;
     }
-    double p[3] = {0.0, 0.0, 0.0};
+    p[0] = p[1] = p[2] = 0.0;
     segment_point(pl.n, pl.d, q, r, p);
     const char res = point_in_polygon(p, (int)fv.n, fv, pl.n, pl.k);
     if (TO.qs) TO.pip[ti] = res;
     if (res == 'v' || res == 'e') {
         atomicOr(&C->gp, 1u);
-        return;
+        return false;
     }
-    if (res != 'i') return;
-    const unsigned long long at = wave_append(&C->n_reg);
-    if (at < R.cap) {
-        R.key[at] = ((unsigned long long)e << R.gbits) | g;
-        R.p[3 * at] = p[0];
-        R.p[3 * at + 1] = p[1];
-        R.p[3 * at + 2] = p[2];
-    } else {
-        atomicOr(&C->ov_reg, 1u);
-    }
+    return res == 'i';
 }
+
+// A block takes a tile of kNarrowTile tests; registry entries and escalated
+// tests (at most one of either per test) are staged in LDS.
+constexpr int kNarrowItems = 4;
+constexpr int kNarrowTile = kBlock * kNarrowItems;
 
 __global__ void __launch_bounds__(kBlock) k_narrow(Soup S, const Plane* __restrict__ planes, const uint32_t* __restrict__ te,
                                                    const uint32_t* __restrict__ tg, uint64_t cap_tests,
                                                    uint32_t* __restrict__ esc, uint64_t cap_esc, TestOut TO, RegOut R,
                                                    Counters* C)
 {
+    __shared__ unsigned long long st_key[kNarrowTile];
+    __shared__ double st_p[3 * kNarrowTile];
+    __shared__ uint32_t st_esc[kNarrowTile];
+    __shared__ uint32_t st_nr, st_ne;
+    __shared__ unsigned long long st_base;
     const uint64_t n = C->n_tests < cap_tests ? C->n_tests : cap_tests;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t e = te[i], g = tg[i];
-        const Plane pl = planes[g];
-        int sq = 0, sr = 0;
-        if (pl.tri[0] == 0xff || pl.k < 0) {
-            // all polygon vertices collinear: the type is '0' (math.cpp:404-410)
-            if (TO.qs) {
-                TO.qs[i] = 0;
-                TO.rs[i] = 0;
-                TO.type[i] = '0';
-                TO.pip[i] = '-';
+    for (uint64_t t0 = (uint64_t)blockIdx.x * kNarrowTile; t0 < n; t0 += (uint64_t)gridDim.x * kNarrowTile) {
+        if (threadIdx.x == 0) st_nr = st_ne = 0;
+        __syncthreads();
+        for (int rr = 0; rr < kNarrowItems; ++rr) {
+            const uint64_t i = t0 + (uint64_t)rr * kBlock + threadIdx.x;
+            if (i >= n) continue;
+            const uint32_t e = te[i], g = tg[i];
+            const Plane pl = planes[g];
+            if (pl.tri[0] == 0xff || pl.k < 0) {
+                // all polygon vertices collinear: the type is '0' (math.cpp:404-410)
+                if (TO.qs) {
+                    TO.qs[i] = 0;
+                    TO.rs[i] = 0;
+                    TO.type[i] = '0';
+                    TO.pip[i] = '-';
+                }
+                continue;
             }
-            continue;
+            const FaceView fv = face_view(S, g);
+            double a[3], b[3], c[3], q[3], r[3];
+            fv(pl.tri[0], a);
+            fv(pl.tri[1], b);
+            fv(pl.tri[2], c);
+            load3(S.ps_v + 3 * (uint64_t)S.ev0[e], q);
+            load3(S.ps_v + 3 * (uint64_t)S.ev1[e], r);
+            const int sq = orient3d_filter(a, b, c, q);
+            const int sr = orient3d_filter(a, b, c, r);
+            if (sq == 0 || sr == 0) {
+                st_esc[wave_reserve(&st_ne)] = (uint32_t)i;
+                continue;
+            }
+            double p[3];
+            if (resolve_test(S, planes, e, g, sq, sr, i, TO, C, p)) {
+                const uint32_t at = wave_reserve(&st_nr);
+                st_key[at] = ((unsigned long long)e << R.gbits) | g;
+                st_p[3 * at] = p[0];
+                st_p[3 * at + 1] = p[1];
+                st_p[3 * at + 2] = p[2];
+            }
         }
-        const FaceView fv = face_view(S, g);
-        double a[3], b[3], c[3], q[3], r[3];
-        fv(pl.tri[0], a);
-        fv(pl.tri[1], b);
-        fv(pl.tri[2], c);
-        load3(S.ps_v + 3 * (uint64_t)S.ev0[e], q);
-        load3(S.ps_v + 3 * (uint64_t)S.ev1[e], r);
-        sq = orient3d_filter(a, b, c, q);
-        sr = orient3d_filter(a, b, c, r);
-        if (sq == 0 || sr == 0) {
-            const unsigned long long at = wave_append(&C->n_esc);
-            if (at < cap_esc) esc[at] = (uint32_t)i;
+        __syncthreads();
+        const uint32_t nr = st_nr, ne = st_ne;
+        const unsigned long long rb = block_reserve(&C->n_reg, nr, &st_base);
+        for (uint32_t q = threadIdx.x; q < nr; q += kBlock) store_registry(R, rb + q, st_key[q], st_p + 3 * q, C);
+        __syncthreads();
+        const unsigned long long eb = block_reserve(&C->n_esc, ne, &st_base);
+        for (uint32_t q = threadIdx.x; q < ne; q += kBlock) {
+            if (eb + q < cap_esc) esc[eb + q] = st_esc[q];
             else atomicOr(&C->ov_esc, 1u);
-            continue;
         }
-        resolve_test(S, planes, e, g, sq, sr, i, TO, R, C);
+        __syncthreads();
     }
 }
 
@@ -1139,20 +1321,15 @@ __global__ void __launch_bounds__(64) k_narrow_exact(Soup S, const Plane* __rest
         int sr = orient3d_filter(a, b, c, r);
         if (sq == 0) sq = orient3d_exact_sign(a, b, c, q);
         if (sr == 0) sr = orient3d_exact_sign(a, b, c, r);
-        resolve_test(S, planes, e, g, sq, sr, i, TO, R, C);
+        double p[3];
+        if (resolve_test(S, planes, e, g, sq, sr, i, TO, C, p))
+            store_registry(R, wave_append(&C->n_reg), ((unsigned long long)e << R.gbits) | g, p, C);
     }
 }
 
 // ===========================================================================
 // 9. registry: sorted order, then the status rules that read it
 // ===========================================================================
-
-__global__ void k_iota(uint32_t* v, const unsigned long long* n_dev, uint64_t cap)
-{
-    const uint64_t n = *n_dev < cap ? *n_dev : cap;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        v[i] = (uint32_t)i;
-}
 
 __global__ void k_clamp_count(const unsigned long long* n_dev, uint64_t cap, unsigned long long* out)
 {

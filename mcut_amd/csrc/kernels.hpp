@@ -31,7 +31,8 @@ struct Counters {
     unsigned long long n_pairs, n_tests, n_esc, n_reg, n_big, n_cand;
     unsigned long long n_edges_src, n_edges_cut;
     unsigned long long n_raw;                              // raw-space (superset) candidate pairs
-    unsigned long long n_visits, n_fallback, n_leaf_tests; // traversal statistics
+    unsigned long long n_fallback, n_lane_walks; // traversal statistics: node splits, per-lane walks
+    unsigned long long pmask; // pair hash slots - 1 (set once the raw pairs are known)
     unsigned int ov_pairs, ov_tests, ov_esc, ov_reg, ov_big;
     unsigned int gp, bad_src, bad_cut, unsupported, nonmanifold, bad_input, pair_fail, partial;
     unsigned int pad[2];
@@ -153,6 +154,35 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long* co
     if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(active));
     base = __shfl(base, leader, 64);
     return base + rank;
+}
+
+// Appends to a global list cost one same-address atomic each, and those
+// serialise device-wide (about 12 ns apiece on MI355X).  Kernels that emit
+// many entries therefore stage them in LDS: a wave reserves staging slots with
+// one LDS atomic (wave_reserve), and the block flushes the staged entries with
+// one global atomic for the whole tile.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* lds_counter)
+{
+    const uint64_t active = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)active) - 1;
+    const uint32_t rank = __popcll(active & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(lds_counter, (uint32_t)__popcll(active));
+    base = __shfl(base, leader, 64);
+    return base + rank;
+}
+
+// Block-wide: reserves `m` entries of a global list (one atomic, by thread 0)
+// and returns the first index to every thread.  All threads of the block call
+// it; `slot` is an LDS word.
+__device__ __forceinline__ unsigned long long block_reserve(unsigned long long* counter, uint32_t m,
+                                                            unsigned long long* slot)
+{
+    if (threadIdx.x == 0) *slot = m ? atomicAdd(counter, (unsigned long long)m) : 0ull;
+    __syncthreads();
+    const unsigned long long base = *slot;
+    return base;
 }
 
 } // namespace mcgpu

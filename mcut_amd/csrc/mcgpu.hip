@@ -22,11 +22,15 @@ struct Buf {
     size_t cap = 0;
 };
 
-// Stages of one dispatch.  Stream A carries the chain that depends on the
-// shift (serial coordinate sums first); stream B builds both trees and runs
-// the broad phase in raw coordinates alongside it; stream C numbers the
-// polygon-soup edges.  A joins B before the exact pair filter and C before
-// the edge-face tests.
+// Stages of one dispatch.  The serial coordinate sums run alone on a stream
+// whose CU mask reserves kSumCUs compute units for them; stream A carries the
+// chain that depends on the shift; stream B builds both trees and runs the
+// broad phase in raw coordinates alongside the sums; stream C numbers the
+// polygon-soup edges.  B and C are masked off the reserved CUs: a serial FP64
+// chain that shares its CU with memory- or atomic-heavy waves runs 4-8 times
+// slower.  A joins B before the exact pair filter and C before the edge-face
+// tests.
+constexpr int kSumCUs = 8;
 enum Stage {
     ST_SUMS,      // A: serial coordinate sums and raw boxes (k_mesh_sums alone)
     ST_PARAMS,    // A: shift parameters
@@ -48,7 +52,7 @@ const char* kStageNames[ST_COUNT] = {"coordinate_sums", "shift_params", "shift_b
                                      "face_planes",     "tests",        "narrow",      "registry"};
 
 // event slots: stage boundaries on the three streams
-enum Ev { EV_A0, EV_A_SUMS, EV_A_PARAMS, EV_A_SHIFT, EV_A_JOIN, EV_A_FILTER, EV_A_PLANES, EV_A_JOIN2, EV_A_TESTS, EV_A_NARROW,
+enum Ev { EV_A0, EV_S0, EV_A_SUMS, EV_A_PARAMS, EV_A_SHIFT, EV_A_JOIN, EV_A_FILTER, EV_A_PLANES, EV_A_JOIN2, EV_A_TESTS, EV_A_NARROW,
           EV_A_END, EV_FORK, EV_B0, EV_B_SORT, EV_B_REFIT, EV_B_TRAV, EV_C0, EV_C_FILL, EV_C_TOPO, EV_COUNT };
 
 uint64_t next_pow2(uint64_t x)
@@ -69,7 +73,7 @@ int bits_for(uint64_t n)
 
 struct mcgpu_context_t {
     int device = 0;
-    hipStream_t s = nullptr, s2 = nullptr, s3 = nullptr;
+    hipStream_t s = nullptr, s2 = nullptr, s3 = nullptr, ssum = nullptr;
     hipEvent_t ev[EV_COUNT];
     hipError_t last_error = hipSuccess;
 
@@ -77,7 +81,7 @@ struct mcgpu_context_t {
     Buf off[2], params, counters, ps_v, box;
     Buf hkeys[2], hfirst[2], eflag[2], eid[2], cedge[2], cface[2], etotal;
     Buf ev0, ev1, ef0, ef1;
-    Buf mcode[2], mface[2], mcode2[2], mface2[2];
+    Buf mcode, mface, mcode2, mface2; // Morton keys and leaf order of both meshes
     Buf nodes[2], rawbox;
     Buf raw_pairs, pairs, phash, cand, cand_list, planes, big_list, big_scratch;
     Buf te, tg, tqs, trs, ttype, tpip, esc;
@@ -98,7 +102,7 @@ struct mcgpu_context_t {
     uint32_t flags = 0, enforcement_count = 0;
     double gp_constant = 1e-4;
     int keep_tests = 0;
-    bool mface_alt[2] = {false, false};
+    bool mface_alt = false;
     int depth[2] = {0, 0};
     double tol[3] = {0, 0, 0}; // raw-space broad-phase tolerance per axis
     bool reg_alt = false;
@@ -249,16 +253,17 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
         CK(ensure(ctx->eflag[m], nk, &u));
         CK(ensure(ctx->eid[m], nk, &u));
         CK(ensure(ctx->cedge[m], nk, &u));
-        CK(ensure(ctx->mcode[m], nf, &u));
-        CK(ensure(ctx->mface[m], nf, &u));
-        CK(ensure(ctx->mcode2[m], nf, &u));
-        CK(ensure(ctx->mface2[m], nf, &u));
         int D = 0;
         while ((1ull << D) < nf) ++D;
         ctx->depth[m] = D;
         CK(ensure(ctx->nodes[m], 6 * ((2ull << D) - 1), &dp));
         scan_need = std::max<uint64_t>(scan_need, scan_tmp_elems(nf + 8));
     }
+    CK(ensure(ctx->mcode, F, &u));
+    CK(ensure(ctx->mface, F, &u));
+    CK(ensure(ctx->mcode2, F, &u));
+    CK(ensure(ctx->mface2, F, &u));
+    scan_need = std::max<uint64_t>(scan_need, scan_tmp_elems(F + 8));
     CK(ensure(ctx->cand, F + 8, &u));
     CK(ensure(ctx->cand_list, F + 8, &u));
     Plane* pl;
@@ -302,7 +307,7 @@ mcgpu_result alloc_list_buffers(mcgpu_context ctx)
     CK(ensure(ctx->big_list, ctx->cap_big, &u));
     Trip* tr;
     CK(ensure(ctx->big_scratch, 256ull * 560, &tr));
-    const uint64_t sort_cap = std::max<uint64_t>({ctx->cap_reg, (uint64_t)ctx->nf[0], (uint64_t)ctx->nf[1]});
+    const uint64_t sort_cap = std::max<uint64_t>(ctx->cap_reg, (uint64_t)ctx->nf[0] + ctx->nf[1]);
     CK(ensure(ctx->hist, radix_hist_elems(sort_cap), &u));
     // the scan scratch also serves the mesh-sized scans (sized in alloc_mesh_buffers)
     const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
@@ -326,16 +331,15 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
     (void)hipEventRecord(ctx->ev[EV_A_PLANES], s);
     if (join_topology) (void)hipStreamWaitEvent(s, ctx->ev[EV_C_TOPO], 0);
     (void)hipEventRecord(ctx->ev[EV_A_JOIN2], s);
-    const uint64_t pmask = next_pow2(2 * ctx->cap_pairs) - 1;
-    k_tests<<<grid_for(6 * ctx->cap_pairs), kBlock, 0, s>>>(S, P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
-                                                        P_<unsigned long long>(ctx->phash), pmask, P_<uint32_t>(ctx->te),
+    k_tests<<<grid_for(6 * ctx->cap_pairs, kTestTile, 2048), kBlock, 0, s>>>(S, P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
+                                                        P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->te),
                                                         P_<uint32_t>(ctx->tg), ctx->cap_tests, C);
     (void)hipEventRecord(ctx->ev[EV_A_TESTS], s);
     TestOut TO{nullptr, nullptr, nullptr, nullptr};
     if (ctx->keep_tests)
         TO = TestOut{P_<int8_t>(ctx->tqs), P_<int8_t>(ctx->trs), P_<char>(ctx->ttype), P_<char>(ctx->tpip)};
-    RegOut R{P_<unsigned long long>(ctx->rkey), P_<double>(ctx->rp), ctx->cap_reg, ctx->gbits};
-    k_narrow<<<grid_for(ctx->cap_tests), kBlock, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
+    RegOut R{P_<unsigned long long>(ctx->rkey), P_<double>(ctx->rp), P_<uint32_t>(ctx->rval), ctx->cap_reg, ctx->gbits};
+    k_narrow<<<grid_for(ctx->cap_tests, kNarrowTile, 2048), kBlock, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
                                                          P_<uint32_t>(ctx->tg), ctx->cap_tests, P_<uint32_t>(ctx->esc),
                                                          ctx->cap_esc, TO, R, C);
     k_narrow_exact<<<grid_for(ctx->cap_esc, 64, 2048), 64, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
@@ -344,7 +348,6 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
     (void)hipEventRecord(ctx->ev[EV_A_NARROW], s);
     unsigned long long* rcount = P_<unsigned long long>(ctx->rcount);
     k_clamp_count<<<1, 1, 0, s>>>(&C->n_reg, ctx->cap_reg, rcount);
-    k_iota<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(P_<uint32_t>(ctx->rval), rcount, ctx->cap_reg);
     RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
     const bool alt = radix_sort<unsigned long long>(P_<unsigned long long>(ctx->rkey), P_<uint32_t>(ctx->rval),
                                                     P_<unsigned long long>(ctx->rkey2), P_<uint32_t>(ctx->rval2),
@@ -398,8 +401,11 @@ void enqueue_frontend(mcgpu_context ctx)
     (void)hipEventRecord(ctx->ev[EV_A0], s);
     fill_u32(C, 0u, offsetof(Counters, rbox) / 4, s); // keeps the raw boxes of the validation pass
     (void)hipEventRecord(ctx->ev[EV_FORK], s);
-    k_mesh_sums<<<6, 64, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd);
-    (void)hipEventRecord(ctx->ev[EV_A_SUMS], s);
+    (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_FORK], 0);
+    (void)hipEventRecord(ctx->ev[EV_S0], ctx->ssum);
+    k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd);
+    (void)hipEventRecord(ctx->ev[EV_A_SUMS], ctx->ssum);
+    (void)hipStreamWaitEvent(s, ctx->ev[EV_A_SUMS], 0);
     const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
     k_shift_params<<<1, 1, 0, s>>>(Pd, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
     (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
@@ -416,22 +422,24 @@ void enqueue_frontend(mcgpu_context ctx)
     double* rawbox = P_<double>(ctx->rawbox);
     for (int m = 0; m < 2; ++m) {
         const uint32_t nf = ctx->nf[m];
+        const uint64_t o = m ? Fs : 0;
         const double* tl = m ? ctx->tol : nullptr;
         k_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf,
-                                                      C->rbox[m], P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]),
-                                                      rawbox + 6 * (uint64_t)(m ? Fs : 0), tl ? tl[0] : 0.0,
-                                                      tl ? tl[1] : 0.0, tl ? tl[2] : 0.0);
-        ctx->mface_alt[m] = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode[m]), P_<uint32_t>(ctx->mface[m]),
-                                                 P_<uint32_t>(ctx->mcode2[m]), P_<uint32_t>(ctx->mface2[m]), nullptr,
-                                                 nf, nf, 30, rt, s2);
+                                                      C->rbox[m], P_<uint32_t>(ctx->mcode) + o,
+                                                      P_<uint32_t>(ctx->mface) + o, rawbox + 6 * o, tl ? tl[0] : 0.0,
+                                                      tl ? tl[1] : 0.0, tl ? tl[2] : 0.0, m ? (1u << 21) : 0u);
     }
+    // one stable sort orders the leaves of both trees (the mesh is the top key bit)
+    ctx->mface_alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode), P_<uint32_t>(ctx->mface), P_<uint32_t>(ctx->mcode2),
+                                          P_<uint32_t>(ctx->mface2), nullptr, F, F, 22, rt, s2);
     (void)hipEventRecord(ctx->ev[EV_B_SORT], s2);
+    const uint32_t* sorted = P_<uint32_t>(ctx->mface_alt ? ctx->mface2 : ctx->mface);
     Tree T[2];
     for (int m = 0; m < 2; ++m) {
         const uint32_t nf = ctx->nf[m];
         const int D = ctx->depth[m];
         const double* fbox = rawbox + 6 * (uint64_t)(m ? Fs : 0);
-        const uint32_t* sface = P_<uint32_t>(ctx->mface_alt[m] ? ctx->mface2[m] : ctx->mface[m]);
+        const uint32_t* sface = sorted + (m ? Fs : 0);
         double* nodes = P_<double>(ctx->nodes[m]);
         int lin = D;
         do {
@@ -457,8 +465,6 @@ void enqueue_frontend(mcgpu_context ctx)
     (void)hipEventRecord(ctx->ev[EV_C0], s3);
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
     const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
-    const uint64_t ps = next_pow2(2 * ctx->cap_pairs);
-    fill_u64(ctx->phash.p, kEmpty64, ps, s3);
     fill_u32(ctx->cand.p, 0u, F, s3);
     (void)hipEventRecord(ctx->ev[EV_C_FILL], s3);
     fill_u32(ctx->ef1.p, 0xffffffffu, K, s3);
@@ -498,10 +504,13 @@ void enqueue_frontend(mcgpu_context ctx)
     (void)hipStreamWaitEvent(s, ctx->ev[EV_B_TRAV], 0);
     (void)hipStreamWaitEvent(s, ctx->ev[EV_C_FILL], 0);
     (void)hipEventRecord(ctx->ev[EV_A_JOIN], s);
-    k_filter_pairs<<<grid_for(ctx->cap_pairs), kBlock, 0, s>>>(P_<double>(ctx->box), P_<unsigned long long>(ctx->raw_pairs),
+    const uint64_t table = next_pow2(2 * ctx->cap_pairs);
+    k_pair_hash_init<<<grid_for(table, kBlock, 1024), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash), ctx->cap_pairs,
+                                                                      table, C);
+    k_filter_pairs<<<grid_for(ctx->cap_pairs, kFilterTile, 2048), kBlock, 0, s>>>(P_<double>(ctx->box), P_<unsigned long long>(ctx->raw_pairs),
                                                                ctx->cap_pairs, Fs, P_<unsigned long long>(ctx->pairs),
-                                                               P_<unsigned long long>(ctx->phash), ps - 1,
-                                                               P_<uint32_t>(ctx->cand), P_<uint32_t>(ctx->cand_list), C);
+                                                               P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->cand),
+                                                               P_<uint32_t>(ctx->cand_list), C);
     (void)hipEventRecord(ctx->ev[EV_A_FILTER], s);
     enqueue_narrow(ctx, S, true);
 }
@@ -511,6 +520,7 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
     CK(hipStreamSynchronize(ctx->s));
     CK(hipStreamSynchronize(ctx->s2));
     CK(hipStreamSynchronize(ctx->s3));
+    CK(hipStreamSynchronize(ctx->ssum));
     CK(hipGetLastError());
     CK(hipMemcpy(&ctx->C, ctx->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
     CK(hipMemcpy(&ctx->P, ctx->params.p, sizeof(Params), hipMemcpyDeviceToHost));
@@ -520,12 +530,12 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         fprintf(stderr,
                 "[mcgpu] raw=%llu pairs=%llu tests=%llu esc=%llu reg=%llu big=%llu cand=%llu edges=%llu+%llu ov=%u/%u/%u/%u/%u "
                 "gp=%u bad=%u/%u unsup=%u nonman=%u badin=%u pairfail=%u partial=%u caps=%llu/%llu/%llu/%llu "
-                "trav levels=%llu fallback_waves=%llu leaf_tests=%llu\n",
+                "trav splits=%llu lane_walks=%llu\n",
                 C.n_raw, C.n_pairs, C.n_tests, C.n_esc, C.n_reg, C.n_big, C.n_cand, C.n_edges_src, C.n_edges_cut, C.ov_pairs,
                 C.ov_tests, C.ov_esc, C.ov_reg, C.ov_big, C.gp, C.bad_src, C.bad_cut, C.unsupported, C.nonmanifold,
                 C.bad_input, C.pair_fail, C.partial, (unsigned long long)ctx->cap_pairs,
                 (unsigned long long)ctx->cap_tests, (unsigned long long)ctx->cap_esc, (unsigned long long)ctx->cap_reg,
-                C.n_visits, C.n_fallback, C.n_leaf_tests);
+                C.n_fallback, C.n_lane_walks);
     if (C.ov_pairs || C.ov_tests || C.ov_esc || C.ov_reg || C.ov_big) {
         *overflow = true;
         if (C.ov_pairs) ctx->cap_pairs = std::max<uint64_t>(2 * ctx->cap_pairs, C.n_raw + C.n_raw / 4);
@@ -545,7 +555,7 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
     // stages of streams B and C run only in a full dispatch; A's stages after a
     // join start at the event recorded once the join completed
     const bool full = ctx->stage_mode == 0;
-    ctx->stage_ms[ST_SUMS] = full ? el(EV_FORK, EV_A_SUMS) : 0.f;
+    ctx->stage_ms[ST_SUMS] = full ? el(EV_S0, EV_A_SUMS) : 0.f;
     ctx->stage_ms[ST_PARAMS] = full ? el(EV_A_SUMS, EV_A_PARAMS) : el(EV_A0, EV_A_PARAMS);
     ctx->stage_ms[ST_SHIFT] = el(EV_A_PARAMS, EV_A_SHIFT);
     ctx->stage_ms[ST_TOPOLOGY] = full ? el(EV_C0, EV_C_TOPO) : 0.f;
@@ -703,9 +713,28 @@ MCGPU_API mcgpu_result mcgpu_create_context(int device, mcgpu_context* out)
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MCGPU_ERROR_DEVICE;
     mcgpu_context ctx = new mcgpu_context_t();
     ctx->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->s2, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->s3, hipStreamNonBlocking) != hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MCGPU_ERROR_DEVICE;
+    }
+    // CU masks: the first kSumCUs compute units for the serial sums, the rest
+    // for the streams that run beside them
+    const int ncu = prop.multiProcessorCount;
+    std::vector<uint32_t> msum((ncu + 31) / 32, 0u), mrest((ncu + 31) / 32, 0u);
+    for (int c = 0; c < ncu; ++c) {
+        if (c < kSumCUs) msum[c / 32] |= 1u << (c % 32);
+        else mrest[c / 32] |= 1u << (c % 32);
+    }
+    if (ncu <= 2 * kSumCUs) std::fill(mrest.begin(), mrest.end(), ~0u); // too small a device to reserve CUs
+    if (hipExtStreamCreateWithCUMask(&ctx->ssum, (uint32_t)msum.size(), msum.data()) != hipSuccess ||
+        hipExtStreamCreateWithCUMask(&ctx->s2, (uint32_t)mrest.size(), mrest.data()) != hipSuccess ||
+        hipExtStreamCreateWithCUMask(&ctx->s3, (uint32_t)mrest.size(), mrest.data()) != hipSuccess) {
+        if (ctx->ssum) (void)hipStreamDestroy(ctx->ssum);
+        if (ctx->s2) (void)hipStreamDestroy(ctx->s2);
+        if (ctx->s3) (void)hipStreamDestroy(ctx->s3);
+        (void)hipStreamDestroy(ctx->s);
         delete ctx;
         return MCGPU_ERROR_DEVICE;
     }
@@ -721,12 +750,12 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
     (void)hipStreamSynchronize(ctx->s);
     (void)hipStreamSynchronize(ctx->s2);
     (void)hipStreamSynchronize(ctx->s3);
+    (void)hipStreamSynchronize(ctx->ssum);
     Buf* all[] = {&ctx->in_v[0], &ctx->in_v[1], &ctx->in_idx[0], &ctx->in_idx[1], &ctx->in_sz[0], &ctx->in_sz[1],
                   &ctx->off[0], &ctx->off[1], &ctx->params, &ctx->counters, &ctx->ps_v, &ctx->box, &ctx->hkeys[0],
                   &ctx->hkeys[1], &ctx->hfirst[0], &ctx->hfirst[1], &ctx->cface[0], &ctx->cface[1], &ctx->eflag[0],
                   &ctx->eflag[1], &ctx->eid[0], &ctx->eid[1], &ctx->cedge[0], &ctx->cedge[1], &ctx->etotal, &ctx->ev0,
-                  &ctx->ev1, &ctx->ef0, &ctx->ef1, &ctx->mcode[0], &ctx->mcode[1], &ctx->mface[0], &ctx->mface[1],
-                  &ctx->mcode2[0], &ctx->mcode2[1], &ctx->mface2[0], &ctx->mface2[1], &ctx->nodes[0], &ctx->nodes[1],
+                  &ctx->ev1, &ctx->ef0, &ctx->ef1, &ctx->mcode, &ctx->mface, &ctx->mcode2, &ctx->mface2, &ctx->nodes[0], &ctx->nodes[1],
                   &ctx->rawbox, &ctx->raw_pairs, &ctx->pairs, &ctx->phash, &ctx->cand, &ctx->cand_list,
                   &ctx->planes, &ctx->big_list, &ctx->big_scratch, &ctx->te, &ctx->tg, &ctx->tqs, &ctx->trs,
                   &ctx->ttype, &ctx->tpip, &ctx->esc, &ctx->rkey, &ctx->rval, &ctx->rkey2, &ctx->rval2, &ctx->rp,
@@ -734,6 +763,7 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
     for (Buf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->ssum);
     (void)hipStreamDestroy(ctx->s3);
     (void)hipStreamDestroy(ctx->s2);
     (void)hipStreamDestroy(ctx->s);
