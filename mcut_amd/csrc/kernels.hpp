@@ -33,6 +33,7 @@ struct Counters {
     unsigned long long n_raw;                              // raw-space (superset) candidate pairs
     unsigned long long n_fallback, n_lane_walks; // traversal statistics: node splits, per-lane walks
     unsigned long long pmask; // pair hash slots - 1 (set once the raw pairs are known)
+    double zero;              // +0.0 (the per-dispatch reset clears it): padding the serial sums load
     unsigned int ov_pairs, ov_tests, ov_esc, ov_reg, ov_big;
     unsigned int gp, bad_src, bad_cut, unsupported, nonmanifold, bad_input, pair_fail, partial;
     unsigned int pad[2];
@@ -100,6 +101,19 @@ struct Plane {
     uint8_t tri[3]; // vertex triple for orient3d; tri[0] = 0xff: all collinear
     uint8_t big;    // triple decided by the large-polygon pass
 };
+
+// order-preserving map of doubles to unsigned integers (for atomic min/max)
+__device__ __forceinline__ unsigned long long ord_key(double x)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double ord_value(unsigned long long k)
+{
+    const unsigned long long u = (k & 0x8000000000000000ull) ? (k & ~0x8000000000000000ull) : ~k;
+    return __longlong_as_double((long long)u);
+}
 
 __device__ __forceinline__ bool boxes_overlap(const double* a, const double* b)
 {

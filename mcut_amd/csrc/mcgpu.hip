@@ -403,11 +403,11 @@ void enqueue_frontend(mcgpu_context ctx)
     (void)hipEventRecord(ctx->ev[EV_FORK], s);
     (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_FORK], 0);
     (void)hipEventRecord(ctx->ev[EV_S0], ctx->ssum);
-    k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd);
+    k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], &C->zero, Pd);
     (void)hipEventRecord(ctx->ev[EV_A_SUMS], ctx->ssum);
     (void)hipStreamWaitEvent(s, ctx->ev[EV_A_SUMS], 0);
     const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
-    k_shift_params<<<1, 1, 0, s>>>(Pd, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
+    k_shift_params<<<1, 1, 0, s>>>(Pd, C, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
     (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
     k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
                                                         P_<double>(ctx->ps_v), 0);
