@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(256) k_raw_bbox(const double* __restrict__ v, 
 // shifted-box test (k_filter_pairs) then leaves exactly the reference's set.
 __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __restrict__ idx,
                              const uint32_t* __restrict__ off, uint32_t nf, const unsigned long long* __restrict__ rbox,
-                             uint32_t* __restrict__ code, uint32_t* __restrict__ face, double* __restrict__ rawbox,
+                             uint32_t* __restrict__ code, uint32_t* __restrict__ face, float* __restrict__ rawbox,
                              double tol0, double tol1, double tol2, uint32_t mesh_bit)
 {
     const double tol[3] = {tol0, tol1, tol2};
@@ -452,10 +452,12 @@ __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __res
         }
         code[f] = (c[0] * 4 + c[1] * 2 + c[2]) | mesh_bit;
         face[f] = (uint32_t)f;
-        double* o = rawbox + 6 * f;
+        // single precision, rounded outward: a float box contains the double
+        // box, so the superset property carries over
+        float* o = rawbox + 6 * f;
         for (int j = 0; j < 3; ++j) {
-            o[j] = mn[j] - tol[j];
-            o[3 + j] = mx[j] + tol[j];
+            o[j] = __double2float_rd(mn[j] - tol[j]);
+            o[3 + j] = __double2float_ru(mx[j] + tol[j]);
         }
     }
 }
@@ -465,11 +467,19 @@ __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __res
 // family of the reference's OIBVH, bvh.cpp:81-193): leaf level D holds 2^D
 // slots, node (l, k) lives at (2^l - 1) + k and has children (l+1, 2k) and
 // (l+1, 2k+1); slots past the last leaf hold empty boxes, which overlap
-// nothing.  The refit needs no atomics and no parent links: a 256-thread block
-// reduces 512 nodes of one level through LDS into the 9 levels above them.
+// nothing.  Boxes are single precision (the raw boxes are already rounded
+// outward).  The refit needs no atomics and no parent links: a 512-thread
+// block reduces 1024 nodes of one level through LDS into the 10 levels above
+// them, and both trees are refitted by the same launches.
 // ---------------------------------------------------------------------------
-constexpr int kRefitWidth = 512;
-constexpr int kRefitLevels = 9;
+constexpr int kRefitWidth = 1024;
+constexpr int kRefitLevels = 10;
+
+__device__ __forceinline__ void empty_box(float* b)
+{
+    b[0] = b[1] = b[2] = 3.402823466e38f;
+    b[3] = b[4] = b[5] = -3.402823466e38f;
+}
 
 __device__ __forceinline__ void empty_box(double* b)
 {
@@ -477,28 +487,44 @@ __device__ __forceinline__ void empty_box(double* b)
     b[3] = b[4] = b[5] = -1.7976931348623157e308;
 }
 
-__global__ void __launch_bounds__(256) k_refit_levels(const double* __restrict__ facebox, const uint32_t* __restrict__ sface,
-                                                      int n, int D, int lin, double* __restrict__ nodes)
+__device__ __forceinline__ bool boxes_overlap(const float* a, const float* b)
 {
-    __shared__ double sb[kRefitWidth * 6];
+    return (a[0] <= b[3] && a[3] >= b[0]) && (a[1] <= b[4] && a[4] >= b[1]) && (a[2] <= b[5] && a[5] >= b[2]);
+}
+
+struct RefitTree {
+    const float* facebox;   // leaf face boxes of this mesh (mesh-local face order)
+    const uint32_t* sface;  // leaf -> mesh-local face
+    float* nodes;
+    int n, D, lin;          // leaves, leaf level, level this launch starts from (lin <= 0: nothing to do)
+    int blocks;             // blocks of this launch for this tree
+};
+
+__global__ void __launch_bounds__(512) k_refit_levels(RefitTree t0, RefitTree t1)
+{
+    __shared__ float sb[kRefitWidth * 6];
+    const bool second = (int)blockIdx.x >= t0.blocks;
+    const RefitTree& T = second ? t1 : t0;
+    const int blk = second ? (int)blockIdx.x - t0.blocks : (int)blockIdx.x;
+    const int lin = T.lin, D = T.D;
     const int width = 1 << lin;
     const int per = width < kRefitWidth ? width : kRefitWidth;
     const int steps = lin < kRefitLevels ? lin : kRefitLevels;
-    const int base = blockIdx.x * per;
+    const int base = blk * per;
     for (int t = threadIdx.x; t < per; t += blockDim.x) {
         const int k = base + t;
-        double b[6];
+        float b[6];
         if (lin == D) {
-            if (k < n) {
-                const double* fb = facebox + 6 * (uint64_t)sface[k];
+            if (k < T.n) {
+                const float* fb = T.facebox + 6 * (uint64_t)T.sface[k];
                 for (int q = 0; q < 6; ++q) b[q] = fb[q];
             } else {
                 empty_box(b);
             }
-            double* o = nodes + 6 * ((uint64_t)(1u << D) - 1 + k);
+            float* o = T.nodes + 6 * ((uint64_t)(1u << D) - 1 + k);
             for (int q = 0; q < 6; ++q) o[q] = b[q];
         } else {
-            const double* src = nodes + 6 * ((uint64_t)(1u << lin) - 1 + k);
+            const float* src = T.nodes + 6 * ((uint64_t)(1u << lin) - 1 + k);
             for (int q = 0; q < 6; ++q) b[q] = src[q];
         }
         for (int q = 0; q < 6; ++q) sb[6 * t + q] = b[q];
@@ -507,12 +533,12 @@ __global__ void __launch_bounds__(256) k_refit_levels(const double* __restrict__
     int cur = per;
     for (int st = 1; st <= steps; ++st) {
         const int half = cur >> 1;
-        double u[6];
+        float u[6];
         const int t = threadIdx.x;
         if (t < half) {
             for (int q = 0; q < 3; ++q) {
-                const double a0 = sb[6 * (2 * t) + q], a1 = sb[6 * (2 * t + 1) + q];
-                const double b0 = sb[6 * (2 * t) + 3 + q], b1 = sb[6 * (2 * t + 1) + 3 + q];
+                const float a0 = sb[6 * (2 * t) + q], a1 = sb[6 * (2 * t + 1) + q];
+                const float b0 = sb[6 * (2 * t) + 3 + q], b1 = sb[6 * (2 * t + 1) + 3 + q];
                 u[q] = a1 < a0 ? a1 : a0;
                 u[3 + q] = b0 < b1 ? b1 : b0;
             }
@@ -521,7 +547,7 @@ __global__ void __launch_bounds__(256) k_refit_levels(const double* __restrict__
         if (t < half) {
             for (int q = 0; q < 6; ++q) sb[6 * t + q] = u[q];
             const int l = lin - st;
-            double* o = nodes + 6 * ((uint64_t)(1u << l) - 1 + (base >> st) + t);
+            float* o = T.nodes + 6 * ((uint64_t)(1u << l) - 1 + (base >> st) + t);
             for (int q = 0; q < 6; ++q) o[q] = u[q];
         }
         __syncthreads();
@@ -539,24 +565,34 @@ __global__ void __launch_bounds__(256) k_refit_levels(const double* __restrict__
 
 struct Tree {
     const uint32_t* sface;  // leaf -> mesh-local face
-    const double* nodes;    // implicit tree boxes
-    const double* facebox;  // ps face boxes of this mesh
+    const float* nodes;     // implicit tree boxes (single precision, rounded outward)
+    const float* facebox;   // raw face boxes of this mesh
     int n, D;
 };
 
-// Wavefront-cooperative BVH x BVH traversal.  A 64-lane wave owns 64
-// Morton-consecutive source leaves; their union is the source-tree node at
-// depth D_s - 6.  The wave walks the cut tree breadth first with that node's
-// box, all lanes testing children in parallel and compacting the survivors
-// into an LDS frontier with ballots; at the leaf level every lane tests its
-// own source face against the surviving cut leaves, staged through LDS.
-// When the frontier outgrows LDS (a node whose leaves are spread out), the
-// wave splits the source node and walks its two children in turn, depth first
-// and without a stack (the next node follows from index arithmetic); only a
-// single source leaf whose frontier still overflows takes a per-lane walk.
-// Found pairs are staged in LDS and flushed with one global atomic per
-// kTravStage - 64 pairs or so.
+// Wavefront-cooperative BVH x BVH traversal.  A 64-lane wave owns
+// kWaveLeaves Morton-consecutive source leaves; their union is the source-tree
+// node at depth D_s - log2(kWaveLeaves).  The wave walks the cut tree breadth
+// first with that node's box, all lanes testing children in parallel and
+// compacting the survivors into an LDS frontier with ballots; at the leaf
+// level every lane owning a source face tests it against the surviving cut
+// leaves, staged through LDS.  A walk starts at cut level kStartLevel with all
+// of its nodes, tested in one memory round trip, instead of descending the
+// nearly empty levels above one round trip each.
+// A frontier wider than kSplitFrontier means the node's leaves are spread
+// out (the Morton curve jumped): the wave then splits the source node and
+// walks its two children in turn, depth first and without a stack (the next
+// node follows from index arithmetic).  A single source leaf keeps going up
+// to the LDS capacity kFrontierCap, beyond which it takes a per-lane walk.
+// Each lane loads kTravILP children before testing them, so one memory round
+// trip covers 64 * kTravILP children.  Found pairs are staged in LDS and
+// flushed with one global atomic per kTravStage - 64 pairs or so.
 constexpr int kFrontierCap = 1024;
+constexpr int kSplitFrontier = 256;
+constexpr int kWaveLeavesLog = 5;
+constexpr int kWaveLeaves = 1 << kWaveLeavesLog;
+constexpr int kStartLevel = 8; // 256 nodes: kTravILP per lane
+constexpr int kTravILP = 4;
 constexpr int kTravStage = 512;
 
 // writes the wave's staged pairs to the global list (all lanes call it)
@@ -575,7 +611,7 @@ __device__ __forceinline__ void flush_pairs(const unsigned long long* stage, int
     __syncthreads();
 }
 
-__device__ __forceinline__ void per_lane_walk(const double* q, uint32_t s, const Tree& cut, unsigned long long* pairs,
+__device__ __forceinline__ void per_lane_walk(const float* q, uint32_t s, const Tree& cut, unsigned long long* pairs,
                                               uint64_t cap, Counters* C)
 {
     const int D = cut.D;
@@ -583,8 +619,8 @@ __device__ __forceinline__ void per_lane_walk(const double* q, uint32_t s, const
     int l = 0;
     uint32_t k = 0;
     for (long long it = 0; it < budget; ++it) {
-        const double* nb = cut.nodes + 6 * ((uint64_t)(1u << l) - 1 + k);
-        double b[6];
+        const float* nb = cut.nodes + 6 * ((uint64_t)(1u << l) - 1 + k);
+        float b[6];
         for (int j = 0; j < 6; ++j) b[j] = nb[j];
         if (boxes_overlap(q, b)) {
             if (l == D) {
@@ -612,15 +648,16 @@ __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsign
                                                       uint64_t cap, Counters* C)
 {
     __shared__ uint32_t front[2][kFrontierCap];
-    __shared__ double lbox[64][6];
+    __shared__ float lbox[64][6];
     __shared__ uint32_t lface[64];
     __shared__ unsigned long long stage[kTravStage];
     int sn = 0; // staged pairs (wave-uniform)
     const int lane = threadIdx.x;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int Ds = src.D;
-    const int lw = Ds >= 6 ? Ds - 6 : 0; // level of a wave's source node
-    const int nwaves = Ds >= 6 ? (src.n + 63) / 64 : 1;
+    const int lw = Ds >= kWaveLeavesLog ? Ds - kWaveLeavesLog : 0; // level of a wave's source node
+    const int nwaves = Ds >= kWaveLeavesLog ? (src.n + kWaveLeaves - 1) / kWaveLeaves : 1;
+    const int s0 = cut.D < kStartLevel ? cut.D : kStartLevel;
     const uint32_t leaf_base = (uint32_t)(1u << cut.D) - 1;
     unsigned long long splits = 0, lane_walks = 0;
     for (int w = blockIdx.x; w < nwaves; w += gridDim.x) {
@@ -631,7 +668,7 @@ __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsign
             const int span = Ds - l;
             const uint32_t i = (k << span) + (uint32_t)lane;
             const bool valid = lane < (1 << span) && i < (uint32_t)src.n;
-            double q[6];
+            float q[6];
             uint32_t s = 0;
             if (valid) {
                 s = src.sface[i];
@@ -639,45 +676,71 @@ __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsign
             } else {
                 empty_box(q);
             }
-            double wb[6];
+            float wb[6];
             {
-                const double* nb = src.nodes + 6 * ((uint64_t)(1u << l) - 1 + k);
+                const float* nb = src.nodes + 6 * ((uint64_t)(1u << l) - 1 + k);
                 for (int j = 0; j < 6; ++j) wb[j] = nb[j];
             }
-            // breadth-first walk of the cut tree with the node box
-            int cur = 0, nf = 0, level = 0;
+            // breadth-first walk of the cut tree with the node box, from the
+            // 2^s0 nodes of level s0
+            const int limit = span > 0 ? kSplitFrontier : kFrontierCap;
+            int cur = 0, nf = 0, level = s0;
             bool overflow = false;
             {
-                double rb[6];
-                for (int j = 0; j < 6; ++j) rb[j] = cut.nodes[j];
-                if (boxes_overlap(wb, rb)) {
-                    if (lane == 0) front[0][0] = 0u;
-                    nf = 1;
+                const int n0 = 1 << s0;
+                const uint32_t base0 = (uint32_t)n0 - 1;
+                for (int c0 = 0; c0 < n0; c0 += 64 * kTravILP) {
+                    float b[kTravILP][6];
+#pragma unroll
+                    for (int u = 0; u < kTravILP; ++u) {
+                        const int c = c0 + 64 * u + lane;
+                        if (c < n0) {
+                            const float* nb = cut.nodes + 6 * ((uint64_t)base0 + c);
+                            for (int j = 0; j < 6; ++j) b[u][j] = nb[j];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kTravILP; ++u) {
+                        const int c = c0 + 64 * u + lane;
+                        const bool keep = c < n0 && boxes_overlap(wb, b[u]);
+                        const uint64_t m = __ballot(keep);
+                        const int pos = nf + __popcll(m & lt);
+                        if (keep && pos < kFrontierCap) front[0][pos] = (uint32_t)c;
+                        nf += __popcll(m);
+                    }
                 }
+                if (nf > limit) overflow = true;
             }
             __syncthreads();
             while (nf > 0 && level < cut.D && !overflow) {
                 int nn = 0;
                 const int nchild = 2 * nf;
                 const uint32_t base_next = (uint32_t)(1u << (level + 1)) - 1;
-                for (int c0 = 0; c0 < nchild; c0 += 64) {
-                    const int c = c0 + lane;
-                    bool keep = false;
-                    uint32_t child = 0;
-                    if (c < nchild) {
-                        child = 2u * front[cur][c >> 1] + (uint32_t)(c & 1);
-                        const double* nb = cut.nodes + 6 * ((uint64_t)base_next + child);
-                        double b[6];
-                        for (int j = 0; j < 6; ++j) b[j] = nb[j];
-                        keep = boxes_overlap(wb, b);
+                for (int c0 = 0; c0 < nchild; c0 += 64 * kTravILP) {
+                    float b[kTravILP][6];
+                    uint32_t child[kTravILP];
+#pragma unroll
+                    for (int u = 0; u < kTravILP; ++u) {
+                        const int c = c0 + 64 * u + lane;
+                        child[u] = 0;
+                        if (c < nchild) {
+                            child[u] = 2u * front[cur][c >> 1] + (uint32_t)(c & 1);
+                            const float* nb = cut.nodes + 6 * ((uint64_t)base_next + child[u]);
+                            for (int j = 0; j < 6; ++j) b[u][j] = nb[j];
+                        }
                     }
-                    const uint64_t m = __ballot(keep);
-                    const int pos = nn + __popcll(m & lt);
-                    if (keep && pos < kFrontierCap) front[cur ^ 1][pos] = child;
-                    nn += __popcll(m);
+#pragma unroll
+                    for (int u = 0; u < kTravILP; ++u) {
+                        const int c = c0 + 64 * u + lane;
+                        const bool keep = c < nchild && boxes_overlap(wb, b[u]);
+                        const uint64_t m = __ballot(keep);
+                        const int pos = nn + __popcll(m & lt);
+                        if (keep && pos < kFrontierCap) front[cur ^ 1][pos] = child[u];
+                        nn += __popcll(m);
+                    }
                 }
                 __syncthreads();
-                if (nn > kFrontierCap) overflow = true;
+                if (nn > limit) overflow = true;
                 nf = nn;
                 cur ^= 1;
                 ++level;
@@ -696,7 +759,7 @@ __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsign
                     const int t = t0 + lane;
                     if (t < nf) {
                         const uint32_t kk = front[cur][t];
-                        const double* nb = cut.nodes + 6 * ((uint64_t)leaf_base + kk);
+                        const float* nb = cut.nodes + 6 * ((uint64_t)leaf_base + kk);
                         for (int j = 0; j < 6; ++j) lbox[lane][j] = nb[j];
                         lface[lane] = cut.sface[kk];
                     }
@@ -707,7 +770,7 @@ __global__ void __launch_bounds__(64) k_traverse_wave(Tree src, Tree cut, unsign
                             flush_pairs(stage, sn, pairs, cap, C);
                             sn = 0;
                         }
-                        double b[6];
+                        float b[6];
                         for (int j = 0; j < 6; ++j) b[j] = lbox[u][j];
                         const bool hit = valid && boxes_overlap(q, b);
                         const uint64_t m = __ballot(hit);
@@ -1309,6 +1372,199 @@ __global__ void __launch_bounds__(64) k_narrow_exact(Soup S, const Plane* __rest
 // 9. registry: sorted order, then the status rules that read it
 // ===========================================================================
 
+// cutpath_edge_creation_info keys (kernel.cpp:2620-2636): every (source face,
+// cut face) pair joined by an intersection point needs at least two points
+// (kernel.cpp:3439-3452); a cut-mesh border edge hit marks a partial cut
+// (kernel.cpp:2646-2650)
+// status hash: 4 slots per registry entry, the power of two above 4 n
+__device__ __forceinline__ uint64_t status_slots(uint64_t n)
+{
+    uint64_t p = 64;
+    while (p < 4 * n) p <<= 1;
+    return p;
+}
+
+// counts one registry entry (e, g) on the face pairs it joins
+__device__ __forceinline__ void status_insert(const Soup& S, uint32_t e, uint32_t g, unsigned long long* hkeys,
+                                              uint32_t* hcnt, uint64_t mask, Counters* C)
+{
+    const uint32_t Fs = S.nf[0], Vs = S.nv[0];
+    const int32_t f0 = S.ef0[e], f1 = S.ef1[e];
+    const bool on_cut = f0 >= (int32_t)Fs;
+    uint64_t k0, k1 = kEmpty64;
+    if (on_cut) {
+        k0 = ((uint64_t)g << 32) | (uint32_t)f0;
+        if (f1 >= 0) k1 = ((uint64_t)g << 32) | (uint32_t)f1;
+    } else {
+        k0 = ((uint64_t)(uint32_t)f0 << 32) | g;
+        if (f1 >= 0) k1 = ((uint64_t)(uint32_t)f1 << 32) | g;
+    }
+    atomicAdd(&hcnt[hash_insert(hkeys, mask, k0)], 1u);
+    if (k1 != kEmpty64) atomicAdd(&hcnt[hash_insert(hkeys, mask, k1)], 1u);
+    if (S.ev0[e] >= Vs && (f0 < 0 || f1 < 0)) atomicOr(&C->partial, 1u);
+}
+
+// Registry order for up to kRegMergeTiles tiles of kRegTile entries, in two
+// launches instead of a multi-pass radix sort.  k_reg_tile_sort sorts each
+// tile with a bitonic network and clears the status hash; k_reg_merge ranks
+// every entry by its place in its own tile plus binary searches in the other
+// tiles (registry keys are distinct), writes (edge, face, point) to that rank
+// and counts the status face pairs.
+//
+// In the tile sort each of 1024 threads holds four consecutive elements in
+// registers: compare-exchange partners at distance 1-2 are in the same
+// thread, at distance 4-128 in the same wave (exchanged with shuffles), and
+// only the ten stages at distance 256 and more exchange through LDS.
+constexpr int kRegTile = 4096;
+constexpr int kRegSortThreads = 1024;
+constexpr int kRegPer = kRegTile / kRegSortThreads; // 4
+constexpr uint64_t kRegMergeTiles = 64;
+
+// keeps the smaller or the larger of (k, v) and its partner's (kp, vp)
+__device__ __forceinline__ void bitonic_keep(unsigned long long& k, uint32_t& v, unsigned long long kp, uint32_t vp,
+                                             bool keep_min)
+{
+    if (keep_min ? (kp < k) : (kp > k)) {
+        k = kp;
+        v = vp;
+    }
+}
+
+__global__ void __launch_bounds__(kRegSortThreads) k_reg_tile_sort(const unsigned long long* __restrict__ key,
+                                                                   const uint32_t* __restrict__ val,
+                                                                   const unsigned long long* __restrict__ n_dev,
+                                                                   uint64_t cap, unsigned long long* __restrict__ skey,
+                                                                   uint32_t* __restrict__ sval,
+                                                                   unsigned long long* __restrict__ hkeys,
+                                                                   uint32_t* __restrict__ hcnt, uint64_t max_slots)
+{
+    __shared__ unsigned long long sk[kRegTile];
+    __shared__ uint32_t sv[kRegTile];
+    const uint64_t n = *n_dev < cap ? *n_dev : cap;
+    uint64_t slots = status_slots(n);
+    if (slots > max_slots) slots = max_slots;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slots; i += (uint64_t)gridDim.x * blockDim.x) {
+        hkeys[i] = kEmpty64;
+        hcnt[i] = 0u;
+    }
+    const uint64_t t0 = (uint64_t)blockIdx.x * kRegTile;
+    if (t0 >= n) return; // whole block
+    const uint32_t cnt = (n - t0) < (uint64_t)kRegTile ? (uint32_t)(n - t0) : (uint32_t)kRegTile;
+    const uint32_t t = threadIdx.x;
+    unsigned long long K[kRegPer];
+    uint32_t V[kRegPer];
+#pragma unroll
+    for (int r = 0; r < kRegPer; ++r) {
+        const uint32_t i = kRegPer * t + r;
+        K[r] = i < cnt ? key[t0 + i] : ~0ull; // padding sorts last
+        V[r] = i < cnt ? val[t0 + i] : 0u;
+    }
+    for (uint32_t k = 2; k <= (uint32_t)kRegTile; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 256) {
+#pragma unroll
+                for (int r = 0; r < kRegPer; ++r) {
+                    sk[kRegPer * t + r] = K[r];
+                    sv[kRegPer * t + r] = V[r];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < kRegPer; ++r) {
+                    const uint32_t i = kRegPer * t + r, p = i ^ j;
+                    bitonic_keep(K[r], V[r], sk[p], sv[p], (i < p) == ((i & k) == 0));
+                }
+                __syncthreads();
+            } else if (j >= (uint32_t)kRegPer) {
+                const int lanes = (int)(j / kRegPer);
+#pragma unroll
+                for (int r = 0; r < kRegPer; ++r) {
+                    const unsigned long long kp = __shfl_xor(K[r], lanes, 64);
+                    const uint32_t vp = __shfl_xor(V[r], lanes, 64);
+                    const uint32_t i = kRegPer * t + r, p = i ^ j;
+                    bitonic_keep(K[r], V[r], kp, vp, (i < p) == ((i & k) == 0));
+                }
+            } else {
+                // partners inside the thread: (0,1),(2,3) for j = 1, (0,2),(1,3) for j = 2
+#pragma unroll
+                for (int r = 0; r < kRegPer; ++r) {
+                    const int pr = r ^ (int)j;
+                    if (pr <= r) continue;
+                    const uint32_t i = kRegPer * t + r;
+                    const bool asc = (i & k) == 0;
+                    if (asc ? (K[pr] < K[r]) : (K[pr] > K[r])) {
+                        const unsigned long long tk = K[r];
+                        K[r] = K[pr];
+                        K[pr] = tk;
+                        const uint32_t tv = V[r];
+                        V[r] = V[pr];
+                        V[pr] = tv;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kRegPer; ++r) {
+        const uint32_t i = kRegPer * t + r;
+        if (i < cnt) {
+            skey[t0 + i] = K[r];
+            sval[t0 + i] = V[r];
+        }
+    }
+}
+
+// first index of tile u (sorted, cnt_u entries at skey + lo_u) whose key is not
+// below k, for up to 8 tiles at once so that their loads overlap
+constexpr int kRegMergeILP = 8;
+
+__global__ void k_reg_merge(Soup S, const unsigned long long* __restrict__ skey, const uint32_t* __restrict__ sval,
+                            const unsigned long long* __restrict__ n_dev, uint64_t cap, int gbits,
+                            const double* __restrict__ p_in, uint32_t* __restrict__ re, uint32_t* __restrict__ rg,
+                            double* __restrict__ rp, unsigned long long* __restrict__ hkeys, uint32_t* __restrict__ hcnt,
+                            uint64_t max_slots, Counters* C)
+{
+    const uint64_t n = *n_dev < cap ? *n_dev : cap;
+    const uint64_t ntiles = (n + kRegTile - 1) / kRegTile;
+    uint64_t slots = status_slots(n);
+    if (slots > max_slots) slots = max_slots;
+    const uint64_t gmask = (1ull << gbits) - 1;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = i / kRegTile;
+        const unsigned long long k = skey[i];
+        uint64_t rank = i - t * kRegTile;
+        for (uint64_t u0 = 0; u0 < ntiles; u0 += kRegMergeILP) {
+            uint32_t lo[kRegMergeILP], hi[kRegMergeILP];
+#pragma unroll
+            for (int q = 0; q < kRegMergeILP; ++q) {
+                const uint64_t u = u0 + q;
+                const uint64_t base = u * kRegTile;
+                lo[q] = 0;
+                hi[q] = (u < ntiles && u != t) ? (uint32_t)((n - base) < (uint64_t)kRegTile ? (n - base) : kRegTile) : 0u;
+            }
+            for (int step = 0; step < 13; ++step) { // 13 halvings cover 4096 entries
+#pragma unroll
+                for (int q = 0; q < kRegMergeILP; ++q) {
+                    if (lo[q] < hi[q]) {
+                        const uint32_t mid = (lo[q] + hi[q]) >> 1;
+                        if (skey[(u0 + q) * kRegTile + mid] < k) lo[q] = mid + 1;
+                        else hi[q] = mid;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kRegMergeILP; ++q) rank += lo[q];
+        }
+        const uint32_t e = (uint32_t)(k >> gbits), g = (uint32_t)(k & gmask);
+        re[rank] = e;
+        rg[rank] = g;
+        const double* p = p_in + 3 * (uint64_t)sval[i];
+        rp[3 * rank] = p[0];
+        rp[3 * rank + 1] = p[1];
+        rp[3 * rank + 2] = p[2];
+        status_insert(S, e, g, hkeys, hcnt, slots - 1, C);
+    }
+}
+
 __global__ void k_clamp_count(const unsigned long long* n_dev, uint64_t cap, unsigned long long* out)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *out = *n_dev < cap ? *n_dev : cap;
@@ -1334,17 +1590,6 @@ __global__ void k_registry_gather(const unsigned long long* __restrict__ skey, c
     }
 }
 
-// cutpath_edge_creation_info keys (kernel.cpp:2620-2636): every (source face,
-// cut face) pair joined by an intersection point needs at least two points
-// (kernel.cpp:3439-3452); a cut-mesh border edge hit marks a partial cut
-// (kernel.cpp:2646-2650)
-// status hash: 4 slots per registry entry, the power of two above 4 n
-__device__ __forceinline__ uint64_t status_slots(uint64_t n)
-{
-    uint64_t p = 64;
-    while (p < 4 * n) p <<= 1;
-    return p;
-}
 
 __global__ void k_status_clear(const unsigned long long* __restrict__ n_dev, unsigned long long* __restrict__ hkeys,
                                uint32_t* __restrict__ hcnt, uint64_t max_slots)
@@ -1364,24 +1609,8 @@ __global__ void k_status_pairs(Soup S, const uint32_t* __restrict__ re, const ui
     const uint64_t n = *n_dev;
     uint64_t slots = status_slots(n);
     if (slots > max_slots) slots = max_slots;
-    const uint64_t mask = slots - 1;
-    const uint32_t Fs = S.nf[0], Vs = S.nv[0];
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t e = re[i], g = rg[i];
-        const int32_t f0 = S.ef0[e], f1 = S.ef1[e];
-        const bool on_cut = f0 >= (int32_t)Fs;
-        uint64_t k0, k1 = kEmpty64;
-        if (on_cut) {
-            k0 = ((uint64_t)g << 32) | (uint32_t)f0;
-            if (f1 >= 0) k1 = ((uint64_t)g << 32) | (uint32_t)f1;
-        } else {
-            k0 = ((uint64_t)(uint32_t)f0 << 32) | g;
-            if (f1 >= 0) k1 = ((uint64_t)(uint32_t)f1 << 32) | g;
-        }
-        atomicAdd(&hcnt[hash_insert(hkeys, mask, k0)], 1u);
-        if (k1 != kEmpty64) atomicAdd(&hcnt[hash_insert(hkeys, mask, k1)], 1u);
-        if (S.ev0[e] >= Vs && (f0 < 0 || f1 < 0)) atomicOr(&C->partial, 1u);
-    }
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        status_insert(S, re[i], rg[i], hkeys, hcnt, slots - 1, C);
 }
 
 __global__ void k_status_check(const unsigned long long* __restrict__ n_dev, const unsigned long long* __restrict__ hkeys,
@@ -1391,16 +1620,6 @@ __global__ void k_status_check(const unsigned long long* __restrict__ n_dev, con
     if (slots > max_slots) slots = max_slots;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slots; i += (uint64_t)gridDim.x * blockDim.x)
         if (hkeys[i] != kEmpty64 && hcnt[i] < 2) atomicOr(&C->pair_fail, 1u);
-}
-
-// packed test keys for the optional sorted test listing
-__global__ void k_test_keys(const uint32_t* __restrict__ te, const uint32_t* __restrict__ tg,
-                            const unsigned long long* __restrict__ n_dev, uint64_t cap, int gbits,
-                            unsigned long long* __restrict__ key)
-{
-    const uint64_t n = *n_dev < cap ? *n_dev : cap;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        key[i] = ((unsigned long long)te[i] << gbits) | tg[i];
 }
 
 } // namespace mcgpu

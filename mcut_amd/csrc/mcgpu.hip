@@ -233,7 +233,8 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
     double* dp;
     CK(ensure(ctx->ps_v, 3 * V, &dp));
     CK(ensure(ctx->box, 6 * F, &dp));
-    CK(ensure(ctx->rawbox, 6 * F, &dp));
+    float* fp;
+    CK(ensure(ctx->rawbox, 6 * F, &fp));
     uint32_t* u;
     int32_t* i32;
     unsigned long long* ull;
@@ -256,7 +257,7 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
         int D = 0;
         while ((1ull << D) < nf) ++D;
         ctx->depth[m] = D;
-        CK(ensure(ctx->nodes[m], 6 * ((2ull << D) - 1), &dp));
+        CK(ensure(ctx->nodes[m], 6 * ((2ull << D) - 1), &fp));
         scan_need = std::max<uint64_t>(scan_need, scan_tmp_elems(nf + 8));
     }
     CK(ensure(ctx->mcode, F, &u));
@@ -347,23 +348,37 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
                                                                    ctx->cap_esc, TO, R, C);
     (void)hipEventRecord(ctx->ev[EV_A_NARROW], s);
     unsigned long long* rcount = P_<unsigned long long>(ctx->rcount);
-    k_clamp_count<<<1, 1, 0, s>>>(&C->n_reg, ctx->cap_reg, rcount);
-    RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
-    const bool alt = radix_sort<unsigned long long>(P_<unsigned long long>(ctx->rkey), P_<uint32_t>(ctx->rval),
-                                                    P_<unsigned long long>(ctx->rkey2), P_<uint32_t>(ctx->rval2),
-                                                    (const uint64_t*)rcount, 0, ctx->cap_reg, ctx->kbits, rt, s);
-    const unsigned long long* sk = P_<unsigned long long>(alt ? ctx->rkey2 : ctx->rkey);
-    const uint32_t* sv = P_<uint32_t>(alt ? ctx->rval2 : ctx->rval);
-    k_registry_gather<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(sk, sv, P_<double>(ctx->rp), rcount, ctx->cap_reg,
-                                                                 ctx->gbits, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg),
-                                                                 P_<double>(ctx->rpts));
     const uint64_t hs = next_pow2(4 * ctx->cap_reg);
-    k_status_clear<<<grid_for(hs), kBlock, 0, s>>>(rcount, P_<unsigned long long>(ctx->shkeys), P_<uint32_t>(ctx->shcnt), hs);
-    k_status_pairs<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg), rcount, hs,
-                                                             P_<unsigned long long>(ctx->shkeys), P_<uint32_t>(ctx->shcnt),
-                                                             C);
-    k_status_check<<<grid_for(hs), kBlock, 0, s>>>(rcount, P_<unsigned long long>(ctx->shkeys), P_<uint32_t>(ctx->shcnt),
-                                                   hs, C);
+    unsigned long long* hk = P_<unsigned long long>(ctx->shkeys);
+    uint32_t* hc = P_<uint32_t>(ctx->shcnt);
+    if (ctx->cap_reg <= kRegMergeTiles * kRegTile) {
+        // tile sort + merge by rank: two launches
+        const unsigned tiles = (unsigned)((ctx->cap_reg + kRegTile - 1) / kRegTile);
+        k_reg_tile_sort<<<tiles, kRegSortThreads, 0, s>>>(P_<unsigned long long>(ctx->rkey), P_<uint32_t>(ctx->rval), &C->n_reg,
+                                                         ctx->cap_reg, P_<unsigned long long>(ctx->rkey2),
+                                                         P_<uint32_t>(ctx->rval2), hk, hc, hs);
+        k_reg_merge<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(S, P_<unsigned long long>(ctx->rkey2), P_<uint32_t>(ctx->rval2),
+                                                              &C->n_reg, ctx->cap_reg, ctx->gbits, P_<double>(ctx->rp),
+                                                              P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg),
+                                                              P_<double>(ctx->rpts), hk, hc, hs, C);
+        k_status_check<<<grid_for(hs), kBlock, 0, s>>>(&C->n_reg, hk, hc, hs, C);
+    } else {
+        // large registries: LSD radix sort of the keys with the entry index as payload
+        k_clamp_count<<<1, 1, 0, s>>>(&C->n_reg, ctx->cap_reg, rcount);
+        RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
+        const bool alt = radix_sort<unsigned long long>(P_<unsigned long long>(ctx->rkey), P_<uint32_t>(ctx->rval),
+                                                        P_<unsigned long long>(ctx->rkey2), P_<uint32_t>(ctx->rval2),
+                                                        (const uint64_t*)rcount, 0, ctx->cap_reg, ctx->kbits, rt, s);
+        const unsigned long long* sk = P_<unsigned long long>(alt ? ctx->rkey2 : ctx->rkey);
+        const uint32_t* sv = P_<uint32_t>(alt ? ctx->rval2 : ctx->rval);
+        k_registry_gather<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(sk, sv, P_<double>(ctx->rp), rcount, ctx->cap_reg,
+                                                                     ctx->gbits, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg),
+                                                                     P_<double>(ctx->rpts));
+        k_status_clear<<<grid_for(hs), kBlock, 0, s>>>(rcount, hk, hc, hs);
+        k_status_pairs<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg), rcount, hs,
+                                                                 hk, hc, C);
+        k_status_check<<<grid_for(hs), kBlock, 0, s>>>(rcount, hk, hc, hs, C);
+    }
     (void)hipEventRecord(ctx->ev[EV_A_END], s);
 }
 
@@ -419,7 +434,7 @@ void enqueue_frontend(mcgpu_context ctx)
     (void)hipStreamWaitEvent(s2, ctx->ev[EV_FORK], 0);
     (void)hipEventRecord(ctx->ev[EV_B0], s2);
     RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
-    double* rawbox = P_<double>(ctx->rawbox);
+    float* rawbox = P_<float>(ctx->rawbox);
     for (int m = 0; m < 2; ++m) {
         const uint32_t nf = ctx->nf[m];
         const uint64_t o = m ? Fs : 0;
@@ -435,27 +450,43 @@ void enqueue_frontend(mcgpu_context ctx)
     (void)hipEventRecord(ctx->ev[EV_B_SORT], s2);
     const uint32_t* sorted = P_<uint32_t>(ctx->mface_alt ? ctx->mface2 : ctx->mface);
     Tree T[2];
+    RefitTree R[2];
     for (int m = 0; m < 2; ++m) {
-        const uint32_t nf = ctx->nf[m];
-        const int D = ctx->depth[m];
-        const double* fbox = rawbox + 6 * (uint64_t)(m ? Fs : 0);
-        const uint32_t* sface = sorted + (m ? Fs : 0);
-        double* nodes = P_<double>(ctx->nodes[m]);
-        int lin = D;
-        do {
-            const int width = 1 << lin;
-            const int per = width < kRefitWidth ? width : kRefitWidth;
-            k_refit_levels<<<width / per, 256, 0, s2>>>(fbox, sface, (int)nf, D, lin, nodes);
-            lin -= (lin < kRefitLevels ? lin : kRefitLevels);
-        } while (lin > 0);
-        T[m].sface = sface;
-        T[m].nodes = nodes;
-        T[m].facebox = fbox;
-        T[m].n = (int)nf;
-        T[m].D = D;
+        R[m].facebox = rawbox + 6 * (uint64_t)(m ? Fs : 0);
+        R[m].sface = sorted + (m ? Fs : 0);
+        R[m].nodes = P_<float>(ctx->nodes[m]);
+        R[m].n = (int)ctx->nf[m];
+        R[m].D = ctx->depth[m];
+        R[m].lin = ctx->depth[m];
+        T[m].sface = R[m].sface;
+        T[m].nodes = R[m].nodes;
+        T[m].facebox = R[m].facebox;
+        T[m].n = R[m].n;
+        T[m].D = R[m].D;
+    }
+    // each round lifts both trees by up to kRefitLevels levels; a tree of
+    // 2^19 leaves needs two rounds
+    for (;;) {
+        int blocks = 0;
+        for (int m = 0; m < 2; ++m) {
+            R[m].blocks = 0;
+            if (R[m].lin > 0 || (R[m].lin == 0 && R[m].D == 0)) {
+                const int width = 1 << R[m].lin;
+                R[m].blocks = width / (width < kRefitWidth ? width : kRefitWidth);
+            }
+            blocks += R[m].blocks;
+        }
+        if (blocks == 0) break;
+        k_refit_levels<<<blocks, 512, 0, s2>>>(R[0], R[1]);
+        for (int m = 0; m < 2; ++m) {
+            if (R[m].blocks == 0) continue;
+            if (R[m].lin == 0) R[m].lin = -1; // a single-leaf tree: its leaf is its root
+            else R[m].lin -= (R[m].lin < kRefitLevels ? R[m].lin : kRefitLevels);
+            if (R[m].lin == 0) R[m].lin = -1;
+        }
     }
     (void)hipEventRecord(ctx->ev[EV_B_REFIT], s2);
-    const unsigned nwaves = (Fs + 63) / 64;
+    const unsigned nwaves = (Fs + kWaveLeaves - 1) / kWaveLeaves;
     k_traverse_wave<<<nwaves < 65536 ? nwaves : 65536, 64, 0, s2>>>(T[0], T[1], P_<unsigned long long>(ctx->raw_pairs),
                                                                     ctx->cap_pairs, C);
     (void)hipEventRecord(ctx->ev[EV_B_TRAV], s2);
@@ -794,7 +825,10 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     if (ctx->cap_pairs < 4 * F) ctx->cap_pairs = std::max<uint64_t>(1 << 14, 4 * F);
     if (ctx->cap_tests < 2 * ctx->cap_pairs) ctx->cap_tests = 2 * ctx->cap_pairs;
     if (ctx->cap_esc < (1u << 14)) ctx->cap_esc = std::max<uint64_t>(1 << 14, ctx->cap_tests / 16);
-    if (ctx->cap_reg < (1u << 14)) ctx->cap_reg = std::max<uint64_t>(1 << 14, ctx->cap_tests / 8);
+    // the registry starts within the two-launch merge sort's range and grows
+    // past it (to the radix path) only when a dispatch overflows it
+    if (ctx->cap_reg < (1u << 14))
+        ctx->cap_reg = std::min<uint64_t>(std::max<uint64_t>(1 << 14, ctx->cap_tests / 8), kRegMergeTiles * kRegTile);
     if (ctx->cap_big < 1024) ctx->cap_big = 1024;
     // validate the face lists before anything indexes vertices through them
     enqueue_validate(ctx);

@@ -44,7 +44,13 @@ def compare(ctx, src, cut, detail=True, **kw):
         v0, v1, f0, f1 = R.edges()
         assert np.array_equal(v0, O["edge_v0"]) and np.array_equal(v1, O["edge_v1"])
         assert np.array_equal(f0, O["edge_f0"]) and np.array_equal(f1, O["edge_f1"])
-    if O["stage"] >= 1:  # the oracle got past the face-property stage
+    if O["stage"] >= 1 and not detail:
+        assert R.counts["tests"] == O["counts"]["tests"]
+        if O["status"] == om.SUCCESS or O["stage"] >= 2:
+            e, g, p = R.registry()
+            assert np.array_equal(e, O["reg_edge"]) and np.array_equal(g, O["reg_face"])
+            assert np.array_equal(p.view(np.uint64), O["reg_point"].view(np.uint64))
+    if O["stage"] >= 1 and detail:  # the oracle got past the face-property stage
         P = R.planes()
         assert np.array_equal(P["face"], O["cand_face"])
         assert np.array_equal(P["normal"], O["plane_n"]) and np.array_equal(P["d"], O["plane_d"])
@@ -108,6 +114,14 @@ def test_benchmark_pairs(ctx, i):
 @pytest.mark.parametrize("lvl,amp", [(3, 0.02), (4, 0.02), (5, 0.005), (6, 0.005), (6, 0.02)])
 def test_icosphere_pairs(ctx, lvl, amp):
     compare(ctx, *meshes.icosphere_pair(lvl, amp=amp))
+
+
+def test_icosphere7_benchmark_pair(ctx):
+    """The benchmark workload at full size: 655,360 triangles, about 20k
+    registry entries (several tiles of the registry sort)."""
+    from mcut_amd import meshgen as mg
+    R, O = compare(ctx, *mg.config2(level=7), detail=False)
+    assert R.status == 0 and R.counts["registry"] > 3 * 4096
 
 
 @pytest.mark.parametrize("seed", range(6))
