@@ -154,6 +154,55 @@ __global__ void k_set_perturbation(Params* P, int has_pert, double p0, double p1
     }
 }
 
+// Per-axis tolerance that makes the raw-space broad phase find every pair
+// whose shifted boxes overlap.  With u = 2^-53 and, per axis, E the extent of
+// both meshes' raw box, W its largest magnitude and n the larger vertex count:
+//   |x - com| <= R = E + (n + 2) 2^-52 W   (com is a mean of sequentially
+//                                           summed coordinates),
+//   |t| <= (R + 1)(1 + 4u),
+//   s(x) = x - com + t + d(x) with |d(x)| <= u (3R + 1)(1 + 4u),
+// and the shifted test s(a_lo) <= fl(s(b_hi) + eps) implies
+//   a_lo <= b_hi + eps (1 + u) + u (8R + 4)(1 + 4u),
+// which the raw test a_lo <= fl(b_hi + tol) accepts once tol covers that
+// bound plus the rounding of b_hi + tol (u (W + tol)).  eps is bounded from
+// the cut mesh's raw extent.  tol keeps a factor of 32 over every rounding
+// term.  A bound that is not finite marks the input unsupported (bit 1).
+__global__ void k_tolerance(Params* P, Counters* C, uint32_t nv0, uint32_t nv1, double gp_constant, int absolute)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double u32 = 3.552713678800501e-15; // 2^-48 = 32 u
+    const double n = (double)(nv0 > nv1 ? nv0 : nv1);
+    double cext2 = 0.0, wmax = 0.0, rmax = 0.0, R[3], W[3];
+    bool finite = true;
+    for (int j = 0; j < 3; ++j) {
+        const double l0 = ord_value(C->rbox[0][j]), l1 = ord_value(C->rbox[1][j]);
+        const double h0 = ord_value(C->rbox[0][3 + j]), h1 = ord_value(C->rbox[1][3 + j]);
+        const double lo = l0 < l1 ? l0 : l1, hi = h0 > h1 ? h0 : h1;
+        W[j] = fmax(fabs(lo), fabs(hi));
+        R[j] = (hi - lo) + (n + 2.0) * 2.220446049250313e-16 * W[j];
+        const double ce = h1 - l1;
+        cext2 += ce * ce;
+        wmax = fmax(wmax, W[j]);
+        rmax = fmax(rmax, R[j]);
+        finite = finite && isfinite(lo) && isfinite(hi);
+    }
+    // diag = |(bmax + t) - (bmin + t)| of the cut mesh, each term within
+    // u (2W + 2|t| + E) of the raw extent
+    const double diag_hi = (sqrt(cext2) + u32 * 2.0 * (wmax + 2.0 * rmax + 2.0)) * (1.0 + u32);
+    const double eps_hi = (absolute ? 1.0 : diag_hi) * gp_constant * (1.0 + u32);
+    for (int j = 0; j < 3; ++j) {
+        P->tol[j] = eps_hi * (1.0 + u32) + u32 * (8.0 * R[j] + W[j] + 4.0);
+        finite = finite && isfinite(P->tol[j]);
+    }
+    if (!finite) C->unsupported |= 2u;
+}
+
+__device__ __forceinline__ bool dispatch_aborted(const Counters* C)
+{
+    // bad face lists, or no finite broad-phase bound: nothing may index vertices
+    return C->bad_input != 0 || (C->unsupported & 2u) != 0;
+}
+
 // ps coordinates: ((x - com) + t) + 0 for source vertices, + perturbation for
 // cut vertices (preproc.cpp:244-276)
 __global__ void k_shift_vertices(const double* __restrict__ vs, uint32_t ns, const double* __restrict__ vc, uint32_t nc,
@@ -336,16 +385,18 @@ __global__ void k_edge_assign(const uint32_t* __restrict__ idx, CornerMap cm, ui
 // 4. face AABBs (bvh.cpp:242-313) and the LBVH over them
 // ===========================================================================
 
-__global__ void k_face_boxes(Soup S, const Params* __restrict__ P, double* __restrict__ box, int cut_only)
+__global__ void k_face_boxes(Soup S, const Params* __restrict__ P, const Counters* __restrict__ C, double* __restrict__ box,
+                             int cut_only)
 {
     const uint32_t F = S.nf[0] + S.nf[1];
     const double eps = P->eps;
+    const bool aborted = dispatch_aborted(C);
     for (uint64_t g = (cut_only ? S.nf[0] : 0) + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < F;
          g += (uint64_t)gridDim.x * blockDim.x) {
         const FaceView fv = face_view(S, (uint32_t)g);
         double lo[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
         double hi[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
-        for (uint32_t i = 0; i < fv.n; ++i) {
+        for (uint32_t i = 0; i < (aborted ? 0u : fv.n); ++i) {
             double p[3];
             fv((int)i, p);
             for (int j = 0; j < 3; ++j) {
@@ -424,9 +475,10 @@ __global__ void __launch_bounds__(256) k_raw_bbox(const double* __restrict__ v, 
 __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __restrict__ idx,
                              const uint32_t* __restrict__ off, uint32_t nf, const unsigned long long* __restrict__ rbox,
                              uint32_t* __restrict__ code, uint32_t* __restrict__ face, float* __restrict__ rawbox,
-                             double tol0, double tol1, double tol2, uint32_t mesh_bit)
+                             const Params* __restrict__ P, const Counters* __restrict__ C, int widen, uint32_t mesh_bit)
 {
-    const double tol[3] = {tol0, tol1, tol2};
+    const double tol[3] = {widen ? P->tol[0] : 0.0, widen ? P->tol[1] : 0.0, widen ? P->tol[2] : 0.0};
+    const bool aborted = dispatch_aborted(C);
     double lo[3], ext[3];
     for (int j = 0; j < 3; ++j) {
         lo[j] = ord_value(rbox[j]);
@@ -434,6 +486,14 @@ __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __res
         ext[j] = h > lo[j] ? h - lo[j] : 1.0;
     }
     for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
+        if (aborted) { // empty box: the broad phase finds nothing
+            code[f] = mesh_bit;
+            face[f] = (uint32_t)f;
+            float* o = rawbox + 6 * f;
+            o[0] = o[1] = o[2] = 3.402823466e38f;
+            o[3] = o[4] = o[5] = -3.402823466e38f;
+            continue;
+        }
         const uint32_t b = off[f], n = off[f + 1] - b;
         double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
         double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};

@@ -25,6 +25,7 @@ struct Params {
     double sum[2][3];   // per-mesh sequential coordinate sums
     double bmin[2][3];  // per-mesh raw bounding boxes
     double bmax[2][3];
+    double tol[3];      // raw-space broad-phase tolerance per axis (k_tolerance)
 };
 
 struct Counters {
@@ -35,11 +36,19 @@ struct Counters {
     unsigned long long pmask; // pair hash slots - 1 (set once the raw pairs are known)
     double zero;              // +0.0 (the per-dispatch reset clears it): padding the serial sums load
     unsigned int ov_pairs, ov_tests, ov_esc, ov_reg, ov_big;
+    // unsupported: bit 0 a candidate polygon of more than 16 vertices, bit 1
+    // coordinates whose broad-phase rounding bound is not finite
     unsigned int gp, bad_src, bad_cut, unsupported, nonmanifold, bad_input, pair_fail, partial;
     unsigned int pad[2];
     // raw bounding box of each mesh as order keys (lo xyz, hi xyz); written
     // while the input is validated and kept by the per-dispatch counter reset
     unsigned long long rbox[2][6];
+};
+
+// the per-dispatch scalars the host reads back, in one allocation (one copy)
+struct DevState {
+    Counters C;
+    Params P;
 };
 
 // polygon soup view: source faces [0,Fs), cut faces [Fs, Fs+Fc); vertices likewise

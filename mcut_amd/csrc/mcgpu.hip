@@ -78,7 +78,7 @@ struct mcgpu_context_t {
     hipError_t last_error = hipSuccess;
 
     Buf in_v[2], in_idx[2], in_sz[2];
-    Buf off[2], params, counters, ps_v, box;
+    Buf off[2], state, ps_v, box; // state: DevState (counters and shift parameters)
     Buf hkeys[2], hfirst[2], eflag[2], eid[2], cedge[2], cface[2], etotal;
     Buf ev0, ev1, ef0, ef1;
     Buf mcode, mface, mcode2, mface2; // Morton keys and leaf order of both meshes
@@ -104,11 +104,11 @@ struct mcgpu_context_t {
     int keep_tests = 0;
     bool mface_alt = false;
     int depth[2] = {0, 0};
-    double tol[3] = {0, 0, 0}; // raw-space broad-phase tolerance per axis
     bool reg_alt = false;
     Counters C;
     Params P;
     float stage_ms[ST_COUNT];
+    bool stages_ready = false;
     int stage_mode = 0; // 0: full dispatch (three streams), 1: perturbed narrow-phase rerun
     float total_ms = 0.f;
 };
@@ -226,10 +226,8 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
     const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1];
     const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
     void* p;
-    Params* pp;
-    Counters* cc;
-    CK(ensure(ctx->params, 1, &pp));
-    CK(ensure(ctx->counters, 1, &cc));
+    DevState* st;
+    CK(ensure(ctx->state, 1, &st));
     double* dp;
     CK(ensure(ctx->ps_v, 3 * V, &dp));
     CK(ensure(ctx->box, 6 * F, &dp));
@@ -324,7 +322,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
 {
     hipStream_t s = ctx->s;
     const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
-    Counters* C = P_<Counters>(ctx->counters);
+    Counters* C = &P_<DevState>(ctx->state)->C;
     k_planes<<<grid_for(F), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->cand_list), P_<Plane>(ctx->planes),
                                              P_<uint32_t>(ctx->big_list), ctx->cap_big, C);
     k_big_polygons<<<4, 64, 0, s>>>(S, P_<uint32_t>(ctx->big_list), C, ctx->cap_big, P_<Plane>(ctx->planes),
@@ -382,13 +380,18 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
     (void)hipEventRecord(ctx->ev[EV_A_END], s);
 }
 
-// face offsets and input validation; the caller syncs and rejects bad input
-// before any kernel indexes vertices through the face lists
+// Per-dispatch reset, raw bounding boxes, face offsets, input validation and
+// the broad-phase tolerance, all on stream A ahead of the fork.  Nothing waits
+// for the host: the kernels that first read coordinates through the face
+// lists write empty boxes when the input was rejected, so no later kernel
+// indexes a vertex through a bad face list, and the host reports the
+// rejection after the dispatch.
 void enqueue_validate(mcgpu_context ctx)
 {
     hipStream_t s = ctx->s;
-    Counters* C = P_<Counters>(ctx->counters);
+    Counters* C = &P_<DevState>(ctx->state)->C;
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
+    (void)hipEventRecord(ctx->ev[EV_A0], s);
     fill_u32(C, 0u, sizeof(Counters) / 4, s);
     for (int m = 0; m < 2; ++m) {
         fill_u64(C->rbox[m], ~0ull, 3, s);
@@ -402,6 +405,9 @@ void enqueue_validate(mcgpu_context ctx)
         }
         k_validate<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(ctx->didx[m], off, ctx->dsz[m], ctx->nf[m], ctx->nv[m], C);
     }
+    const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
+    k_tolerance<<<1, 1, 0, s>>>(&P_<DevState>(ctx->state)->P, C, ctx->nv[0], ctx->nv[1], ctx->gp_constant,
+                                absolute ? 1 : 0);
 }
 
 void enqueue_frontend(mcgpu_context ctx)
@@ -409,12 +415,10 @@ void enqueue_frontend(mcgpu_context ctx)
     hipStream_t s = ctx->s, s2 = ctx->s2, s3 = ctx->s3;
     const uint32_t Fs = ctx->nf[0];
     const uint64_t F = (uint64_t)Fs + ctx->nf[1], V = (uint64_t)ctx->nv[0] + ctx->nv[1];
-    Counters* C = P_<Counters>(ctx->counters);
-    Params* Pd = P_<Params>(ctx->params);
+    Counters* C = &P_<DevState>(ctx->state)->C;
+    Params* Pd = &P_<DevState>(ctx->state)->P;
 
     // ---- stream A, enqueued first: the serial sums start at once ----
-    (void)hipEventRecord(ctx->ev[EV_A0], s);
-    fill_u32(C, 0u, offsetof(Counters, rbox) / 4, s); // keeps the raw boxes of the validation pass
     (void)hipEventRecord(ctx->ev[EV_FORK], s);
     (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_FORK], 0);
     (void)hipEventRecord(ctx->ev[EV_S0], ctx->ssum);
@@ -427,7 +431,7 @@ void enqueue_frontend(mcgpu_context ctx)
     k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
                                                         P_<double>(ctx->ps_v), 0);
     const Soup S = make_soup(ctx);
-    k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, Pd, P_<double>(ctx->box), 0);
+    k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, Pd, C, P_<double>(ctx->box), 0);
     (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
 
     // ---- stream B: raw-space trees and broad phase ----
@@ -438,11 +442,10 @@ void enqueue_frontend(mcgpu_context ctx)
     for (int m = 0; m < 2; ++m) {
         const uint32_t nf = ctx->nf[m];
         const uint64_t o = m ? Fs : 0;
-        const double* tl = m ? ctx->tol : nullptr;
         k_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf,
                                                       C->rbox[m], P_<uint32_t>(ctx->mcode) + o,
-                                                      P_<uint32_t>(ctx->mface) + o, rawbox + 6 * o, tl ? tl[0] : 0.0,
-                                                      tl ? tl[1] : 0.0, tl ? tl[2] : 0.0, m ? (1u << 21) : 0u);
+                                                      P_<uint32_t>(ctx->mface) + o, rawbox + 6 * o, Pd, C, m,
+                                                      m ? (1u << 21) : 0u);
     }
     // one stable sort orders the leaves of both trees (the mesh is the top key bit)
     ctx->mface_alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode), P_<uint32_t>(ctx->mface), P_<uint32_t>(ctx->mcode2),
@@ -548,13 +551,13 @@ void enqueue_frontend(mcgpu_context ctx)
 
 mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow)
 {
+    // stream A joins the other three before its last kernel, so it finishes last
     CK(hipStreamSynchronize(ctx->s));
-    CK(hipStreamSynchronize(ctx->s2));
-    CK(hipStreamSynchronize(ctx->s3));
-    CK(hipStreamSynchronize(ctx->ssum));
     CK(hipGetLastError());
-    CK(hipMemcpy(&ctx->C, ctx->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
-    CK(hipMemcpy(&ctx->P, ctx->params.p, sizeof(Params), hipMemcpyDeviceToHost));
+    DevState hs;
+    CK(hipMemcpy(&hs, ctx->state.p, sizeof(DevState), hipMemcpyDeviceToHost));
+    ctx->C = hs.C;
+    ctx->P = hs.P;
     const Counters& C = ctx->C;
     *overflow = false;
     if (getenv("MCGPU_DEBUG"))
@@ -567,6 +570,8 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
                 C.bad_input, C.pair_fail, C.partial, (unsigned long long)ctx->cap_pairs,
                 (unsigned long long)ctx->cap_tests, (unsigned long long)ctx->cap_esc, (unsigned long long)ctx->cap_reg,
                 C.n_fallback, C.n_lane_walks);
+    if (C.bad_input || C.nonmanifold) return MCGPU_ERROR_INVALID_VALUE;
+    if (C.unsupported & 2u) return MCGPU_ERROR_UNSUPPORTED;
     if (C.ov_pairs || C.ov_tests || C.ov_esc || C.ov_reg || C.ov_big) {
         *overflow = true;
         if (C.ov_pairs) ctx->cap_pairs = std::max<uint64_t>(2 * ctx->cap_pairs, C.n_raw + C.n_raw / 4);
@@ -576,28 +581,13 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         if (C.ov_big) ctx->cap_big = std::max<uint64_t>(2 * ctx->cap_big, C.n_big + C.n_big / 4);
         return MCGPU_SUCCESS;
     }
-    if (C.bad_input || C.nonmanifold) return MCGPU_ERROR_INVALID_VALUE;
     if (C.unsupported) return MCGPU_ERROR_UNSUPPORTED;
     auto el = [&](int a, int b) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) ms = 0.f;
         return ms;
     };
-    // stages of streams B and C run only in a full dispatch; A's stages after a
-    // join start at the event recorded once the join completed
-    const bool full = ctx->stage_mode == 0;
-    ctx->stage_ms[ST_SUMS] = full ? el(EV_S0, EV_A_SUMS) : 0.f;
-    ctx->stage_ms[ST_PARAMS] = full ? el(EV_A_SUMS, EV_A_PARAMS) : el(EV_A0, EV_A_PARAMS);
-    ctx->stage_ms[ST_SHIFT] = el(EV_A_PARAMS, EV_A_SHIFT);
-    ctx->stage_ms[ST_TOPOLOGY] = full ? el(EV_C0, EV_C_TOPO) : 0.f;
-    ctx->stage_ms[ST_BVH_SORT] = full ? el(EV_B0, EV_B_SORT) : 0.f;
-    ctx->stage_ms[ST_REFIT] = full ? el(EV_B_SORT, EV_B_REFIT) : 0.f;
-    ctx->stage_ms[ST_TRAVERSE] = full ? el(EV_B_REFIT, EV_B_TRAV) : 0.f;
-    ctx->stage_ms[ST_FILTER] = full ? el(EV_A_JOIN, EV_A_FILTER) : 0.f;
-    ctx->stage_ms[ST_PLANES] = el(EV_A_FILTER, EV_A_PLANES);
-    ctx->stage_ms[ST_TESTS] = el(EV_A_JOIN2, EV_A_TESTS);
-    ctx->stage_ms[ST_NARROW] = el(EV_A_TESTS, EV_A_NARROW);
-    ctx->stage_ms[ST_REGISTRY] = el(EV_A_NARROW, EV_A_END);
+    ctx->stages_ready = false; // stage times are read from the events on request
     float tot = el(EV_A0, EV_A_END);
     ctx->total_ms = tot;
     ctx->have = true;
@@ -622,51 +612,31 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
     return MCGPU_SUCCESS;
 }
 
-double host_ord_value(unsigned long long k)
+// per-stage times of the last dispatch from its events (the events stay
+// valid until the next dispatch records them again)
+void compute_stage_times(mcgpu_context ctx)
 {
-    const unsigned long long u = (k & 0x8000000000000000ull) ? (k & ~0x8000000000000000ull) : ~k;
-    double x;
-    std::memcpy(&x, &u, sizeof x);
-    return x;
-}
-
-// Per-axis tolerance that makes the raw-space broad phase find every pair
-// whose shifted boxes overlap.  With u = 2^-53 and, per axis, E the extent of
-// both meshes' raw box, W its largest magnitude and n the larger vertex count:
-//   |x - com| <= R = E + (n + 2) 2^-52 W   (com is a mean of sequentially
-//                                           summed coordinates),
-//   |t| <= (R + 1)(1 + 4u),
-//   s(x) = x - com + t + d(x) with |d(x)| <= u (3R + 1)(1 + 4u),
-// and the shifted test s(a_lo) <= fl(s(b_hi) + eps) implies
-//   a_lo <= b_hi + eps (1 + u) + u (8R + 4)(1 + 4u),
-// which the raw test a_lo <= fl(b_hi + tol) accepts once tol covers that
-// bound plus the rounding of b_hi + tol (u (W + tol)).  eps is bounded from
-// the cut mesh's raw extent.  tol below keeps a factor of 32 over every
-// rounding term.  Returns false when the bound is not finite.
-bool broad_phase_tolerance(const Counters& c0, const uint32_t* nv, double gp_constant, bool absolute, double* tol)
-{
-    const double u32 = std::ldexp(1.0, -48); // 32 u
-    const double n = (double)std::max(nv[0], nv[1]);
-    double lo[3], hi[3], cext2 = 0.0, wmax = 0.0, rmax = 0.0, R[3], W[3];
-    for (int j = 0; j < 3; ++j) {
-        lo[j] = std::min(host_ord_value(c0.rbox[0][j]), host_ord_value(c0.rbox[1][j]));
-        hi[j] = std::max(host_ord_value(c0.rbox[0][3 + j]), host_ord_value(c0.rbox[1][3 + j]));
-        W[j] = std::max(std::fabs(lo[j]), std::fabs(hi[j]));
-        R[j] = (hi[j] - lo[j]) + (n + 2.0) * std::ldexp(1.0, -52) * W[j];
-        const double ce = host_ord_value(c0.rbox[1][3 + j]) - host_ord_value(c0.rbox[1][j]);
-        cext2 += ce * ce;
-        wmax = std::max(wmax, W[j]);
-        rmax = std::max(rmax, R[j]);
-    }
-    // diag = |(bmax + t) - (bmin + t)| of the cut mesh, each term within
-    // u (2W + 2|t| + E) of the raw extent
-    const double diag_hi = (std::sqrt(cext2) + u32 * 2.0 * (wmax + 2.0 * rmax + 2.0)) * (1.0 + u32);
-    const double eps_hi = (absolute ? 1.0 : diag_hi) * gp_constant * (1.0 + u32);
-    for (int j = 0; j < 3; ++j) {
-        tol[j] = eps_hi * (1.0 + u32) + u32 * (8.0 * R[j] + W[j] + 4.0);
-        if (!std::isfinite(tol[j]) || !std::isfinite(lo[j]) || !std::isfinite(hi[j])) return false;
-    }
-    return true;
+    auto el = [&](int a, int b) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) ms = 0.f;
+        return ms;
+    };
+    // stages of streams B and C run only in a full dispatch; A's stages after a
+    // join start at the event recorded once the join completed
+    const bool full = ctx->stage_mode == 0;
+    ctx->stage_ms[ST_SUMS] = full ? el(EV_S0, EV_A_SUMS) : 0.f;
+    ctx->stage_ms[ST_PARAMS] = full ? el(EV_A_SUMS, EV_A_PARAMS) : el(EV_A0, EV_A_PARAMS);
+    ctx->stage_ms[ST_SHIFT] = el(EV_A_PARAMS, EV_A_SHIFT);
+    ctx->stage_ms[ST_TOPOLOGY] = full ? el(EV_C0, EV_C_TOPO) : 0.f;
+    ctx->stage_ms[ST_BVH_SORT] = full ? el(EV_B0, EV_B_SORT) : 0.f;
+    ctx->stage_ms[ST_REFIT] = full ? el(EV_B_SORT, EV_B_REFIT) : 0.f;
+    ctx->stage_ms[ST_TRAVERSE] = full ? el(EV_B_REFIT, EV_B_TRAV) : 0.f;
+    ctx->stage_ms[ST_FILTER] = full ? el(EV_A_JOIN, EV_A_FILTER) : 0.f;
+    ctx->stage_ms[ST_PLANES] = el(EV_A_FILTER, EV_A_PLANES);
+    ctx->stage_ms[ST_TESTS] = el(EV_A_JOIN2, EV_A_TESTS);
+    ctx->stage_ms[ST_NARROW] = el(EV_A_TESTS, EV_A_NARROW);
+    ctx->stage_ms[ST_REGISTRY] = el(EV_A_NARROW, EV_A_END);
+    ctx->stages_ready = true;
 }
 
 mcgpu_result upload_mesh(mcgpu_context ctx, int m, const mcgpu_mesh* mesh, bool on_device)
@@ -783,7 +753,7 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
     (void)hipStreamSynchronize(ctx->s3);
     (void)hipStreamSynchronize(ctx->ssum);
     Buf* all[] = {&ctx->in_v[0], &ctx->in_v[1], &ctx->in_idx[0], &ctx->in_idx[1], &ctx->in_sz[0], &ctx->in_sz[1],
-                  &ctx->off[0], &ctx->off[1], &ctx->params, &ctx->counters, &ctx->ps_v, &ctx->box, &ctx->hkeys[0],
+                  &ctx->off[0], &ctx->off[1], &ctx->state, &ctx->ps_v, &ctx->box, &ctx->hkeys[0],
                   &ctx->hkeys[1], &ctx->hfirst[0], &ctx->hfirst[1], &ctx->cface[0], &ctx->cface[1], &ctx->eflag[0],
                   &ctx->eflag[1], &ctx->eid[0], &ctx->eid[1], &ctx->cedge[0], &ctx->cedge[1], &ctx->etotal, &ctx->ev0,
                   &ctx->ev1, &ctx->ef0, &ctx->ef1, &ctx->mcode, &ctx->mface, &ctx->mcode2, &ctx->mface2, &ctx->nodes[0], &ctx->nodes[1],
@@ -830,21 +800,10 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     if (ctx->cap_reg < (1u << 14))
         ctx->cap_reg = std::min<uint64_t>(std::max<uint64_t>(1 << 14, ctx->cap_tests / 8), kRegMergeTiles * kRegTile);
     if (ctx->cap_big < 1024) ctx->cap_big = 1024;
-    // validate the face lists before anything indexes vertices through them
-    enqueue_validate(ctx);
-    CK(hipStreamSynchronize(ctx->s));
-    CK(hipGetLastError());
-    {
-        Counters c0;
-        CK(hipMemcpy(&c0, ctx->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
-        if (c0.bad_input) return MCGPU_ERROR_INVALID_VALUE;
-        if (!broad_phase_tolerance(c0, ctx->nv, ctx->gp_constant,
-                                   (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0, ctx->tol))
-            return MCGPU_ERROR_UNSUPPORTED;
-    }
     for (int attempt = 0; attempt < 6; ++attempt) {
         if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
         ctx->stage_mode = 0;
+        enqueue_validate(ctx);
         enqueue_frontend(ctx);
         bool overflow = false;
         if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
@@ -865,14 +824,14 @@ MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const doubl
         if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
         hipStream_t s = ctx->s;
         const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1];
-        Counters* C = P_<Counters>(ctx->counters);
+        Counters* C = &P_<DevState>(ctx->state)->C;
         ctx->stage_mode = 1;
         (void)hipEventRecord(ctx->ev[EV_A0], s);
         k_reset_narrow<<<1, 1, 0, s>>>(C);
-        k_set_perturbation<<<1, 1, 0, s>>>(P_<Params>(ctx->params), 1, perturbation[0], perturbation[1], perturbation[2]);
+        k_set_perturbation<<<1, 1, 0, s>>>(&P_<DevState>(ctx->state)->P, 1, perturbation[0], perturbation[1], perturbation[2]);
         (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
         k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1],
-                                                            P_<Params>(ctx->params), P_<double>(ctx->ps_v), 1);
+                                                            &P_<DevState>(ctx->state)->P, P_<double>(ctx->ps_v), 1);
         (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
         (void)hipEventRecord(ctx->ev[EV_A_JOIN], s);
         (void)hipEventRecord(ctx->ev[EV_A_FILTER], s);
@@ -1010,6 +969,7 @@ MCGPU_API mcgpu_result mcgpu_get_ps_vertices(mcgpu_context ctx, double* xyz, uin
 MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity)
 {
     if (!ctx || !ms || !ctx->have) return 0;
+    if (!ctx->stages_ready) compute_stage_times(ctx);
     int n = std::min(capacity, (int)ST_COUNT);
     for (int i = 0; i < n; ++i) ms[i] = ctx->stage_ms[i];
     return n;

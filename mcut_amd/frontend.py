@@ -177,9 +177,26 @@ class Result:
         return out
 
     def stage_ms(self):
+        """Device time of each pipeline stage of this dispatch, in ms (HIP events)."""
         ms = np.zeros(16, np.float32)
         n = lib().mcgpu_stage_times(self._ctx.h, _ptr(ms), 16)
-        return {lib().mcgpu_stage_name(i).decode(): float(ms[i]) for i in range(n)}
+        return {name: float(ms[i]) for i, name in enumerate(stage_names()[:n])}
+
+
+_stage_names = None
+
+
+def stage_names():
+    global _stage_names
+    if _stage_names is None:
+        names = []
+        while True:
+            nm = lib().mcgpu_stage_name(len(names)).decode()
+            if not nm:
+                break
+            names.append(nm)
+        _stage_names = names
+    return _stage_names
 
 
 class Context:

@@ -174,3 +174,16 @@ def test_invalid_input_rejected(ctx):
     assert ei.value.code == -1
     # the context stays usable
     compare(ctx, *meshes.config1_rotated())
+
+
+def test_coordinates_beyond_the_broad_phase_bound(ctx):
+    """Coordinates whose squared extent overflows (1e200) leave the raw-space
+    broad phase without a finite rounding bound: the dispatch reports
+    MCGPU_ERROR_UNSUPPORTED and the context stays usable."""
+    from mcut_amd import frontend, meshgen as mg
+    V, F, _ = mg.uv_sphere(8, 4)
+    Vc, Fc, Sc = mg.tessellated_box((0, 0, 0), 1.0, 2)
+    with pytest.raises(frontend.McgpuError) as ei:
+        ctx.dispatch((V * 1e200, F, None), (Vc * 1e200, Fc, Sc))
+    assert ei.value.code == -4
+    compare(ctx, *meshes.config1_rotated())
