@@ -132,6 +132,18 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
 MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const double* perturbation,
                                                   uint32_t enforcement_count, mcgpu_frontend_info* info);
 
+/* Broad phase alone, at the seam of the reference's build_oibvh (bvh.cpp:219)
+ * and intersectOIBVHs (bvh.cpp:638): given the face AABBs of both meshes
+ * (6 doubles per face: min xyz, max xyz, as build_oibvh's face_bboxes), finds
+ * every (source face, cut face) pair whose boxes overlap, with the inclusive
+ * test of intersect_bounding_boxes (math.h:931-941).  The set is the one
+ * intersectOIBVHs reports; read it with mcgpu_get_candidate_pairs.  Boxes are
+ * device pointers when input_on_device is 1.  After this call only the
+ * candidate pairs are available from ctx. */
+MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_boxes, uint32_t num_src,
+                                         const double* cut_boxes, uint32_t num_cut, uint32_t input_on_device,
+                                         uint64_t* num_pairs);
+
 /* --- results of the last dispatch (host copies; capacity in elements) ----- */
 
 /* Intersection registry sorted by (edge, face): m0_ivtx_to_intersection_registry_entry

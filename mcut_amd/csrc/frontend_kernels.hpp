@@ -522,6 +522,75 @@ __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __res
     }
 }
 
+// Broad phase over caller-given face boxes (min xyz, max xyz, doubles): the
+// union box of one mesh's face boxes, as order keys ...
+__global__ void __launch_bounds__(256) k_box_bounds(const double* __restrict__ box, uint32_t nf,
+                                                    unsigned long long* __restrict__ rbox)
+{
+    __shared__ double red[4][6];
+    double lo[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+    double hi[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x)
+        for (int j = 0; j < 3; ++j) {
+            const double a = box[6 * f + j], b = box[6 * f + 3 + j];
+            lo[j] = a < lo[j] ? a : lo[j];
+            hi[j] = b > hi[j] ? b : hi[j];
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int j = 0; j < 3; ++j) {
+            const double a = __shfl_xor(lo[j], o, 64), b = __shfl_xor(hi[j], o, 64);
+            lo[j] = a < lo[j] ? a : lo[j];
+            hi[j] = b > hi[j] ? b : hi[j];
+        }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int j = 0; j < 3; ++j) {
+            red[w][j] = lo[j];
+            red[w][3 + j] = hi[j];
+        }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int j = threadIdx.x;
+        double a = red[0][j], b = red[0][3 + j];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            a = red[q][j] < a ? red[q][j] : a;
+            b = red[q][3 + j] > b ? red[q][3 + j] : b;
+        }
+        atomicMin(&rbox[j], ord_key(a));
+        atomicMax(&rbox[3 + j], ord_key(b));
+    }
+}
+
+// ... and the leaf keys and single-precision leaf boxes (rounded outward) of
+// those face boxes, as k_morton_raw makes them from vertices
+__global__ void k_morton_boxes(const double* __restrict__ box, uint32_t nf, const unsigned long long* __restrict__ rbox,
+                               uint32_t* __restrict__ code, uint32_t* __restrict__ face, float* __restrict__ fbox,
+                               uint32_t mesh_bit)
+{
+    double lo[3], ext[3];
+    for (int j = 0; j < 3; ++j) {
+        lo[j] = ord_value(rbox[j]);
+        const double h = ord_value(rbox[3 + j]);
+        ext[j] = h > lo[j] ? h - lo[j] : 1.0;
+    }
+    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
+        const double* b = box + 6 * f;
+        uint32_t c[3];
+        for (int j = 0; j < 3; ++j) {
+            float x = (float)((((b[j] + b[3 + j]) * 0.5) - lo[j]) / ext[j]);
+            x = fminf(fmaxf(x * 128.0f, 0.0f), 127.0f); // NaN centres clamp to a cell
+            c[j] = expand_bits10((uint32_t)x);
+        }
+        code[f] = (c[0] * 4 + c[1] * 2 + c[2]) | mesh_bit;
+        face[f] = (uint32_t)f;
+        float* o = fbox + 6 * f;
+        for (int j = 0; j < 3; ++j) {
+            o[j] = __double2float_rd(b[j]);
+            o[3 + j] = __double2float_ru(b[3 + j]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Implicit complete binary tree over the Morton-ordered leaves (the layout
 // family of the reference's OIBVH, bvh.cpp:81-193): leaf level D holds 2^D
@@ -882,10 +951,15 @@ __global__ void k_pair_hash_init(unsigned long long* __restrict__ phash, uint64_
         phash[i] = kEmpty64;
 }
 
-constexpr int kFilterItems = 4;
-constexpr int kFilterTile = kBlock * kFilterItems;
+// The staged-append kernels run 1024-thread blocks with one item per thread:
+// every item is a short chain of dependent gathers, so more threads in flight
+// shorten the kernel, while the tile stays large enough that the one global
+// atomic per tile flush does not serialise the device.
+constexpr int kStageBlock = 1024;
+constexpr int kFilterItems = 1;
+constexpr int kFilterTile = kStageBlock * kFilterItems;
 
-__global__ void __launch_bounds__(kBlock) k_filter_pairs(const double* __restrict__ box,
+__global__ void __launch_bounds__(kStageBlock) k_filter_pairs(const double* __restrict__ box,
                                                          const unsigned long long* __restrict__ raw, uint64_t cap,
                                                          uint32_t Fs, unsigned long long* __restrict__ pairs,
                                                          unsigned long long* __restrict__ phash,
@@ -902,7 +976,7 @@ __global__ void __launch_bounds__(kBlock) k_filter_pairs(const double* __restric
         if (threadIdx.x == 0) st_np = st_nf = 0;
         __syncthreads();
         for (int rr = 0; rr < kFilterItems; ++rr) {
-            const uint64_t i = t0 + (uint64_t)rr * kBlock + threadIdx.x;
+            const uint64_t i = t0 + (uint64_t)rr * kStageBlock + threadIdx.x;
             if (i >= n) continue;
             const uint64_t k = raw[i];
             const uint32_t f[2] = {(uint32_t)(k >> 32), Fs + (uint32_t)k};
@@ -921,10 +995,10 @@ __global__ void __launch_bounds__(kBlock) k_filter_pairs(const double* __restric
         __syncthreads();
         const uint32_t np = st_np, nfc = st_nf;
         const unsigned long long pb = block_reserve(&C->n_pairs, np, &st_base);
-        for (uint32_t q = threadIdx.x; q < np; q += kBlock) pairs[pb + q] = st_pair[q]; // pb + np <= n <= cap
+        for (uint32_t q = threadIdx.x; q < np; q += kStageBlock) pairs[pb + q] = st_pair[q]; // pb + np <= n <= cap
         __syncthreads();
         const unsigned long long fb = block_reserve(&C->n_cand, nfc, &st_base);
-        for (uint32_t q = threadIdx.x; q < nfc; q += kBlock) cand_list[fb + q] = st_face[q]; // one entry per face
+        for (uint32_t q = threadIdx.x; q < nfc; q += kStageBlock) cand_list[fb + q] = st_face[q]; // one entry per face
         __syncthreads();
     }
 }
@@ -1173,11 +1247,11 @@ __device__ __forceinline__ bool edge_face_cull(const Soup& S, uint32_t e, uint32
 // six work items per pair: (side, corner slot); a polygon's corners beyond
 // the third are taken by the same slot in steps of three.  A block takes a
 // tile of kTestTile items and stages the tests it emits in LDS.
-constexpr int kTestItems = 4;
-constexpr int kTestTile = kBlock * kTestItems;
+constexpr int kTestItems = 1;
+constexpr int kTestTile = kStageBlock * kTestItems;
 constexpr int kTestStage = 2048;
 
-__global__ void __launch_bounds__(kBlock) k_tests(Soup S, const unsigned long long* __restrict__ pairs, uint64_t cap_pairs,
+__global__ void __launch_bounds__(kStageBlock) k_tests(Soup S, const unsigned long long* __restrict__ pairs, uint64_t cap_pairs,
                                                   const unsigned long long* __restrict__ phash,
                                                   uint32_t* __restrict__ te, uint32_t* __restrict__ tg, uint64_t cap_tests,
                                                   Counters* C)
@@ -1193,7 +1267,7 @@ __global__ void __launch_bounds__(kBlock) k_tests(Soup S, const unsigned long lo
         if (threadIdx.x == 0) st_n = 0;
         __syncthreads();
         for (int rr = 0; rr < kTestItems; ++rr) {
-            const uint64_t w = t0 + (uint64_t)rr * kBlock + threadIdx.x;
+            const uint64_t w = t0 + (uint64_t)rr * kStageBlock + threadIdx.x;
             if (w >= items) continue;
             const uint64_t i = w / 6;
             const uint32_t slot = (uint32_t)(w - 6 * i);
@@ -1234,7 +1308,7 @@ __global__ void __launch_bounds__(kBlock) k_tests(Soup S, const unsigned long lo
         __syncthreads();
         const uint32_t m = st_n < (uint32_t)kTestStage ? st_n : (uint32_t)kTestStage;
         const unsigned long long base = block_reserve(&C->n_tests, m, &st_base);
-        for (uint32_t q = threadIdx.x; q < m; q += kBlock) {
+        for (uint32_t q = threadIdx.x; q < m; q += kStageBlock) {
             const unsigned long long at = base + q;
             if (at < cap_tests) {
                 te[at] = st_e[q];
@@ -1333,10 +1407,10 @@ __device__ inline bool resolve_test(const Soup& S, const Plane* planes, uint32_t
 
 // A block takes a tile of kNarrowTile tests; registry entries and escalated
 // tests (at most one of either per test) are staged in LDS.
-constexpr int kNarrowItems = 4;
-constexpr int kNarrowTile = kBlock * kNarrowItems;
+constexpr int kNarrowItems = 1;
+constexpr int kNarrowTile = kStageBlock * kNarrowItems;
 
-__global__ void __launch_bounds__(kBlock) k_narrow(Soup S, const Plane* __restrict__ planes, const uint32_t* __restrict__ te,
+__global__ void __launch_bounds__(kStageBlock) k_narrow(Soup S, const Plane* __restrict__ planes, const uint32_t* __restrict__ te,
                                                    const uint32_t* __restrict__ tg, uint64_t cap_tests,
                                                    uint32_t* __restrict__ esc, uint64_t cap_esc, TestOut TO, RegOut R,
                                                    Counters* C)
@@ -1351,7 +1425,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(Soup S, const Plane* __restri
         if (threadIdx.x == 0) st_nr = st_ne = 0;
         __syncthreads();
         for (int rr = 0; rr < kNarrowItems; ++rr) {
-            const uint64_t i = t0 + (uint64_t)rr * kBlock + threadIdx.x;
+            const uint64_t i = t0 + (uint64_t)rr * kStageBlock + threadIdx.x;
             if (i >= n) continue;
             const uint32_t e = te[i], g = tg[i];
             const Plane pl = planes[g];
@@ -1390,10 +1464,10 @@ __global__ void __launch_bounds__(kBlock) k_narrow(Soup S, const Plane* __restri
         __syncthreads();
         const uint32_t nr = st_nr, ne = st_ne;
         const unsigned long long rb = block_reserve(&C->n_reg, nr, &st_base);
-        for (uint32_t q = threadIdx.x; q < nr; q += kBlock) store_registry(R, rb + q, st_key[q], st_p + 3 * q, C);
+        for (uint32_t q = threadIdx.x; q < nr; q += kStageBlock) store_registry(R, rb + q, st_key[q], st_p + 3 * q, C);
         __syncthreads();
         const unsigned long long eb = block_reserve(&C->n_esc, ne, &st_base);
-        for (uint32_t q = threadIdx.x; q < ne; q += kBlock) {
+        for (uint32_t q = threadIdx.x; q < ne; q += kStageBlock) {
             if (eb + q < cap_esc) esc[eb + q] = st_esc[q];
             else atomicOr(&C->ov_esc, 1u);
         }

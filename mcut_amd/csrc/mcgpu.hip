@@ -93,6 +93,7 @@ struct mcgpu_context_t {
 
     // state of the last dispatch
     bool have = false;
+    bool broad_only = false; // the last call was mcgpu_broad_phase: only its pairs are valid
     uint32_t nv[2] = {0, 0}, nf[2] = {0, 0};
     uint64_t ncorner[2] = {0, 0};
     const double* dv[2] = {nullptr, nullptr};
@@ -330,7 +331,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
     (void)hipEventRecord(ctx->ev[EV_A_PLANES], s);
     if (join_topology) (void)hipStreamWaitEvent(s, ctx->ev[EV_C_TOPO], 0);
     (void)hipEventRecord(ctx->ev[EV_A_JOIN2], s);
-    k_tests<<<grid_for(6 * ctx->cap_pairs, kTestTile, 2048), kBlock, 0, s>>>(S, P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
+    k_tests<<<grid_for(6 * ctx->cap_pairs, kTestTile, 2048), kStageBlock, 0, s>>>(S, P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
                                                         P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->te),
                                                         P_<uint32_t>(ctx->tg), ctx->cap_tests, C);
     (void)hipEventRecord(ctx->ev[EV_A_TESTS], s);
@@ -338,7 +339,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
     if (ctx->keep_tests)
         TO = TestOut{P_<int8_t>(ctx->tqs), P_<int8_t>(ctx->trs), P_<char>(ctx->ttype), P_<char>(ctx->tpip)};
     RegOut R{P_<unsigned long long>(ctx->rkey), P_<double>(ctx->rp), P_<uint32_t>(ctx->rval), ctx->cap_reg, ctx->gbits};
-    k_narrow<<<grid_for(ctx->cap_tests, kNarrowTile, 2048), kBlock, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
+    k_narrow<<<grid_for(ctx->cap_tests, kNarrowTile, 2048), kStageBlock, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
                                                          P_<uint32_t>(ctx->tg), ctx->cap_tests, P_<uint32_t>(ctx->esc),
                                                          ctx->cap_esc, TO, R, C);
     k_narrow_exact<<<grid_for(ctx->cap_esc, 64, 2048), 64, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
@@ -410,47 +411,19 @@ void enqueue_validate(mcgpu_context ctx)
                                 absolute ? 1 : 0);
 }
 
-void enqueue_frontend(mcgpu_context ctx)
+// Both trees and the raw-space traversal, on stream st, once the leaf keys
+// (mcode, mface) and single-precision leaf boxes (rawbox) are written: one
+// stable sort orders the leaves of both trees (the mesh is the top key bit),
+// the implicit trees are refitted, and the traversal appends raw pairs.
+void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C)
 {
-    hipStream_t s = ctx->s, s2 = ctx->s2, s3 = ctx->s3;
     const uint32_t Fs = ctx->nf[0];
-    const uint64_t F = (uint64_t)Fs + ctx->nf[1], V = (uint64_t)ctx->nv[0] + ctx->nv[1];
-    Counters* C = &P_<DevState>(ctx->state)->C;
-    Params* Pd = &P_<DevState>(ctx->state)->P;
-
-    // ---- stream A, enqueued first: the serial sums start at once ----
-    (void)hipEventRecord(ctx->ev[EV_FORK], s);
-    (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_FORK], 0);
-    (void)hipEventRecord(ctx->ev[EV_S0], ctx->ssum);
-    k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], &C->zero, Pd);
-    (void)hipEventRecord(ctx->ev[EV_A_SUMS], ctx->ssum);
-    (void)hipStreamWaitEvent(s, ctx->ev[EV_A_SUMS], 0);
-    const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
-    k_shift_params<<<1, 1, 0, s>>>(Pd, C, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
-    (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
-    k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
-                                                        P_<double>(ctx->ps_v), 0);
-    const Soup S = make_soup(ctx);
-    k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, Pd, C, P_<double>(ctx->box), 0);
-    (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
-
-    // ---- stream B: raw-space trees and broad phase ----
-    (void)hipStreamWaitEvent(s2, ctx->ev[EV_FORK], 0);
-    (void)hipEventRecord(ctx->ev[EV_B0], s2);
-    RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
+    const uint64_t F = (uint64_t)Fs + ctx->nf[1];
     float* rawbox = P_<float>(ctx->rawbox);
-    for (int m = 0; m < 2; ++m) {
-        const uint32_t nf = ctx->nf[m];
-        const uint64_t o = m ? Fs : 0;
-        k_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf,
-                                                      C->rbox[m], P_<uint32_t>(ctx->mcode) + o,
-                                                      P_<uint32_t>(ctx->mface) + o, rawbox + 6 * o, Pd, C, m,
-                                                      m ? (1u << 21) : 0u);
-    }
-    // one stable sort orders the leaves of both trees (the mesh is the top key bit)
+    RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
     ctx->mface_alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode), P_<uint32_t>(ctx->mface), P_<uint32_t>(ctx->mcode2),
-                                          P_<uint32_t>(ctx->mface2), nullptr, F, F, 22, rt, s2);
-    (void)hipEventRecord(ctx->ev[EV_B_SORT], s2);
+                                          P_<uint32_t>(ctx->mface2), nullptr, F, F, 22, rt, st);
+    (void)hipEventRecord(ctx->ev[EV_B_SORT], st);
     const uint32_t* sorted = P_<uint32_t>(ctx->mface_alt ? ctx->mface2 : ctx->mface);
     Tree T[2];
     RefitTree R[2];
@@ -480,7 +453,7 @@ void enqueue_frontend(mcgpu_context ctx)
             blocks += R[m].blocks;
         }
         if (blocks == 0) break;
-        k_refit_levels<<<blocks, 512, 0, s2>>>(R[0], R[1]);
+        k_refit_levels<<<blocks, 512, 0, st>>>(R[0], R[1]);
         for (int m = 0; m < 2; ++m) {
             if (R[m].blocks == 0) continue;
             if (R[m].lin == 0) R[m].lin = -1; // a single-leaf tree: its leaf is its root
@@ -488,11 +461,50 @@ void enqueue_frontend(mcgpu_context ctx)
             if (R[m].lin == 0) R[m].lin = -1;
         }
     }
-    (void)hipEventRecord(ctx->ev[EV_B_REFIT], s2);
+    (void)hipEventRecord(ctx->ev[EV_B_REFIT], st);
     const unsigned nwaves = (Fs + kWaveLeaves - 1) / kWaveLeaves;
-    k_traverse_wave<<<nwaves < 65536 ? nwaves : 65536, 64, 0, s2>>>(T[0], T[1], P_<unsigned long long>(ctx->raw_pairs),
+    k_traverse_wave<<<nwaves < 65536 ? nwaves : 65536, 64, 0, st>>>(T[0], T[1], P_<unsigned long long>(ctx->raw_pairs),
                                                                     ctx->cap_pairs, C);
-    (void)hipEventRecord(ctx->ev[EV_B_TRAV], s2);
+    (void)hipEventRecord(ctx->ev[EV_B_TRAV], st);
+}
+
+void enqueue_frontend(mcgpu_context ctx)
+{
+    hipStream_t s = ctx->s, s2 = ctx->s2, s3 = ctx->s3;
+    const uint32_t Fs = ctx->nf[0];
+    const uint64_t F = (uint64_t)Fs + ctx->nf[1], V = (uint64_t)ctx->nv[0] + ctx->nv[1];
+    Counters* C = &P_<DevState>(ctx->state)->C;
+    Params* Pd = &P_<DevState>(ctx->state)->P;
+
+    // ---- stream A, enqueued first: the serial sums start at once ----
+    (void)hipEventRecord(ctx->ev[EV_FORK], s);
+    (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_FORK], 0);
+    (void)hipEventRecord(ctx->ev[EV_S0], ctx->ssum);
+    k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], &C->zero, Pd);
+    (void)hipEventRecord(ctx->ev[EV_A_SUMS], ctx->ssum);
+    (void)hipStreamWaitEvent(s, ctx->ev[EV_A_SUMS], 0);
+    const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
+    k_shift_params<<<1, 1, 0, s>>>(Pd, C, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
+    (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
+    k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
+                                                        P_<double>(ctx->ps_v), 0);
+    const Soup S = make_soup(ctx);
+    k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, Pd, C, P_<double>(ctx->box), 0);
+    (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
+
+    // ---- stream B: raw-space trees and broad phase ----
+    (void)hipStreamWaitEvent(s2, ctx->ev[EV_FORK], 0);
+    (void)hipEventRecord(ctx->ev[EV_B0], s2);
+    float* rawbox = P_<float>(ctx->rawbox);
+    for (int m = 0; m < 2; ++m) {
+        const uint32_t nf = ctx->nf[m];
+        const uint64_t o = m ? Fs : 0;
+        k_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf,
+                                                      C->rbox[m], P_<uint32_t>(ctx->mcode) + o,
+                                                      P_<uint32_t>(ctx->mface) + o, rawbox + 6 * o, Pd, C, m,
+                                                      m ? (1u << 21) : 0u);
+    }
+    enqueue_bvh_traverse(ctx, s2, C);
 
     // ---- stream C: polygon-soup edge numbering, and the tables the filter fills ----
     (void)hipStreamWaitEvent(s3, ctx->ev[EV_FORK], 0);
@@ -541,7 +553,7 @@ void enqueue_frontend(mcgpu_context ctx)
     const uint64_t table = next_pow2(2 * ctx->cap_pairs);
     k_pair_hash_init<<<grid_for(table, kBlock, 1024), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash), ctx->cap_pairs,
                                                                       table, C);
-    k_filter_pairs<<<grid_for(ctx->cap_pairs, kFilterTile, 2048), kBlock, 0, s>>>(P_<double>(ctx->box), P_<unsigned long long>(ctx->raw_pairs),
+    k_filter_pairs<<<grid_for(ctx->cap_pairs, kFilterTile, 2048), kStageBlock, 0, s>>>(P_<double>(ctx->box), P_<unsigned long long>(ctx->raw_pairs),
                                                                ctx->cap_pairs, Fs, P_<unsigned long long>(ctx->pairs),
                                                                P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->cand),
                                                                P_<uint32_t>(ctx->cand_list), C);
@@ -781,6 +793,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     if (!(desc->gp_constant > 0.0)) return MCGPU_ERROR_INVALID_VALUE; // frontend.cpp:391-397
     CK(hipSetDevice(ctx->device));
     ctx->have = false;
+    ctx->broad_only = false;
     ctx->flags = desc->flags;
     ctx->gp_constant = desc->gp_constant;
     ctx->enforcement_count = desc->enforcement_count;
@@ -812,11 +825,92 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     return MCGPU_ERROR_OUT_OF_MEMORY;
 }
 
+MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_boxes, uint32_t num_src,
+                                         const double* cut_boxes, uint32_t num_cut, uint32_t input_on_device,
+                                         uint64_t* num_pairs)
+{
+    if (!ctx || !src_boxes || !cut_boxes || num_src == 0 || num_cut == 0) return MCGPU_ERROR_INVALID_VALUE;
+    CK(hipSetDevice(ctx->device));
+    ctx->have = false;
+    ctx->broad_only = true;
+    ctx->nf[0] = num_src;
+    ctx->nf[1] = num_cut;
+    ctx->nv[0] = ctx->nv[1] = 0;
+    ctx->ncorner[0] = ctx->ncorner[1] = 0;
+    const uint64_t F = (uint64_t)num_src + num_cut;
+    hipStream_t s = ctx->s;
+    DevState* st;
+    double* dp;
+    float* fp;
+    uint32_t* u;
+    unsigned long long* ull;
+    CK(ensure(ctx->state, 1, &st));
+    CK(ensure(ctx->box, 6 * F, &dp));
+    CK(ensure(ctx->rawbox, 6 * F, &fp));
+    CK(ensure(ctx->mcode, F, &u));
+    CK(ensure(ctx->mface, F, &u));
+    CK(ensure(ctx->mcode2, F, &u));
+    CK(ensure(ctx->mface2, F, &u));
+    CK(ensure(ctx->cand, F + 8, &u));
+    CK(ensure(ctx->cand_list, F + 8, &u));
+    CK(ensure(ctx->hist, radix_hist_elems(F), &u));
+    CK(ensure(ctx->scan_tmp, std::max(scan_tmp_elems(radix_hist_elems(F)), scan_tmp_elems(F + 8)) + 1024, &u));
+    for (int m = 0; m < 2; ++m) {
+        int D = 0;
+        while ((1ull << D) < ctx->nf[m]) ++D;
+        ctx->depth[m] = D;
+        CK(ensure(ctx->nodes[m], 6 * ((2ull << D) - 1), &fp));
+    }
+    if (ctx->cap_pairs < 4 * F) ctx->cap_pairs = std::max<uint64_t>(1 << 14, 4 * F);
+    const hipMemcpyKind kind = input_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        CK(ensure(ctx->raw_pairs, ctx->cap_pairs, &ull));
+        CK(ensure(ctx->pairs, ctx->cap_pairs, &ull));
+        CK(ensure(ctx->phash, next_pow2(2 * ctx->cap_pairs), &ull));
+        Counters* C = &st->C;
+        (void)hipEventRecord(ctx->ev[EV_A0], s);
+        fill_u32(C, 0u, sizeof(Counters) / 4, s);
+        CK(hipMemcpyAsync(dp, src_boxes, 48ull * num_src, kind, s));
+        CK(hipMemcpyAsync(dp + 6ull * num_src, cut_boxes, 48ull * num_cut, kind, s));
+        for (int m = 0; m < 2; ++m) {
+            const uint64_t o = m ? num_src : 0;
+            fill_u64(C->rbox[m], ~0ull, 3, s);
+            k_box_bounds<<<grid_for(ctx->nf[m], kBlock, 64), kBlock, 0, s>>>(dp + 6 * o, ctx->nf[m], C->rbox[m]);
+            k_morton_boxes<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(dp + 6 * o, ctx->nf[m], C->rbox[m], P_<uint32_t>(ctx->mcode) + o,
+                                                                  P_<uint32_t>(ctx->mface) + o, P_<float>(ctx->rawbox) + 6 * o,
+                                                                  m ? (1u << 21) : 0u);
+        }
+        enqueue_bvh_traverse(ctx, s, C);
+        const uint64_t table = next_pow2(2 * ctx->cap_pairs);
+        k_pair_hash_init<<<grid_for(table, kBlock, 1024), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash), ctx->cap_pairs,
+                                                                          table, C);
+        fill_u32(ctx->cand.p, 0u, F, s);
+        k_filter_pairs<<<grid_for(ctx->cap_pairs, kFilterTile, 2048), kStageBlock, 0, s>>>(
+            dp, P_<unsigned long long>(ctx->raw_pairs), ctx->cap_pairs, num_src, P_<unsigned long long>(ctx->pairs),
+            P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->cand), P_<uint32_t>(ctx->cand_list), C);
+        (void)hipEventRecord(ctx->ev[EV_A_END], s);
+        CK(hipStreamSynchronize(s));
+        CK(hipGetLastError());
+        CK(hipMemcpy(&ctx->C, C, sizeof(Counters), hipMemcpyDeviceToHost));
+        if (ctx->C.ov_pairs) {
+            ctx->cap_pairs = std::max<uint64_t>(2 * ctx->cap_pairs, ctx->C.n_raw + ctx->C.n_raw / 4);
+            continue;
+        }
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[EV_A0], ctx->ev[EV_A_END]) != hipSuccess) ms = 0.f;
+        ctx->total_ms = ms;
+        ctx->have = true;
+        if (num_pairs) *num_pairs = ctx->C.n_pairs;
+        return MCGPU_SUCCESS;
+    }
+    return MCGPU_ERROR_OUT_OF_MEMORY;
+}
+
 MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const double* perturbation,
                                                   uint32_t enforcement_count, mcgpu_frontend_info* info)
 {
     if (!ctx || !perturbation) return MCGPU_ERROR_INVALID_VALUE;
-    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    if (!ctx->have || ctx->broad_only) return MCGPU_ERROR_NO_RESULT;
     CK(hipSetDevice(ctx->device));
     ctx->enforcement_count = enforcement_count;
     for (int attempt = 0; attempt < 6; ++attempt) {
@@ -848,7 +942,7 @@ MCGPU_API mcgpu_result mcgpu_get_registry(mcgpu_context ctx, uint32_t* edges, ui
                                           uint64_t capacity)
 {
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
-    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    if (!ctx->have || ctx->broad_only) return MCGPU_ERROR_NO_RESULT;
     const uint64_t n = std::min<uint64_t>(ctx->C.n_reg, capacity);
     if (edges) CK(hipMemcpy(edges, ctx->re.p, 4 * n, hipMemcpyDeviceToHost));
     if (faces) CK(hipMemcpy(faces, ctx->rg.p, 4 * n, hipMemcpyDeviceToHost));
@@ -860,7 +954,7 @@ MCGPU_API mcgpu_result mcgpu_registry_device(mcgpu_context ctx, const uint32_t**
                                              const double** points, uint64_t* count)
 {
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
-    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    if (!ctx->have || ctx->broad_only) return MCGPU_ERROR_NO_RESULT;
     if (edges) *edges = P_<uint32_t>(ctx->re);
     if (faces) *faces = P_<uint32_t>(ctx->rg);
     if (points) *points = P_<double>(ctx->rpts);
@@ -888,7 +982,7 @@ MCGPU_API mcgpu_result mcgpu_get_edges(mcgpu_context ctx, uint32_t* v0, uint32_t
                                        uint64_t capacity)
 {
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
-    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    if (!ctx->have || ctx->broad_only) return MCGPU_ERROR_NO_RESULT;
     const uint64_t n = std::min<uint64_t>(ctx->C.n_edges_src + ctx->C.n_edges_cut, capacity);
     if (v0) CK(hipMemcpy(v0, ctx->ev0.p, 4 * n, hipMemcpyDeviceToHost));
     if (v1) CK(hipMemcpy(v1, ctx->ev1.p, 4 * n, hipMemcpyDeviceToHost));
@@ -901,7 +995,7 @@ MCGPU_API mcgpu_result mcgpu_get_tests(mcgpu_context ctx, uint32_t* edges, uint3
                                        int8_t* rsign, char* type, char* pip, uint64_t capacity)
 {
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
-    if (!ctx->have || !ctx->keep_tests) return MCGPU_ERROR_NO_RESULT;
+    if (!ctx->have || ctx->broad_only || !ctx->keep_tests) return MCGPU_ERROR_NO_RESULT;
     const uint64_t n = ctx->C.n_tests;
     std::vector<uint32_t> e(n), g(n);
     std::vector<int8_t> qs(n), rs(n);
@@ -936,7 +1030,7 @@ MCGPU_API mcgpu_result mcgpu_get_face_planes(mcgpu_context ctx, uint32_t* faces,
                                              int32_t* max_comp, int32_t* triple, uint64_t capacity)
 {
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
-    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    if (!ctx->have || ctx->broad_only) return MCGPU_ERROR_NO_RESULT;
     const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
     std::vector<uint32_t> cand(F);
     std::vector<Plane> pl(F);
@@ -960,7 +1054,7 @@ MCGPU_API mcgpu_result mcgpu_get_face_planes(mcgpu_context ctx, uint32_t* faces,
 MCGPU_API mcgpu_result mcgpu_get_ps_vertices(mcgpu_context ctx, double* xyz, uint64_t capacity)
 {
     if (!ctx || !xyz) return MCGPU_ERROR_INVALID_VALUE;
-    if (!ctx->have) return MCGPU_ERROR_NO_RESULT;
+    if (!ctx->have || ctx->broad_only) return MCGPU_ERROR_NO_RESULT;
     const uint64_t n = std::min<uint64_t>((uint64_t)ctx->nv[0] + ctx->nv[1], capacity);
     CK(hipMemcpy(xyz, ctx->ps_v.p, 24 * n, hipMemcpyDeviceToHost));
     return MCGPU_SUCCESS;
@@ -968,7 +1062,7 @@ MCGPU_API mcgpu_result mcgpu_get_ps_vertices(mcgpu_context ctx, double* xyz, uin
 
 MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity)
 {
-    if (!ctx || !ms || !ctx->have) return 0;
+    if (!ctx || !ms || !ctx->have || ctx->broad_only) return 0;
     if (!ctx->stages_ready) compute_stage_times(ctx);
     int n = std::min(capacity, (int)ST_COUNT);
     for (int i = 0; i < n; ++i) ms[i] = ctx->stage_ms[i];

@@ -79,13 +79,14 @@ def lib():
         L.mcgpu_get_face_planes.argtypes = [vp, vp, vp, vp, vp, vp, u64]
         L.mcgpu_get_ps_vertices.argtypes = [vp, vp, u64]
         L.mcgpu_registry_device.argtypes = [vp, vp, vp, vp, vp]
+        L.mcgpu_broad_phase.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, C.c_uint32, C.POINTER(u64)]
         L.mcgpu_stage_times.argtypes = [vp, vp, C.c_int]
         L.mcgpu_stage_name.argtypes = [C.c_int]
         L.mcgpu_stage_name.restype = C.c_char_p
         for fn in ("mcgpu_create_context", "mcgpu_release_context", "mcgpu_dispatch_frontend",
                    "mcgpu_redispatch_perturbed", "mcgpu_get_registry", "mcgpu_get_candidate_pairs", "mcgpu_get_edges",
                    "mcgpu_get_tests", "mcgpu_get_face_planes", "mcgpu_get_ps_vertices", "mcgpu_registry_device",
-                   "mcgpu_stage_times"):
+                   "mcgpu_stage_times", "mcgpu_broad_phase"):
             getattr(L, fn).restype = C.c_int
         _lib = L
     return _lib
@@ -233,6 +234,20 @@ class Context:
         _check(lib().mcgpu_dispatch_frontend(self.h, C.byref(src), C.byref(cut), C.byref(d), C.byref(info)),
                "mcgpu_dispatch_frontend")
         return Result(self, info, src.num_vertices + cut.num_vertices)
+
+    def broad_phase(self, src_boxes, cut_boxes):
+        """Candidate (source face, cut face) pairs of two sets of face AABBs
+        (F x 6 float64: min xyz, max xyz), sorted; the seam of build_oibvh +
+        intersectOIBVHs (bvh.cpp:219, 638)."""
+        a = np.ascontiguousarray(src_boxes, dtype=np.float64).reshape(-1, 6)
+        b = np.ascontiguousarray(cut_boxes, dtype=np.float64).reshape(-1, 6)
+        n = C.c_uint64(0)
+        _check(lib().mcgpu_broad_phase(self.h, _ptr(a), a.shape[0], _ptr(b), b.shape[0], 0, C.byref(n)),
+               "mcgpu_broad_phase")
+        out = np.zeros((n.value, 2), np.uint32)
+        if n.value:
+            _check(lib().mcgpu_get_candidate_pairs(self.h, _ptr(out), n.value), "mcgpu_get_candidate_pairs")
+        return out
 
     def redispatch_perturbed(self, perturbation, enforcement_count, nv):
         p = np.ascontiguousarray(perturbation, dtype=np.float64)

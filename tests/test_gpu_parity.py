@@ -187,3 +187,51 @@ def test_coordinates_beyond_the_broad_phase_bound(ctx):
         ctx.dispatch((V * 1e200, F, None), (Vc * 1e200, Fc, Sc))
     assert ei.value.code == -4
     compare(ctx, *meshes.config1_rotated())
+
+
+def _brute_pairs(a, b):
+    """All (i, j) with inclusive overlap of boxes a[i] and b[j] (exact float64 compares)."""
+    ov = np.ones((a.shape[0], b.shape[0]), bool)
+    for j in range(3):
+        ov &= (a[:, None, j] <= b[None, :, 3 + j]) & (a[:, None, 3 + j] >= b[None, :, j])
+    i, k = np.nonzero(ov)
+    return np.stack([i, k], 1).astype(np.uint32)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_broad_phase_boxes(ctx, seed):
+    """mcgpu_broad_phase against brute force on random boxes, with boxes that
+    touch exactly (shared faces, zero-width boxes) and a few empty boxes."""
+    rng = np.random.default_rng(300 + seed)
+    n_s, n_c = int(rng.integers(1, 3000)), int(rng.integers(1, 3000))
+
+    def boxes(n):
+        lo = rng.integers(0, 64, (n, 3)).astype(np.float64) / 8.0  # coarse grid: exact touching is common
+        ext = rng.integers(0, 6, (n, 3)).astype(np.float64) / 8.0
+        b = np.concatenate([lo, lo + ext], 1)
+        b[rng.random(n) < 0.01, :3] += 100.0  # inverted (empty) boxes
+        return b
+    a, b = boxes(n_s), boxes(n_c)
+    got = ctx.broad_phase(a, b)
+    want = _brute_pairs(a, b)
+    want = want[np.lexsort((want[:, 1], want[:, 0]))]
+    assert np.array_equal(got, want), (got.shape, want.shape)
+
+
+def test_broad_phase_mesh_boxes(ctx):
+    """Face boxes of a mesh pair, as build_oibvh makes them: the pairs equal
+    the oracle's broad phase on the same pair."""
+    from mcut_amd import meshgen as mg
+    src, cut = meshes.icosphere_pair(5, amp=0.005)
+    O = om.frontend(src, cut, detail=True)
+    ps = O["ps_vertices"]
+    ns = src[0].shape[0]
+
+    def face_boxes(F, off, eps):
+        T = F.reshape(-1, 3) + off
+        P = ps[T]
+        return np.concatenate([P.min(1) - eps, P.max(1) + eps], 1)
+    a = face_boxes(src[1], 0, 0.0)
+    b = face_boxes(cut[1], ns, O["eps"])
+    got = ctx.broad_phase(a, b)
+    assert np.array_equal(got, O["pairs"])
