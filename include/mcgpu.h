@@ -109,6 +109,24 @@ typedef struct {
     float reserved;
 } mcgpu_frontend_info;
 
+/* A batch of independent (source, cut) pairs, each dispatched as
+ * mcgpu_dispatch_frontend would dispatch it alone.  `src` holds every pair's
+ * source mesh concatenated in pair order and `cut` every cut mesh; face
+ * indices stay local to their own mesh (as each pair's mcDispatch would take
+ * them), num_vertices / num_faces are the totals.  The offset arrays are
+ * host memory with num_pairs + 1 entries: pair p's source vertices are
+ * [src_vertex_offsets[p], src_vertex_offsets[p + 1]), and likewise for faces
+ * and the cut meshes.  Every mesh needs at least 3 vertices and one face. */
+typedef struct {
+    uint32_t num_pairs;
+    mcgpu_mesh src;
+    mcgpu_mesh cut;
+    const uint32_t* src_vertex_offsets;
+    const uint32_t* src_face_offsets;
+    const uint32_t* cut_vertex_offsets;
+    const uint32_t* cut_face_offsets;
+} mcgpu_batch;
+
 /* --- context (one per MCUT context / API thread) ---------------------------- */
 
 /* mcCreateContext (mcut.h:567) counterpart: binds a device and creates the stream. */
@@ -144,7 +162,32 @@ MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_bo
                                          const double* cut_boxes, uint32_t num_cut, uint32_t input_on_device,
                                          uint64_t* num_pairs);
 
-/* --- results of the last dispatch (host copies; capacity in elements) ----- */
+/* Batched front end: every pair of `batch` through the front end in one
+ * pass of the pipeline (each stage one launch over all pairs), with results
+ * bit-identical to dispatching each pair alone.  This has no single
+ * counterpart in the reference, whose mcDispatch (mcut.h:988-1020) takes
+ * one pair per call: it replaces a loop of those calls' front ends.  The
+ * general-position retry is per pair with mcgpu_dispatch_frontend and
+ * mcgpu_redispatch_perturbed.  info holds totals over the batch, its status
+ * is the first pair's status that is not SUCCESS (com, translation and eps are
+ * zero); per-pair results come from the getters below. */
+MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch* batch, const mcgpu_dispatch_desc* desc,
+                                            mcgpu_frontend_info* info);
+/* After mcgpu_dispatch_batch: each pair's status (mcgpu_status) and its
+ * entries [registry_offsets[p], registry_offsets[p + 1]) of the batch
+ * registry.  capacity >= num_pairs; registry_offsets needs num_pairs + 1. */
+MCGPU_API mcgpu_result mcgpu_get_batch_status(mcgpu_context ctx, int32_t* status, uint64_t* registry_offsets,
+                                              uint64_t capacity);
+/* After mcgpu_dispatch_batch: candidate pairs as (pair, source face, cut
+ * face) triples, face ids local to the pair, sorted. */
+MCGPU_API mcgpu_result mcgpu_get_batch_candidate_pairs(mcgpu_context ctx, uint32_t* triples, uint64_t capacity);
+
+/* --- results of the last dispatch (host copies; capacity in elements) -----
+ * After mcgpu_dispatch_batch, mcgpu_get_registry and mcgpu_registry_device
+ * give the batch registry: pairs in order, each pair's entries with the
+ * edge and face ids of that pair's own polygon soup; the other getters
+ * describe the batch's concatenated soup (every source mesh, then every cut
+ * mesh). */
 
 /* Intersection registry sorted by (edge, face): m0_ivtx_to_intersection_registry_entry
  * and the intersection points (kernel.cpp:2603-2615).  points: 3 doubles per entry. */

@@ -58,14 +58,10 @@ namespace mcgpu {
 
 constexpr int kSumRing = 16;
 
-__global__ void __launch_bounds__(64) k_mesh_sums(const double* __restrict__ v0, uint32_t n0,
-                                                  const double* __restrict__ v1, uint32_t n1,
-                                                  const double* __restrict__ zero, Params* P)
+// left-to-right sum of v[0], v[3], ..., v[3 (n - 1)] by one wavefront (the
+// whole wave returns it)
+__device__ __forceinline__ double serial_sum(const double* __restrict__ v, uint32_t n, const double* __restrict__ zero)
 {
-    __builtin_amdgcn_s_setprio(3);
-    const int m = blockIdx.x / 3, j = blockIdx.x % 3;
-    const double* __restrict__ v = (m ? v1 : v0) + j;
-    const uint32_t n = m ? n1 : n0;
     const uint32_t c = threadIdx.x & 15;
     const uint32_t nfull = n / 64;
     const uint32_t last = nfull ? nfull - 1 : 0;
@@ -107,18 +103,57 @@ __global__ void __launch_bounds__(64) k_mesh_sums(const double* __restrict__ v0,
         MCGPU_SUM_ROW(acc, T[2], one);
         MCGPU_SUM_ROW(acc, T[3], one);
     }
+    return acc;
+}
+
+__global__ void __launch_bounds__(64) k_mesh_sums(const double* __restrict__ v0, uint32_t n0,
+                                                  const double* __restrict__ v1, uint32_t n1,
+                                                  const double* __restrict__ zero, Params* P)
+{
+    __builtin_amdgcn_s_setprio(3);
+    const int m = blockIdx.x / 3, j = blockIdx.x % 3;
+    const double acc = serial_sum((m ? v1 : v0) + j, m ? n1 : n0, zero);
     if (threadIdx.x == 0) P->sum[m][j] = acc;
 }
 
-__global__ void k_shift_params(Params* P, const Counters* __restrict__ C, uint32_t n0, uint32_t n1, double gp_constant,
-                               int eps_absolute, int has_pert, double p0, double p1, double p2)
+// Batches: the 6 chains of every pair (pair p, mesh m, coordinate j is chain
+// 6 p + 3 m + j), handed out longest first (order[]) by a work queue to a
+// persistent grid of waves, so that long chains start early and short ones
+// fill in behind them.
+__global__ void __launch_bounds__(64) k_batch_sums(const double* __restrict__ vs, const double* __restrict__ vc,
+                                                   const BatchPair* __restrict__ bp, const uint32_t* __restrict__ order,
+                                                   uint32_t nchains, Counters* C, Params* P)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __builtin_amdgcn_s_setprio(3);
+    for (;;) {
+        // Everything that steers this loop is a scalar (readfirstlane) and the
+        // body has no lane-dependent branch: with a chain index read back from
+        // lane 0 by a shuffle, or an `if (lane 0)` around the atomic or the
+        // store, the compiler treats the exit as divergent and may structure
+        // the loop so that lanes 1-63 keep iterating without lane 0, which
+        // never ends.  Every lane issues the atomic (lane 0 adds 1, the others
+        // 0) and lane 0's result is the wave's chain.
+        const uint32_t w = __builtin_amdgcn_readfirstlane(atomicAdd(&C->sum_next, threadIdx.x == 0 ? 1u : 0u));
+        if (w >= nchains) break;
+        const uint32_t chain = __builtin_amdgcn_readfirstlane(order[w]);
+        const uint32_t p = chain / 6, m = (chain % 6) / 3, j = chain % 3;
+        const uint32_t v0 = __builtin_amdgcn_readfirstlane(m ? bp[p].vc0 : bp[p].vs0);
+        const uint32_t n = __builtin_amdgcn_readfirstlane(m ? bp[p].nvc : bp[p].nvs);
+        const double acc = serial_sum((m ? vc : vs) + 3ull * v0 + j, n, &C->zero);
+        P[p].sum[m][j] = acc; // the same value in every lane
+    }
+}
+
+// com, t, diag and eps of a pair from its coordinate sums (P->sum) and the raw
+// boxes of its two meshes (rbox[m] = order keys lo xyz, hi xyz)
+__device__ inline void shift_params_of(Params* P, const unsigned long long (*rbox)[6], uint32_t n0, uint32_t n1,
+                                       double gp_constant, int eps_absolute)
+{
     // raw bounding boxes from the validation pass (min/max do not depend on order)
     for (int m = 0; m < 2; ++m)
         for (int j = 0; j < 3; ++j) {
-            P->bmin[m][j] = ord_value(C->rbox[m][j]);
-            P->bmax[m][j] = ord_value(C->rbox[m][3 + j]);
+            P->bmin[m][j] = ord_value(rbox[m][j]);
+            P->bmax[m][j] = ord_value(rbox[m][3 + j]);
         }
     double c0[3], c1[3], tpq[3];
     for (int j = 0; j < 3; ++j) {
@@ -138,10 +173,28 @@ __global__ void k_shift_params(Params* P, const Counters* __restrict__ C, uint32
     P->gp_constant = gp_constant;
     P->eps_absolute = eps_absolute;
     P->eps = (eps_absolute ? 1.0 : P->diag) * gp_constant; // preproc.cpp:2667-2676
+}
+
+__global__ void k_shift_params(Params* P, const Counters* __restrict__ C, uint32_t n0, uint32_t n1, double gp_constant,
+                               int eps_absolute, int has_pert, double p0, double p1, double p2)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    shift_params_of(P, C->rbox, n0, n1, gp_constant, eps_absolute);
     P->has_pert = has_pert;
     P->pert[0] = p0;
     P->pert[1] = p1;
     P->pert[2] = p2;
+}
+
+// batches: one thread per pair
+__global__ void k_batch_shift_params(Params* P, const unsigned long long (*rbox)[2][6], const BatchPair* __restrict__ bp,
+                                     uint32_t npairs, double gp_constant, int eps_absolute)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
+        shift_params_of(P + p, rbox[p], bp[p].nvs, bp[p].nvc, gp_constant, eps_absolute);
+        P[p].has_pert = 0;
+        P[p].pert[0] = P[p].pert[1] = P[p].pert[2] = 0.0;
+    }
 }
 
 __global__ void k_set_perturbation(Params* P, int has_pert, double p0, double p1, double p2)
@@ -167,16 +220,16 @@ __global__ void k_set_perturbation(Params* P, int has_pert, double p0, double p1
 // bound plus the rounding of b_hi + tol (u (W + tol)).  eps is bounded from
 // the cut mesh's raw extent.  tol keeps a factor of 32 over every rounding
 // term.  A bound that is not finite marks the input unsupported (bit 1).
-__global__ void k_tolerance(Params* P, Counters* C, uint32_t nv0, uint32_t nv1, double gp_constant, int absolute)
+__device__ inline bool tolerance_of(const unsigned long long (*rbox)[6], uint32_t nv0, uint32_t nv1, double gp_constant,
+                                    int absolute, double* tol)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double u32 = 3.552713678800501e-15; // 2^-48 = 32 u
     const double n = (double)(nv0 > nv1 ? nv0 : nv1);
     double cext2 = 0.0, wmax = 0.0, rmax = 0.0, R[3], W[3];
     bool finite = true;
     for (int j = 0; j < 3; ++j) {
-        const double l0 = ord_value(C->rbox[0][j]), l1 = ord_value(C->rbox[1][j]);
-        const double h0 = ord_value(C->rbox[0][3 + j]), h1 = ord_value(C->rbox[1][3 + j]);
+        const double l0 = ord_value(rbox[0][j]), l1 = ord_value(rbox[1][j]);
+        const double h0 = ord_value(rbox[0][3 + j]), h1 = ord_value(rbox[1][3 + j]);
         const double lo = l0 < l1 ? l0 : l1, hi = h0 > h1 ? h0 : h1;
         W[j] = fmax(fabs(lo), fabs(hi));
         R[j] = (hi - lo) + (n + 2.0) * 2.220446049250313e-16 * W[j];
@@ -191,10 +244,24 @@ __global__ void k_tolerance(Params* P, Counters* C, uint32_t nv0, uint32_t nv1, 
     const double diag_hi = (sqrt(cext2) + u32 * 2.0 * (wmax + 2.0 * rmax + 2.0)) * (1.0 + u32);
     const double eps_hi = (absolute ? 1.0 : diag_hi) * gp_constant * (1.0 + u32);
     for (int j = 0; j < 3; ++j) {
-        P->tol[j] = eps_hi * (1.0 + u32) + u32 * (8.0 * R[j] + W[j] + 4.0);
-        finite = finite && isfinite(P->tol[j]);
+        tol[j] = eps_hi * (1.0 + u32) + u32 * (8.0 * R[j] + W[j] + 4.0);
+        finite = finite && isfinite(tol[j]);
     }
-    if (!finite) C->unsupported |= 2u;
+    return finite;
+}
+
+__global__ void k_tolerance(Params* P, Counters* C, uint32_t nv0, uint32_t nv1, double gp_constant, int absolute)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!tolerance_of(C->rbox, nv0, nv1, gp_constant, absolute, P->tol)) C->unsupported |= 2u;
+}
+
+// batches: one thread per pair; any pair without a finite bound rejects the batch
+__global__ void k_batch_tolerance(Params* P, const unsigned long long (*rbox)[2][6], const BatchPair* __restrict__ bp,
+                                  uint32_t npairs, double gp_constant, int absolute, Counters* C)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x)
+        if (!tolerance_of(rbox[p], bp[p].nvs, bp[p].nvc, gp_constant, absolute, P[p].tol)) atomicOr(&C->unsupported, 2u);
 }
 
 __device__ __forceinline__ bool dispatch_aborted(const Counters* C)
@@ -221,6 +288,22 @@ __global__ void k_shift_vertices(const double* __restrict__ vs, uint32_t ns, con
         const double x = cut ? vc[i - split] : vs[i];
         const double y = (x - com[j]) + t[j];
         ps_v[i] = y + (cut ? pe[j] : 0.0);
+    }
+}
+
+// batches: every pair's vertices with its own shift (no perturbation)
+__global__ void k_batch_shift_vertices(const double* __restrict__ vs, uint32_t ns, const double* __restrict__ vc,
+                                       uint32_t nc, const uint32_t* __restrict__ vpair, const Params* __restrict__ P,
+                                       double* __restrict__ ps_v)
+{
+    const uint64_t total = (uint64_t)(ns + nc) * 3, split = (uint64_t)ns * 3;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        const bool cut = i >= split;
+        const int j = (int)(i % 3);
+        const Params* q = P + vpair[i / 3];
+        const double x = cut ? vc[i - split] : vs[i];
+        const double y = (x - q->com[j]) + q->t[j];
+        ps_v[i] = y + 0.0;
     }
 }
 
@@ -389,7 +472,7 @@ __global__ void k_face_boxes(Soup S, const Params* __restrict__ P, const Counter
                              int cut_only)
 {
     const uint32_t F = S.nf[0] + S.nf[1];
-    const double eps = P->eps;
+    const double eps0 = P->eps;
     const bool aborted = dispatch_aborted(C);
     for (uint64_t g = (cut_only ? S.nf[0] : 0) + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < F;
          g += (uint64_t)gridDim.x * blockDim.x) {
@@ -404,6 +487,7 @@ __global__ void k_face_boxes(Soup S, const Params* __restrict__ P, const Counter
                 lo[j] = (p[j] < lo[j]) ? p[j] : lo[j];
             }
         }
+        const double eps = S.fpair ? P[S.fpair[g]].eps : eps0; // batches: the pair's eps
         if (g >= S.nf[0] && eps > 0.0)
             for (int j = 0; j < 3; ++j) {
                 hi[j] = hi[j] + eps;
@@ -518,6 +602,71 @@ __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __res
         for (int j = 0; j < 3; ++j) {
             o[j] = __double2float_rd(mn[j] - tol[j]);
             o[3 + j] = __double2float_ru(mx[j] + tol[j]);
+        }
+    }
+}
+
+// Batches: every pair's raw boxes are moved by the pair's offset P[p].boff
+// into a cell of their own, so that one pair of trees over all pairs never
+// pairs faces of different pairs (the pair filter checks it regardless).  The
+// offset is added in double, then the bound is pushed out by 2^-48 of its
+// magnitude, which covers the rounding of that addition, before the outward
+// rounding to single precision: the float box still contains the exact moved
+// box.  Leaf keys: mesh as bit 31, pair above mb bits per axis of Morton
+// code of the centre within the pair's own mesh box.
+__device__ __forceinline__ float float_below(double x)
+{
+    return __double2float_rd(x - fabs(x) * 3.552713678800501e-15);
+}
+
+__device__ __forceinline__ float float_above(double x)
+{
+    return __double2float_ru(x + fabs(x) * 3.552713678800501e-15);
+}
+
+__global__ void k_batch_morton_raw(const double* __restrict__ v, const uint32_t* __restrict__ idx,
+                                   const uint32_t* __restrict__ off, uint32_t nf, int m, uint32_t fofs,
+                                   const uint32_t* __restrict__ fpair, const unsigned long long (*rbox)[2][6],
+                                   const Params* __restrict__ P, const Counters* __restrict__ C, int mb,
+                                   uint32_t* __restrict__ code, uint32_t* __restrict__ face, float* __restrict__ rawbox)
+{
+    const bool aborted = dispatch_aborted(C);
+    const float cells = (float)(1u << mb);
+    const uint32_t mesh_bit = m ? (1u << 31) : 0u;
+    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
+        face[f] = (uint32_t)f;
+        float* o = rawbox + 6 * f;
+        if (aborted) {
+            code[f] = mesh_bit;
+            o[0] = o[1] = o[2] = 3.402823466e38f;
+            o[3] = o[4] = o[5] = -3.402823466e38f;
+            continue;
+        }
+        const uint32_t p = fpair[fofs + f];
+        const Params* q = P + p;
+        const uint32_t b = off[f], n = off[f + 1] - b;
+        double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+        double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+        for (uint32_t i = 0; i < n; ++i) {
+            const double* x = v + 3 * (uint64_t)idx[b + i];
+            for (int j = 0; j < 3; ++j) {
+                mn[j] = x[j] < mn[j] ? x[j] : mn[j];
+                mx[j] = x[j] > mx[j] ? x[j] : mx[j];
+            }
+        }
+        uint32_t c[3];
+        for (int j = 0; j < 3; ++j) {
+            const double lo = ord_value(rbox[p][m][j]), hi = ord_value(rbox[p][m][3 + j]);
+            const double ext = hi > lo ? hi - lo : 1.0;
+            float x = (float)((((mn[j] + mx[j]) * 0.5) - lo) / ext);
+            x = fminf(fmaxf(x * cells, 0.0f), cells - 1.0f);
+            c[j] = expand_bits10((uint32_t)x);
+        }
+        code[f] = mesh_bit | (p << (3 * mb)) | (c[0] * 4 + c[1] * 2 + c[2]);
+        for (int j = 0; j < 3; ++j) {
+            const double t = m ? q->tol[j] : 0.0;
+            o[j] = float_below((mn[j] - t) + q->boff[j]);
+            o[3 + j] = float_above((mx[j] + t) + q->boff[j]);
         }
     }
 }
@@ -961,7 +1110,8 @@ constexpr int kFilterTile = kStageBlock * kFilterItems;
 
 __global__ void __launch_bounds__(kStageBlock) k_filter_pairs(const double* __restrict__ box,
                                                          const unsigned long long* __restrict__ raw, uint64_t cap,
-                                                         uint32_t Fs, unsigned long long* __restrict__ pairs,
+                                                         uint32_t Fs, const uint32_t* __restrict__ fpair,
+                                                         unsigned long long* __restrict__ pairs,
                                                          unsigned long long* __restrict__ phash,
                                                          uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_list,
                                                          Counters* C)
@@ -980,6 +1130,7 @@ __global__ void __launch_bounds__(kStageBlock) k_filter_pairs(const double* __re
             if (i >= n) continue;
             const uint64_t k = raw[i];
             const uint32_t f[2] = {(uint32_t)(k >> 32), Fs + (uint32_t)k};
+            if (fpair && fpair[f[0]] != fpair[f[1]]) continue; // batches: faces of different pairs
             double a[6], b[6];
             for (int j = 0; j < 6; ++j) {
                 a[j] = box[6 * (uint64_t)f[0] + j];
@@ -1047,6 +1198,7 @@ __global__ void k_planes(Soup S, const uint32_t* __restrict__ cand_list, Plane* 
             // kernel.cpp:2320-2331 names the mesh with "face id > source face count"
             if (g > S.nf[0]) atomicOr(&C->bad_cut, 1u);
             else atomicOr(&C->bad_src, 1u);
+            pair_flag(S, g, g > S.nf[0] ? PF_BAD_CUT : PF_BAD_SRC);
         } else if (fv.n > 3 && fv.n <= 5) {
             if (!best_triple_small((int)fv.n, fv, pl.n, k, pl.tri)) pl.tri[0] = 0xff;
         } else if (fv.n > 5 && fv.n <= kMaxPoly) {
@@ -1056,6 +1208,7 @@ __global__ void k_planes(Soup S, const uint32_t* __restrict__ cand_list, Plane* 
             else atomicOr(&C->ov_big, 1u);
         } else if (fv.n > kMaxPoly) {
             atomicOr(&C->unsupported, 1u);
+            pair_flag(S, g, PF_UNSUPPORTED);
         }
         planes[g] = pl;
     }
@@ -1333,15 +1486,32 @@ struct TestOut {
 };
 
 struct RegOut {
-    unsigned long long* key; // (edge << gbits) | face
+    unsigned long long* key; // sort key (reg_key)
     double* p;
     uint32_t* val;           // sort payload: the entry's own index
+    uint32_t* e;             // edge and face of each entry (ps ids)
+    uint32_t* g;
     uint64_t cap;
     int gbits;
+    const BatchPair* bp;     // batches: pair ranges (the key orders pairs first)
 };
 
+// Registry order: by (edge, face).  In a batch the soup holds all source
+// meshes before all cut meshes, so the key uses each id's rank in the
+// pair-major order [pair 0 source, pair 0 cut, pair 1 source, ...]: one sort
+// then leaves every pair's entries together and in the order of that pair's
+// own dispatch (both maps are increasing within a pair).
+__device__ __forceinline__ unsigned long long reg_key(const RegOut& R, const Soup& S, uint32_t e, uint32_t g, uint32_t Es)
+{
+    if (!R.bp) return ((unsigned long long)e << R.gbits) | g;
+    const BatchPair b = R.bp[S.fpair[g]];
+    const uint32_t rg = g < S.nf[0] ? g + b.fc0 : (g - S.nf[0]) + b.fs0 + b.nfs;
+    const uint32_t re = e < Es ? e + b.ec0 : (e - Es) + b.es0 + b.nes;
+    return ((unsigned long long)re << R.gbits) | rg;
+}
+
 __device__ __forceinline__ void store_registry(const RegOut& R, unsigned long long at, unsigned long long key,
-                                               const double* p, Counters* C)
+                                               uint32_t e, uint32_t g, const double* p, Counters* C)
 {
     if (at < R.cap) {
         R.key[at] = key;
@@ -1349,6 +1519,8 @@ __device__ __forceinline__ void store_registry(const RegOut& R, unsigned long lo
         R.p[3 * at + 1] = p[1];
         R.p[3 * at + 2] = p[2];
         R.val[at] = (uint32_t)at;
+        R.e[at] = e;
+        R.g[at] = g;
     } else {
         atomicOr(&C->ov_reg, 1u);
     }
@@ -1391,7 +1563,10 @@ __device__ inline bool resolve_test(const Soup& S, const Plane* planes, uint32_t
             violated = res == 'i' || res == 'v' || res == 'e';
         }
         if (TO.qs) TO.pip[ti] = res;
-        if (violated) atomicOr(&C->gp, 1u);
+        if (violated) {
+            atomicOr(&C->gp, 1u);
+            pair_flag(S, g, PF_GP);
+        }
         return false;
     }
     p[0] = p[1] = p[2] = 0.0;
@@ -1400,6 +1575,7 @@ __device__ inline bool resolve_test(const Soup& S, const Plane* planes, uint32_t
     if (TO.qs) TO.pip[ti] = res;
     if (res == 'v' || res == 'e') {
         atomicOr(&C->gp, 1u);
+        pair_flag(S, g, PF_GP);
         return false;
     }
     return res == 'i';
@@ -1415,12 +1591,13 @@ __global__ void __launch_bounds__(kStageBlock) k_narrow(Soup S, const Plane* __r
                                                    uint32_t* __restrict__ esc, uint64_t cap_esc, TestOut TO, RegOut R,
                                                    Counters* C)
 {
-    __shared__ unsigned long long st_key[kNarrowTile];
+    __shared__ uint32_t st_e[kNarrowTile], st_g[kNarrowTile];
     __shared__ double st_p[3 * kNarrowTile];
     __shared__ uint32_t st_esc[kNarrowTile];
     __shared__ uint32_t st_nr, st_ne;
     __shared__ unsigned long long st_base;
     const uint64_t n = C->n_tests < cap_tests ? C->n_tests : cap_tests;
+    const uint32_t Es = (uint32_t)C->n_edges_src;
     for (uint64_t t0 = (uint64_t)blockIdx.x * kNarrowTile; t0 < n; t0 += (uint64_t)gridDim.x * kNarrowTile) {
         if (threadIdx.x == 0) st_nr = st_ne = 0;
         __syncthreads();
@@ -1455,7 +1632,8 @@ __global__ void __launch_bounds__(kStageBlock) k_narrow(Soup S, const Plane* __r
             double p[3];
             if (resolve_test(S, planes, e, g, sq, sr, i, TO, C, p)) {
                 const uint32_t at = wave_reserve(&st_nr);
-                st_key[at] = ((unsigned long long)e << R.gbits) | g;
+                st_e[at] = e;
+                st_g[at] = g;
                 st_p[3 * at] = p[0];
                 st_p[3 * at + 1] = p[1];
                 st_p[3 * at + 2] = p[2];
@@ -1464,7 +1642,8 @@ __global__ void __launch_bounds__(kStageBlock) k_narrow(Soup S, const Plane* __r
         __syncthreads();
         const uint32_t nr = st_nr, ne = st_ne;
         const unsigned long long rb = block_reserve(&C->n_reg, nr, &st_base);
-        for (uint32_t q = threadIdx.x; q < nr; q += kStageBlock) store_registry(R, rb + q, st_key[q], st_p + 3 * q, C);
+        for (uint32_t q = threadIdx.x; q < nr; q += kStageBlock)
+            store_registry(R, rb + q, reg_key(R, S, st_e[q], st_g[q], Es), st_e[q], st_g[q], st_p + 3 * q, C);
         __syncthreads();
         const unsigned long long eb = block_reserve(&C->n_esc, ne, &st_base);
         for (uint32_t q = threadIdx.x; q < ne; q += kStageBlock) {
@@ -1498,7 +1677,7 @@ __global__ void __launch_bounds__(64) k_narrow_exact(Soup S, const Plane* __rest
         if (sr == 0) sr = orient3d_exact_sign(a, b, c, r);
         double p[3];
         if (resolve_test(S, planes, e, g, sq, sr, i, TO, C, p))
-            store_registry(R, wave_append(&C->n_reg), ((unsigned long long)e << R.gbits) | g, p, C);
+            store_registry(R, wave_append(&C->n_reg), reg_key(R, S, e, g, (uint32_t)C->n_edges_src), e, g, p, C);
     }
 }
 
@@ -1535,7 +1714,10 @@ __device__ __forceinline__ void status_insert(const Soup& S, uint32_t e, uint32_
     }
     atomicAdd(&hcnt[hash_insert(hkeys, mask, k0)], 1u);
     if (k1 != kEmpty64) atomicAdd(&hcnt[hash_insert(hkeys, mask, k1)], 1u);
-    if (S.ev0[e] >= Vs && (f0 < 0 || f1 < 0)) atomicOr(&C->partial, 1u);
+    if (S.ev0[e] >= Vs && (f0 < 0 || f1 < 0)) {
+        if (!C->partial) atomicOr(&C->partial, 1u);
+        pair_flag(S, g, PF_PARTIAL);
+    }
 }
 
 // Registry order for up to kRegMergeTiles tiles of kRegTile entries, in two
@@ -1652,8 +1834,9 @@ __global__ void __launch_bounds__(kRegSortThreads) k_reg_tile_sort(const unsigne
 constexpr int kRegMergeILP = 8;
 
 __global__ void k_reg_merge(Soup S, const unsigned long long* __restrict__ skey, const uint32_t* __restrict__ sval,
-                            const unsigned long long* __restrict__ n_dev, uint64_t cap, int gbits,
-                            const double* __restrict__ p_in, uint32_t* __restrict__ re, uint32_t* __restrict__ rg,
+                            const unsigned long long* __restrict__ n_dev, uint64_t cap, const uint32_t* __restrict__ e_in,
+                            const uint32_t* __restrict__ g_in, const double* __restrict__ p_in,
+                            uint32_t* __restrict__ re, uint32_t* __restrict__ rg,
                             double* __restrict__ rp, unsigned long long* __restrict__ hkeys, uint32_t* __restrict__ hcnt,
                             uint64_t max_slots, Counters* C)
 {
@@ -1661,7 +1844,6 @@ __global__ void k_reg_merge(Soup S, const unsigned long long* __restrict__ skey,
     const uint64_t ntiles = (n + kRegTile - 1) / kRegTile;
     uint64_t slots = status_slots(n);
     if (slots > max_slots) slots = max_slots;
-    const uint64_t gmask = (1ull << gbits) - 1;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t t = i / kRegTile;
         const unsigned long long k = skey[i];
@@ -1688,10 +1870,11 @@ __global__ void k_reg_merge(Soup S, const unsigned long long* __restrict__ skey,
 #pragma unroll
             for (int q = 0; q < kRegMergeILP; ++q) rank += lo[q];
         }
-        const uint32_t e = (uint32_t)(k >> gbits), g = (uint32_t)(k & gmask);
+        const uint32_t j = sval[i];
+        const uint32_t e = e_in[j], g = g_in[j];
         re[rank] = e;
         rg[rank] = g;
-        const double* p = p_in + 3 * (uint64_t)sval[i];
+        const double* p = p_in + 3 * (uint64_t)j;
         rp[3 * rank] = p[0];
         rp[3 * rank + 1] = p[1];
         rp[3 * rank + 2] = p[2];
@@ -1704,20 +1887,17 @@ __global__ void k_clamp_count(const unsigned long long* n_dev, uint64_t cap, uns
     if (threadIdx.x == 0 && blockIdx.x == 0) *out = *n_dev < cap ? *n_dev : cap;
 }
 
-// sorted keys + permutation -> (edge, face, point) arrays; the status
-// side structures of kernel.cpp:2603-2654 are keyed the same way
-__global__ void k_registry_gather(const unsigned long long* __restrict__ skey, const uint32_t* __restrict__ perm,
-                                  const double* __restrict__ p_in, const unsigned long long* __restrict__ n_dev,
-                                  uint64_t cap, int gbits, uint32_t* __restrict__ re, uint32_t* __restrict__ rg,
-                                  double* __restrict__ rp)
+// sort permutation -> (edge, face, point) arrays in registry order
+__global__ void k_registry_gather(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ e_in,
+                                  const uint32_t* __restrict__ g_in, const double* __restrict__ p_in,
+                                  const unsigned long long* __restrict__ n_dev, uint64_t cap, uint32_t* __restrict__ re,
+                                  uint32_t* __restrict__ rg, double* __restrict__ rp)
 {
     const uint64_t n = *n_dev < cap ? *n_dev : cap;
-    const uint64_t gmask = (1ull << gbits) - 1;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const unsigned long long k = skey[i];
-        re[i] = (uint32_t)(k >> gbits);
-        rg[i] = (uint32_t)(k & gmask);
         const uint32_t j = perm[i];
+        re[i] = e_in[j];
+        rg[i] = g_in[j];
         rp[3 * i] = p_in[3 * (uint64_t)j];
         rp[3 * i + 1] = p_in[3 * (uint64_t)j + 1];
         rp[3 * i + 2] = p_in[3 * (uint64_t)j + 2];
@@ -1747,13 +1927,184 @@ __global__ void k_status_pairs(Soup S, const uint32_t* __restrict__ re, const ui
         status_insert(S, re[i], rg[i], hkeys, hcnt, slots - 1, C);
 }
 
-__global__ void k_status_check(const unsigned long long* __restrict__ n_dev, const unsigned long long* __restrict__ hkeys,
+__global__ void k_status_check(Soup S, const unsigned long long* __restrict__ n_dev, const unsigned long long* __restrict__ hkeys,
                                const uint32_t* __restrict__ hcnt, uint64_t max_slots, Counters* C)
 {
     uint64_t slots = status_slots(*n_dev);
     if (slots > max_slots) slots = max_slots;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slots; i += (uint64_t)gridDim.x * blockDim.x)
-        if (hkeys[i] != kEmpty64 && hcnt[i] < 2) atomicOr(&C->pair_fail, 1u);
+        if (hkeys[i] != kEmpty64 && hcnt[i] < 2) {
+            atomicOr(&C->pair_fail, 1u);
+            pair_flag(S, (uint32_t)(hkeys[i] >> 32), PF_PAIR_FAIL); // the key's source face
+        }
+}
+
+// ===========================================================================
+// 10. batches: pair tables, input, raw boxes, layout, per-pair results
+// ===========================================================================
+
+// out[base + i] = the segment of element i, starts[] being the first element
+// of each of the nseg segments (increasing)
+__global__ void k_segment_ids(const uint32_t* __restrict__ starts, uint32_t nseg, uint32_t n, uint32_t* __restrict__ out)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nseg; // last segment whose start is <= i
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (starts[mid] <= (uint32_t)i) lo = mid;
+            else hi = mid;
+        }
+        out[i] = lo;
+    }
+}
+
+// Face lists of a batch mesh part: checks every pair's faces against its own
+// vertex count (as k_validate does for one mesh) and writes the indices as
+// ids of the concatenated part.
+__global__ void k_batch_globalize(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off,
+                                  const uint32_t* __restrict__ sizes, uint32_t nf, int m, uint32_t fofs,
+                                  const uint32_t* __restrict__ fpair, const BatchPair* __restrict__ bp,
+                                  uint32_t* __restrict__ out, Counters* C)
+{
+    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
+        const BatchPair b = bp[fpair[fofs + f]];
+        const uint32_t nv = m ? b.nvc : b.nvs, v0 = m ? b.vc0 : b.vs0;
+        const uint32_t o = off[f], n = sizes ? sizes[f] : 3u;
+        bool bad = n < 3;
+        for (uint32_t i = 0; i < n && !bad; ++i) {
+            const uint32_t a = idx[o + i];
+            bad = a >= nv;
+            for (uint32_t k = 0; k < i && !bad; ++k) bad = idx[o + k] == a;
+            out[o + i] = v0 + a;
+        }
+        if (bad) atomicOr(&C->bad_input, 1u);
+    }
+}
+
+// raw box of every (pair, mesh) segment: one block each, order keys out
+__global__ void __launch_bounds__(256) k_batch_bbox(const double* __restrict__ vs, const double* __restrict__ vc,
+                                                    const BatchPair* __restrict__ bp, unsigned long long (*rbox)[2][6])
+{
+    __shared__ double red[4][6];
+    const uint32_t p = blockIdx.x >> 1, m = blockIdx.x & 1;
+    const BatchPair b = bp[p];
+    const double* v = m ? vc + 3ull * b.vc0 : vs + 3ull * b.vs0;
+    const uint32_t n = m ? b.nvc : b.nvs;
+    double lo[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+    double hi[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        for (int j = 0; j < 3; ++j) {
+            const double x = v[3ull * i + j];
+            lo[j] = x < lo[j] ? x : lo[j];
+            hi[j] = x > hi[j] ? x : hi[j];
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int j = 0; j < 3; ++j) {
+            const double a = __shfl_xor(lo[j], o, 64), c = __shfl_xor(hi[j], o, 64);
+            lo[j] = a < lo[j] ? a : lo[j];
+            hi[j] = c > hi[j] ? c : hi[j];
+        }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int j = 0; j < 3; ++j) {
+            red[w][j] = lo[j];
+            red[w][3 + j] = hi[j];
+        }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int j = threadIdx.x;
+        double a = red[0][j], c = red[0][3 + j];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            a = red[q][j] < a ? red[q][j] : a;
+            c = red[q][3 + j] > c ? red[q][3 + j] : c;
+        }
+        rbox[p][m][j] = ord_key(a);
+        rbox[p][m][3 + j] = ord_key(c);
+    }
+}
+
+// One block: the cell edge is the largest widened extent of any pair (both
+// meshes, cut side widened by tol) times 1.0625, and pair p's cell is the
+// one at the Morton-order position p, so that the pair-major leaf order
+// walks neighbouring cells.  boff moves the pair's widened low corner to its
+// cell's low corner.
+__device__ __forceinline__ uint32_t compact_bits3(uint32_t x)
+{
+    x &= 0x09249249u;
+    x = (x ^ (x >> 2)) & 0x030c30c3u;
+    x = (x ^ (x >> 4)) & 0x0300f00fu;
+    x = (x ^ (x >> 8)) & 0xff0000ffu;
+    x = (x ^ (x >> 16)) & 0x000003ffu;
+    return x;
+}
+
+__global__ void __launch_bounds__(1024) k_batch_layout(Params* P, const unsigned long long (*rbox)[2][6], uint32_t npairs)
+{
+    __shared__ double red[16];
+    double ext = 0.0;
+    for (uint32_t p = threadIdx.x; p < npairs; p += blockDim.x)
+        for (int j = 0; j < 3; ++j) {
+            const double lo = fmin(ord_value(rbox[p][0][j]), ord_value(rbox[p][1][j])) - P[p].tol[j];
+            const double hi = fmax(ord_value(rbox[p][0][3 + j]), ord_value(rbox[p][1][3 + j])) + P[p].tol[j];
+            ext = fmax(ext, hi - lo);
+        }
+    for (int o = 32; o > 0; o >>= 1) ext = fmax(ext, __shfl_xor(ext, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ext;
+    __syncthreads();
+    double cell = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) cell = fmax(cell, red[q]);
+    cell = cell > 0.0 ? cell * 1.0625 : 1.0;
+    for (uint32_t p = threadIdx.x; p < npairs; p += blockDim.x) {
+        const uint32_t cx = compact_bits3(p >> 2), cy = compact_bits3(p >> 1), cz = compact_bits3(p);
+        const double cc[3] = {(double)cx * cell, (double)cy * cell, (double)cz * cell};
+        for (int j = 0; j < 3; ++j) {
+            const double lo = fmin(ord_value(rbox[p][0][j]), ord_value(rbox[p][1][j])) - P[p].tol[j];
+            P[p].boff[j] = cc[j] - lo;
+        }
+    }
+}
+
+// polygon-soup edge ranges of every pair, once the edges are numbered: the
+// edges first used by a pair's corners are numbered in one run, starting at
+// the count of edges first used before its first corner
+__global__ void k_batch_edge_bases(BatchPair* bp, uint32_t npairs, const uint32_t* __restrict__ off_s,
+                                   const uint32_t* __restrict__ off_c, uint32_t nfs, uint32_t nfc,
+                                   const uint32_t* __restrict__ eid_s, const uint32_t* __restrict__ eid_c,
+                                   const Counters* __restrict__ C)
+{
+    const uint32_t Es = (uint32_t)C->n_edges_src, Ec = (uint32_t)C->n_edges_cut;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
+        const BatchPair b = bp[p];
+        const uint32_t s0 = b.fs0 < nfs ? eid_s[off_s[b.fs0]] : Es;
+        const uint32_t s1 = b.fs0 + b.nfs < nfs ? eid_s[off_s[b.fs0 + b.nfs]] : Es;
+        const uint32_t c0 = b.fc0 < nfc ? eid_c[off_c[b.fc0]] : Ec;
+        const uint32_t c1 = b.fc0 + b.nfc < nfc ? eid_c[off_c[b.fc0 + b.nfc]] : Ec;
+        bp[p].es0 = s0;
+        bp[p].nes = s1 - s0;
+        bp[p].ec0 = c0;
+        bp[p].nec = c1 - c0;
+    }
+}
+
+// Pair-major registry -> pair-local ids (the ids of that pair's own dispatch)
+// and each pair's range [rstart, rend) (both filled with 0 beforehand)
+__global__ void k_batch_localize(const uint32_t* __restrict__ re, const uint32_t* __restrict__ rg,
+                                 const unsigned long long* __restrict__ n_dev, uint64_t cap, uint32_t Fs,
+                                 const uint32_t* __restrict__ fpair, const BatchPair* __restrict__ bp,
+                                 const Counters* __restrict__ C, uint32_t* __restrict__ le, uint32_t* __restrict__ lg,
+                                 uint32_t* __restrict__ rstart, uint32_t* __restrict__ rend)
+{
+    const uint64_t n = *n_dev < cap ? *n_dev : cap;
+    const uint32_t Es = (uint32_t)C->n_edges_src;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = re[i], g = rg[i];
+        const uint32_t p = fpair[g];
+        const BatchPair b = bp[p];
+        lg[i] = g < Fs ? g - b.fs0 : (g - Fs - b.fc0) + b.nfs;
+        le[i] = e < Es ? e - b.es0 : (e - Es - b.ec0) + b.nes;
+        if (i == 0 || fpair[rg[i - 1]] != p) rstart[p] = (uint32_t)i;
+        if (i + 1 == n || fpair[rg[i + 1]] != p) rend[p] = (uint32_t)(i + 1);
+    }
 }
 
 } // namespace mcgpu

@@ -26,6 +26,7 @@ struct Params {
     double bmin[2][3];  // per-mesh raw bounding boxes
     double bmax[2][3];
     double tol[3];      // raw-space broad-phase tolerance per axis (k_tolerance)
+    double boff[3];     // batches: raw-space offset that places this pair's boxes in a cell of its own
 };
 
 struct Counters {
@@ -39,7 +40,8 @@ struct Counters {
     // unsupported: bit 0 a candidate polygon of more than 16 vertices, bit 1
     // coordinates whose broad-phase rounding bound is not finite
     unsigned int gp, bad_src, bad_cut, unsupported, nonmanifold, bad_input, pair_fail, partial;
-    unsigned int pad[2];
+    unsigned int sum_next; // batches: work-queue head of the serial sums
+    unsigned int pad;
     // raw bounding box of each mesh as order keys (lo xyz, hi xyz); written
     // while the input is validated and kept by the per-dispatch counter reset
     unsigned long long rbox[2][6];
@@ -49,6 +51,25 @@ struct Counters {
 struct DevState {
     Counters C;
     Params P;
+};
+
+// One mesh pair of a batch.  A batch is run as one polygon soup whose source
+// part is every source mesh in pair order and whose cut part is every cut mesh
+// in pair order; these are the pair's ranges in the two parts.
+struct BatchPair {
+    uint32_t fs0, nfs, fc0, nfc; // faces: first source face, count; first cut face (cut-part index), count
+    uint32_t vs0, nvs, vc0, nvc; // vertices, likewise
+    uint32_t es0, nes, ec0, nec; // polygon-soup edges, likewise (written once the edges are numbered)
+};
+
+// per-pair status bits of a batch (the dispatch-wide flags of Counters, per pair)
+enum PairFlag : uint32_t {
+    PF_GP = 1u,
+    PF_BAD_SRC = 2u,
+    PF_BAD_CUT = 4u,
+    PF_PAIR_FAIL = 8u,
+    PF_PARTIAL = 16u,
+    PF_UNSUPPORTED = 32u,
 };
 
 // polygon soup view: source faces [0,Fs), cut faces [Fs, Fs+Fc); vertices likewise
@@ -63,7 +84,17 @@ struct Soup {
     const uint32_t* ev1;
     const int32_t* ef0;      // face of h0, h1 (-1 none)
     const int32_t* ef1;
+    const uint32_t* fpair;   // batches: pair of each ps face (null for one pair)
+    uint32_t* pflags;        // batches: PairFlag bits per pair
 };
+
+// records a status bit on the pair of ps face g (batches only)
+__device__ __forceinline__ void pair_flag(const Soup& S, uint32_t g, uint32_t bit)
+{
+    if (!S.pflags) return;
+    uint32_t* f = S.pflags + S.fpair[g];
+    if (!(*f & bit)) atomicOr(f, bit);
+}
 
 __device__ __forceinline__ void load3(const double* p, double* o)
 {

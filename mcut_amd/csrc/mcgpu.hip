@@ -31,6 +31,7 @@ struct Buf {
 // slower.  A joins B before the exact pair filter and C before the edge-face
 // tests.
 constexpr int kSumCUs = 8;
+constexpr uint32_t kBatchSumWaves = 128; // persistent waves of the batched serial sums (on the reserved CUs)
 enum Stage {
     ST_SUMS,      // A: serial coordinate sums and raw boxes (k_mesh_sums alone)
     ST_PARAMS,    // A: shift parameters
@@ -85,7 +86,7 @@ struct mcgpu_context_t {
     Buf nodes[2], rawbox;
     Buf raw_pairs, pairs, phash, cand, cand_list, planes, big_list, big_scratch;
     Buf te, tg, tqs, trs, ttype, tpip, esc;
-    Buf rkey, rval, rkey2, rval2, rp, re, rg, rpts, rcount;
+    Buf rkey, rval, rkey2, rval2, rp, re, rg, rpts, rcount, re_in, rg_in;
     Buf shkeys, shcnt;
     Buf scan_tmp, scan_tmp_c, hist;
 
@@ -112,6 +113,19 @@ struct mcgpu_context_t {
     bool stages_ready = false;
     int stage_mode = 0; // 0: full dispatch (three streams), 1: perturbed narrow-phase rerun
     float total_ms = 0.f;
+    int sort_bits = 22;  // leaf key bits
+
+    // batches (mcgpu_dispatch_batch): the soup is every source mesh, then
+    // every cut mesh, in pair order
+    bool batch = false;
+    uint32_t npairs = 0;
+    int batch_mb = 7;                    // Morton bits per axis within a pair
+    std::vector<BatchPair> hpairs;       // pair ranges (host copy, edge ranges read back)
+    std::vector<uint32_t> hstarts, horder;
+    std::vector<uint32_t> hflags, hrstart, hrend;
+    std::vector<int32_t> hstatus;
+    Buf bpairs, bstarts, fpair, vpair, bparams, brbox, pflags, border, bidx[2], rle, rlg, rstart, rend;
+    const uint32_t* bin_idx[2] = {nullptr, nullptr}; // the caller's pair-local face lists
 };
 
 #define CK(x)                                                                                                          \
@@ -182,7 +196,8 @@ mcgpu_result check_mesh(const mcgpu_mesh* m)
     return MCGPU_SUCCESS;
 }
 
-int32_t decide_status(const Counters& C, uint32_t flags, uint32_t enforcement_count)
+int32_t decide_status(bool bad_cut, bool bad_src, bool gp, uint64_t n_reg, bool partial, bool pair_fail, uint32_t flags,
+                      uint32_t enforcement_count)
 {
     const bool enforce =
         (flags & (MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION | MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE)) != 0;
@@ -190,15 +205,20 @@ int32_t decide_status(const Counters& C, uint32_t flags, uint32_t enforcement_co
     // general-position violation), 3211-3229 (empty registry), 3269-3278
     // (partial cut with through-cuts required), 3439-3452 (fewer than two
     // intersection points on a face pair)
-    if (C.bad_cut) return MCGPU_STATUS_INVALID_CUT_MESH;
-    if (C.bad_src) return MCGPU_STATUS_INVALID_SRC_MESH;
-    if (C.gp) return MCGPU_STATUS_GENERAL_POSITION_VIOLATION;
-    if (C.n_reg == 0)
+    if (bad_cut) return MCGPU_STATUS_INVALID_CUT_MESH;
+    if (bad_src) return MCGPU_STATUS_INVALID_SRC_MESH;
+    if (gp) return MCGPU_STATUS_GENERAL_POSITION_VIOLATION;
+    if (n_reg == 0)
         return (enforce && enforcement_count > 0) ? MCGPU_STATUS_GENERAL_POSITION_VIOLATION : MCGPU_STATUS_SUCCESS;
-    if (C.partial && (flags & MCGPU_DISPATCH_REQUIRE_THROUGH_CUTS)) return MCGPU_STATUS_SUCCESS;
-    if (C.pair_fail)
+    if (partial && (flags & MCGPU_DISPATCH_REQUIRE_THROUGH_CUTS)) return MCGPU_STATUS_SUCCESS;
+    if (pair_fail)
         return enforce ? MCGPU_STATUS_GENERAL_POSITION_VIOLATION : MCGPU_STATUS_INVALID_MESH_INTERSECTION;
     return MCGPU_STATUS_SUCCESS;
+}
+
+int32_t decide_status(const Counters& C, uint32_t flags, uint32_t enforcement_count)
+{
+    return decide_status(C.bad_cut, C.bad_src, C.gp, C.n_reg, C.partial, C.pair_fail, flags, enforcement_count);
 }
 
 Soup make_soup(mcgpu_context ctx)
@@ -217,6 +237,8 @@ Soup make_soup(mcgpu_context ctx)
     S.ev1 = P_<uint32_t>(ctx->ev1);
     S.ef0 = P_<int32_t>(ctx->ef0);
     S.ef1 = P_<int32_t>(ctx->ef1);
+    S.fpair = ctx->batch ? P_<uint32_t>(ctx->fpair) : nullptr;
+    S.pflags = ctx->batch ? P_<uint32_t>(ctx->pflags) : nullptr;
     return S;
 }
 
@@ -301,6 +323,12 @@ mcgpu_result alloc_list_buffers(mcgpu_context ctx)
     CK(ensure(ctx->re, ctx->cap_reg, &u));
     CK(ensure(ctx->rg, ctx->cap_reg, &u));
     CK(ensure(ctx->rpts, 3 * ctx->cap_reg, &dp));
+    CK(ensure(ctx->re_in, ctx->cap_reg, &u));
+    CK(ensure(ctx->rg_in, ctx->cap_reg, &u));
+    if (ctx->batch) {
+        CK(ensure(ctx->rle, ctx->cap_reg, &u));
+        CK(ensure(ctx->rlg, ctx->cap_reg, &u));
+    }
     CK(ensure(ctx->rcount, 1, &ull));
     CK(ensure(ctx->shkeys, next_pow2(4 * ctx->cap_reg), &ull));
     CK(ensure(ctx->shcnt, next_pow2(4 * ctx->cap_reg), &u));
@@ -338,7 +366,9 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
     TestOut TO{nullptr, nullptr, nullptr, nullptr};
     if (ctx->keep_tests)
         TO = TestOut{P_<int8_t>(ctx->tqs), P_<int8_t>(ctx->trs), P_<char>(ctx->ttype), P_<char>(ctx->tpip)};
-    RegOut R{P_<unsigned long long>(ctx->rkey), P_<double>(ctx->rp), P_<uint32_t>(ctx->rval), ctx->cap_reg, ctx->gbits};
+    RegOut R{P_<unsigned long long>(ctx->rkey), P_<double>(ctx->rp), P_<uint32_t>(ctx->rval), P_<uint32_t>(ctx->re_in),
+             P_<uint32_t>(ctx->rg_in), ctx->cap_reg, ctx->gbits,
+             ctx->batch ? P_<BatchPair>(ctx->bpairs) : nullptr};
     k_narrow<<<grid_for(ctx->cap_tests, kNarrowTile, 2048), kStageBlock, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
                                                          P_<uint32_t>(ctx->tg), ctx->cap_tests, P_<uint32_t>(ctx->esc),
                                                          ctx->cap_esc, TO, R, C);
@@ -357,10 +387,10 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
                                                          ctx->cap_reg, P_<unsigned long long>(ctx->rkey2),
                                                          P_<uint32_t>(ctx->rval2), hk, hc, hs);
         k_reg_merge<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(S, P_<unsigned long long>(ctx->rkey2), P_<uint32_t>(ctx->rval2),
-                                                              &C->n_reg, ctx->cap_reg, ctx->gbits, P_<double>(ctx->rp),
+                                                              &C->n_reg, ctx->cap_reg, R.e, R.g, P_<double>(ctx->rp),
                                                               P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg),
                                                               P_<double>(ctx->rpts), hk, hc, hs, C);
-        k_status_check<<<grid_for(hs), kBlock, 0, s>>>(&C->n_reg, hk, hc, hs, C);
+        k_status_check<<<grid_for(hs), kBlock, 0, s>>>(S, &C->n_reg, hk, hc, hs, C);
     } else {
         // large registries: LSD radix sort of the keys with the entry index as payload
         k_clamp_count<<<1, 1, 0, s>>>(&C->n_reg, ctx->cap_reg, rcount);
@@ -368,15 +398,23 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
         const bool alt = radix_sort<unsigned long long>(P_<unsigned long long>(ctx->rkey), P_<uint32_t>(ctx->rval),
                                                         P_<unsigned long long>(ctx->rkey2), P_<uint32_t>(ctx->rval2),
                                                         (const uint64_t*)rcount, 0, ctx->cap_reg, ctx->kbits, rt, s);
-        const unsigned long long* sk = P_<unsigned long long>(alt ? ctx->rkey2 : ctx->rkey);
         const uint32_t* sv = P_<uint32_t>(alt ? ctx->rval2 : ctx->rval);
-        k_registry_gather<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(sk, sv, P_<double>(ctx->rp), rcount, ctx->cap_reg,
-                                                                     ctx->gbits, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg),
+        k_registry_gather<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(sv, R.e, R.g, P_<double>(ctx->rp), rcount, ctx->cap_reg,
+                                                                     P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg),
                                                                      P_<double>(ctx->rpts));
         k_status_clear<<<grid_for(hs), kBlock, 0, s>>>(rcount, hk, hc, hs);
         k_status_pairs<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg), rcount, hs,
                                                                  hk, hc, C);
-        k_status_check<<<grid_for(hs), kBlock, 0, s>>>(rcount, hk, hc, hs, C);
+        k_status_check<<<grid_for(hs), kBlock, 0, s>>>(S, rcount, hk, hc, hs, C);
+    }
+    if (ctx->batch) {
+        // pair-local ids and each pair's range of the pair-major registry
+        fill_u32(ctx->rstart.p, 0u, ctx->npairs, s);
+        fill_u32(ctx->rend.p, 0u, ctx->npairs, s);
+        k_batch_localize<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(
+            P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg), &C->n_reg, ctx->cap_reg, ctx->nf[0], P_<uint32_t>(ctx->fpair),
+            P_<BatchPair>(ctx->bpairs), C, P_<uint32_t>(ctx->rle), P_<uint32_t>(ctx->rlg), P_<uint32_t>(ctx->rstart),
+            P_<uint32_t>(ctx->rend));
     }
     (void)hipEventRecord(ctx->ev[EV_A_END], s);
 }
@@ -411,6 +449,76 @@ void enqueue_validate(mcgpu_context ctx)
                                 absolute ? 1 : 0);
 }
 
+// Batches: the same reset and validation, per pair: segment ids of every
+// vertex and face, face lists checked against their own pair's vertex count
+// and rewritten as ids of the concatenated part, raw boxes of every (pair,
+// mesh), each pair's broad-phase tolerance, and the cell layout.
+void enqueue_validate_batch(mcgpu_context ctx)
+{
+    hipStream_t s = ctx->s;
+    Counters* C = &P_<DevState>(ctx->state)->C;
+    uint32_t* etotal = P_<uint32_t>(ctx->etotal);
+    const uint32_t np = ctx->npairs;
+    const uint32_t* st = P_<uint32_t>(ctx->bstarts); // vertex starts (source, cut), face starts (source, cut)
+    uint32_t* fp = P_<uint32_t>(ctx->fpair);
+    uint32_t* vp = P_<uint32_t>(ctx->vpair);
+    const BatchPair* bp = P_<BatchPair>(ctx->bpairs);
+    auto* brbox = (unsigned long long(*)[2][6])ctx->brbox.p;
+    Params* Pb = P_<Params>(ctx->bparams);
+    (void)hipEventRecord(ctx->ev[EV_A0], s);
+    fill_u32(C, 0u, sizeof(Counters) / 4, s);
+    fill_u32(ctx->pflags.p, 0u, np, s);
+    k_segment_ids<<<grid_for(ctx->nv[0]), kBlock, 0, s>>>(st, np, ctx->nv[0], vp);
+    k_segment_ids<<<grid_for(ctx->nv[1]), kBlock, 0, s>>>(st + np, np, ctx->nv[1], vp + ctx->nv[0]);
+    k_segment_ids<<<grid_for(ctx->nf[0]), kBlock, 0, s>>>(st + 2 * np, np, ctx->nf[0], fp);
+    k_segment_ids<<<grid_for(ctx->nf[1]), kBlock, 0, s>>>(st + 3 * np, np, ctx->nf[1], fp + ctx->nf[0]);
+    for (int m = 0; m < 2; ++m) {
+        uint32_t* off = P_<uint32_t>(ctx->off[m]);
+        if (ctx->dsz[m]) {
+            scan_exclusive(ctx->dsz[m], off, ctx->nf[m], P_<uint32_t>(ctx->scan_tmp), etotal + m, s);
+            k_offsets_tail<<<1, 1, 0, s>>>(off, ctx->nf[m], etotal + m);
+        } else {
+            k_triangle_offsets<<<grid_for((uint64_t)ctx->nf[m] + 1), kBlock, 0, s>>>(off, ctx->nf[m]);
+        }
+        k_batch_globalize<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(ctx->bin_idx[m], off, ctx->dsz[m], ctx->nf[m], m,
+                                                                  m ? ctx->nf[0] : 0u, fp, bp, P_<uint32_t>(ctx->bidx[m]), C);
+    }
+    k_batch_bbox<<<2 * np, 256, 0, s>>>(ctx->dv[0], ctx->dv[1], bp, brbox);
+    const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
+    k_batch_tolerance<<<grid_for(np), kBlock, 0, s>>>(Pb, brbox, bp, np, ctx->gp_constant, absolute ? 1 : 0, C);
+    k_batch_layout<<<1, 1024, 0, s>>>(Pb, brbox, np);
+}
+
+// per-pair results of a batch, after finish(): status bits and registry
+// ranges read back, statuses decided as for one pair
+mcgpu_result finish_batch(mcgpu_context ctx, mcgpu_frontend_info* info)
+{
+    const uint32_t np = ctx->npairs;
+    ctx->hflags.resize(np);
+    ctx->hrstart.resize(np);
+    ctx->hrend.resize(np);
+    ctx->hstatus.resize(np);
+    CK(hipMemcpy(ctx->hflags.data(), ctx->pflags.p, 4ull * np, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(ctx->hrstart.data(), ctx->rstart.p, 4ull * np, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(ctx->hrend.data(), ctx->rend.p, 4ull * np, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(ctx->hpairs.data(), ctx->bpairs.p, sizeof(BatchPair) * np, hipMemcpyDeviceToHost));
+    int32_t first = MCGPU_STATUS_SUCCESS;
+    for (uint32_t p = 0; p < np; ++p) {
+        const uint32_t f = ctx->hflags[p];
+        const int32_t st = decide_status((f & PF_BAD_CUT) != 0, (f & PF_BAD_SRC) != 0, (f & PF_GP) != 0,
+                                         ctx->hrend[p] - ctx->hrstart[p], (f & PF_PARTIAL) != 0,
+                                         (f & PF_PAIR_FAIL) != 0, ctx->flags, ctx->enforcement_count);
+        ctx->hstatus[p] = st;
+        if (first == MCGPU_STATUS_SUCCESS) first = st;
+    }
+    if (info) {
+        info->status = first;
+        for (int j = 0; j < 3; ++j) info->com[j] = info->translation[j] = 0.0;
+        info->eps = 0.0;
+    }
+    return MCGPU_SUCCESS;
+}
+
 // Both trees and the raw-space traversal, on stream st, once the leaf keys
 // (mcode, mface) and single-precision leaf boxes (rawbox) are written: one
 // stable sort orders the leaves of both trees (the mesh is the top key bit),
@@ -422,7 +530,7 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C)
     float* rawbox = P_<float>(ctx->rawbox);
     RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
     ctx->mface_alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode), P_<uint32_t>(ctx->mface), P_<uint32_t>(ctx->mcode2),
-                                          P_<uint32_t>(ctx->mface2), nullptr, F, F, 22, rt, st);
+                                          P_<uint32_t>(ctx->mface2), nullptr, F, F, ctx->sort_bits, rt, st);
     (void)hipEventRecord(ctx->ev[EV_B_SORT], st);
     const uint32_t* sorted = P_<uint32_t>(ctx->mface_alt ? ctx->mface2 : ctx->mface);
     Tree T[2];
@@ -480,16 +588,32 @@ void enqueue_frontend(mcgpu_context ctx)
     (void)hipEventRecord(ctx->ev[EV_FORK], s);
     (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_FORK], 0);
     (void)hipEventRecord(ctx->ev[EV_S0], ctx->ssum);
-    k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], &C->zero, Pd);
+    const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
+    Params* Pb = P_<Params>(ctx->bparams);
+    const auto* brbox = (const unsigned long long(*)[2][6])ctx->brbox.p;
+    if (ctx->batch) {
+        const uint32_t nch = 6 * ctx->npairs;
+        k_batch_sums<<<std::min<uint32_t>(nch, kBatchSumWaves), 64, 0, ctx->ssum>>>(
+            ctx->dv[0], ctx->dv[1], P_<BatchPair>(ctx->bpairs), P_<uint32_t>(ctx->border), nch, C, Pb);
+    } else {
+        k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], &C->zero, Pd);
+    }
     (void)hipEventRecord(ctx->ev[EV_A_SUMS], ctx->ssum);
     (void)hipStreamWaitEvent(s, ctx->ev[EV_A_SUMS], 0);
-    const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
-    k_shift_params<<<1, 1, 0, s>>>(Pd, C, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
-    (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
-    k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
-                                                        P_<double>(ctx->ps_v), 0);
+    if (ctx->batch) {
+        k_batch_shift_params<<<grid_for(ctx->npairs), kBlock, 0, s>>>(Pb, brbox, P_<BatchPair>(ctx->bpairs), ctx->npairs,
+                                                                      ctx->gp_constant, absolute ? 1 : 0);
+        (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
+        k_batch_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1],
+                                                                  P_<uint32_t>(ctx->vpair), Pb, P_<double>(ctx->ps_v));
+    } else {
+        k_shift_params<<<1, 1, 0, s>>>(Pd, C, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
+        (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
+        k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
+                                                            P_<double>(ctx->ps_v), 0);
+    }
     const Soup S = make_soup(ctx);
-    k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, Pd, C, P_<double>(ctx->box), 0);
+    k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, ctx->batch ? Pb : Pd, C, P_<double>(ctx->box), 0);
     (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
 
     // ---- stream B: raw-space trees and broad phase ----
@@ -499,10 +623,16 @@ void enqueue_frontend(mcgpu_context ctx)
     for (int m = 0; m < 2; ++m) {
         const uint32_t nf = ctx->nf[m];
         const uint64_t o = m ? Fs : 0;
-        k_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf,
-                                                      C->rbox[m], P_<uint32_t>(ctx->mcode) + o,
-                                                      P_<uint32_t>(ctx->mface) + o, rawbox + 6 * o, Pd, C, m,
-                                                      m ? (1u << 21) : 0u);
+        if (ctx->batch)
+            k_batch_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf, m,
+                                                                (uint32_t)o, P_<uint32_t>(ctx->fpair), brbox, Pb, C,
+                                                                ctx->batch_mb, P_<uint32_t>(ctx->mcode) + o,
+                                                                P_<uint32_t>(ctx->mface) + o, rawbox + 6 * o);
+        else
+            k_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf,
+                                                          C->rbox[m], P_<uint32_t>(ctx->mcode) + o,
+                                                          P_<uint32_t>(ctx->mface) + o, rawbox + 6 * o, Pd, C, m,
+                                                          m ? (1u << 21) : 0u);
     }
     enqueue_bvh_traverse(ctx, s2, C);
 
@@ -544,6 +674,10 @@ void enqueue_frontend(mcgpu_context ctx)
             hs - 1, P_<uint32_t>(ctx->eid[m]), C, m ? ctx->nv[0] : 0, m ? ctx->nf[0] : 0, P_<uint32_t>(ctx->cedge[m]),
             P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
     }
+    if (ctx->batch)
+        k_batch_edge_bases<<<grid_for(ctx->npairs), kBlock, 0, s3>>>(
+            P_<BatchPair>(ctx->bpairs), ctx->npairs, P_<uint32_t>(ctx->off[0]), P_<uint32_t>(ctx->off[1]), ctx->nf[0],
+            ctx->nf[1], P_<uint32_t>(ctx->eid[0]), P_<uint32_t>(ctx->eid[1]), C);
     (void)hipEventRecord(ctx->ev[EV_C_TOPO], s3);
 
     // ---- stream A: join the broad phase, exact filter, narrow phase ----
@@ -554,7 +688,9 @@ void enqueue_frontend(mcgpu_context ctx)
     k_pair_hash_init<<<grid_for(table, kBlock, 1024), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash), ctx->cap_pairs,
                                                                       table, C);
     k_filter_pairs<<<grid_for(ctx->cap_pairs, kFilterTile, 2048), kStageBlock, 0, s>>>(P_<double>(ctx->box), P_<unsigned long long>(ctx->raw_pairs),
-                                                               ctx->cap_pairs, Fs, P_<unsigned long long>(ctx->pairs),
+                                                               ctx->cap_pairs, Fs,
+                                                               ctx->batch ? P_<uint32_t>(ctx->fpair) : nullptr,
+                                                               P_<unsigned long long>(ctx->pairs),
                                                                P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->cand),
                                                                P_<uint32_t>(ctx->cand_list), C);
     (void)hipEventRecord(ctx->ev[EV_A_FILTER], s);
@@ -772,7 +908,10 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
                   &ctx->rawbox, &ctx->raw_pairs, &ctx->pairs, &ctx->phash, &ctx->cand, &ctx->cand_list,
                   &ctx->planes, &ctx->big_list, &ctx->big_scratch, &ctx->te, &ctx->tg, &ctx->tqs, &ctx->trs,
                   &ctx->ttype, &ctx->tpip, &ctx->esc, &ctx->rkey, &ctx->rval, &ctx->rkey2, &ctx->rval2, &ctx->rp,
-                  &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->scan_tmp_c, &ctx->hist};
+                  &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->scan_tmp_c, &ctx->hist,
+                  &ctx->re_in, &ctx->rg_in, &ctx->bpairs, &ctx->bstarts, &ctx->fpair, &ctx->vpair, &ctx->bparams,
+                  &ctx->brbox, &ctx->pflags, &ctx->border, &ctx->bidx[0], &ctx->bidx[1], &ctx->rle, &ctx->rlg,
+                  &ctx->rstart, &ctx->rend};
     for (Buf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
@@ -794,6 +933,8 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     CK(hipSetDevice(ctx->device));
     ctx->have = false;
     ctx->broad_only = false;
+    ctx->batch = false;
+    ctx->sort_bits = 22;
     ctx->flags = desc->flags;
     ctx->gp_constant = desc->gp_constant;
     ctx->enforcement_count = desc->enforcement_count;
@@ -825,6 +966,144 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     return MCGPU_ERROR_OUT_OF_MEMORY;
 }
 
+MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch* batch, const mcgpu_dispatch_desc* desc,
+                                            mcgpu_frontend_info* info)
+{
+    if (!ctx || !batch || !desc || batch->num_pairs == 0) return MCGPU_ERROR_INVALID_VALUE;
+    const uint32_t np = batch->num_pairs;
+    const uint32_t* vo[2] = {batch->src_vertex_offsets, batch->cut_vertex_offsets};
+    const uint32_t* fo[2] = {batch->src_face_offsets, batch->cut_face_offsets};
+    const mcgpu_mesh* mesh[2] = {&batch->src, &batch->cut};
+    mcgpu_result r;
+    for (int m = 0; m < 2; ++m) {
+        if ((r = check_mesh(mesh[m])) != MCGPU_SUCCESS) return r;
+        if (!vo[m] || !fo[m] || vo[m][0] != 0 || fo[m][0] != 0 || vo[m][np] != mesh[m]->num_vertices ||
+            fo[m][np] != mesh[m]->num_faces)
+            return MCGPU_ERROR_INVALID_VALUE;
+        for (uint32_t p = 0; p < np; ++p) // every pair: a mesh of at least 3 vertices and one face
+            if (vo[m][p + 1] < vo[m][p] + 3 || fo[m][p + 1] < fo[m][p] + 1) return MCGPU_ERROR_INVALID_VALUE;
+    }
+    if (!(desc->gp_constant > 0.0)) return MCGPU_ERROR_INVALID_VALUE;
+    const int pb = bits_for(np);
+    if (pb > 28) return MCGPU_ERROR_UNSUPPORTED;
+    CK(hipSetDevice(ctx->device));
+    ctx->have = false;
+    ctx->broad_only = false;
+    ctx->batch = true;
+    ctx->npairs = np;
+    ctx->batch_mb = std::min(7, (31 - pb) / 3);
+    ctx->sort_bits = 32;
+    ctx->flags = desc->flags;
+    ctx->gp_constant = desc->gp_constant;
+    ctx->enforcement_count = desc->enforcement_count;
+    ctx->keep_tests = 0;
+    for (int m = 0; m < 2; ++m) {
+        if ((r = upload_mesh(ctx, m, mesh[m], desc->input_on_device != 0)) != MCGPU_SUCCESS) return r;
+        ctx->bin_idx[m] = ctx->didx[m];
+    }
+    // pair tables
+    ctx->hpairs.assign(np, BatchPair{});
+    ctx->hstarts.resize(4ull * np);
+    std::vector<uint32_t> len(6ull * np);
+    for (uint32_t p = 0; p < np; ++p) {
+        BatchPair& b = ctx->hpairs[p];
+        b.fs0 = fo[0][p], b.nfs = fo[0][p + 1] - fo[0][p];
+        b.fc0 = fo[1][p], b.nfc = fo[1][p + 1] - fo[1][p];
+        b.vs0 = vo[0][p], b.nvs = vo[0][p + 1] - vo[0][p];
+        b.vc0 = vo[1][p], b.nvc = vo[1][p + 1] - vo[1][p];
+        ctx->hstarts[p] = b.vs0;
+        ctx->hstarts[np + p] = b.vc0;
+        ctx->hstarts[2ull * np + p] = b.fs0;
+        ctx->hstarts[3ull * np + p] = b.fc0;
+        for (int j = 0; j < 6; ++j) len[6ull * p + j] = j < 3 ? b.nvs : b.nvc;
+    }
+    // serial-sum chains, longest first
+    ctx->horder.resize(6ull * np);
+    for (uint32_t c = 0; c < 6 * np; ++c) ctx->horder[c] = c;
+    std::stable_sort(ctx->horder.begin(), ctx->horder.end(), [&](uint32_t a, uint32_t b) { return len[a] > len[b]; });
+    const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
+    const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1];
+    ctx->gbits = bits_for(F + 1);
+    ctx->kbits = ctx->gbits + bits_for(ctx->ncorner[0] + ctx->ncorner[1] + 1);
+    if (ctx->kbits > 64) return MCGPU_ERROR_UNSUPPORTED;
+    if ((r = alloc_mesh_buffers(ctx)) != MCGPU_SUCCESS) return r;
+    {
+        BatchPair* bp;
+        uint32_t* u;
+        Params* pp;
+        unsigned long long* ull;
+        CK(ensure(ctx->bpairs, np, &bp));
+        CK(ensure(ctx->bstarts, 4ull * np, &u));
+        CK(ensure(ctx->border, 6ull * np, &u));
+        CK(ensure(ctx->pflags, np, &u));
+        CK(ensure(ctx->rstart, np, &u));
+        CK(ensure(ctx->rend, np, &u));
+        CK(ensure(ctx->fpair, F, &u));
+        CK(ensure(ctx->vpair, V, &u));
+        CK(ensure(ctx->bparams, np, &pp));
+        CK(ensure(ctx->brbox, 12ull * np, &ull));
+        for (int m = 0; m < 2; ++m) {
+            CK(ensure(ctx->bidx[m], ctx->ncorner[m], &u));
+            ctx->didx[m] = P_<uint32_t>(ctx->bidx[m]);
+        }
+        CK(hipMemcpyAsync(ctx->bpairs.p, ctx->hpairs.data(), sizeof(BatchPair) * np, hipMemcpyHostToDevice, ctx->s));
+        CK(hipMemcpyAsync(ctx->bstarts.p, ctx->hstarts.data(), 16ull * np, hipMemcpyHostToDevice, ctx->s));
+        CK(hipMemcpyAsync(ctx->border.p, ctx->horder.data(), 24ull * np, hipMemcpyHostToDevice, ctx->s));
+    }
+    if (ctx->cap_pairs < 4 * F) ctx->cap_pairs = std::max<uint64_t>(1 << 14, 4 * F);
+    if (ctx->cap_tests < 2 * ctx->cap_pairs) ctx->cap_tests = 2 * ctx->cap_pairs;
+    if (ctx->cap_esc < (1u << 14)) ctx->cap_esc = std::max<uint64_t>(1 << 14, ctx->cap_tests / 16);
+    if (ctx->cap_reg < ctx->cap_tests / 32) ctx->cap_reg = std::max<uint64_t>(1 << 14, ctx->cap_tests / 32);
+    if (ctx->cap_big < 1024) ctx->cap_big = 1024;
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
+        ctx->stage_mode = 0;
+        enqueue_validate_batch(ctx);
+        enqueue_frontend(ctx);
+        bool overflow = false;
+        if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
+        if (!overflow) return finish_batch(ctx, info);
+    }
+    return MCGPU_ERROR_OUT_OF_MEMORY;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_batch_status(mcgpu_context ctx, int32_t* status, uint64_t* registry_offsets,
+                                              uint64_t capacity)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have || !ctx->batch) return MCGPU_ERROR_NO_RESULT;
+    const uint32_t np = ctx->npairs;
+    if (capacity < np) return MCGPU_ERROR_INVALID_VALUE;
+    uint64_t at = 0;
+    for (uint32_t p = 0; p < np; ++p) {
+        if (status) status[p] = ctx->hstatus[p];
+        if (registry_offsets) registry_offsets[p] = at;
+        at += ctx->hrend[p] - ctx->hrstart[p];
+    }
+    if (registry_offsets && capacity > np) registry_offsets[np] = at;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_batch_candidate_pairs(mcgpu_context ctx, uint32_t* triples, uint64_t capacity)
+{
+    if (!ctx || !triples) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have || !ctx->batch) return MCGPU_ERROR_NO_RESULT;
+    const uint64_t n = ctx->C.n_pairs;
+    std::vector<unsigned long long> h(n);
+    if (n) CK(hipMemcpy(h.data(), ctx->pairs.p, 8 * n, hipMemcpyDeviceToHost));
+    std::sort(h.begin(), h.end()); // source faces ascend with the pair
+    const uint64_t m = std::min<uint64_t>(n, capacity);
+    uint32_t p = 0;
+    for (uint64_t i = 0; i < m; ++i) {
+        const uint32_t sf = (uint32_t)(h[i] >> 32), cf = (uint32_t)h[i];
+        while (p + 1 < ctx->npairs && ctx->hpairs[p + 1].fs0 <= sf) ++p;
+        triples[3 * i] = p;
+        triples[3 * i + 1] = sf - ctx->hpairs[p].fs0;
+        triples[3 * i + 2] = cf - ctx->hpairs[p].fc0;
+    }
+    return MCGPU_SUCCESS;
+}
+
 MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_boxes, uint32_t num_src,
                                          const double* cut_boxes, uint32_t num_cut, uint32_t input_on_device,
                                          uint64_t* num_pairs)
@@ -833,6 +1112,8 @@ MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_bo
     CK(hipSetDevice(ctx->device));
     ctx->have = false;
     ctx->broad_only = true;
+    ctx->batch = false;
+    ctx->sort_bits = 22;
     ctx->nf[0] = num_src;
     ctx->nf[1] = num_cut;
     ctx->nv[0] = ctx->nv[1] = 0;
@@ -886,7 +1167,7 @@ MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_bo
                                                                           table, C);
         fill_u32(ctx->cand.p, 0u, F, s);
         k_filter_pairs<<<grid_for(ctx->cap_pairs, kFilterTile, 2048), kStageBlock, 0, s>>>(
-            dp, P_<unsigned long long>(ctx->raw_pairs), ctx->cap_pairs, num_src, P_<unsigned long long>(ctx->pairs),
+            dp, P_<unsigned long long>(ctx->raw_pairs), ctx->cap_pairs, num_src, nullptr, P_<unsigned long long>(ctx->pairs),
             P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->cand), P_<uint32_t>(ctx->cand_list), C);
         (void)hipEventRecord(ctx->ev[EV_A_END], s);
         CK(hipStreamSynchronize(s));
@@ -910,7 +1191,7 @@ MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const doubl
                                                   uint32_t enforcement_count, mcgpu_frontend_info* info)
 {
     if (!ctx || !perturbation) return MCGPU_ERROR_INVALID_VALUE;
-    if (!ctx->have || ctx->broad_only) return MCGPU_ERROR_NO_RESULT;
+    if (!ctx->have || ctx->broad_only || ctx->batch) return MCGPU_ERROR_NO_RESULT;
     CK(hipSetDevice(ctx->device));
     ctx->enforcement_count = enforcement_count;
     for (int attempt = 0; attempt < 6; ++attempt) {
@@ -944,8 +1225,10 @@ MCGPU_API mcgpu_result mcgpu_get_registry(mcgpu_context ctx, uint32_t* edges, ui
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
     if (!ctx->have || ctx->broad_only) return MCGPU_ERROR_NO_RESULT;
     const uint64_t n = std::min<uint64_t>(ctx->C.n_reg, capacity);
-    if (edges) CK(hipMemcpy(edges, ctx->re.p, 4 * n, hipMemcpyDeviceToHost));
-    if (faces) CK(hipMemcpy(faces, ctx->rg.p, 4 * n, hipMemcpyDeviceToHost));
+    const Buf& be = ctx->batch ? ctx->rle : ctx->re; // batches: pair-local ids, pairs in order
+    const Buf& bg = ctx->batch ? ctx->rlg : ctx->rg;
+    if (edges) CK(hipMemcpy(edges, be.p, 4 * n, hipMemcpyDeviceToHost));
+    if (faces) CK(hipMemcpy(faces, bg.p, 4 * n, hipMemcpyDeviceToHost));
     if (points) CK(hipMemcpy(points, ctx->rpts.p, 24 * n, hipMemcpyDeviceToHost));
     return MCGPU_SUCCESS;
 }
@@ -955,8 +1238,8 @@ MCGPU_API mcgpu_result mcgpu_registry_device(mcgpu_context ctx, const uint32_t**
 {
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
     if (!ctx->have || ctx->broad_only) return MCGPU_ERROR_NO_RESULT;
-    if (edges) *edges = P_<uint32_t>(ctx->re);
-    if (faces) *faces = P_<uint32_t>(ctx->rg);
+    if (edges) *edges = P_<uint32_t>(ctx->batch ? ctx->rle : ctx->re);
+    if (faces) *faces = P_<uint32_t>(ctx->batch ? ctx->rlg : ctx->rg);
     if (points) *points = P_<double>(ctx->rpts);
     if (count) *count = ctx->C.n_reg;
     return MCGPU_SUCCESS;

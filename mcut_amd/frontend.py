@@ -55,6 +55,11 @@ class FrontendInfo(C.Structure):
                 ("reserved", C.c_float)]
 
 
+class Batch(C.Structure):
+    _fields_ = [("num_pairs", C.c_uint32), ("src", Mesh), ("cut", Mesh), ("src_vertex_offsets", C.c_void_p),
+                ("src_face_offsets", C.c_void_p), ("cut_vertex_offsets", C.c_void_p), ("cut_face_offsets", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -80,13 +85,17 @@ def lib():
         L.mcgpu_get_ps_vertices.argtypes = [vp, vp, u64]
         L.mcgpu_registry_device.argtypes = [vp, vp, vp, vp, vp]
         L.mcgpu_broad_phase.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, C.c_uint32, C.POINTER(u64)]
+        L.mcgpu_dispatch_batch.argtypes = [vp, C.POINTER(Batch), C.POINTER(DispatchDesc), C.POINTER(FrontendInfo)]
+        L.mcgpu_get_batch_status.argtypes = [vp, vp, vp, u64]
+        L.mcgpu_get_batch_candidate_pairs.argtypes = [vp, vp, u64]
         L.mcgpu_stage_times.argtypes = [vp, vp, C.c_int]
         L.mcgpu_stage_name.argtypes = [C.c_int]
         L.mcgpu_stage_name.restype = C.c_char_p
         for fn in ("mcgpu_create_context", "mcgpu_release_context", "mcgpu_dispatch_frontend",
                    "mcgpu_redispatch_perturbed", "mcgpu_get_registry", "mcgpu_get_candidate_pairs", "mcgpu_get_edges",
                    "mcgpu_get_tests", "mcgpu_get_face_planes", "mcgpu_get_ps_vertices", "mcgpu_registry_device",
-                   "mcgpu_stage_times", "mcgpu_broad_phase"):
+                   "mcgpu_stage_times", "mcgpu_broad_phase", "mcgpu_dispatch_batch", "mcgpu_get_batch_status",
+                   "mcgpu_get_batch_candidate_pairs"):
             getattr(L, fn).restype = C.c_int
         _lib = L
     return _lib
@@ -200,6 +209,71 @@ def stage_names():
     return _stage_names
 
 
+def pack_batch(pairs):
+    """Concatenates (src, cut) mesh pairs into the arrays of an mcgpu_batch:
+    every source mesh in pair order, every cut mesh in pair order, face
+    indices local to their mesh, and the vertex / face offsets per pair."""
+    out = {}
+    for side, k in (("src", 0), ("cut", 1)):
+        meshes = [pr[k] for pr in pairs]
+        V = np.ascontiguousarray(np.concatenate([np.asarray(m[0], np.float64).reshape(-1, 3) for m in meshes]))
+        F = np.ascontiguousarray(np.concatenate([np.asarray(m[1], np.uint32).ravel() for m in meshes]))
+        nf = [(np.asarray(m[1]).size // 3 if m[2] is None else len(m[2])) for m in meshes]
+        S = None
+        if any(m[2] is not None for m in meshes):
+            S = np.ascontiguousarray(np.concatenate([
+                np.full(n, 3, np.uint32) if m[2] is None else np.asarray(m[2], np.uint32) for m, n in zip(meshes, nf)]))
+        out[side] = (V, F, S)
+        out[side + "_vo"] = np.concatenate([[0], np.cumsum([np.asarray(m[0]).reshape(-1, 3).shape[0]
+                                                            for m in meshes])]).astype(np.uint32)
+        out[side + "_fo"] = np.concatenate([[0], np.cumsum(nf)]).astype(np.uint32)
+    out["num_pairs"] = len(pairs)
+    return out
+
+
+class BatchResult:
+    """Per-pair outputs of one batched dispatch."""
+
+    def __init__(self, ctx, info: FrontendInfo, num_pairs: int):
+        self._ctx = ctx
+        self.info = info
+        self.num_pairs = num_pairs
+        self.counts = dict(pairs=info.num_candidate_pairs, cand=info.num_candidate_faces, edges=info.num_edges,
+                           tests=info.num_tests, escalated=info.num_escalated, registry=info.num_registry,
+                           raw_pairs=info.num_raw_pairs)
+        self.device_ms = info.device_ms
+        self.status = np.zeros(num_pairs, np.int32)
+        self.reg_offsets = np.zeros(num_pairs + 1, np.uint64)
+        _check(lib().mcgpu_get_batch_status(ctx.h, _ptr(self.status), _ptr(self.reg_offsets), num_pairs + 1),
+               "mcgpu_get_batch_status")
+        self._reg = None
+
+    def registry(self):
+        """The whole batch registry (pair-local ids, pairs in order)."""
+        if self._reg is None:
+            n = self.info.num_registry
+            e, g, p = np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros((n, 3), np.float64)
+            _check(lib().mcgpu_get_registry(self._ctx.h, _ptr(e), _ptr(g), _ptr(p), n), "mcgpu_get_registry")
+            self._reg = (e, g, p)
+        return self._reg
+
+    def pair_registry(self, i):
+        e, g, p = self.registry()
+        a, b = int(self.reg_offsets[i]), int(self.reg_offsets[i + 1])
+        return e[a:b], g[a:b], p[a:b]
+
+    def candidate_pairs(self):
+        """(pair, source face, cut face) triples, pair-local ids, sorted."""
+        n = self.info.num_candidate_pairs
+        out = np.zeros((n, 3), np.uint32)
+        if n:
+            _check(lib().mcgpu_get_batch_candidate_pairs(self._ctx.h, _ptr(out), n), "mcgpu_get_batch_candidate_pairs")
+        return out
+
+    def stage_ms(self):
+        return Result.stage_ms(self)
+
+
 class Context:
     """One libmcgpu context: a HIP stream and its device buffers on one GPU."""
 
@@ -248,6 +322,27 @@ class Context:
         if n.value:
             _check(lib().mcgpu_get_candidate_pairs(self.h, _ptr(out), n.value), "mcgpu_get_candidate_pairs")
         return out
+
+    def dispatch_batch(self, pairs=None, packed=None, flags=0, gp_constant=1e-4):
+        """Batched front end over a list of (src, cut) pairs (or pack_batch's output)."""
+        P = packed if packed is not None else pack_batch(pairs)
+        keep = []
+        ms, mc = _host_mesh(P["src"], keep), _host_mesh(P["cut"], keep)
+        offs = [np.ascontiguousarray(P[k], np.uint32) for k in ("src_vo", "src_fo", "cut_vo", "cut_fo")]
+        b = Batch(P["num_pairs"], ms, mc, *[_ptr(o) for o in offs])
+        return self._dispatch_batch(b, DispatchDesc(flags, gp_constant, 0, 0, 0, 0))
+
+    def dispatch_batch_device(self, num_pairs, src: Mesh, cut: Mesh, offsets, flags=0, gp_constant=1e-4):
+        """Batched front end over device-resident meshes; offsets = host uint32
+        arrays (src vertex, src face, cut vertex, cut face), num_pairs + 1 each."""
+        offs = [np.ascontiguousarray(o, np.uint32) for o in offsets]
+        b = Batch(num_pairs, src, cut, *[_ptr(o) for o in offs])
+        return self._dispatch_batch(b, DispatchDesc(flags, gp_constant, 0, 1, 0, 0))
+
+    def _dispatch_batch(self, b, d):
+        info = FrontendInfo()
+        _check(lib().mcgpu_dispatch_batch(self.h, C.byref(b), C.byref(d), C.byref(info)), "mcgpu_dispatch_batch")
+        return BatchResult(self, info, b.num_pairs)
 
     def redispatch_perturbed(self, perturbation, enforcement_count, nv):
         p = np.ascontiguousarray(perturbation, dtype=np.float64)

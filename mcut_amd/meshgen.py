@@ -163,46 +163,72 @@ def icosphere(level: int):
     return V, F.astype(np.uint32).ravel(), None
 
 
+_GEODESIC = {}
+
+
 def geodesic_sphere(n: int):
     """Class-I geodesic sphere of frequency n: each icosahedron face is split
     into n^2 triangles on a barycentric grid, points projected to the unit
-    sphere.  20*n^2 faces, 10*n^2+2 vertices (grid points shared)."""
+    sphere.  20*n^2 faces, 10*n^2+2 vertices (grid points shared between
+    faces through their (icosahedron vertex, weight) sets, so a shared point
+    gets one id and one coordinate)."""
+    if n in _GEODESIC:
+        V, F = _GEODESIC[n]
+        return V.copy(), F.copy(), None
     V0, F0 = _icosahedron()
-    # vertex ids: global dedup through integer barycentric keys on shared edges
-    key_to_id = {}
-    pts = []
-    faces = []
+    I, J = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    ok = (I + J) <= n
+    gi, gj = I[ok], J[ok]                          # grid points of one face, i-major
+    G = gi.size
+    lmap = -np.ones((n + 2, n + 2), np.int64)
+    lmap[gi, gj] = np.arange(G)
+    w = np.stack([n - gi - gj, gi, gj], 1)         # weights of (a, b, c)
+    vid = F0[:, None, :].repeat(G, 1)              # (20, G, 3)
+    ww = np.broadcast_to(w[None], (20, G, 3))
+    code = np.where(ww > 0, vid * (n + 1) + ww, -1).reshape(-1, 3)
+    code.sort(axis=1)
+    uk, inv = np.unique(code, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    P = np.zeros((uk.shape[0], 3))
+    for k in range(3):
+        c = uk[:, k]
+        has = c >= 0
+        P[has] += V0[c[has] // (n + 1)] * (c[has] % (n + 1))[:, None].astype(np.float64)
+    P /= n
+    P /= np.linalg.norm(P, axis=1)[:, None]
+    # cells (i, j), i + j <= n - 1, i-major: an up triangle each, a down one when i + j <= n - 2
+    ci, cj = gi[gi + gj <= n - 1], gj[gi + gj <= n - 1]
+    up = np.stack([lmap[ci, cj], lmap[ci + 1, cj], lmap[ci, cj + 1]], 1)
+    dn = np.stack([lmap[ci + 1, cj], lmap[ci + 1, cj + 1], lmap[ci, cj + 1]], 1)
+    both = np.stack([up, dn], 1)                   # (cells, 2, 3)
+    keep = np.stack([np.ones(ci.size, bool), ci + cj <= n - 2], 1)
+    local = both[keep]                             # per-face local triangles in order
+    Fall = (local[None, :, :] + (np.arange(20) * G)[:, None, None]).reshape(-1)
+    F = inv[Fall].astype(np.uint32)
+    _GEODESIC[n] = (P, F)
+    return P.copy(), F.copy(), None
 
-    def gid(face, i, j):
-        # point a + (b-a)*i/n + (c-a)*j/n; canonical key from the (vertex-id, weight) set
-        a, b, c = F0[face]
-        w = {}
-        for vv, ww in ((a, n - i - j), (b, i), (c, j)):
-            if ww:
-                w[vv] = w.get(vv, 0) + ww
-        key = tuple(sorted(w.items()))
-        if key not in key_to_id:
-            key_to_id[key] = len(pts)
-            p = sum(V0[vv] * ww for vv, ww in key) / n
-            pts.append(p / np.linalg.norm(p))
-        return key_to_id[key]
 
-    for f in range(20):
-        for i in range(n):
-            for j in range(n - i):
-                p0 = gid(f, i, j)
-                p1 = gid(f, i + 1, j)
-                p2 = gid(f, i, j + 1)
-                faces.append((p0, p1, p2))
-                if i + j < n - 1:
-                    p3 = gid(f, i + 1, j + 1)
-                    faces.append((p1, p3, p2))
-    return np.asarray(pts, dtype=np.float64), np.asarray(faces, dtype=np.uint32).ravel(), None
+_BOX = {}
+
+
+def _box_grid(m: int):
+    """Integer grid coordinates (0..m) and triangles of tessellated_box's
+    m x m-per-face cube (cached per m)."""
+    if m not in _BOX:
+        V, F, _ = tessellated_box((0.0, 0.0, 0.0), float(m), m)
+        _BOX[m] = (V + m / 2.0, F)
+    return _BOX[m]
 
 
 def random_rotation(seed: int, stream: int = 0) -> np.ndarray:
     """Uniform random rotation (Shoemake's subgroup algorithm for a unit quaternion)."""
     u1, u2, u3 = u01(seed, stream, 3)
+    return rotation_from_uniforms(u1, u2, u3)
+
+
+def rotation_from_uniforms(u1: float, u2: float, u3: float) -> np.ndarray:
+    """Shoemake's map of three U[0,1) draws to a uniform random rotation matrix."""
     a = math.sqrt(1.0 - u1)
     b = math.sqrt(u1)
     w = a * math.sin(2 * math.pi * u2)
@@ -284,6 +310,47 @@ def config2(level: int = 7, amp: float = 0.02):
     B = B @ R.T
     B[:, 0] += 0.5
     return (A, F, None), (B.copy(), F.copy(), None)
+
+
+def config4_pairs(num_pairs: int = 4096, seed: int = 5, amp: float = 0.02, tri_range=(200, 200_000),
+                  box_range=(12, 12_000)):
+    """Batched mix: independent (sphere, box) pairs.  Pair k draws, from its
+    own stream: sphere triangles T ~ logU(tri_range) -> Class-I geodesic
+    frequency n = round(sqrt(T / 20)) clamped to [4, 100], radius 1, radial
+    noise U(-amp, amp); box triangles T_b ~ logU(box_range) -> m = round(sqrt(
+    T_b / 12)) quads per face edge, triangulated, uniform random rotation,
+    edge s ~ U(0.5, 1.5), centre at distance 1 + s/2 - delta along a uniform
+    random direction, delta ~ U(0.1, 0.6).  A draw whose sphere has a face of
+    squared Newell length below 1e-9 (rejected by the reference,
+    math.cpp:151-152) is redrawn."""
+    pairs = []
+    lt, lb = math.log(tri_range[0]), math.log(box_range[0])
+    rt, rb = math.log(tri_range[1] / tri_range[0]), math.log(box_range[1] / box_range[0])
+    for k in range(num_pairs):
+        for a in range(16):
+            st = 1000 + 16 * k + a
+            u = u01(seed, st, 12)
+            n = int(min(100, max(4, round(math.sqrt(math.exp(lt + u[0] * rt) / 20.0)))))
+            m = int(max(1, round(math.sqrt(math.exp(lb + u[1] * rb) / 12.0))))
+            R = rotation_from_uniforms(u[2], u[3], u[4])
+            sc = 0.5 + u[5]
+            z = 2.0 * u[6] - 1.0
+            ph = 2.0 * math.pi * u[7]
+            rz = math.sqrt(max(0.0, 1.0 - z * z))
+            d = np.array([rz * math.cos(ph), rz * math.sin(ph), z])
+            delta = 0.1 + 0.5 * u[8]
+            V, F, _ = geodesic_sphere(n)
+            V = radial_displace(V, amp, seed=seed, stream=st + (1 << 40))
+            if newell_min_sq(V, F) < 1e-9:
+                continue
+            G, Fb = _box_grid(m)
+            Vb = ((G / m) - 0.5) * sc
+            Vb = Vb @ R.T + d * (1.0 + sc / 2.0 - delta)
+            pairs.append(((V, F, None), (np.ascontiguousarray(Vb), Fb.copy(), None)))
+            break
+        else:
+            raise RuntimeError(f"pair {k}: no valid draw")
+    return pairs
 
 
 def read_off(path: str):
