@@ -10,6 +10,10 @@ resident in HBM.
 
 Prints one JSON line.  The CPU baseline is the single-threaded C restatement
 of the same front end (oracle/) timed on this host on the same input.
+
+The line also carries the batched mix ("batched"): 4096 seeded sphere/box
+pairs of uneven size, one batched dispatch against one plain dispatch per
+pair, both over inputs resident in HBM.
 """
 import argparse
 import json
@@ -75,6 +79,10 @@ def stage_bytes(name, n):
     return table.get(name, 0)
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(src, cut, registry):
     from oracle import oracle as om
     t0 = time.perf_counter()
@@ -85,6 +93,82 @@ def cpu_baseline(src, cut, registry):
             "sample": f"the whole config front end once on the host ({dt:.2f} s, single thread)"}
 
 
+def bench_batch(args, torch, frontend, mg):
+    """Batched mix: num_pairs seeded (displaced geodesic sphere, rotated
+    tessellated box) pairs resident in HBM.  One mcgpu_dispatch_batch call
+    against a loop of one mcgpu_dispatch_frontend call per pair over the same
+    device buffers; per-pair statuses and registry counts of the two must
+    agree."""
+    t0 = time.perf_counter()
+    pairs = mg.config4_pairs(args.batch_pairs, seed=5)
+    P = frontend.pack_batch(pairs)
+    gen_s = time.perf_counter() - t0
+    progress(f"batched mix: {args.batch_pairs} pairs generated in {gen_s:.1f} s")
+    del pairs
+    dev = torch.device("cuda:0")
+    Vs, Fs_, _ = P["src"]
+    Vc, Fc_, _ = P["cut"]
+    tVs = torch.from_numpy(Vs).to(dev)
+    tFs = torch.from_numpy(Fs_.view(np.int32)).to(dev)
+    tVc = torch.from_numpy(Vc).to(dev)
+    tFc = torch.from_numpy(Fc_.view(np.int32)).to(dev)
+    torch.cuda.synchronize()
+    nfs, nfc = int(P["src_fo"][-1]), int(P["cut_fo"][-1])
+    ms = frontend.device_mesh(tVs.data_ptr(), tFs.data_ptr(), None, Vs.shape[0], nfs)
+    mc = frontend.device_mesh(tVc.data_ptr(), tFc.data_ptr(), None, Vc.shape[0], nfc)
+    offs = [P["src_vo"], P["src_fo"], P["cut_vo"], P["cut_fo"]]
+    np_ = P["num_pairs"]
+    bctx = frontend.Context(0)
+    B = bctx.dispatch_batch_device(np_, ms, mc, offs)  # warm-up: sizes the buffers
+    torch.cuda.synchronize()
+    progress(f"batched mix: warm-up batch done ({B.counts})")
+    bt = []
+    for _ in range(args.batch_steps):
+        t1 = time.perf_counter()
+        B = bctx.dispatch_batch_device(np_, ms, mc, offs)
+        bt.append(time.perf_counter() - t1)
+    batch_s = min(bt)
+    bstages = B.stage_ms()
+    progress(f"batched mix: batch {batch_s * 1e3:.1f} ms")
+    reg_b = np.diff(B.reg_offsets.astype(np.int64))
+    status_b = B.status.copy()
+    counts_b = dict(B.counts)
+    device_ms_b = B.device_ms
+    bctx.close()
+
+    pctx = frontend.Context(0)
+
+    def plain(p):
+        vs0, vs1, fs0, fs1 = int(P["src_vo"][p]), int(P["src_vo"][p + 1]), int(P["src_fo"][p]), int(P["src_fo"][p + 1])
+        vc0, vc1, fc0, fc1 = int(P["cut_vo"][p]), int(P["cut_vo"][p + 1]), int(P["cut_fo"][p]), int(P["cut_fo"][p + 1])
+        a = frontend.device_mesh(tVs.data_ptr() + 24 * vs0, tFs.data_ptr() + 12 * fs0, None, vs1 - vs0, fs1 - fs0)
+        b = frontend.device_mesh(tVc.data_ptr() + 24 * vc0, tFc.data_ptr() + 12 * fc0, None, vc1 - vc0, fc1 - fc0)
+        return pctx.dispatch_device(a, b)
+    for p in range(np_):  # warm-up pass: sizes the buffers for every pair
+        plain(p)
+    torch.cuda.synchronize()
+    progress(f"batched mix: plain warm-up pass done")
+    reg_p = np.zeros(np_, np.int64)
+    status_p = np.zeros(np_, np.int32)
+    t1 = time.perf_counter()
+    for p in range(np_):
+        R = plain(p)
+        reg_p[p] = R.counts["registry"]
+        status_p[p] = R.status
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t1
+    pctx.close()
+    assert np.array_equal(reg_b, reg_p) and np.array_equal(status_b, status_p), "batch and plain results differ"
+    I = int(reg_b.sum())
+    return {"workload": f"{np_} pairs: geodesic sphere (logU 200-200k tris, +-2% radial noise) vs rotated "
+                        f"tessellated box (logU 12-12k tris), seed 5",
+            "faces": nfs + nfc, "registry": I, "pairs_not_success": int(np.count_nonzero(status_b)),
+            "batch_ms": batch_s * 1e3, "batch_device_ms": device_ms_b, "plain_loop_ms": plain_s * 1e3,
+            "batched_vs_plain": plain_s / batch_s, "batch_pairs_per_s": I / batch_s,
+            "plain_pairs_per_s": I / plain_s, "counts": counts_b, "stage_ms": bstages,
+            "generate_s": gen_s}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +176,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--level", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-pairs", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (use --gpus 1)", file=sys.stderr)
@@ -113,6 +200,7 @@ def main():
 
     ms, mc = to_dev(src), to_dev(cut)
     torch.cuda.synchronize()
+    progress("single pair: inputs resident")
     ctx = frontend.Context(0)
     for _ in range(args.warmup):
         R = ctx.dispatch_device(ms, mc)
@@ -176,10 +264,15 @@ def main():
                      "traffic": None, "alg_bytes": dom_bytes,
                      "frontend_alg_bytes": sum(stage_bytes(k, n) for k in stages)},
     }
+    progress(f"single pair: {ms_step:.3f} ms per step")
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(src, cut, I)
+        progress("single pair: cpu baseline done")
         line["speedup_vs_cpu_baseline"] = line["value"] / line["cpu_baseline"]["value"]
     ctx.close()
+    del keep
+    if not args.no_batch:
+        line["batched"] = bench_batch(args, torch, frontend, mg)
     print(json.dumps(line), flush=True)
 
 
