@@ -182,6 +182,56 @@ MCGPU_API mcgpu_result mcgpu_get_batch_status(mcgpu_context ctx, int32_t* status
  * face) triples, face ids local to the pair, sorted. */
 MCGPU_API mcgpu_result mcgpu_get_batch_candidate_pairs(mcgpu_context ctx, uint32_t* triples, uint64_t capacity);
 
+/* --- serving ------------------------------------------------------------------
+ * One resident source mesh cut by requests as they arrive.  The server runs
+ * the source through the front end once and keeps every part of it that no
+ * request changes -- coordinate sums, raw box, polygon-soup edges, raw-space
+ * leaf boxes and BVH -- resident in HBM.  Each request is then only its cut
+ * mesh: its sums, shift, cut-mesh edges and BVH, the traversal against the
+ * resident source tree, and the narrow phase.  Requests run in slots (one
+ * HIP stream and buffers sized once for the largest request each), so
+ * requests that arrive while others run overlap on the device.  No reference
+ * counterpart: it replaces a host loop of mcDispatch calls with one source
+ * mesh; every request's status and registry are those of mcgpu_dispatch_frontend
+ * on (source, cut). */
+typedef struct mcgpu_server_t* mcgpu_server;
+
+typedef struct {
+    uint32_t max_cut_vertices;   /* largest request a slot holds */
+    uint32_t max_cut_faces;
+    uint32_t max_cut_corners;    /* sum of face sizes (0: 3 * max_cut_faces) */
+    uint32_t max_slots;          /* requests in flight at most */
+    uint64_t hbm_budget;         /* bytes the in-flight requests' footprints may add up to */
+} mcgpu_server_limits;
+
+typedef struct {
+    double arrival_s;            /* from the schedule */
+    double start_s;              /* admitted to a slot (seconds since the run began) */
+    double done_s;               /* results read back */
+    int32_t status;              /* mcgpu_status, or a negative mcgpu_result */
+    uint32_t reserved;
+    uint64_t num_registry;
+    uint64_t num_candidate_pairs;
+} mcgpu_request_result;
+
+/* desc: flags, gp_constant and input_on_device for the source and every
+ * request (device pointers must outlive the server / the run). */
+MCGPU_API mcgpu_result mcgpu_server_create(int device, const mcgpu_mesh* src, const mcgpu_dispatch_desc* desc,
+                                           const mcgpu_server_limits* limits, mcgpu_server* out);
+MCGPU_API mcgpu_result mcgpu_server_release(mcgpu_server srv);
+MCGPU_API mcgpu_result mcgpu_server_info(mcgpu_server srv, uint32_t* slots, uint64_t* slot_bytes, uint32_t* source_edges);
+/* Runs a schedule to completion: request i (cuts[i]) arrives arrival_s[i]
+ * seconds after the call.  Arrived requests are admitted oldest first while a
+ * slot is free and the sum of in-flight footprints (an estimate from each
+ * request's face, corner and vertex counts) stays within hbm_budget; an idle
+ * server always admits.  Completion is polled; out[i] gets its times, status
+ * and counts.  keep_registries: copy each registry to the host for
+ * mcgpu_server_get_registry. */
+MCGPU_API mcgpu_result mcgpu_server_run(mcgpu_server srv, const mcgpu_mesh* cuts, const double* arrival_s, uint32_t n,
+                                        uint32_t keep_registries, mcgpu_request_result* out);
+MCGPU_API mcgpu_result mcgpu_server_get_registry(mcgpu_server srv, uint32_t request, uint32_t* edges, uint32_t* faces,
+                                                 double* points, uint64_t capacity);
+
 /* --- results of the last dispatch (host copies; capacity in elements) -----
  * After mcgpu_dispatch_batch, mcgpu_get_registry and mcgpu_registry_device
  * give the batch registry: pairs in order, each pair's entries with the

@@ -112,6 +112,7 @@ __global__ void __launch_bounds__(64) k_mesh_sums(const double* __restrict__ v0,
 {
     __builtin_amdgcn_s_setprio(3);
     const int m = blockIdx.x / 3, j = blockIdx.x % 3;
+    if (m == 0 && !v0) return; // a resident source: its sums are set already
     const double acc = serial_sum((m ? v1 : v0) + j, m ? n1 : n0, zero);
     if (threadIdx.x == 0) P->sum[m][j] = acc;
 }
@@ -289,6 +290,22 @@ __global__ void k_shift_vertices(const double* __restrict__ vs, uint32_t ns, con
         const double y = (x - com[j]) + t[j];
         ps_v[i] = y + (cut ? pe[j] : 0.0);
     }
+}
+
+// serving: the resident source's sums, raw box and edge count into this
+// dispatch's parameters and counters (after the counter reset)
+struct ResidentGpuView {
+    double sum[3];
+    unsigned long long rbox[6];
+    unsigned long long ne;
+};
+
+__global__ void k_resident_source(Counters* C, Params* P, const ResidentGpuView* r)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int j = 0; j < 3; ++j) P->sum[0][j] = r->sum[j];
+    for (int j = 0; j < 6; ++j) C->rbox[0][j] = r->rbox[j];
+    C->n_edges_src = r->ne;
 }
 
 // batches: every pair's vertices with its own shift (no perturbation)

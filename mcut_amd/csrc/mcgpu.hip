@@ -3,7 +3,10 @@
 #include "mcgpu.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <deque>
+#include <thread>
 #include <cstddef>
 #include <cstdio>
 #include <cstdlib>
@@ -70,6 +73,27 @@ int bits_for(uint64_t n)
     return b;
 }
 
+// Serving: the request-independent parts of a resident source mesh, computed
+// once by the server's master context and read by every request slot.
+using ResidentGpu = ResidentGpuView; // sums, raw box as order keys, source edge count
+
+struct Resident {
+    uint32_t nv = 0, nf = 0, ne = 0;
+    uint64_t ncorner = 0;
+    const double* dv = nullptr;
+    const uint32_t* didx = nullptr;
+    const uint32_t* dsz = nullptr;
+    const uint32_t* off = nullptr;   // face offsets
+    const uint32_t* cedge = nullptr; // corner -> edge
+    const uint32_t *ev0 = nullptr, *ev1 = nullptr;
+    const int32_t *ef0 = nullptr, *ef1 = nullptr; // source edges [0, ne)
+    const float* rawbox = nullptr;   // float leaf boxes of the source faces
+    const uint32_t* sface = nullptr; // Morton order of the source faces
+    const float* nodes = nullptr;    // source tree
+    int depth = 0;
+    ResidentGpu* dres = nullptr;
+};
+
 } // namespace
 
 struct mcgpu_context_t {
@@ -126,6 +150,11 @@ struct mcgpu_context_t {
     std::vector<int32_t> hstatus;
     Buf bpairs, bstarts, fpair, vpair, bparams, brbox, pflags, border, bidx[2], rle, rlg, rstart, rend;
     const uint32_t* bin_idx[2] = {nullptr, nullptr}; // the caller's pair-local face lists
+
+    // serving: a request slot of an mcgpu_server; one stream (s2, s3 and ssum
+    // are s), the source mesh is the server's resident one
+    const Resident* res = nullptr;
+    bool own_streams = true;
 };
 
 #define CK(x)                                                                                                          \
@@ -230,6 +259,10 @@ Soup make_soup(mcgpu_context ctx)
         S.nv[m] = ctx->nv[m];
         S.nf[m] = ctx->nf[m];
         S.cedge[m] = P_<uint32_t>(ctx->cedge[m]);
+    }
+    if (ctx->res) { // the resident source's face offsets and corner edges
+        S.off[0] = ctx->res->off;
+        S.cedge[0] = ctx->res->cedge;
     }
     S.ps_v = P_<double>(ctx->ps_v);
     S.box = P_<double>(ctx->box);
@@ -432,7 +465,9 @@ void enqueue_validate(mcgpu_context ctx)
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
     (void)hipEventRecord(ctx->ev[EV_A0], s);
     fill_u32(C, 0u, sizeof(Counters) / 4, s);
+    if (ctx->res) k_resident_source<<<1, 1, 0, s>>>(C, &P_<DevState>(ctx->state)->P, ctx->res->dres);
     for (int m = 0; m < 2; ++m) {
+        if (ctx->res && m == 0) continue; // resident source: validated, boxed and summed once
         fill_u64(C->rbox[m], ~0ull, 3, s);
         k_raw_bbox<<<grid_for(ctx->nv[m], kBlock, 64), kBlock, 0, s>>>(ctx->dv[m], ctx->nv[m], C->rbox[m]);
         uint32_t* off = P_<uint32_t>(ctx->off[m]);
@@ -529,8 +564,12 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C)
     const uint64_t F = (uint64_t)Fs + ctx->nf[1];
     float* rawbox = P_<float>(ctx->rawbox);
     RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
-    ctx->mface_alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode), P_<uint32_t>(ctx->mface), P_<uint32_t>(ctx->mcode2),
-                                          P_<uint32_t>(ctx->mface2), nullptr, F, F, ctx->sort_bits, rt, st);
+    // a resident source keeps its own sorted leaves and tree: only the cut
+    // leaves are sorted and only the cut tree is refitted
+    const uint64_t o0 = ctx->res ? Fs : 0;
+    ctx->mface_alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode) + o0, P_<uint32_t>(ctx->mface) + o0,
+                                          P_<uint32_t>(ctx->mcode2) + o0, P_<uint32_t>(ctx->mface2) + o0, nullptr,
+                                          F - o0, F - o0, ctx->sort_bits, rt, st);
     (void)hipEventRecord(ctx->ev[EV_B_SORT], st);
     const uint32_t* sorted = P_<uint32_t>(ctx->mface_alt ? ctx->mface2 : ctx->mface);
     Tree T[2];
@@ -542,6 +581,13 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C)
         R[m].n = (int)ctx->nf[m];
         R[m].D = ctx->depth[m];
         R[m].lin = ctx->depth[m];
+        if (m == 0 && ctx->res) {
+            R[0].facebox = ctx->res->rawbox;
+            R[0].sface = ctx->res->sface;
+            R[0].nodes = const_cast<float*>(ctx->res->nodes);
+            R[0].D = ctx->res->depth;
+            R[0].lin = -1; // already refitted
+        }
         T[m].sface = R[m].sface;
         T[m].nodes = R[m].nodes;
         T[m].facebox = R[m].facebox;
@@ -595,6 +641,8 @@ void enqueue_frontend(mcgpu_context ctx)
         const uint32_t nch = 6 * ctx->npairs;
         k_batch_sums<<<std::min<uint32_t>(nch, kBatchSumWaves), 64, 0, ctx->ssum>>>(
             ctx->dv[0], ctx->dv[1], P_<BatchPair>(ctx->bpairs), P_<uint32_t>(ctx->border), nch, C, Pb);
+    } else if (ctx->res) { // the resident source's sums are set already
+        k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(nullptr, 0, ctx->dv[1], ctx->nv[1], &C->zero, Pd);
     } else {
         k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], &C->zero, Pd);
     }
@@ -623,6 +671,7 @@ void enqueue_frontend(mcgpu_context ctx)
     for (int m = 0; m < 2; ++m) {
         const uint32_t nf = ctx->nf[m];
         const uint64_t o = m ? Fs : 0;
+        if (ctx->res && m == 0) continue;
         if (ctx->batch)
             k_batch_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf, m,
                                                                 (uint32_t)o, P_<uint32_t>(ctx->fpair), brbox, Pb, C,
@@ -643,17 +692,24 @@ void enqueue_frontend(mcgpu_context ctx)
     const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
     fill_u32(ctx->cand.p, 0u, F, s3);
     (void)hipEventRecord(ctx->ev[EV_C_FILL], s3);
-    fill_u32(ctx->ef1.p, 0xffffffffu, K, s3);
+    const uint32_t res_ne = ctx->res ? ctx->res->ne : 0u;
+    if (ctx->res) { // the resident source's edges occupy [0, ne) of the edge table
+        (void)hipMemcpyAsync(ctx->ev0.p, ctx->res->ev0, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
+        (void)hipMemcpyAsync(ctx->ev1.p, ctx->res->ev1, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
+        (void)hipMemcpyAsync(ctx->ef0.p, ctx->res->ef0, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
+        (void)hipMemcpyAsync(ctx->ef1.p, ctx->res->ef1, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
+    }
+    fill_u32(P_<uint32_t>(ctx->ef1) + res_ne, 0xffffffffu, K - res_ne, s3);
     CornerMap cmap[2];
     for (int m = 0; m < 2; ++m) {
         cmap[m].off = P_<uint32_t>(ctx->off[m]);
         cmap[m].cface = ctx->dsz[m] ? P_<uint32_t>(ctx->cface[m]) : nullptr;
-        if (ctx->dsz[m])
+        if (ctx->dsz[m] && !(ctx->res && m == 0))
             k_corner_faces<<<grid_for(ctx->nf[m]), kBlock, 0, s3>>>(cmap[m].off, ctx->nf[m], P_<uint32_t>(ctx->cface[m]));
     }
     // the scans of this stream use their own scratch: stream B's sort uses scan_tmp
     uint32_t* ctmp = P_<uint32_t>(ctx->scan_tmp_c);
-    for (int m = 0; m < 2; ++m) {
+    for (int m = ctx->res ? 1 : 0; m < 2; ++m) {
         const uint64_t nk = ctx->ncorner[m];
         const uint64_t hs = next_pow2(2 * nk + 2);
         fill_u64(ctx->hkeys[m].p, kEmpty64, hs, s3);
@@ -666,7 +722,7 @@ void enqueue_frontend(mcgpu_context ctx)
         scan_exclusive(P_<uint32_t>(ctx->eflag[m]), P_<uint32_t>(ctx->eid[m]), nk, ctmp, etotal + m, s3);
         k_store_edge_count<<<1, 1, 0, s3>>>(etotal + m, C, m);
     }
-    for (int m = 0; m < 2; ++m) {
+    for (int m = ctx->res ? 1 : 0; m < 2; ++m) {
         const uint64_t nk = ctx->ncorner[m];
         const uint64_t hs = next_pow2(2 * nk + 2);
         k_edge_assign<<<grid_for(nk), kBlock, 0, s3>>>(
@@ -897,9 +953,11 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->s);
-    (void)hipStreamSynchronize(ctx->s2);
-    (void)hipStreamSynchronize(ctx->s3);
-    (void)hipStreamSynchronize(ctx->ssum);
+    if (ctx->own_streams) {
+        (void)hipStreamSynchronize(ctx->s2);
+        (void)hipStreamSynchronize(ctx->s3);
+        (void)hipStreamSynchronize(ctx->ssum);
+    }
     Buf* all[] = {&ctx->in_v[0], &ctx->in_v[1], &ctx->in_idx[0], &ctx->in_idx[1], &ctx->in_sz[0], &ctx->in_sz[1],
                   &ctx->off[0], &ctx->off[1], &ctx->state, &ctx->ps_v, &ctx->box, &ctx->hkeys[0],
                   &ctx->hkeys[1], &ctx->hfirst[0], &ctx->hfirst[1], &ctx->cface[0], &ctx->cface[1], &ctx->eflag[0],
@@ -915,9 +973,11 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
     for (Buf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
-    (void)hipStreamDestroy(ctx->ssum);
-    (void)hipStreamDestroy(ctx->s3);
-    (void)hipStreamDestroy(ctx->s2);
+    if (ctx->own_streams) {
+        (void)hipStreamDestroy(ctx->ssum);
+        (void)hipStreamDestroy(ctx->s3);
+        (void)hipStreamDestroy(ctx->s2);
+    }
     (void)hipStreamDestroy(ctx->s);
     delete ctx;
     return MCGPU_SUCCESS;
@@ -1355,6 +1415,384 @@ MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity)
 MCGPU_API const char* mcgpu_stage_name(int stage)
 {
     return (stage >= 0 && stage < ST_COUNT) ? kStageNames[stage] : "";
+}
+
+// ===========================================================================
+// Serving: one resident source mesh, cut requests as they arrive
+// ===========================================================================
+
+} // extern "C"
+
+struct mcgpu_server_t {
+    int device = 0;
+    mcgpu_context master = nullptr; // its buffers hold the resident source
+    Resident res;
+    mcgpu_server_limits lim{};
+    mcgpu_dispatch_desc desc{};
+    std::vector<mcgpu_context> slots;
+    std::vector<hipEvent_t> done;
+    std::vector<std::vector<uint32_t>> reg_e, reg_g;
+    std::vector<std::vector<double>> reg_p;
+    uint64_t slot_bytes = 0;
+};
+
+namespace {
+
+uint64_t ctx_bytes(mcgpu_context c)
+{
+    uint64_t t = 0;
+    const Buf* all[] = {&c->in_v[1], &c->in_idx[1], &c->in_sz[1], &c->off[1], &c->state, &c->ps_v, &c->box,
+                        &c->hkeys[1], &c->hfirst[1], &c->cface[1], &c->eflag[1], &c->eid[1], &c->cedge[1],
+                        &c->etotal, &c->ev0, &c->ev1, &c->ef0, &c->ef1, &c->mcode, &c->mface, &c->mcode2,
+                        &c->mface2, &c->nodes[1], &c->rawbox, &c->raw_pairs, &c->pairs, &c->phash, &c->cand,
+                        &c->cand_list, &c->planes, &c->big_list, &c->big_scratch, &c->te, &c->tg, &c->esc,
+                        &c->rkey, &c->rval, &c->rkey2, &c->rval2, &c->rp, &c->re, &c->rg, &c->rpts, &c->rcount,
+                        &c->re_in, &c->rg_in, &c->shkeys, &c->shcnt, &c->scan_tmp, &c->scan_tmp_c, &c->hist};
+    for (const Buf* b : all) t += b->cap;
+    return t;
+}
+
+// HBM a request touches: the per-face, per-corner and per-vertex arrays of
+// its polygon soup (source and cut) and the lists sized from them
+uint64_t request_footprint(const Resident& r, const mcgpu_mesh* cut, uint64_t cut_corners)
+{
+    const uint64_t F = (uint64_t)r.nf + cut->num_faces, V = (uint64_t)r.nv + cut->num_vertices;
+    const uint64_t K = r.ncorner + cut_corners;
+    return 400 * F + 100 * K + 48 * V;
+}
+
+uint64_t mesh_corners(const mcgpu_mesh* m)
+{
+    if (!m->face_sizes) return 3ull * m->num_faces;
+    uint64_t k = 0;
+    for (uint32_t f = 0; f < m->num_faces; ++f) k += m->face_sizes[f];
+    return k;
+}
+
+mcgpu_result slot_create(int device, mcgpu_context* out)
+{
+    mcgpu_context ctx = new mcgpu_context_t();
+    ctx->device = device;
+    ctx->own_streams = false;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MCGPU_ERROR_DEVICE;
+    }
+    ctx->s2 = ctx->s3 = ctx->ssum = ctx->s;
+    for (int i = 0; i < EV_COUNT; ++i) (void)hipEventCreate(&ctx->ev[i]);
+    *out = ctx;
+    return MCGPU_SUCCESS;
+}
+
+// a slot's view of one request: the resident source as mesh 0, the cut as mesh 1
+mcgpu_result slot_bind(mcgpu_server srv, mcgpu_context ctx, const mcgpu_mesh* cut, bool on_device)
+{
+    const Resident& r = srv->res;
+    ctx->res = &srv->res;
+    ctx->have = false;
+    ctx->broad_only = false;
+    ctx->batch = false;
+    ctx->sort_bits = 22;
+    ctx->flags = srv->desc.flags;
+    ctx->gp_constant = srv->desc.gp_constant;
+    ctx->enforcement_count = 0;
+    ctx->keep_tests = 0;
+    ctx->nv[0] = r.nv;
+    ctx->nf[0] = r.nf;
+    ctx->ncorner[0] = r.ncorner;
+    ctx->dv[0] = r.dv;
+    ctx->didx[0] = r.didx;
+    ctx->dsz[0] = r.dsz;
+    mcgpu_result rr;
+    if ((rr = upload_mesh(ctx, 1, cut, on_device)) != MCGPU_SUCCESS) return rr;
+    const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
+    ctx->gbits = bits_for(F + 1);
+    ctx->kbits = ctx->gbits + bits_for(ctx->ncorner[0] + ctx->ncorner[1] + 1);
+    if (ctx->kbits > 64) return MCGPU_ERROR_UNSUPPORTED;
+    if ((rr = alloc_mesh_buffers(ctx)) != MCGPU_SUCCESS) return rr;
+    if (ctx->cap_pairs < 4 * F) ctx->cap_pairs = std::max<uint64_t>(1 << 14, 4 * F);
+    if (ctx->cap_tests < 2 * ctx->cap_pairs) ctx->cap_tests = 2 * ctx->cap_pairs;
+    if (ctx->cap_esc < (1u << 14)) ctx->cap_esc = std::max<uint64_t>(1 << 14, ctx->cap_tests / 16);
+    if (ctx->cap_reg < (1u << 14))
+        ctx->cap_reg = std::min<uint64_t>(std::max<uint64_t>(1 << 14, ctx->cap_tests / 8), kRegMergeTiles * kRegTile);
+    if (ctx->cap_big < 1024) ctx->cap_big = 1024;
+    return alloc_list_buffers(ctx);
+}
+
+void slot_enqueue(mcgpu_context ctx, hipEvent_t done)
+{
+    ctx->stage_mode = 0;
+    enqueue_validate(ctx);
+    enqueue_frontend(ctx);
+    (void)hipEventRecord(done, ctx->s);
+}
+
+} // namespace
+
+extern "C" {
+
+MCGPU_API mcgpu_result mcgpu_server_create(int device, const mcgpu_mesh* src, const mcgpu_dispatch_desc* desc,
+                                           const mcgpu_server_limits* limits, mcgpu_server* out)
+{
+    if (!out || !src || !desc || !limits || limits->max_slots == 0 || limits->max_cut_faces == 0 ||
+        limits->max_cut_vertices < 3)
+        return MCGPU_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    mcgpu_result r;
+    if ((r = check_mesh(src)) != MCGPU_SUCCESS) return r;
+    mcgpu_server srv = new mcgpu_server_t();
+    srv->device = device;
+    srv->lim = *limits;
+    srv->desc = *desc;
+    srv->desc.keep_tests = 0;
+    // the resident source: one dispatch of it against a far-away triangle
+    // leaves every source-only product in the master context's buffers
+    if ((r = mcgpu_create_context(device, &srv->master)) != MCGPU_SUCCESS) {
+        delete srv;
+        return r;
+    }
+    const double tri[9] = {1e6, 1e6, 1e6, 1e6 + 1.0, 1e6, 1e6, 1e6, 1e6 + 1.0, 1e6};
+    const uint32_t tf[3] = {0, 1, 2};
+    mcgpu_context m = srv->master;
+    {
+        mcgpu_mesh dummy{tri, tf, nullptr, 3, 1};
+        mcgpu_dispatch_desc d = srv->desc;
+        double* dv = nullptr;
+        uint32_t* di = nullptr;
+        if (desc->input_on_device) { // the dummy follows the source's residency
+            if (hipMalloc(&dv, sizeof(tri)) != hipSuccess || hipMalloc(&di, sizeof(tf)) != hipSuccess ||
+                hipMemcpy(dv, tri, sizeof(tri), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(di, tf, sizeof(tf), hipMemcpyHostToDevice) != hipSuccess) {
+                if (dv) (void)hipFree(dv);
+                if (di) (void)hipFree(di);
+                mcgpu_release_context(m);
+                delete srv;
+                return MCGPU_ERROR_OUT_OF_MEMORY;
+            }
+            dummy.vertices = dv;
+            dummy.face_indices = di;
+        }
+        mcgpu_frontend_info info;
+        r = mcgpu_dispatch_frontend(m, src, &dummy, &d, &info);
+        if (dv) (void)hipFree(dv);
+        if (di) (void)hipFree(di);
+        if (r != MCGPU_SUCCESS) {
+            mcgpu_release_context(m);
+            delete srv;
+            return r;
+        }
+    }
+    Resident& R = srv->res;
+    R.nv = m->nv[0];
+    R.nf = m->nf[0];
+    R.ncorner = m->ncorner[0];
+    R.dv = m->dv[0];
+    R.didx = m->didx[0];
+    R.dsz = m->dsz[0];
+    R.off = P_<uint32_t>(m->off[0]);
+    R.cedge = P_<uint32_t>(m->cedge[0]);
+    R.ev0 = P_<uint32_t>(m->ev0);
+    R.ev1 = P_<uint32_t>(m->ev1);
+    R.ef0 = P_<int32_t>(m->ef0);
+    R.ef1 = P_<int32_t>(m->ef1);
+    R.ne = (uint32_t)m->C.n_edges_src;
+    R.rawbox = P_<float>(m->rawbox);
+    R.sface = P_<uint32_t>(m->mface_alt ? m->mface2 : m->mface);
+    R.nodes = P_<float>(m->nodes[0]);
+    R.depth = m->depth[0];
+    ResidentGpu h;
+    for (int j = 0; j < 3; ++j) h.sum[j] = m->P.sum[0][j];
+    for (int j = 0; j < 6; ++j) h.rbox[j] = m->C.rbox[0][j];
+    h.ne = m->C.n_edges_src;
+    if (hipMalloc(&R.dres, sizeof(ResidentGpu)) != hipSuccess ||
+        hipMemcpy(R.dres, &h, sizeof(ResidentGpu), hipMemcpyHostToDevice) != hipSuccess) {
+        mcgpu_server_release(srv);
+        return MCGPU_ERROR_DEVICE;
+    }
+    // request slots, each sized once for the largest request it may hold
+    std::vector<double> mv(3ull * limits->max_cut_vertices, 0.0);
+    const uint64_t mk = limits->max_cut_corners ? limits->max_cut_corners : 3ull * limits->max_cut_faces;
+    for (uint32_t i = 0; i < limits->max_slots; ++i) {
+        mcgpu_context c;
+        if ((r = slot_create(device, &c)) != MCGPU_SUCCESS) {
+            mcgpu_server_release(srv);
+            return r;
+        }
+        srv->slots.push_back(c);
+        hipEvent_t e;
+        (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        srv->done.push_back(e);
+        c->res = &srv->res;
+        c->nv[0] = R.nv, c->nf[0] = R.nf, c->ncorner[0] = R.ncorner;
+        c->nv[1] = limits->max_cut_vertices, c->nf[1] = limits->max_cut_faces, c->ncorner[1] = mk;
+        double* dp;
+        uint32_t* u;
+        const uint64_t F = (uint64_t)R.nf + limits->max_cut_faces;
+        if (ensure(c->in_v[1], 3ull * limits->max_cut_vertices, &dp) != hipSuccess ||
+            ensure(c->in_idx[1], mk, &u) != hipSuccess || ensure(c->in_sz[1], limits->max_cut_faces, &u) != hipSuccess ||
+            alloc_mesh_buffers(c) != MCGPU_SUCCESS) {
+            mcgpu_server_release(srv);
+            return MCGPU_ERROR_OUT_OF_MEMORY;
+        }
+        c->cap_pairs = std::max<uint64_t>(1 << 14, 4 * F);
+        c->cap_tests = 2 * c->cap_pairs;
+        c->cap_esc = std::max<uint64_t>(1 << 14, c->cap_tests / 16);
+        c->cap_reg = std::min<uint64_t>(std::max<uint64_t>(1 << 14, c->cap_tests / 8), kRegMergeTiles * kRegTile);
+        c->cap_big = 1024;
+        if (alloc_list_buffers(c) != MCGPU_SUCCESS) {
+            mcgpu_server_release(srv);
+            return MCGPU_ERROR_OUT_OF_MEMORY;
+        }
+        srv->slot_bytes = std::max(srv->slot_bytes, ctx_bytes(c));
+    }
+    *out = srv;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_server_release(mcgpu_server srv)
+{
+    if (!srv) return MCGPU_ERROR_INVALID_VALUE;
+    (void)hipSetDevice(srv->device);
+    for (mcgpu_context c : srv->slots) mcgpu_release_context(c);
+    for (hipEvent_t e : srv->done) (void)hipEventDestroy(e);
+    if (srv->res.dres) (void)hipFree(srv->res.dres);
+    if (srv->master) mcgpu_release_context(srv->master);
+    delete srv;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_server_info(mcgpu_server srv, uint32_t* slots, uint64_t* slot_bytes, uint32_t* source_edges)
+{
+    if (!srv) return MCGPU_ERROR_INVALID_VALUE;
+    if (slots) *slots = (uint32_t)srv->slots.size();
+    if (slot_bytes) *slot_bytes = srv->slot_bytes;
+    if (source_edges) *source_edges = srv->res.ne;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_server_run(mcgpu_server srv, const mcgpu_mesh* cuts, const double* arrival_s, uint32_t n,
+                                        uint32_t keep_registries, mcgpu_request_result* out)
+{
+    if (!srv || !cuts || !arrival_s || !out) return MCGPU_ERROR_INVALID_VALUE;
+    for (uint32_t i = 0; i < n; ++i) {
+        mcgpu_result r;
+        if ((r = check_mesh(&cuts[i])) != MCGPU_SUCCESS) return r;
+        if (cuts[i].num_faces > srv->lim.max_cut_faces || cuts[i].num_vertices > srv->lim.max_cut_vertices)
+            return MCGPU_ERROR_INVALID_VALUE;
+    }
+    if (hipSetDevice(srv->device) != hipSuccess) return MCGPU_ERROR_DEVICE;
+    const bool on_device = srv->desc.input_on_device != 0;
+    srv->reg_e.assign(keep_registries ? n : 0, {});
+    srv->reg_g.assign(keep_registries ? n : 0, {});
+    srv->reg_p.assign(keep_registries ? n : 0, {});
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return arrival_s[a] < arrival_s[b]; });
+    std::vector<uint64_t> est(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i] = mcgpu_request_result{};
+        out[i].arrival_s = arrival_s[i];
+        est[i] = request_footprint(srv->res, &cuts[i], on_device ? 3ull * cuts[i].num_faces : mesh_corners(&cuts[i]));
+    }
+    const size_t S = srv->slots.size();
+    std::vector<int64_t> busy(S, -1);
+    std::vector<uint64_t> held(S, 0);
+    std::vector<int> reruns(S, 0);
+    uint64_t inflight = 0;
+    std::deque<uint32_t> waiting;
+    uint32_t next = 0, finished = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto now = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    while (finished < n) {
+        bool progressed = false;
+        // completions: a finished slot is read back and freed
+        for (size_t s = 0; s < S; ++s) {
+            if (busy[s] < 0 || hipEventQuery(srv->done[s]) != hipSuccess) continue;
+            const uint32_t i = (uint32_t)busy[s];
+            mcgpu_context c = srv->slots[s];
+            mcgpu_frontend_info info;
+            bool overflow = false;
+            mcgpu_result r = finish(c, &info, &overflow);
+            if (r == MCGPU_SUCCESS && overflow) { // a list outgrew its capacity: grow and run it again
+                if (++reruns[s] > 5) r = MCGPU_ERROR_OUT_OF_MEMORY;
+                else if ((r = alloc_list_buffers(c)) == MCGPU_SUCCESS) {
+                    slot_enqueue(c, srv->done[s]);
+                    continue;
+                }
+            }
+            out[i].done_s = now();
+            if (r != MCGPU_SUCCESS) {
+                out[i].status = (int32_t)r; // negative: an mcgpu_result
+            } else {
+                out[i].status = info.status;
+                out[i].num_registry = info.num_registry;
+                out[i].num_candidate_pairs = info.num_candidate_pairs;
+                if (keep_registries && info.num_registry) {
+                    const uint64_t m = info.num_registry;
+                    srv->reg_e[i].resize(m);
+                    srv->reg_g[i].resize(m);
+                    srv->reg_p[i].resize(3 * m);
+                    mcgpu_get_registry(c, srv->reg_e[i].data(), srv->reg_g[i].data(), srv->reg_p[i].data(), m);
+                }
+            }
+            inflight -= held[s];
+            held[s] = 0;
+            busy[s] = -1;
+            ++finished;
+            progressed = true;
+        }
+        // arrivals
+        const double t = now();
+        while (next < n && arrival_s[order[next]] <= t) waiting.push_back(order[next++]);
+        // admissions: oldest first, while a slot is free and the in-flight
+        // footprint stays within the budget (an idle server always admits)
+        while (!waiting.empty()) {
+            size_t s = 0;
+            while (s < S && busy[s] >= 0) ++s;
+            if (s == S) break;
+            const uint32_t i = waiting.front();
+            if (inflight > 0 && inflight + est[i] > srv->lim.hbm_budget) break;
+            waiting.pop_front();
+            out[i].start_s = now();
+            mcgpu_context c = srv->slots[s];
+            const mcgpu_result r = slot_bind(srv, c, &cuts[i], on_device);
+            if (r != MCGPU_SUCCESS) {
+                out[i].status = (int32_t)r;
+                out[i].done_s = now();
+                ++finished;
+                continue;
+            }
+            slot_enqueue(c, srv->done[s]);
+            reruns[s] = 0;
+            busy[s] = i;
+            held[s] = est[i];
+            inflight += est[i];
+            progressed = true;
+        }
+        if (!progressed) {
+            bool any = false;
+            for (size_t s = 0; s < S; ++s) any = any || busy[s] >= 0;
+            if (!any && waiting.empty() && next < n) {
+                const double wait = arrival_s[order[next]] - now();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
+            } else {
+                std::this_thread::sleep_for(std::chrono::microseconds(10));
+            }
+        }
+    }
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_server_get_registry(mcgpu_server srv, uint32_t request, uint32_t* edges, uint32_t* faces,
+                                                 double* points, uint64_t capacity)
+{
+    if (!srv) return MCGPU_ERROR_INVALID_VALUE;
+    if (request >= srv->reg_e.size()) return MCGPU_ERROR_NO_RESULT;
+    const uint64_t m = std::min<uint64_t>(srv->reg_e[request].size(), capacity);
+    if (edges) std::copy(srv->reg_e[request].begin(), srv->reg_e[request].begin() + m, edges);
+    if (faces) std::copy(srv->reg_g[request].begin(), srv->reg_g[request].begin() + m, faces);
+    if (points) std::copy(srv->reg_p[request].begin(), srv->reg_p[request].begin() + 3 * m, points);
+    return MCGPU_SUCCESS;
 }
 
 } // extern "C"

@@ -60,6 +60,16 @@ class Batch(C.Structure):
                 ("src_face_offsets", C.c_void_p), ("cut_vertex_offsets", C.c_void_p), ("cut_face_offsets", C.c_void_p)]
 
 
+class ServerLimits(C.Structure):
+    _fields_ = [("max_cut_vertices", C.c_uint32), ("max_cut_faces", C.c_uint32), ("max_cut_corners", C.c_uint32),
+                ("max_slots", C.c_uint32), ("hbm_budget", C.c_uint64)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double), ("status", C.c_int32),
+                ("reserved", C.c_uint32), ("num_registry", C.c_uint64), ("num_candidate_pairs", C.c_uint64)]
+
+
 _lib = None
 
 
@@ -88,6 +98,12 @@ def lib():
         L.mcgpu_dispatch_batch.argtypes = [vp, C.POINTER(Batch), C.POINTER(DispatchDesc), C.POINTER(FrontendInfo)]
         L.mcgpu_get_batch_status.argtypes = [vp, vp, vp, u64]
         L.mcgpu_get_batch_candidate_pairs.argtypes = [vp, vp, u64]
+        L.mcgpu_server_create.argtypes = [C.c_int, C.POINTER(Mesh), C.POINTER(DispatchDesc), C.POINTER(ServerLimits),
+                                          C.POINTER(vp)]
+        L.mcgpu_server_release.argtypes = [vp]
+        L.mcgpu_server_info.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(u64), C.POINTER(C.c_uint32)]
+        L.mcgpu_server_run.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp]
+        L.mcgpu_server_get_registry.argtypes = [vp, C.c_uint32, vp, vp, vp, u64]
         L.mcgpu_stage_times.argtypes = [vp, vp, C.c_int]
         L.mcgpu_stage_name.argtypes = [C.c_int]
         L.mcgpu_stage_name.restype = C.c_char_p
@@ -95,7 +111,8 @@ def lib():
                    "mcgpu_redispatch_perturbed", "mcgpu_get_registry", "mcgpu_get_candidate_pairs", "mcgpu_get_edges",
                    "mcgpu_get_tests", "mcgpu_get_face_planes", "mcgpu_get_ps_vertices", "mcgpu_registry_device",
                    "mcgpu_stage_times", "mcgpu_broad_phase", "mcgpu_dispatch_batch", "mcgpu_get_batch_status",
-                   "mcgpu_get_batch_candidate_pairs"):
+                   "mcgpu_get_batch_candidate_pairs", "mcgpu_server_create", "mcgpu_server_release",
+                   "mcgpu_server_info", "mcgpu_server_run", "mcgpu_server_get_registry"):
             getattr(L, fn).restype = C.c_int
         _lib = L
     return _lib
@@ -350,3 +367,50 @@ class Context:
         _check(lib().mcgpu_redispatch_perturbed(self.h, _ptr(p), enforcement_count, C.byref(info)),
                "mcgpu_redispatch_perturbed")
         return Result(self, info, nv)
+
+
+class Server:
+    """mcgpu_server: one resident source mesh, cut requests run in overlapping
+    slots as they arrive (C ABI in include/mcgpu.h)."""
+
+    def __init__(self, src, max_cut_vertices, max_cut_faces, slots=4, hbm_budget=8 << 30, device=0, flags=0,
+                 gp_constant=1e-4, max_cut_corners=0):
+        self._keep = []
+        ms = _host_mesh(src, self._keep)
+        d = DispatchDesc(flags, gp_constant, 0, 0, 0, 0)
+        lim = ServerLimits(max_cut_vertices, max_cut_faces, max_cut_corners, slots, hbm_budget)
+        h = C.c_void_p()
+        _check(lib().mcgpu_server_create(device, C.byref(ms), C.byref(d), C.byref(lim), C.byref(h)),
+               "mcgpu_server_create")
+        self.h = h
+        n, b, e = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        lib().mcgpu_server_info(self.h, C.byref(n), C.byref(b), C.byref(e))
+        self.slots, self.slot_bytes, self.source_edges = n.value, b.value, e.value
+
+    def run(self, cuts, arrival_s, keep_registries=False):
+        """Runs the schedule; returns a structured array of per-request results."""
+        keep = []
+        meshes = (Mesh * len(cuts))(*[_host_mesh(c, keep) for c in cuts])
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        out = (RequestResult * len(cuts))()
+        _check(lib().mcgpu_server_run(self.h, meshes, _ptr(arr), len(cuts), 1 if keep_registries else 0, out),
+               "mcgpu_server_run")
+        dt = np.dtype([("arrival_s", "f8"), ("start_s", "f8"), ("done_s", "f8"), ("status", "i4"),
+                       ("reserved", "u4"), ("num_registry", "u8"), ("num_candidate_pairs", "u8")])
+        return np.frombuffer(bytes(out), dtype=dt).copy()
+
+    def registry(self, i, n):
+        e, g, p = np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros((n, 3), np.float64)
+        _check(lib().mcgpu_server_get_registry(self.h, i, _ptr(e), _ptr(g), _ptr(p), n), "mcgpu_server_get_registry")
+        return e, g, p
+
+    def close(self):
+        if self.h:
+            lib().mcgpu_server_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

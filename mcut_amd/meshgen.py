@@ -353,6 +353,39 @@ def config4_pairs(num_pairs: int = 4096, seed: int = 5, amp: float = 0.02, tri_r
     return pairs
 
 
+def config5_source(n: int = 91, amp: float = 0.02, seed: int = 6):
+    """Served stream's resident source: Class-I geodesic sphere of frequency n
+    (n = 91: 165,620 triangles), radial noise U(-amp, amp)."""
+    V, F, _ = geodesic_sphere(n)
+    return radial_displace(V, amp, seed=seed, stream=7), F, None
+
+
+def crack_sheet(k: int, seed: int = 6, tri_range=(2_000, 50_000), side: float = 3.0, amp: float = 0.03):
+    """Request k of the served stream: an open m x m quad patch (two triangles
+    per quad), m = round(sqrt(T / 2)) with T ~ logU(tri_range), edge `side`,
+    normal uniform on the sphere, offset along the normal U(-0.6, 0.6), and
+    heights amp * fBm (4 octaves, centred) along the normal."""
+    u = u01(seed, 50_000 + k, 8)
+    T = math.exp(math.log(tri_range[0]) + u[0] * math.log(tri_range[1] / tri_range[0]))
+    m = max(1, int(round(math.sqrt(T / 2.0))))
+    z = 2.0 * u[1] - 1.0
+    ph = 2.0 * math.pi * u[2]
+    rz = math.sqrt(max(0.0, 1.0 - z * z))
+    nrm = np.array([rz * math.cos(ph), rz * math.sin(ph), z])
+    a = np.array([1.0, 0.0, 0.0]) if abs(nrm[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    e1 = np.cross(nrm, a)
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(nrm, e1)
+    rot = 2.0 * math.pi * u[3]
+    e1, e2 = math.cos(rot) * e1 + math.sin(rot) * e2, -math.sin(rot) * e1 + math.cos(rot) * e2
+    off = -0.6 + 1.2 * u[4]
+    G, F, _ = grid_patch(m, side)
+    h = value_noise_fbm(G[:, 0] / side + 0.5, G[:, 1] / side + 0.5, 4, seed * 1000 + k)
+    h = amp * (h - 0.9375) / 0.9375
+    V = off * nrm[None, :] + G[:, 0:1] * e1[None, :] + G[:, 1:2] * e2[None, :] + h[:, None] * nrm[None, :]
+    return np.ascontiguousarray(V), F, None
+
+
 def read_off(path: str):
     """Read an OFF file; returns (V, F, S) with S None when all faces are triangles."""
     with open(path, "r") as fh:

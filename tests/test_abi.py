@@ -45,6 +45,11 @@ def _c_layout(tmp_path):
         "mcgpu_frontend_info": ["status", "gp_violation", "num_candidate_pairs", "num_candidate_faces", "num_edges",
                                 "num_tests", "num_escalated", "num_registry", "num_raw_pairs", "com",
                                 "translation", "eps", "device_ms", "reserved"],
+        "mcgpu_batch": ["num_pairs", "src", "cut", "src_vertex_offsets", "src_face_offsets", "cut_vertex_offsets",
+                        "cut_face_offsets"],
+        "mcgpu_server_limits": ["max_cut_vertices", "max_cut_faces", "max_cut_corners", "max_slots", "hbm_budget"],
+        "mcgpu_request_result": ["arrival_s", "start_s", "done_s", "status", "reserved", "num_registry",
+                                 "num_candidate_pairs"],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mcgpu.h"', "int main(void) {"]
     for s, fs in fields.items():
@@ -68,7 +73,8 @@ def _c_layout(tmp_path):
 def test_struct_layouts_match_the_header(tmp_path):
     from mcut_amd import frontend
     py = {"mcgpu_mesh": frontend.Mesh, "mcgpu_dispatch_desc": frontend.DispatchDesc,
-          "mcgpu_frontend_info": frontend.FrontendInfo}
+          "mcgpu_frontend_info": frontend.FrontendInfo, "mcgpu_batch": frontend.Batch,
+          "mcgpu_server_limits": frontend.ServerLimits, "mcgpu_request_result": frontend.RequestResult}
     fields, res = _c_layout(tmp_path)
     for s, fs in fields.items():
         T = py[s]
