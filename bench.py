@@ -13,7 +13,9 @@ of the same front end (oracle/) timed on this host on the same input.
 
 The line also carries the batched mix ("batched"): 4096 seeded sphere/box
 pairs of uneven size, one batched dispatch against one plain dispatch per
-pair, both over inputs resident in HBM.
+pair, both over inputs resident in HBM; and the served stream ("served"): a
+resident source mesh cut by crack-sheet requests arriving as a Poisson
+process, p50/p99 latency and throughput at each rate.
 """
 import argparse
 import json
@@ -169,6 +171,46 @@ def bench_batch(args, torch, frontend, mg):
             "generate_s": gen_s}
 
 
+def bench_serve(args, frontend, mg):
+    """Served stream: one resident displaced geodesic sphere (n = 91, 165,620
+    triangles) cut by crack-sheet requests (2k-50k triangles, a pool of 256
+    seeded sheets cycled) arriving as a Poisson process, swept over rates.
+    Latency = completion - arrival on the host clock; every repeat of a pool
+    sheet must report the same registry count."""
+    src = mg.config5_source(91)
+    pool = [mg.crack_sheet(k, seed=6) for k in range(args.serve_pool)]
+    mv = max(c[0].shape[0] for c in pool)
+    mf = max(c[1].size // 3 for c in pool)
+    srv = frontend.Server(src, mv, mf, slots=args.serve_slots, hbm_budget=args.serve_budget_gb << 30)
+    progress(f"serving: {srv.slots} slots of {srv.slot_bytes / 2**20:.0f} MiB, pool of {len(pool)} sheets")
+    # service time of each pool sheet alone (one in flight)
+    res = srv.run(pool, np.arange(len(pool)) * 0.0 + np.arange(len(pool)) * 1e-9, keep_registries=False)
+    ref_counts = res["num_registry"].astype(np.int64)
+    rng = np.random.default_rng(6)
+    rates = []
+    for lam in args.serve_rates:
+        n = int(min(args.serve_max, max(50, lam * args.serve_seconds)))
+        arr = np.cumsum(rng.exponential(1.0 / lam, n))
+        cuts = [pool[i % len(pool)] for i in range(n)]
+        r = srv.run(cuts, arr)
+        lat = (r["done_s"] - r["arrival_s"]) * 1e3
+        span = r["done_s"].max() - r["arrival_s"].min()
+        counts = r["num_registry"].astype(np.int64)
+        assert np.all(r["status"] >= 0) and np.array_equal(counts, ref_counts[np.arange(n) % len(pool)]), \
+            "a request's result changed in the mix"
+        rates.append({"rate_rps": lam, "requests": n, "p50_ms": float(np.percentile(lat, 50)),
+                      "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                      "throughput_rps": n / span, "pairs_per_s": float(counts.sum()) / span,
+                      "queue_wait_p50_ms": float(np.percentile((r["start_s"] - r["arrival_s"]) * 1e3, 50))})
+        progress(f"serving: {lam} req/s -> p50 {rates[-1]['p50_ms']:.2f} ms, p99 {rates[-1]['p99_ms']:.2f} ms")
+    srv.close()
+    return {"workload": "resident geodesic sphere n=91 (165,620 tris, +-2% radial noise) cut by open crack sheets "
+                        "(logU 2k-50k tris, side 3, random orientation, offset U(-0.6,0.6), 0.03 fBm), "
+                        f"Poisson arrivals, seed 6, {len(pool)} distinct sheets cycled",
+            "slots": srv.slots, "slot_mib": srv.slot_bytes / 2**20, "hbm_budget_gib": args.serve_budget_gb,
+            "alone_registry_mean": float(ref_counts.mean()), "rates": rates}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +221,13 @@ def main():
     ap.add_argument("--batch-pairs", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[10, 30, 100, 300, 1000, 3000])
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-max", type=int, default=1000)
+    ap.add_argument("--serve-pool", type=int, default=256)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (use --gpus 1)", file=sys.stderr)
@@ -273,6 +322,8 @@ def main():
     del keep
     if not args.no_batch:
         line["batched"] = bench_batch(args, torch, frontend, mg)
+    if not args.no_serve:
+        line["served"] = bench_serve(args, frontend, mg)
     print(json.dumps(line), flush=True)
 
 
