@@ -15,7 +15,9 @@ The line also carries the batched mix ("batched"): 4096 seeded sphere/box
 pairs of uneven size, one batched dispatch against one plain dispatch per
 pair, both over inputs resident in HBM; and the served stream ("served"): a
 resident source mesh cut by crack-sheet requests arriving as a Poisson
-process, p50/p99 latency and throughput at each rate.
+process, p50/p99 latency and throughput at each rate; and the precision stage
+("precision"): the degenerate-heavy heightfield sectioning and the predicate
+stage alone with its per-stage decision rates.
 """
 import argparse
 import json
@@ -211,6 +213,41 @@ def bench_serve(args, frontend, mg):
             "alone_registry_mean": float(ref_counts.mean()), "rates": rates}
 
 
+def bench_precision(args, frontend, mg):
+    """Degenerate-heavy sectioning: the 1024 x 1024 heightfield cut at
+    z = 410/4096 (status as the reference reports it), its x/y-scaled variant
+    (faces 1 x 1, valid planes, many vertices exactly on the cut plane), and
+    the predicate stage alone over both: 2 T orient3d signs through the FP32
+    pre-filter, the FP64 filter and exact arithmetic."""
+    ctx = frontend.Context(0)
+    out = {"bounds": {"f32": "16 * 2^-24 * permanent (9.54e-7 * permanent), coordinate differences in [2^-40, 2^40]",
+                      "f64": "(7 + 56 eps) eps * permanent, eps = 2^-53 (7.77e-16 * permanent; shewchuk.c:424)"}}
+    for name, scale in (("heightfield", 1.0), ("heightfield_scaled", 1024.0)):
+        src, cut = mg.config3(scale=scale)
+        R = ctx.dispatch(src, cut)
+        dev = []
+        for _ in range(3):
+            R = ctx.dispatch(src, cut)
+            dev.append(R.device_ms)
+        c = dict(R.counts)
+        T = c["tests"]
+        P = R.predicate_stage()
+        P2 = R.predicate_stage()
+        P["device_ms"] = min(P["device_ms"], P2["device_ms"])
+        out[name] = {"workload": f"1024 x 1024 heightfield (fBm heights in [0, 0.2] on a 2^-12 grid, seed 4), "
+                                 f"x/y scale {scale:g}, cut by a 64 x 64 quad plane at z = 410/4096",
+                     "status": R.status_name, "frontend_device_ms": min(dev), "counts": c,
+                     "narrow_stage_rates": {"f32": c["decided_f32"] / max(1, T), "f64": c["decided_f64"] / max(1, T),
+                                            "exact": c["escalated"] / max(1, T)},
+                     "predicate_stage": {**P, "predicates": 2 * T,
+                                         "predicates_per_s": 2 * T / (P["device_ms"] * 1e-3) if P["device_ms"] else None,
+                                         "escalation_rate": P["exact"] / max(1, 2 * T),
+                                         "f32_rate": P["f32"] / max(1, 2 * T)}}
+        progress(f"precision: {name} {R.status_name}, {T} tests, predicate stage {P['device_ms']:.3f} ms")
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +259,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[10, 30, 100, 300, 1000, 3000])
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-max", type=int, default=1000)
@@ -324,6 +362,8 @@ def main():
         line["batched"] = bench_batch(args, torch, frontend, mg)
     if not args.no_serve:
         line["served"] = bench_serve(args, frontend, mg)
+    if not args.no_precision:
+        line["precision"] = bench_precision(args, frontend, mg)
     print(json.dumps(line), flush=True)
 
 

@@ -102,6 +102,8 @@ typedef struct {
     uint64_t num_escalated;      /* tests whose orient3d signs needed exact arithmetic */
     uint64_t num_registry;       /* intersecting (edge, face) pairs */
     uint64_t num_raw_pairs;      /* pairs of the raw-coordinate broad phase before the exact shifted-box test */
+    uint64_t num_decided_f32;    /* tests whose two orient3d signs the FP32 pre-filter settled */
+    uint64_t num_decided_f64;    /* tests settled by the FP64 semi-static filter (num_escalated: by exact arithmetic) */
     double com[3];               /* shift: x' = (x - com) + translation */
     double translation[3];
     double eps;                  /* cut-face AABB enlargement */
@@ -267,6 +269,19 @@ MCGPU_API mcgpu_result mcgpu_registry_device(mcgpu_context ctx, const uint32_t**
                                              const double** points, uint64_t* count);
 
 /* --- measurement -------------------------------------------------------------- */
+
+/* The predicate stage alone, over the edge-face tests of the last
+ * mcgpu_dispatch_frontend: both orient3d signs of every test (its edge's
+ * endpoints against its face's plane triplet, or the face's first three
+ * vertices where the plane has none) through the staged filter -- the FP32
+ * pre-filter (error bound 16 * 2^-24 * permanent), the FP64 semi-static
+ * filter ((7 + 56 eps) eps * permanent, shewchuk.c:424) and exact expansion
+ * arithmetic.  counts[6] = decided in FP32, in FP64, exactly; signs negative,
+ * zero, positive (all over 2 * num_tests predicates).  qsign / rsign
+ * (optional) get the signs in mcgpu_get_tests order.  device_ms: the
+ * kernel's time (this call replaces the stage times of the dispatch). */
+MCGPU_API mcgpu_result mcgpu_predicate_stage(mcgpu_context ctx, uint64_t* counts, int8_t* qsign, int8_t* rsign,
+                                             uint64_t capacity, float* device_ms);
 
 /* Per-stage device times of the last dispatch in ms (stage names via
  * mcgpu_stage_name); returns the number of stages written. */

@@ -1611,10 +1611,11 @@ __global__ void __launch_bounds__(kStageBlock) k_narrow(Soup S, const Plane* __r
     __shared__ uint32_t st_e[kNarrowTile], st_g[kNarrowTile];
     __shared__ double st_p[3 * kNarrowTile];
     __shared__ uint32_t st_esc[kNarrowTile];
-    __shared__ uint32_t st_nr, st_ne;
+    __shared__ uint32_t st_nr, st_ne, st_n32, st_n64;
     __shared__ unsigned long long st_base;
     const uint64_t n = C->n_tests < cap_tests ? C->n_tests : cap_tests;
     const uint32_t Es = (uint32_t)C->n_edges_src;
+    if (threadIdx.x == 0) st_n32 = st_n64 = 0;
     for (uint64_t t0 = (uint64_t)blockIdx.x * kNarrowTile; t0 < n; t0 += (uint64_t)gridDim.x * kNarrowTile) {
         if (threadIdx.x == 0) st_nr = st_ne = 0;
         __syncthreads();
@@ -1640,12 +1641,16 @@ __global__ void __launch_bounds__(kStageBlock) k_narrow(Soup S, const Plane* __r
             fv(pl.tri[2], c);
             load3(S.ps_v + 3 * (uint64_t)S.ev0[e], q);
             load3(S.ps_v + 3 * (uint64_t)S.ev1[e], r);
-            const int sq = orient3d_filter(a, b, c, q);
-            const int sr = orient3d_filter(a, b, c, r);
+            int stq, str_;
+            const int sq = orient3d_staged(a, b, c, q, stq);
+            const int sr = orient3d_staged(a, b, c, r, str_);
             if (sq == 0 || sr == 0) {
                 st_esc[wave_reserve(&st_ne)] = (uint32_t)i;
                 continue;
             }
+            // the test's stage: the later of its two signs' stages
+            if ((stq | str_) == 0) wave_reserve(&st_n32);
+            else wave_reserve(&st_n64);
             double p[3];
             if (resolve_test(S, planes, e, g, sq, sr, i, TO, C, p)) {
                 const uint32_t at = wave_reserve(&st_nr);
@@ -1669,6 +1674,8 @@ __global__ void __launch_bounds__(kStageBlock) k_narrow(Soup S, const Plane* __r
         }
         __syncthreads();
     }
+    if (threadIdx.x == 0 && st_n32) atomicAdd(&C->n_f32, (unsigned long long)st_n32);
+    if (threadIdx.x == 0 && st_n64) atomicAdd(&C->n_f64, (unsigned long long)st_n64);
 }
 
 // tests whose filter was inconclusive: exact signs, then the same decision
@@ -1688,14 +1695,62 @@ __global__ void __launch_bounds__(64) k_narrow_exact(Soup S, const Plane* __rest
         fv(pl.tri[2], c);
         load3(S.ps_v + 3 * (uint64_t)S.ev0[e], q);
         load3(S.ps_v + 3 * (uint64_t)S.ev1[e], r);
-        int sq = orient3d_filter(a, b, c, q);
-        int sr = orient3d_filter(a, b, c, r);
+        int stq, str_;
+        int sq = orient3d_staged(a, b, c, q, stq);
+        int sr = orient3d_staged(a, b, c, r, str_);
         if (sq == 0) sq = orient3d_exact_sign(a, b, c, q);
         if (sr == 0) sr = orient3d_exact_sign(a, b, c, r);
         double p[3];
         if (resolve_test(S, planes, e, g, sq, sr, i, TO, C, p))
             store_registry(R, wave_append(&C->n_reg), reg_key(R, S, e, g, (uint32_t)C->n_edges_src), e, g, p, C);
     }
+}
+
+// The predicate stage alone: both orient3d signs of every edge-face test
+// through the staged filter (FP32, FP64, exact), against the face's plane
+// triplet, or its first three vertices where the plane has none (a
+// degenerate or collinear face, which the narrow phase does not test).
+// counts: [decided in FP32, in FP64, exactly, negative, zero, positive] over
+// the 2 T predicates; qs / rs (optional) receive the signs per test.
+__global__ void __launch_bounds__(256) k_predicates(Soup S, const Plane* __restrict__ planes, const uint32_t* __restrict__ te,
+                                                    const uint32_t* __restrict__ tg, uint64_t cap_tests,
+                                                    const Counters* __restrict__ C, int8_t* __restrict__ qs,
+                                                    int8_t* __restrict__ rs, unsigned long long* __restrict__ counts)
+{
+    __shared__ unsigned int cnt[6];
+    if (threadIdx.x < 6) cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    unsigned int mine[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    const uint64_t n = C->n_tests < cap_tests ? C->n_tests : cap_tests;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = te[i], g = tg[i];
+        const Plane pl = planes[g];
+        const bool tri_ok = pl.k >= 0 && pl.tri[0] != 0xff;
+        const FaceView fv = face_view(S, g);
+        double a[3], b[3], c[3], x[2][3];
+        fv(tri_ok ? pl.tri[0] : 0, a);
+        fv(tri_ok ? pl.tri[1] : 1, b);
+        fv(tri_ok ? pl.tri[2] : 2, c);
+        load3(S.ps_v + 3 * (uint64_t)S.ev0[e], x[0]);
+        load3(S.ps_v + 3 * (uint64_t)S.ev1[e], x[1]);
+        int sg[2];
+        for (int k = 0; k < 2; ++k) {
+            int stage;
+            int sgn = orient3d_staged(a, b, c, x[k], stage);
+            if (stage == 2) sgn = orient3d_exact_sign(a, b, c, x[k]);
+            ++mine[stage];
+            ++mine[4 + sgn];
+            sg[k] = sgn;
+        }
+        if (qs) {
+            qs[i] = (int8_t)sg[0];
+            rs[i] = (int8_t)sg[1];
+        }
+    }
+    for (int k = 0; k < 6; ++k)
+        if (mine[k]) atomicAdd(&cnt[k], mine[k]);
+    __syncthreads();
+    if (threadIdx.x < 6 && cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
 }
 
 // ===========================================================================

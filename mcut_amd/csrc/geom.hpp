@@ -199,6 +199,64 @@ __device__ __forceinline__ int orient3d_filter(const double* pa, const double* p
     return 0;
 }
 
+// orient3d pre-filter in single precision, ahead of the FP64 filter.  The
+// nine coordinate differences are formed in double, as the FP64 filter forms
+// them, and rounded to float: each is within 2^-24 (1 + 2^-29) of the exact
+// difference, one float rounding plus the double one.  The determinant and
+// its permanent are then evaluated in float in the order of orient3dfast.
+// Shewchuk's stage-A analysis bounds the error of that evaluation by
+// (7 + 56 eps) eps * permanent for entries each within eps of exact; the
+// bound used here is 16 eps * permanent with eps = 2^-24, which covers the
+// slightly larger entry error with a factor of two to spare.  The analysis is
+// relative, so it needs every product to be a normal float: each difference
+// must be 0 or have magnitude in [2^-40, 2^40] (products of three then lie in
+// [2^-120, 2^120]; a product difference that cancels to a subnormal has an
+// absolute error of at most a few 2^-149, below the bound whenever the
+// permanent is nonzero).  Outside that range, or for a NaN, the pre-filter
+// declines.  Returns +1/-1 when the sign is certain, 0 when it is not.
+constexpr float kO3dErrBoundF32 = 9.5367431640625e-07f; // 16 * 2^-24
+
+__device__ __forceinline__ bool f32_in_range(float x)
+{
+    const float a = fabsf(x);
+    return a == 0.0f || (a >= 9.094947017729282e-13f && a <= 1.099511627776e12f); // [2^-40, 2^40]
+}
+
+__device__ __forceinline__ int orient3d_filter_f32(const double* pa, const double* pb, const double* pc, const double* pd)
+{
+    const float adx = (float)(pa[0] - pd[0]), bdx = (float)(pb[0] - pd[0]), cdx = (float)(pc[0] - pd[0]);
+    const float ady = (float)(pa[1] - pd[1]), bdy = (float)(pb[1] - pd[1]), cdy = (float)(pc[1] - pd[1]);
+    const float adz = (float)(pa[2] - pd[2]), bdz = (float)(pb[2] - pd[2]), cdz = (float)(pc[2] - pd[2]);
+    if (!(f32_in_range(adx) && f32_in_range(bdx) && f32_in_range(cdx) && f32_in_range(ady) && f32_in_range(bdy) &&
+          f32_in_range(cdy) && f32_in_range(adz) && f32_in_range(bdz) && f32_in_range(cdz)))
+        return 0; // NaN fails the range test too
+    const float bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    const float cdxady = cdx * ady, adxcdy = adx * cdy;
+    const float adxbdy = adx * bdy, bdxady = bdx * ady;
+    const float det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+    const float perm = (fabsf(bdxcdy) + fabsf(cdxbdy)) * fabsf(adz) + (fabsf(cdxady) + fabsf(adxcdy)) * fabsf(bdz) +
+                       (fabsf(adxbdy) + fabsf(bdxady)) * fabsf(cdz);
+    const float eb = kO3dErrBoundF32 * perm;
+    if (det > eb) return 1;
+    if (-det > eb) return -1;
+    return 0;
+}
+
+// the staged sign: the FP32 pre-filter, then the FP64 filter; 0 = undecided
+// (escalate).  stage receives 0 (FP32 decided), 1 (FP64 decided) or 2.
+__device__ __forceinline__ int orient3d_staged(const double* pa, const double* pb, const double* pc, const double* pd,
+                                               int& stage)
+{
+    int s = orient3d_filter_f32(pa, pb, pc, pd);
+    if (s != 0) {
+        stage = 0;
+        return s;
+    }
+    s = orient3d_filter(pa, pb, pc, pd);
+    stage = s != 0 ? 1 : 2;
+    return s;
+}
+
 // --- exact orient3d sign: expansions of the coordinate differences ----------
 
 __device__ inline int grow_zeroelim(int n, const double* e, double b, double* h)

@@ -34,6 +34,7 @@ struct Counters {
     unsigned long long n_edges_src, n_edges_cut;
     unsigned long long n_raw;                              // raw-space (superset) candidate pairs
     unsigned long long n_fallback, n_lane_walks; // traversal statistics: node splits, per-lane walks
+    unsigned long long n_f32, n_f64;            // tests whose two signs the FP32 / FP64 filter settled
     unsigned long long pmask; // pair hash slots - 1 (set once the raw pairs are known)
     double zero;              // +0.0 (the per-dispatch reset clears it): padding the serial sums load
     unsigned int ov_pairs, ov_tests, ov_esc, ov_reg, ov_big;

@@ -112,6 +112,7 @@ struct mcgpu_context_t {
     Buf te, tg, tqs, trs, ttype, tpip, esc;
     Buf rkey, rval, rkey2, rval2, rp, re, rg, rpts, rcount, re_in, rg_in;
     Buf shkeys, shcnt;
+    Buf pqs, prs, pcounts; // predicate stage: per-test signs, stage counts
     Buf scan_tmp, scan_tmp_c, hist;
 
     uint64_t cap_pairs = 0, cap_tests = 0, cap_esc = 0, cap_reg = 0, cap_big = 0;
@@ -203,6 +204,7 @@ __global__ void k_reset_narrow(Counters* C)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         C->n_tests = C->n_esc = C->n_reg = C->n_big = 0; // candidate faces are kept
+        C->n_f32 = C->n_f64 = 0;
         C->ov_tests = C->ov_esc = C->ov_reg = C->ov_big = 0;
         C->gp = C->bad_src = C->bad_cut = C->unsupported = C->pair_fail = C->partial = 0;
     }
@@ -806,6 +808,8 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         info->num_escalated = C.n_esc;
         info->num_registry = C.n_reg;
         info->num_raw_pairs = C.n_raw;
+        info->num_decided_f32 = C.n_f32;
+        info->num_decided_f64 = C.n_f64;
         for (int j = 0; j < 3; ++j) {
             info->com[j] = ctx->P.com[j];
             info->translation[j] = ctx->P.t[j];
@@ -969,7 +973,7 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
                   &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->scan_tmp_c, &ctx->hist,
                   &ctx->re_in, &ctx->rg_in, &ctx->bpairs, &ctx->bstarts, &ctx->fpair, &ctx->vpair, &ctx->bparams,
                   &ctx->brbox, &ctx->pflags, &ctx->border, &ctx->bidx[0], &ctx->bidx[1], &ctx->rle, &ctx->rlg,
-                  &ctx->rstart, &ctx->rend};
+                  &ctx->rstart, &ctx->rend, &ctx->pqs, &ctx->prs, &ctx->pcounts};
     for (Buf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
@@ -1277,6 +1281,60 @@ MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const doubl
         // a list overflowed: the pairs are intact, so only the narrow part reruns
     }
     return MCGPU_ERROR_OUT_OF_MEMORY;
+}
+
+MCGPU_API mcgpu_result mcgpu_predicate_stage(mcgpu_context ctx, uint64_t* counts, int8_t* qsign, int8_t* rsign,
+                                             uint64_t capacity, float* device_ms)
+{
+    if (!ctx || !counts) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have || ctx->broad_only || ctx->batch) return MCGPU_ERROR_NO_RESULT;
+    CK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->s;
+    const uint64_t n = ctx->C.n_tests;
+    int8_t* q = nullptr;
+    int8_t* r = nullptr;
+    unsigned long long* cnt;
+    CK(ensure(ctx->pcounts, 6, &cnt));
+    const bool signs = qsign || rsign;
+    if (signs) {
+        CK(ensure(ctx->pqs, n, &q));
+        CK(ensure(ctx->prs, n, &r));
+    }
+    fill_u64(cnt, 0ull, 6, s);
+    (void)hipEventRecord(ctx->ev[EV_A0], s);
+    k_predicates<<<grid_for(n, 256, 8192), 256, 0, s>>>(make_soup(ctx), P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
+                                                        P_<uint32_t>(ctx->tg), ctx->cap_tests,
+                                                        &P_<DevState>(ctx->state)->C, q, r, cnt);
+    (void)hipEventRecord(ctx->ev[EV_A_END], s);
+    CK(hipStreamSynchronize(s));
+    CK(hipGetLastError());
+    unsigned long long h[6];
+    CK(hipMemcpy(h, cnt, sizeof(h), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 6; ++k) counts[k] = h[k];
+    if (device_ms) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[EV_A0], ctx->ev[EV_A_END]) != hipSuccess) ms = 0.f;
+        *device_ms = ms;
+    }
+    ctx->stages_ready = false; // the events now time this kernel; stage times are gone
+    if (signs && n) { // in the order of mcgpu_get_tests: sorted by (edge, face)
+        std::vector<uint32_t> e(n), g(n);
+        std::vector<int8_t> hq(n), hr(n);
+        CK(hipMemcpy(e.data(), ctx->te.p, 4 * n, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(g.data(), ctx->tg.p, 4 * n, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(hq.data(), q, n, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(hr.data(), r, n, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> perm(n);
+        for (uint64_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+        std::sort(perm.begin(), perm.end(),
+                  [&](uint32_t a, uint32_t b) { return ((uint64_t)e[a] << 32 | g[a]) < ((uint64_t)e[b] << 32 | g[b]); });
+        const uint64_t m = std::min<uint64_t>(n, capacity);
+        for (uint64_t i = 0; i < m; ++i) {
+            if (qsign) qsign[i] = hq[perm[i]];
+            if (rsign) rsign[i] = hr[perm[i]];
+        }
+    }
+    return MCGPU_SUCCESS;
 }
 
 MCGPU_API mcgpu_result mcgpu_get_registry(mcgpu_context ctx, uint32_t* edges, uint32_t* faces, double* points,

@@ -50,6 +50,7 @@ class FrontendInfo(C.Structure):
     _fields_ = [("status", C.c_int32), ("gp_violation", C.c_uint32), ("num_candidate_pairs", C.c_uint64),
                 ("num_candidate_faces", C.c_uint64), ("num_edges", C.c_uint64), ("num_tests", C.c_uint64),
                 ("num_escalated", C.c_uint64), ("num_registry", C.c_uint64), ("num_raw_pairs", C.c_uint64),
+                ("num_decided_f32", C.c_uint64), ("num_decided_f64", C.c_uint64),
                 ("com", C.c_double * 3),
                 ("translation", C.c_double * 3), ("eps", C.c_double), ("device_ms", C.c_float),
                 ("reserved", C.c_float)]
@@ -104,6 +105,7 @@ def lib():
         L.mcgpu_server_info.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(u64), C.POINTER(C.c_uint32)]
         L.mcgpu_server_run.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp]
         L.mcgpu_server_get_registry.argtypes = [vp, C.c_uint32, vp, vp, vp, u64]
+        L.mcgpu_predicate_stage.argtypes = [vp, vp, vp, vp, u64, C.POINTER(C.c_float)]
         L.mcgpu_stage_times.argtypes = [vp, vp, C.c_int]
         L.mcgpu_stage_name.argtypes = [C.c_int]
         L.mcgpu_stage_name.restype = C.c_char_p
@@ -112,7 +114,7 @@ def lib():
                    "mcgpu_get_tests", "mcgpu_get_face_planes", "mcgpu_get_ps_vertices", "mcgpu_registry_device",
                    "mcgpu_stage_times", "mcgpu_broad_phase", "mcgpu_dispatch_batch", "mcgpu_get_batch_status",
                    "mcgpu_get_batch_candidate_pairs", "mcgpu_server_create", "mcgpu_server_release",
-                   "mcgpu_server_info", "mcgpu_server_run", "mcgpu_server_get_registry"):
+                   "mcgpu_server_info", "mcgpu_server_run", "mcgpu_server_get_registry", "mcgpu_predicate_stage"):
             getattr(L, fn).restype = C.c_int
         _lib = L
     return _lib
@@ -152,7 +154,8 @@ class Result:
         self.status_name = STATUS.get(info.status, str(info.status))
         self.counts = dict(pairs=info.num_candidate_pairs, cand=info.num_candidate_faces, edges=info.num_edges,
                            tests=info.num_tests, escalated=info.num_escalated, registry=info.num_registry,
-                           raw_pairs=info.num_raw_pairs)
+                           raw_pairs=info.num_raw_pairs, decided_f32=info.num_decided_f32,
+                           decided_f64=info.num_decided_f64)
         self.com = np.array(info.com[:])
         self.t = np.array(info.translation[:])
         self.eps = info.eps
@@ -201,6 +204,23 @@ class Result:
     def ps_vertices(self):
         out = np.zeros((self._nv, 3))
         _check(lib().mcgpu_get_ps_vertices(self._ctx.h, _ptr(out), self._nv), "mcgpu_get_ps_vertices")
+        return out
+
+    def predicate_stage(self, signs=False):
+        """Both orient3d signs of every test through the staged filter, alone:
+        counts per stage and per sign, the kernel time, and (signs=True) the
+        signs in tests() order."""
+        cnt = np.zeros(6, np.uint64)
+        n = self.info.num_tests
+        qs = np.zeros(n, np.int8) if signs else None
+        rs = np.zeros(n, np.int8) if signs else None
+        ms = C.c_float(0)
+        _check(lib().mcgpu_predicate_stage(self._ctx.h, _ptr(cnt), _ptr(qs), _ptr(rs), n, C.byref(ms)),
+               "mcgpu_predicate_stage")
+        out = dict(f32=int(cnt[0]), f64=int(cnt[1]), exact=int(cnt[2]), negative=int(cnt[3]), zero=int(cnt[4]),
+                   positive=int(cnt[5]), device_ms=ms.value)
+        if signs:
+            out["qsign"], out["rsign"] = qs, rs
         return out
 
     def stage_ms(self):
@@ -257,7 +277,8 @@ class BatchResult:
         self.num_pairs = num_pairs
         self.counts = dict(pairs=info.num_candidate_pairs, cand=info.num_candidate_faces, edges=info.num_edges,
                            tests=info.num_tests, escalated=info.num_escalated, registry=info.num_registry,
-                           raw_pairs=info.num_raw_pairs)
+                           raw_pairs=info.num_raw_pairs, decided_f32=info.num_decided_f32,
+                           decided_f64=info.num_decided_f64)
         self.device_ms = info.device_ms
         self.status = np.zeros(num_pairs, np.int32)
         self.reg_offsets = np.zeros(num_pairs + 1, np.uint64)

@@ -353,6 +353,31 @@ def config4_pairs(num_pairs: int = 4096, seed: int = 5, amp: float = 0.02, tri_r
     return pairs
 
 
+def config3(cells: int = 1024, scale: float = 1.0, seed: int = 4, cut_quads: int = 64):
+    """Degenerate-heavy sectioning.  Source: heightfield over a cells x cells
+    grid on [0, scale]^2 (two triangles per cell, normals up), heights =
+    6-octave fBm (value noise, lacunarity 2, gain 0.5, seed) rescaled to
+    [0, 0.2] and rounded to multiples of 2^-12.  Cut: a planar cut_quads x
+    cut_quads quad grid, triangulated, at z = 410/4096 (the representable
+    height nearest 0.1, which many heightfield vertices have exactly) over
+    [-0.005, 1.005]^2 * scale.  scale = cells makes every source cell 1 x 1,
+    which keeps the source's face normals above the reference's 1e-9
+    threshold (at scale 1 they are about cells^-4)."""
+    i = np.arange(cells + 1, dtype=np.float64)
+    X, Y = np.meshgrid(i / cells, i / cells, indexing="ij")
+    h = value_noise_fbm(X.ravel(), Y.ravel(), 6, seed)
+    h = (h - h.min()) / (h.max() - h.min()) * 0.2
+    h = np.round(h * 4096.0) / 4096.0
+    V = np.stack([X.ravel() * scale, Y.ravel() * scale, h], axis=1)
+    a_i, a_j = np.meshgrid(np.arange(cells), np.arange(cells), indexing="ij")
+    a = (a_i * (cells + 1) + a_j).ravel()
+    b = a + (cells + 1)
+    F = np.stack([np.stack([a, b, b + 1], 1), np.stack([a, b + 1, a + 1], 1)], 1).reshape(-1)
+    G, Fc, _ = grid_patch(cut_quads, 1.01 * scale)
+    Vc = G + np.array([0.5 * scale, 0.5 * scale, 410.0 / 4096.0])
+    return (V, F.astype(np.uint32), None), (np.ascontiguousarray(Vc), Fc, None)
+
+
 def config5_source(n: int = 91, amp: float = 0.02, seed: int = 6):
     """Served stream's resident source: Class-I geodesic sphere of frequency n
     (n = 91: 165,620 triangles), radial noise U(-amp, amp)."""

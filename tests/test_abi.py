@@ -43,7 +43,8 @@ def _c_layout(tmp_path):
         "mcgpu_dispatch_desc": ["flags", "gp_constant", "enforcement_count", "input_on_device", "keep_tests",
                                 "reserved"],
         "mcgpu_frontend_info": ["status", "gp_violation", "num_candidate_pairs", "num_candidate_faces", "num_edges",
-                                "num_tests", "num_escalated", "num_registry", "num_raw_pairs", "com",
+                                "num_tests", "num_escalated", "num_registry", "num_raw_pairs", "num_decided_f32",
+                                "num_decided_f64", "com",
                                 "translation", "eps", "device_ms", "reserved"],
         "mcgpu_batch": ["num_pairs", "src", "cut", "src_vertex_offsets", "src_face_offsets", "cut_vertex_offsets",
                         "cut_face_offsets"],
