@@ -248,6 +248,43 @@ def bench_precision(args, frontend, mg):
     return out
 
 
+def bench_mcdispatch(args, frontend, mg):
+    """mcDispatch end to end (input meshes to connected components ready):
+    MCUT built from its own sources, unchanged, against the same MCUT with its
+    broad phase on the GPU (the bvh.cpp seam), on a pair MCUT completes:
+    icosphere-5 vs a rotated icosphere-5 (20,480 triangles each, +-0.5%
+    radial noise, offset +0.5 x).  The GPU front end alone on the same pair
+    is timed beside them."""
+    from mcut_amd.seam import mcut_api
+    ref_lib = os.path.join(ROOT, "oracle", "_ref", "libmcut_ref.so")
+    if not (os.path.exists(ref_lib) and os.path.exists(mcut_api.SEAM_LIB)):
+        return {"skipped": "MCUT builds not present"}
+    V, F, _ = mg.icosphere(args.mcd_level)
+    A = mg.radial_displace(V, 0.005, seed=1)
+    B = mg.radial_displace(V, 0.005, seed=2) @ mg.random_rotation(seed=3).T
+    B[:, 0] += 0.5
+    src, cut = (A, F, None), (np.ascontiguousarray(B), F.copy(), None)
+    helpers = max(1, min(os.cpu_count() or 1, 16) - 1)
+    ref, seam = mcut_api.Mcut(ref_lib), mcut_api.Mcut(mcut_api.SEAM_LIB)
+    out = {"workload": f"icosphere-{args.mcd_level} pair ({F.size // 3} tris each, +-0.5% radial noise, rotated, "
+                       f"offset +0.5x), MC_DISPATCH_FILTER_ALL",
+           "helpers": helpers}
+    r = args.mcd_repeats
+    out["reference_cpu_ms"] = min(ref.time_dispatch(src, cut, helpers=0, repeats=r))
+    out["reference_cpu_helpers_ms"] = min(ref.time_dispatch(src, cut, helpers=helpers, repeats=r))
+    progress(f"mcDispatch: reference {out['reference_cpu_ms']:.0f} ms (1 thread), "
+             f"{out['reference_cpu_helpers_ms']:.0f} ms ({helpers} helpers)")
+    out["seam_gpu_broad_phase_ms"] = min(seam.time_dispatch(src, cut, helpers=helpers, repeats=r))
+    ctx = frontend.Context(0)
+    ctx.dispatch(src, cut)
+    fe = [ctx.dispatch(src, cut).device_ms for _ in range(5)]
+    ctx.close()
+    out["gpu_frontend_device_ms"] = min(fe)
+    out["seam_vs_reference"] = out["reference_cpu_helpers_ms"] / out["seam_gpu_broad_phase_ms"]
+    progress(f"mcDispatch: seam {out['seam_gpu_broad_phase_ms']:.0f} ms, GPU front end {min(fe):.2f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +297,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-mcdispatch", action="store_true")
+    ap.add_argument("--mcd-level", type=int, default=5)
+    ap.add_argument("--mcd-repeats", type=int, default=2)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[10, 30, 100, 300, 1000, 3000])
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-max", type=int, default=1000)
@@ -364,6 +404,8 @@ def main():
         line["served"] = bench_serve(args, frontend, mg)
     if not args.no_precision:
         line["precision"] = bench_precision(args, frontend, mg)
+    if not args.no_mcdispatch:
+        line["mcdispatch"] = bench_mcdispatch(args, frontend, mg)
     print(json.dumps(line), flush=True)
 
 

@@ -55,6 +55,52 @@ class Mcut:
         if code != MC_NO_ERROR:
             raise McutError(code, what)
 
+    def time_dispatch(self, src, cut, flags=MC_DISPATCH_FILTER_ALL, helpers=0, repeats=3):
+        """Wall-clock ms of mcDispatch alone, `repeats` times in one context
+        (connected components released between calls, outside the timing).
+        The library's banner and any debug text go to stderr, not stdout."""
+        import time
+        L = self.L
+        out_fd = os.dup(1)
+        os.dup2(2, 1)
+        cwd = os.getcwd()
+        ctx = C.c_void_p()
+        times = []
+        try:
+            if helpers:
+                self._ok(L.mcCreateContextWithHelpers(C.byref(ctx), 0, helpers), "mcCreateContextWithHelpers")
+            else:
+                self._ok(L.mcCreateContext(C.byref(ctx), 0), "mcCreateContext")
+            keep = []
+
+            def arrs(m):
+                V, F, S = m
+                v = np.ascontiguousarray(V, np.float64)
+                f = np.ascontiguousarray(F, np.uint32)
+                s = None if S is None else np.ascontiguousarray(S, np.uint32)
+                keep.extend([v, f, s])
+                nf = f.size // 3 if s is None else s.size
+                return v.ctypes.data, f.ctypes.data, (None if s is None else s.ctypes.data), v.shape[0], nf
+            sv, sf, ss, snv, snf = arrs(src)
+            cv, cf, cs, cnv, cnf = arrs(cut)
+            with tempfile.TemporaryDirectory() as tmp:
+                os.chdir(tmp)
+                for _ in range(repeats):
+                    t0 = time.perf_counter()
+                    self._ok(L.mcDispatch(ctx, flags | MC_DISPATCH_VERTEX_ARRAY_DOUBLE, sv, sf, ss, snv, snf, cv, cf,
+                                          cs, cnv, cnf), "mcDispatch")
+                    times.append((time.perf_counter() - t0) * 1e3)
+                    self._ok(L.mcReleaseConnectedComponents(ctx, 0, None), "mcReleaseConnectedComponents")
+                os.chdir(cwd)
+        finally:
+            os.chdir(cwd)
+            if ctx.value:
+                L.mcReleaseContext(ctx)
+            C.CDLL(None).fflush(None)  # C-level buffered text leaves while fd 1 is still stderr
+            os.dup2(out_fd, 1)
+            os.close(out_fd)
+        return times
+
     def dispatch(self, src, cut, flags=MC_DISPATCH_FILTER_ALL, helpers=0):
         """mcDispatch on (V, F, S) meshes (S None = triangles) in a fresh context;
         returns the connected components as dicts of numpy arrays."""
