@@ -152,6 +152,20 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
 MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const double* perturbation,
                                                   uint32_t enforcement_count, mcgpu_frontend_info* info);
 
+/* mcDispatch's general-position loop (preproc.cpp:2562-2631) around the front
+ * end: dispatch; then, while the status is a general-position violation, an
+ * enforce flag is set and fewer than `attempts` perturbations were made
+ * (the context setting MC_CONTEXT_GENERAL_POSITION_ENFORCEMENT_ATTEMPTS,
+ * default 4), draw the next perturbation exactly as the reference draws it
+ * (per context, the sequence continuing across dispatches like the
+ * reference's thread-local generator) and rerun the narrow phase with the
+ * cut mesh moved by it.  A violation left after `attempts` perturbations is
+ * where the reference throws "max perturbation iterations reached".
+ * perturbation (3 doubles) and perturbations (count) are optional outputs. */
+MCGPU_API mcgpu_result mcgpu_dispatch_enforced(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
+                                               const mcgpu_dispatch_desc* desc, uint32_t attempts,
+                                               mcgpu_frontend_info* info, double* perturbation, uint32_t* perturbations);
+
 /* Broad phase alone, at the seam of the reference's build_oibvh (bvh.cpp:219)
  * and intersectOIBVHs (bvh.cpp:638): given the face AABBs of both meshes
  * (6 doubles per face: min xyz, max xyz, as build_oibvh's face_bboxes), finds

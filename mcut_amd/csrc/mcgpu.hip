@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <random>
 #include <thread>
 #include <cstddef>
 #include <cstdio>
@@ -151,6 +152,15 @@ struct mcgpu_context_t {
     std::vector<int32_t> hstatus;
     Buf bpairs, bstarts, fpair, vpair, bparams, brbox, pflags, border, bidx[2], rle, rlg, rstart, rend;
     const uint32_t* bin_idx[2] = {nullptr, nullptr}; // the caller's pair-local face lists
+
+    // the general-position retry's perturbation draws, as the reference makes
+    // them on its context's API thread (preproc.cpp:2621-2631): a
+    // default_random_engine seeded with 1 seeds an mt19937, whose draws go
+    // through uniform_real_distribution(-1, 1); the sequence continues across
+    // the context's dispatches
+    std::default_random_engine gp_seed_engine{1};
+    std::mt19937 gp_twister{gp_seed_engine()};
+    std::uniform_real_distribution<double> gp_uniform{-1.0, 1.0};
 
     // serving: a request slot of an mcgpu_server; one stream (s2, s3 and ssum
     // are s), the source mesh is the server's resident one
@@ -1333,6 +1343,38 @@ MCGPU_API mcgpu_result mcgpu_predicate_stage(mcgpu_context ctx, uint64_t* counts
             if (qsign) qsign[i] = hq[perm[i]];
             if (rsign) rsign[i] = hr[perm[i]];
         }
+    }
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_dispatch_enforced(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
+                                               const mcgpu_dispatch_desc* desc, uint32_t attempts,
+                                               mcgpu_frontend_info* info, double* perturbation, uint32_t* perturbations)
+{
+    if (!ctx || !desc || !info) return MCGPU_ERROR_INVALID_VALUE;
+    mcgpu_dispatch_desc d = *desc;
+    d.enforcement_count = 0;
+    mcgpu_result r = mcgpu_dispatch_frontend(ctx, src, cut, &d, info);
+    if (perturbations) *perturbations = 0;
+    if (perturbation) perturbation[0] = perturbation[1] = perturbation[2] = 0.0;
+    if (r != MCGPU_SUCCESS) return r;
+    const bool enforce =
+        (desc->flags & (MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION | MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE)) != 0;
+    if (!enforce) return MCGPU_SUCCESS;
+    // preproc.cpp:2588-2631: on a violation, stop once `attempts` perturbations
+    // were made, else draw the next vector (scaled by the same diag * constant
+    // that sets eps, preproc.cpp:2667-2676) and rerun the narrow phase on the
+    // kept candidates with the count of earlier perturbations
+    uint32_t count = 0;
+    while (info->status == MCGPU_STATUS_GENERAL_POSITION_VIOLATION && count < attempts) {
+        const double scale = info->eps;
+        double p[3];
+        for (int i = 0; i < 3; ++i) p[i] = ctx->gp_uniform(ctx->gp_twister) * scale;
+        if ((r = mcgpu_redispatch_perturbed(ctx, p, count, info)) != MCGPU_SUCCESS) return r;
+        info->eps = scale;
+        ++count;
+        if (perturbation) perturbation[0] = p[0], perturbation[1] = p[1], perturbation[2] = p[2];
+        if (perturbations) *perturbations = count;
     }
     return MCGPU_SUCCESS;
 }

@@ -105,6 +105,8 @@ def lib():
         L.mcgpu_server_info.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(u64), C.POINTER(C.c_uint32)]
         L.mcgpu_server_run.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp]
         L.mcgpu_server_get_registry.argtypes = [vp, C.c_uint32, vp, vp, vp, u64]
+        L.mcgpu_dispatch_enforced.argtypes = [vp, C.POINTER(Mesh), C.POINTER(Mesh), C.POINTER(DispatchDesc), C.c_uint32,
+                                              C.POINTER(FrontendInfo), vp, C.POINTER(C.c_uint32)]
         L.mcgpu_predicate_stage.argtypes = [vp, vp, vp, vp, u64, C.POINTER(C.c_float)]
         L.mcgpu_stage_times.argtypes = [vp, vp, C.c_int]
         L.mcgpu_stage_name.argtypes = [C.c_int]
@@ -114,7 +116,8 @@ def lib():
                    "mcgpu_get_tests", "mcgpu_get_face_planes", "mcgpu_get_ps_vertices", "mcgpu_registry_device",
                    "mcgpu_stage_times", "mcgpu_broad_phase", "mcgpu_dispatch_batch", "mcgpu_get_batch_status",
                    "mcgpu_get_batch_candidate_pairs", "mcgpu_server_create", "mcgpu_server_release",
-                   "mcgpu_server_info", "mcgpu_server_run", "mcgpu_server_get_registry", "mcgpu_predicate_stage"):
+                   "mcgpu_server_info", "mcgpu_server_run", "mcgpu_server_get_registry", "mcgpu_predicate_stage",
+                   "mcgpu_dispatch_enforced"):
             getattr(L, fn).restype = C.c_int
         _lib = L
     return _lib
@@ -339,6 +342,19 @@ class Context:
         _check(lib().mcgpu_dispatch_frontend(self.h, C.byref(ms), C.byref(mc), C.byref(d), C.byref(info)),
                "mcgpu_dispatch_frontend")
         return Result(self, info, ms.num_vertices + mc.num_vertices)
+
+    def dispatch_enforced(self, src, cut, flags=1 << 15, gp_constant=1e-4, attempts=4):
+        """mcDispatch's general-position loop on the GPU front end; returns
+        (Result, perturbation vector of the last rerun, number of perturbations)."""
+        keep = []
+        ms, mc = _host_mesh(src, keep), _host_mesh(cut, keep)
+        d = DispatchDesc(flags, gp_constant, 0, 0, 0, 0)
+        info = FrontendInfo()
+        p = np.zeros(3)
+        k = C.c_uint32(0)
+        _check(lib().mcgpu_dispatch_enforced(self.h, C.byref(ms), C.byref(mc), C.byref(d), attempts, C.byref(info),
+                                             _ptr(p), C.byref(k)), "mcgpu_dispatch_enforced")
+        return Result(self, info, ms.num_vertices + mc.num_vertices), p, k.value
 
     def dispatch_device(self, src: Mesh, cut: Mesh, flags=0, gp_constant=1e-4, enforcement_count=0, keep_tests=False):
         d = DispatchDesc(flags, gp_constant, enforcement_count, 1, 1 if keep_tests else 0, 0)

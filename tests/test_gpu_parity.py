@@ -106,6 +106,31 @@ def test_config1_perturbed_retry(ctx):
     assert {tuple(x) for x in p.tolist()} == want
 
 
+def test_enforced_retry_draws_the_reference_perturbation():
+    """mcgpu_dispatch_enforced on config 1 with ENFORCE_GENERAL_POSITION in a
+    fresh context: one perturbation, bit-identical to the one the reference
+    drew (golden fixture), and the oracle's registry for it."""
+    import os
+    from mcut_amd import frontend, meshgen as mg
+    cases = np.load(os.path.join(meshes.GOLDEN, "ref_cases.npz"))
+    want = cases["config1_enforce/perturbation"]
+    src, cut = mg.config1()
+    c = frontend.Context(0)
+    try:
+        R, p, k = c.dispatch_enforced(src, cut)
+        assert k == 1 and np.array_equal(p.view(np.uint64), np.asarray(want, np.float64).view(np.uint64))
+        O = om.frontend(src, cut, enforce_gp=True, enforcement_count=0, perturbation=p)
+        assert R.status == O["status"] == 0
+        e, g, pt = R.registry()
+        assert np.array_equal(e, O["reg_edge"]) and np.array_equal(g, O["reg_face"])
+        assert np.array_equal(pt.view(np.uint64), O["reg_point"].view(np.uint64))
+        # the sequence continues in the context: a second enforced dispatch draws the next vector
+        R2, p2, k2 = c.dispatch_enforced(src, cut)
+        assert k2 >= 1 and not np.array_equal(p2, p)
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("i", range(64))
 def test_benchmark_pairs(ctx, i):
     compare(ctx, *meshes.bench_pair(i))
