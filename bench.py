@@ -298,6 +298,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-mcdispatch", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (three times): the known-size kernel that "
+                         "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--mcd-level", type=int, default=5)
     ap.add_argument("--mcd-repeats", type=int, default=2)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[10, 30, 100, 300, 1000, 3000])
@@ -328,6 +331,13 @@ def main():
     ms, mc = to_dev(src), to_dev(cut)
     torch.cuda.synchronize()
     progress("single pair: inputs resident")
+    if args.copy_calibration:
+        a = torch.ones(32 << 20, dtype=torch.float64, device=dev)  # 256 MiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     ctx = frontend.Context(0)
     for _ in range(args.warmup):
         R = ctx.dispatch_device(ms, mc)
@@ -392,6 +402,12 @@ def main():
                      "frontend_alg_bytes": sum(stage_bytes(k, n) for k in stages)},
     }
     progress(f"single pair: {ms_step:.3f} ms per step")
+    traffic_file = os.path.join(ROOT, "profiles", "roofline_traffic.json")
+    if os.path.exists(traffic_file):  # HBM bytes per launch of the roofline kernel, from rocprofv3 --pmc runs
+        t = json.load(open(traffic_file))
+        if t.get("kernel") == STAGE_KERNEL[dom] and t.get("workload") == line["config"]["workload"]:
+            line["roofline"]["traffic"] = t["bytes_per_launch"]
+            line["roofline"]["traffic_source"] = t["source"]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(src, cut, I)
         progress("single pair: cpu baseline done")
