@@ -566,33 +566,42 @@ __global__ void __launch_bounds__(256) k_raw_bbox(const double* __restrict__ v, 
 // widened by a tolerance tol covering eps plus every rounding error of the
 // shift find a superset of the pairs whose shifted boxes overlap.  The exact
 // shifted-box test (k_filter_pairs) then leaves exactly the reference's set.
-__global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __restrict__ idx,
-                             const uint32_t* __restrict__ off, uint32_t nf, const unsigned long long* __restrict__ rbox,
-                             uint32_t* __restrict__ code, uint32_t* __restrict__ face, float* __restrict__ rawbox,
-                             const Params* __restrict__ P, const Counters* __restrict__ C, int widen, uint32_t mesh_bit)
+// one mesh's inputs to the leaf pass
+struct MortonMesh {
+    const double* v;
+    const uint32_t* idx;
+    const uint32_t* off;
+    const unsigned long long* rbox; // raw box as order keys
+    uint32_t nf;
+    uint32_t mesh_bit;
+    int widen;                      // cut mesh: boxes widened by tol
+};
+
+// leaves of both meshes in one launch: faces [start, nf0 + nf1) of the soup
+// (start = nf0 when the source's leaves are already made)
+__global__ void k_morton_raw(MortonMesh M0, MortonMesh M1, uint32_t start, uint32_t* __restrict__ code,
+                             uint32_t* __restrict__ face, float* __restrict__ rawbox, const Params* __restrict__ P,
+                             const Counters* __restrict__ C)
 {
-    const double tol[3] = {widen ? P->tol[0] : 0.0, widen ? P->tol[1] : 0.0, widen ? P->tol[2] : 0.0};
     const bool aborted = dispatch_aborted(C);
-    double lo[3], ext[3];
-    for (int j = 0; j < 3; ++j) {
-        lo[j] = ord_value(rbox[j]);
-        const double h = ord_value(rbox[3 + j]);
-        ext[j] = h > lo[j] ? h - lo[j] : 1.0;
-    }
-    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t F = M0.nf + M1.nf;
+    for (uint64_t g = start + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < F; g += (uint64_t)gridDim.x * blockDim.x) {
+        const bool cutm = g >= M0.nf;
+        const MortonMesh& M = cutm ? M1 : M0;
+        const uint32_t f = (uint32_t)(cutm ? g - M0.nf : g);
+        face[g] = f;
+        float* o = rawbox + 6 * g;
         if (aborted) { // empty box: the broad phase finds nothing
-            code[f] = mesh_bit;
-            face[f] = (uint32_t)f;
-            float* o = rawbox + 6 * f;
+            code[g] = M.mesh_bit;
             o[0] = o[1] = o[2] = 3.402823466e38f;
             o[3] = o[4] = o[5] = -3.402823466e38f;
             continue;
         }
-        const uint32_t b = off[f], n = off[f + 1] - b;
+        const uint32_t b = M.off[f], n = M.off[f + 1] - b;
         double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
         double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
         for (uint32_t i = 0; i < n; ++i) {
-            const double* p = v + 3 * (uint64_t)idx[b + i];
+            const double* p = M.v + 3 * (uint64_t)M.idx[b + i];
             for (int j = 0; j < 3; ++j) {
                 mn[j] = p[j] < mn[j] ? p[j] : mn[j];
                 mx[j] = p[j] > mx[j] ? p[j] : mx[j];
@@ -600,18 +609,19 @@ __global__ void k_morton_raw(const double* __restrict__ v, const uint32_t* __res
         }
         uint32_t c[3];
         for (int j = 0; j < 3; ++j) {
-            float x = (float)((((mn[j] + mx[j]) * 0.5) - lo[j]) / ext[j]);
+            const double lo = ord_value(M.rbox[j]), hi = ord_value(M.rbox[3 + j]);
+            const double ext = hi > lo ? hi - lo : 1.0;
+            float x = (float)((((mn[j] + mx[j]) * 0.5) - lo) / ext);
             x = fminf(fmaxf(x * 128.0f, 0.0f), 127.0f);
             c[j] = expand_bits10((uint32_t)x);
         }
-        code[f] = (c[0] * 4 + c[1] * 2 + c[2]) | mesh_bit;
-        face[f] = (uint32_t)f;
+        code[g] = (c[0] * 4 + c[1] * 2 + c[2]) | M.mesh_bit;
         // single precision, rounded outward: a float box contains the double
         // box, so the superset property carries over
-        float* o = rawbox + 6 * f;
         for (int j = 0; j < 3; ++j) {
-            o[j] = __double2float_rd(mn[j] - tol[j]);
-            o[3 + j] = __double2float_ru(mx[j] + tol[j]);
+            const double t = M.widen ? P->tol[j] : 0.0;
+            o[j] = __double2float_rd(mn[j] - t);
+            o[3 + j] = __double2float_ru(mx[j] + t);
         }
     }
 }
@@ -788,13 +798,12 @@ struct RefitTree {
     int blocks;             // blocks of this launch for this tree
 };
 
-__global__ void __launch_bounds__(512) k_refit_levels(RefitTree t0, RefitTree t1)
+// one block's share of a refit round: 1024 nodes of level lin (blk-th
+// group) reduced through LDS into the up to ten levels above them
+__device__ inline void refit_block(const RefitTree& T, int lin, int blk)
 {
     __shared__ float sb[kRefitWidth * 6];
-    const bool second = (int)blockIdx.x >= t0.blocks;
-    const RefitTree& T = second ? t1 : t0;
-    const int blk = second ? (int)blockIdx.x - t0.blocks : (int)blockIdx.x;
-    const int lin = T.lin, D = T.D;
+    const int D = T.D;
     const int width = 1 << lin;
     const int per = width < kRefitWidth ? width : kRefitWidth;
     const int steps = lin < kRefitLevels ? lin : kRefitLevels;
@@ -841,6 +850,19 @@ __global__ void __launch_bounds__(512) k_refit_levels(RefitTree t0, RefitTree t1
         __syncthreads();
         cur = half;
     }
+}
+
+// One round for both trees: blocks [0, t0.blocks) refit tree 0, the rest
+// tree 1.  (Finishing the top levels in the round's last block would need
+// every block to fence its writes at device scope, which on this GPU writes
+// back the L2 of its XCD: measured 4x slower, and it slows every kernel
+// running beside it.)
+__global__ void __launch_bounds__(512) k_refit_levels(RefitTree t0, RefitTree t1)
+{
+    const bool second = (int)blockIdx.x >= t0.blocks;
+    const RefitTree& T = second ? t1 : t0;
+    const int blk = second ? (int)blockIdx.x - t0.blocks : (int)blockIdx.x;
+    refit_block(T, T.lin, blk);
 }
 
 // ===========================================================================

@@ -680,20 +680,23 @@ void enqueue_frontend(mcgpu_context ctx)
     (void)hipStreamWaitEvent(s2, ctx->ev[EV_FORK], 0);
     (void)hipEventRecord(ctx->ev[EV_B0], s2);
     float* rawbox = P_<float>(ctx->rawbox);
-    for (int m = 0; m < 2; ++m) {
-        const uint32_t nf = ctx->nf[m];
-        const uint64_t o = m ? Fs : 0;
-        if (ctx->res && m == 0) continue;
-        if (ctx->batch)
+    if (ctx->batch) {
+        for (int m = 0; m < 2; ++m) {
+            const uint32_t nf = ctx->nf[m];
+            const uint64_t o = m ? Fs : 0;
             k_batch_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf, m,
                                                                 (uint32_t)o, P_<uint32_t>(ctx->fpair), brbox, Pb, C,
                                                                 ctx->batch_mb, P_<uint32_t>(ctx->mcode) + o,
                                                                 P_<uint32_t>(ctx->mface) + o, rawbox + 6 * o);
-        else
-            k_morton_raw<<<grid_for(nf), kBlock, 0, s2>>>(ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), nf,
-                                                          C->rbox[m], P_<uint32_t>(ctx->mcode) + o,
-                                                          P_<uint32_t>(ctx->mface) + o, rawbox + 6 * o, Pd, C, m,
-                                                          m ? (1u << 21) : 0u);
+        }
+    } else { // both meshes' leaves in one launch (the cut mesh's only, for a resident source)
+        MortonMesh M[2];
+        for (int m = 0; m < 2; ++m)
+            M[m] = MortonMesh{ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), C->rbox[m], ctx->nf[m],
+                              m ? (1u << 21) : 0u, m};
+        const uint32_t start = ctx->res ? Fs : 0u;
+        k_morton_raw<<<grid_for(F - start), kBlock, 0, s2>>>(M[0], M[1], start, P_<uint32_t>(ctx->mcode),
+                                                             P_<uint32_t>(ctx->mface), rawbox, Pd, C);
     }
     enqueue_bvh_traverse(ctx, s2, C);
 
