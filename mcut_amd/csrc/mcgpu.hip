@@ -32,9 +32,14 @@ struct Buf {
 // broad phase in raw coordinates alongside the sums; stream C numbers the
 // polygon-soup edges.  B and C are masked off the reserved CUs: a serial FP64
 // chain that shares its CU with memory- or atomic-heavy waves runs 4-8 times
-// slower.  A joins B before the exact pair filter and C before the edge-face
-// tests.
+// slower.  B and C also get disjoint CU sets: sharing all of them, the tree
+// build's small launches queue behind the edge numbering's large hash grids
+// and the sort stretches from 0.12 to 0.25 ms.  B takes kBroadShare of the
+// CUs left after the sums, the split that balanced the three paths on the
+// 256-CU part (B 144 CUs, C 104).  A joins B before the exact pair filter and
+// C before the edge-face tests.
 constexpr int kSumCUs = 8;
+constexpr double kBroadShare = 0.58;
 constexpr uint32_t kBatchSumWaves = 128; // persistent waves of the batched serial sums (on the reserved CUs)
 enum Stage {
     ST_SUMS,      // A: serial coordinate sums and raw boxes (k_mesh_sums alone)
@@ -944,15 +949,20 @@ MCGPU_API mcgpu_result mcgpu_create_context(int device, mcgpu_context* out)
     // CU masks: the first kSumCUs compute units for the serial sums, the rest
     // for the streams that run beside them
     const int ncu = prop.multiProcessorCount;
-    std::vector<uint32_t> msum((ncu + 31) / 32, 0u), mrest((ncu + 31) / 32, 0u);
+    const int nb = kSumCUs + (int)(kBroadShare * (ncu - kSumCUs) + 0.5);
+    std::vector<uint32_t> msum((ncu + 31) / 32, 0u), mb((ncu + 31) / 32, 0u), mc((ncu + 31) / 32, 0u);
     for (int c = 0; c < ncu; ++c) {
         if (c < kSumCUs) msum[c / 32] |= 1u << (c % 32);
-        else mrest[c / 32] |= 1u << (c % 32);
+        else if (c < nb) mb[c / 32] |= 1u << (c % 32);
+        else mc[c / 32] |= 1u << (c % 32);
     }
-    if (ncu <= 2 * kSumCUs) std::fill(mrest.begin(), mrest.end(), ~0u); // too small a device to reserve CUs
+    if (ncu <= 4 * kSumCUs) { // too small a device to partition
+        std::fill(mb.begin(), mb.end(), ~0u);
+        std::fill(mc.begin(), mc.end(), ~0u);
+    }
     if (hipExtStreamCreateWithCUMask(&ctx->ssum, (uint32_t)msum.size(), msum.data()) != hipSuccess ||
-        hipExtStreamCreateWithCUMask(&ctx->s2, (uint32_t)mrest.size(), mrest.data()) != hipSuccess ||
-        hipExtStreamCreateWithCUMask(&ctx->s3, (uint32_t)mrest.size(), mrest.data()) != hipSuccess) {
+        hipExtStreamCreateWithCUMask(&ctx->s2, (uint32_t)mb.size(), mb.data()) != hipSuccess ||
+        hipExtStreamCreateWithCUMask(&ctx->s3, (uint32_t)mc.size(), mc.data()) != hipSuccess) {
         if (ctx->ssum) (void)hipStreamDestroy(ctx->ssum);
         if (ctx->s2) (void)hipStreamDestroy(ctx->s2);
         if (ctx->s3) (void)hipStreamDestroy(ctx->s3);

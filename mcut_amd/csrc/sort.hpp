@@ -170,11 +170,14 @@ __device__ __forceinline__ uint32_t digit_of(K key, int shift)
     return (uint32_t)(key >> shift) & (kRadix - 1);
 }
 
-// histogram of one digit per tile, stored digit-major: hist[d * ntiles + t]
+// histogram of one digit per tile, stored digit-major: hist[d * ntiles + t].
+// With ghist, each tile's counts are also added into the totals of its group
+// of G consecutive tiles: ghist[d * ngc + t / G].
 template <typename K>
 __global__ void __launch_bounds__(kSortBlock) k_radix_hist(const K* __restrict__ keys, const uint64_t* n_dev,
                                                            uint64_t n_host, int shift, uint32_t* __restrict__ hist,
-                                                           uint32_t ntiles)
+                                                           uint32_t ntiles, uint32_t* __restrict__ ghist, uint32_t G,
+                                                           uint32_t ngc)
 {
     __shared__ uint32_t h[kRadix];
     const uint64_t n = read_count(n_dev, n_host);
@@ -191,7 +194,9 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_hist(const K* __restrict__
             }
         }
         __syncthreads();
-        hist[(uint64_t)threadIdx.x * nt + t] = h[threadIdx.x];
+        const uint32_t c = h[threadIdx.x];
+        hist[(uint64_t)threadIdx.x * nt + t] = c;
+        if (ghist && c) atomicAdd(&ghist[(uint64_t)threadIdx.x * ngc + t / G], c);
         __syncthreads();
     }
 }
@@ -201,7 +206,8 @@ template <typename K, bool HasVal>
 __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
                                                               K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                               const uint64_t* n_dev, uint64_t n_host, int shift,
-                                                              const uint32_t* __restrict__ offs, uint32_t ntiles)
+                                                              const uint32_t* __restrict__ offs, uint32_t ntiles,
+                                                              const uint32_t* __restrict__ ghist, uint32_t G, uint32_t ngc)
 {
     __shared__ uint32_t wcount[kSortBlock / 64][kRadix];
     __shared__ uint32_t running[kRadix];
@@ -211,11 +217,32 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restric
     (void)ntiles;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // with group totals, offs holds the raw tile histograms: this thread's
+    // digit starts after every smaller digit (dstart), then after its own
+    // digit in the groups and in the tiles of its group before the tile
+    uint32_t dstart = 0;
+    const uint32_t ng = ghist ? (nt + G - 1) / G : 0;
+    if (ghist) {
+        const uint32_t* gd = ghist + (uint64_t)threadIdx.x * ngc;
+        uint32_t tot = 0;
+        for (uint32_t q = 0; q < ng; ++q) tot += gd[q];
+        dstart = block_exclusive_scan(tot, nullptr);
+    }
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint64_t base = (uint64_t)t * kSortTile;
         if (base >= n) break; // tiles are visited in increasing order
         running[threadIdx.x] = 0;
-        gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * nt + t];
+        if (ghist) {
+            const uint32_t g = t / G;
+            const uint32_t* gd = ghist + (uint64_t)threadIdx.x * ngc;
+            const uint32_t* hd = offs + (uint64_t)threadIdx.x * nt;
+            uint32_t off = dstart;
+            for (uint32_t q = 0; q < g; ++q) off += gd[q];
+            for (uint32_t q = g * G; q < t; ++q) off += hd[q];
+            gbase[threadIdx.x] = off;
+        } else {
+            gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * nt + t];
+        }
         for (int w = 0; w < kSortBlock / 64; ++w) wcount[w][threadIdx.x] = 0;
         __syncthreads();
         for (int j = 0; j < kSortItems; ++j) {
@@ -257,12 +284,35 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restric
 }
 
 struct RadixTemp {
-    uint32_t* hist;      // kRadix * ntiles
+    uint32_t* hist;      // radix_hist_elems(cap): tile histograms, then group totals
     uint32_t* scan_tmp;  // scan_tmp_elems(kRadix * ntiles)
 };
 
+// Tile counts up to this use group totals in place of a scan between the
+// histogram and scatter passes: two launches per digit instead of five.
+// Every scatter block reads about 2 sqrt(tiles) counts per digit, so the
+// group path stops where that outgrows the scan.
+constexpr uint32_t kGroupedMaxTiles = 1024;
+constexpr int kMaxPasses = 8;
+
 inline uint32_t radix_tiles(uint64_t cap) { return (uint32_t)((cap + kSortTile - 1) / kSortTile) + 1; }
-inline uint64_t radix_hist_elems(uint64_t cap) { return (uint64_t)kRadix * radix_tiles(cap); }
+inline uint32_t radix_group(uint32_t ntiles)
+{
+    uint32_t g = 1;
+    while (g * g < ntiles) ++g;
+    return g;
+}
+inline uint32_t radix_groups(uint32_t ntiles) { return (ntiles + radix_group(ntiles) - 1) / radix_group(ntiles); }
+inline uint64_t radix_hist_elems(uint64_t cap)
+{
+    const uint32_t nt = radix_tiles(cap);
+    return (uint64_t)kRadix * nt + (uint64_t)kRadix * kMaxPasses * radix_groups(nt);
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
 
 // Sorts keys[0..n) (n on device or host) by bits [0, bits) with an optional
 // u32 payload.  Ping-pongs between (keys, vals) and (keys_alt, vals_alt);
@@ -273,23 +323,36 @@ inline bool radix_sort(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
 {
     const uint32_t ntiles = radix_tiles(cap);
     const uint32_t grid = ntiles < 2048 ? ntiles : 2048;
+    const int passes = (bits + kRadixBits - 1) / kRadixBits;
+    const bool grouped = ntiles <= kGroupedMaxTiles && passes <= kMaxPasses;
+    const uint32_t G = radix_group(ntiles), ngc = radix_groups(ntiles);
+    uint32_t* ghist = tmp.hist + (uint64_t)kRadix * ntiles;
+    if (grouped) {
+        const uint64_t nz = (uint64_t)kRadix * ngc * passes;
+        k_zero_u32<<<(unsigned)((nz + 255) / 256 < 1024 ? (nz + 255) / 256 : 1024), 256, 0, s>>>(ghist, nz);
+    }
     bool alt = false;
-    for (int shift = 0; shift < bits; shift += kRadixBits) {
+    for (int p = 0; p < passes; ++p) {
+        const int shift = p * kRadixBits;
         K* kin = alt ? keys_alt : keys;
         K* kout = alt ? keys : keys_alt;
         uint32_t* vin = alt ? vals_alt : vals;
         uint32_t* vout = alt ? vals : vals_alt;
-        k_radix_hist<K><<<grid, kSortBlock, 0, s>>>(kin, n_dev, n_host, shift, tmp.hist, ntiles);
-        const uint64_t nh_host = (uint64_t)kRadix * ((n_host + kSortTile - 1) / kSortTile);
-        if (!n_dev && nh_host > 16384)
-            scan_exclusive(tmp.hist, tmp.hist, nh_host, tmp.scan_tmp, nullptr, s);
-        else
-            k_scan_hist<<<1, kScan1Block, 0, s>>>(tmp.hist, n_dev, n_host, kSortTile);
+        uint32_t* gh = grouped ? ghist + (uint64_t)kRadix * ngc * p : nullptr;
+        k_radix_hist<K><<<grid, kSortBlock, 0, s>>>(kin, n_dev, n_host, shift, tmp.hist, ntiles, gh, G, ngc);
+        if (!grouped) {
+            const uint64_t nh_host = (uint64_t)kRadix * ((n_host + kSortTile - 1) / kSortTile);
+            if (!n_dev && nh_host > 16384)
+                scan_exclusive(tmp.hist, tmp.hist, nh_host, tmp.scan_tmp, nullptr, s);
+            else
+                k_scan_hist<<<1, kScan1Block, 0, s>>>(tmp.hist, n_dev, n_host, kSortTile);
+        }
         if (vals)
-            k_radix_scatter<K, true><<<grid, kSortBlock, 0, s>>>(kin, vin, kout, vout, n_dev, n_host, shift, tmp.hist, ntiles);
+            k_radix_scatter<K, true><<<grid, kSortBlock, 0, s>>>(kin, vin, kout, vout, n_dev, n_host, shift, tmp.hist, ntiles,
+                                                                 gh, G, ngc);
         else
             k_radix_scatter<K, false><<<grid, kSortBlock, 0, s>>>(kin, nullptr, kout, nullptr, n_dev, n_host, shift, tmp.hist,
-                                                                  ntiles);
+                                                                  ntiles, gh, G, ngc);
         alt = !alt;
     }
     return alt;
