@@ -435,7 +435,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
         const unsigned tiles = (unsigned)((ctx->cap_reg + kRegTile - 1) / kRegTile);
         k_reg_tile_sort<<<tiles, kRegSortThreads, 0, s>>>(P_<unsigned long long>(ctx->rkey), P_<uint32_t>(ctx->rval), &C->n_reg,
                                                          ctx->cap_reg, P_<unsigned long long>(ctx->rkey2),
-                                                         P_<uint32_t>(ctx->rval2), hk, hc, hs);
+                                                         P_<uint32_t>(ctx->rval2), hk, hc, hs, ctx->kbits);
         k_reg_merge<<<grid_for(ctx->cap_reg), kBlock, 0, s>>>(S, P_<unsigned long long>(ctx->rkey2), P_<uint32_t>(ctx->rval2),
                                                               &C->n_reg, ctx->cap_reg, R.e, R.g, P_<double>(ctx->rp),
                                                               P_<uint32_t>(ctx->re), P_<uint32_t>(ctx->rg),
