@@ -475,6 +475,143 @@ __global__ void k_edge_assign(const uint32_t* __restrict__ idx, CornerMap cm, ui
     }
 }
 
+// Packed edge table, for meshes whose vertex ids fit 21 bits and whose corner
+// ranks fit 22: one 64-bit word per edge holds (min vertex, max vertex) in
+// its top 42 bits and the smallest insertion rank in its low 22, so one
+// atomicCAS (plus an atomicMin when the edge is already there) replaces the
+// key and first-use tables: 53 us instead of 103 us for the 983,040 corners of
+// an icosphere-7.  Both meshes go through one launch per pass: corners
+// [0, K0) are the source's, [K0, K0 + K1) the cut mesh's (is_cut = 1).  The
+// insert leaves each corner's slot in its cedge entry; the flag and assign
+// passes read it instead of probing, and the assign pass overwrites it with
+// the edge id.  Edge ids come from one exclusive scan over both meshes'
+// first-use flags, so a cut edge's id already counts the source's edges
+// (with a resident source only the cut mesh is scanned, after its ne edges).
+constexpr uint32_t kPackedVertexBits = 21, kPackedRankBits = 22;
+constexpr unsigned long long kPackedRankMask = (1ull << kPackedRankBits) - 1;
+
+struct TopoMesh {
+    const uint32_t* idx;
+    CornerMap cm;
+    unsigned long long* keys;
+    uint64_t mask;
+    uint32_t* cedge;
+    uint64_t nk;
+    uint32_t vofs, fofs; // soup offsets of the mesh's vertices and faces
+};
+
+__device__ __forceinline__ uint64_t packed_edge(uint32_t a, uint32_t b)
+{
+    return a < b ? ((uint64_t)a << kPackedVertexBits | b) : ((uint64_t)b << kPackedVertexBits | a);
+}
+
+// the corner whose insertion rank is r: a face's ranks are a permutation of
+// its corner positions, so r lies in its own face (the inverse of corner_rank)
+__device__ __forceinline__ uint64_t rank_corner(const CornerMap& cm, uint32_t r, int is_cut)
+{
+    uint32_t f, b, n, i;
+    cm(r, f, b, n, i);
+    if (!is_cut) return r;
+    const uint32_t j = r - b;
+    return b + (j == n - 1 ? 0 : j + 1);
+}
+
+__global__ void k_fill2_u64(unsigned long long* p0, uint64_t n0, unsigned long long* p1, uint64_t n1, unsigned long long v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n0 + n1; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i < n0) p0[i] = v;
+        else p1[i - n0] = v;
+    }
+}
+
+__global__ void k_edge_insert_packed(TopoMesh M0, TopoMesh M1, uint64_t start)
+{
+    const uint64_t K = M0.nk + M1.nk;
+    for (uint64_t q = start + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < K; q += (uint64_t)gridDim.x * blockDim.x) {
+        const int m = q >= M0.nk;
+        const TopoMesh& M = m ? M1 : M0;
+        const uint64_t c = m ? q - M0.nk : q;
+        uint32_t f, b, n, i;
+        M.cm(c, f, b, n, i);
+        const uint32_t a = M.idx[c], d = M.idx[b + (i + 1 == n ? 0 : i + 1)];
+        const uint64_t ek = packed_edge(a, d);
+        const uint32_t rank = corner_rank(b, i, n, m);
+        const unsigned long long w = (ek << kPackedRankBits) | rank;
+        uint64_t h = hash64(ek) & M.mask;
+        for (uint64_t probe = 0; probe <= M.mask; ++probe) {
+            const unsigned long long prev = atomicCAS(&M.keys[h], kEmpty64, w);
+            if (prev == kEmpty64) break;
+            if ((prev >> kPackedRankBits) == ek) {
+                if ((prev & kPackedRankMask) > rank) atomicMin(&M.keys[h], w);
+                break;
+            }
+            h = (h + 1) & M.mask;
+        }
+        M.cedge[c] = (uint32_t)h;
+    }
+}
+
+// flag[moff + rank] = 1 at an edge's first use (moff = K0 for the cut mesh)
+__global__ void k_edge_flag_packed(TopoMesh M0, TopoMesh M1, uint64_t start, uint32_t* __restrict__ flag, Counters* C)
+{
+    const uint64_t K = M0.nk + M1.nk;
+    for (uint64_t q = start + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < K; q += (uint64_t)gridDim.x * blockDim.x) {
+        const int m = q >= M0.nk;
+        const TopoMesh& M = m ? M1 : M0;
+        const uint64_t c = m ? q - M0.nk : q;
+        uint32_t f, b, n, i;
+        M.cm(c, f, b, n, i);
+        const uint32_t a = M.idx[c];
+        const uint32_t rank = corner_rank(b, i, n, m);
+        const uint32_t r0 = (uint32_t)(M.keys[M.cedge[c]] & kPackedRankMask);
+        flag[(m ? M0.nk : 0) + rank] = (r0 == rank) ? 1u : 0u;
+        if (r0 != rank && M.idx[rank_corner(M.cm, r0, m)] == a) atomicOr(&C->nonmanifold, 1u); // used twice in one direction
+    }
+}
+
+// edge counts from the scan: both meshes scanned together, or (resident
+// source) the cut mesh alone
+__global__ void k_store_edge_counts_packed(const uint32_t* __restrict__ eid, const uint32_t* __restrict__ flag, uint64_t K0,
+                                           const uint32_t* __restrict__ total, int cut_only, Counters* C)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (cut_only) {
+            C->n_edges_cut = *total;
+        } else {
+            const uint32_t ns = K0 ? eid[K0 - 1] + flag[K0 - 1] : 0u;
+            C->n_edges_src = ns;
+            C->n_edges_cut = *total - ns;
+        }
+    }
+}
+
+__global__ void k_edge_assign_packed(TopoMesh M0, TopoMesh M1, uint64_t start, const uint32_t* __restrict__ eid,
+                                     int cut_only, Counters* C, uint32_t* __restrict__ ev0, uint32_t* __restrict__ ev1,
+                                     int32_t* __restrict__ ef0, int32_t* __restrict__ ef1)
+{
+    const uint64_t K = M0.nk + M1.nk;
+    const uint32_t eb1 = cut_only ? (uint32_t)C->n_edges_src : 0u;
+    for (uint64_t q = start + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < K; q += (uint64_t)gridDim.x * blockDim.x) {
+        const int m = q >= M0.nk;
+        const TopoMesh& M = m ? M1 : M0;
+        const uint64_t c = m ? q - M0.nk : q;
+        uint32_t f, b, n, i;
+        M.cm(c, f, b, n, i);
+        const uint32_t a = M.idx[c], d = M.idx[b + (i + 1 == n ? 0 : i + 1)];
+        const uint32_t rank = corner_rank(b, i, n, m);
+        const uint32_t r0 = (uint32_t)(M.keys[M.cedge[c]] & kPackedRankMask);
+        const uint32_t e = (m ? eb1 : 0u) + eid[(m ? M0.nk : 0) + r0];
+        M.cedge[c] = e;
+        if (r0 == rank) {
+            ev0[e] = M.vofs + a;
+            ev1[e] = M.vofs + d;
+            ef0[e] = (int32_t)(M.fofs + f);
+        } else if (atomicExch(&ef1[e], (int32_t)(M.fofs + f)) != -1) {
+            atomicOr(&C->nonmanifold, 4u);
+        }
+    }
+}
+
 // ===========================================================================
 // 4. face AABBs (bvh.cpp:242-313) and the LBVH over them
 // ===========================================================================

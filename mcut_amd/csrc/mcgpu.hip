@@ -342,7 +342,11 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
     CK(ensure(ctx->planes, F, &pl));
     (void)p;
     CK(ensure(ctx->scan_tmp, scan_need + 1024, &u));
-    CK(ensure(ctx->scan_tmp_c, scan_tmp_elems(std::max(ctx->ncorner[0], ctx->ncorner[1]) + 8) + 1024, &u));
+    // the packed edge numbering scans both meshes' first-use flags as one array
+    const uint64_t K2 = ctx->ncorner[0] + ctx->ncorner[1];
+    CK(ensure(ctx->eflag[0], K2 + 8, &u));
+    CK(ensure(ctx->eid[0], K2 + 8, &u));
+    CK(ensure(ctx->scan_tmp_c, scan_tmp_elems(K2 + 8) + 1024, &u));
     return MCGPU_SUCCESS;
 }
 
@@ -392,6 +396,16 @@ mcgpu_result alloc_list_buffers(mcgpu_context ctx)
     const uint64_t need = std::max<uint64_t>(scan_tmp_elems(radix_hist_elems(sort_cap)), scan_tmp_elems(K + 8));
     CK(ensure(ctx->scan_tmp, need + 1024, &u));
     return MCGPU_SUCCESS;
+}
+
+// one packed word per edge (frontend_kernels.hpp, TopoMesh) when every vertex
+// id fits 21 bits and every corner rank 22 bits; batches keep the two tables
+bool topology_packed(mcgpu_context ctx)
+{
+    if (ctx->batch) return false;
+    for (int m = ctx->res ? 1 : 0; m < 2; ++m)
+        if (ctx->nv[m] >= (1u << kPackedVertexBits) || ctx->ncorner[m] >= (1ull << kPackedRankBits)) return false;
+    return true;
 }
 
 // the part of the pipeline that runs again for a perturbed cut mesh (whose
@@ -729,6 +743,26 @@ void enqueue_frontend(mcgpu_context ctx)
     }
     // the scans of this stream use their own scratch: stream B's sort uses scan_tmp
     uint32_t* ctmp = P_<uint32_t>(ctx->scan_tmp_c);
+    if (topology_packed(ctx)) {
+        TopoMesh TM[2];
+        for (int m = 0; m < 2; ++m)
+            TM[m] = TopoMesh{ctx->didx[m], cmap[m], P_<unsigned long long>(ctx->hkeys[m]),
+                             next_pow2(2 * ctx->ncorner[m] + 2) - 1, P_<uint32_t>(ctx->cedge[m]), ctx->ncorner[m],
+                             m ? ctx->nv[0] : 0u, m ? ctx->nf[0] : 0u};
+        const int cut_only = ctx->res ? 1 : 0;
+        const uint64_t K0 = ctx->ncorner[0], start = cut_only ? K0 : 0;
+        uint32_t* flag = P_<uint32_t>(ctx->eflag[0]);
+        uint32_t* eid = P_<uint32_t>(ctx->eid[0]);
+        const uint64_t t0 = cut_only ? 0 : TM[0].mask + 1, t1 = TM[1].mask + 1;
+        k_fill2_u64<<<grid_for(t0 + t1), kBlock, 0, s3>>>(TM[0].keys, t0, TM[1].keys, t1, kEmpty64);
+        k_edge_insert_packed<<<grid_for(K - start), kBlock, 0, s3>>>(TM[0], TM[1], start);
+        k_edge_flag_packed<<<grid_for(K - start), kBlock, 0, s3>>>(TM[0], TM[1], start, flag, C);
+        scan_exclusive(flag + start, eid + start, K - start, ctmp, etotal, s3);
+        k_store_edge_counts_packed<<<1, 1, 0, s3>>>(eid, flag, K0, etotal, cut_only, C);
+        k_edge_assign_packed<<<grid_for(K - start), kBlock, 0, s3>>>(TM[0], TM[1], start, eid, cut_only, C,
+                                                                     P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1),
+                                                                     P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
+    } else {
     for (int m = ctx->res ? 1 : 0; m < 2; ++m) {
         const uint64_t nk = ctx->ncorner[m];
         const uint64_t hs = next_pow2(2 * nk + 2);
@@ -749,6 +783,7 @@ void enqueue_frontend(mcgpu_context ctx)
             ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]), P_<unsigned long long>(ctx->hfirst[m]),
             hs - 1, P_<uint32_t>(ctx->eid[m]), C, m ? ctx->nv[0] : 0, m ? ctx->nf[0] : 0, P_<uint32_t>(ctx->cedge[m]),
             P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
+    }
     }
     if (ctx->batch)
         k_batch_edge_bases<<<grid_for(ctx->npairs), kBlock, 0, s3>>>(
