@@ -102,13 +102,339 @@ __device__ __forceinline__ double serial_sum(const double* __restrict__ v, uint3
 
 __global__ void __launch_bounds__(64) k_mesh_sums(const double* __restrict__ v0, uint32_t n0,
                                                   const double* __restrict__ v1, uint32_t n1,
-                                                  const double* __restrict__ zero, Params* P)
+                                                  const double* __restrict__ zero, Params* P,
+                                                  const uint32_t* __restrict__ only_failed)
 {
     __builtin_amdgcn_s_setprio(3);
     const int m = blockIdx.x / 3, j = blockIdx.x % 3;
     if (m == 0 && !v0) return; // a resident source: its sums are set already
+    if (only_failed && __builtin_amdgcn_readfirstlane(only_failed[blockIdx.x]) == 0u) return;
     const double acc = serial_sum((m ? v1 : v0) + j, m ? n1 : n0, zero);
     if (threadIdx.x == 0) P->sum[m][j] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// The same sums from a chain about five times shorter.  A step s = RN(s' + x)
+// whose result stays in the binade of s' (|s| and |s'| in [2^e, 2^(e+1)), same
+// sign) rounds to the multiples of u = 2^(e-52), of which s' already is one:
+// s = s' + RN_u(x), whatever s' is, unless RN_u(x) is a tie.  A run of such
+// steps adds sum RN_u(x) to its start value exactly, one rounding-free add.
+// Binade changes and ties stay single steps of the chain.
+//
+// k_sum_blocks and k_sum_items predict the binade of every partial sum from
+// an approximate prefix (any summation order) and write each 32-value thread
+// range as items: raw values, and for every same-binade run its exact sum
+// (integer multiples of u) with the interval [lo, hi] of start values for
+// which every partial sum of the run stays in [2^e + u, 2^(e+1)].  k_sum_chain adds the items left to right with the serial sum's DPP
+// chain and checkpoints every 64 items; k_sum_verify replays every 16 items
+// from the checkpoint before them and checks each run's actual start value
+// against its interval.  A start inside it proves the run exact (the prediction only
+// chooses the runs); a failure sends that coordinate to the serial sum
+// (k_mesh_sums with the failure flags).  On the icosphere-7 pair the longest
+// chain holds 35k items for 164k vertices.
+constexpr int kSumThreads = 128, kSumPer = 32;
+constexpr uint32_t kSumTile = kSumThreads * kSumPer; // values per block and per item region
+constexpr int kSumLds = kSumPer + 1;                 // LDS stride of a thread's values (no bank conflicts)
+
+struct SumBufs {
+    const double* v[2]; // vertex arrays (xyz), null when that mesh's sums are preset
+    uint32_t n[2];      // vertex counts
+    uint32_t nblk;      // blocks per chain (of the larger mesh)
+    double* bsum;       // [6][nblk]
+    double* items;      // [6][nblk * kSumTile]
+    double* lo;
+    double* hi;
+    uint32_t* nitem;    // [6][nblk]: items of each block region, padded to 64
+    uint32_t* list;     // [6][lcap]: 64-item blocks in chain order
+    uint32_t lcap;
+    double* ckpt;       // [6][lcap]: chain value after each 64 items
+    uint32_t* fail;     // [6]
+    uint32_t* nb64;     // [6]: 64-item blocks of each chain
+};
+
+__device__ __forceinline__ bool sum_chain_active(const SumBufs& B, int ch) { return B.v[ch / 3] != nullptr && B.n[ch / 3] > 0; }
+
+__global__ void __launch_bounds__(kSumThreads) k_sum_blocks(SumBufs B)
+{
+    __shared__ double red[kSumThreads / 64];
+    const int ch = blockIdx.y, m = ch / 3, j = ch % 3;
+    if (blockIdx.x == 0 && threadIdx.x == 0) B.fail[ch] = 0u;
+    if (!sum_chain_active(B, ch)) return;
+    const uint32_t n = B.n[m], b = blockIdx.x;
+    if ((uint64_t)b * kSumTile >= n) return;
+    const double* v = B.v[m] + j;
+    const uint64_t i0 = (uint64_t)b * kSumTile;
+    double t = 0.0;
+#pragma unroll 8
+    for (uint32_t k = threadIdx.x; k < kSumTile; k += kSumThreads)
+        if (i0 + k < n) t += v[3 * (i0 + k)];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int w = 0; w < kSumThreads / 64; ++w) a += red[w];
+        B.bsum[(uint64_t)ch * B.nblk + b] = a;
+    }
+}
+
+// (sign, exponent) of a predicted partial sum; kSumTiny for zero, |a| < 2^-900
+// or |a| >= 2^1001 (keeps 2^(e+3) and 2^(52-e) finite)
+constexpr int kSumTiny = 1 << 20;
+__device__ __forceinline__ int sum_binade(double a)
+{
+    const uint64_t bits = (uint64_t)__double_as_longlong(a);
+    const int ex = (int)((bits >> 52) & 0x7ff);
+    if (ex < 1023 - 900 || ex > 1023 + 1000) return kSumTiny;
+    return (ex << 1) | (int)(bits >> 63);
+}
+
+struct SumRun {
+    long long q, qmin, qmax; // running sum of RN_u(x) / u and its extremes (0 included)
+    int key;                 // binade; kSumTiny when no run is open
+    bool bad;
+};
+
+// one thread's items for its values x[0..cnt) (LDS), starting from the
+// approximate prefix a: emission through emit(item, lo, hi)
+template <class Emit>
+__device__ __forceinline__ void sum_items_walk(const double* x, uint32_t cnt, double a, Emit emit)
+{
+    SumRun R{0, 0, 0, kSumTiny, false};
+    int prev = sum_binade(a);
+    const double inf = __longlong_as_double(0x7ff0000000000000ll);
+    auto close = [&]() {
+        if (R.key == kSumTiny) return;
+        const int e = (R.key >> 1) - 1023;
+        const double u = __longlong_as_double((long long)(e - 52 + 1023) << 52);
+        // in units of u: every partial sum v + Q within [2^e + u, 2^(e+1)]
+        // (mirrored for a negative binade).  The exact step value is within u/2
+        // of v + Q, and there the nearest double is v + Q: inside the binade the
+        // doubles are the multiples of u, and just above 2^(e+1) they are 2u
+        // apart.  Exact in integers, rounded inward to doubles.
+        const long long L = (R.key & 1) ? -(1ll << 53) : (1ll << 52) + 1;
+        const long long H = (R.key & 1) ? -((1ll << 52) + 1) : (1ll << 53);
+        const long long lo_u = L - R.qmin, hi_u = H - R.qmax;
+        double lod = (double)lo_u, hid = (double)hi_u;
+        if ((long long)lod < lo_u) lod = nextafter(lod, 1.0e300);
+        if ((long long)hid > hi_u) hid = nextafter(hid, -1.0e300);
+        double lo = lod * u, hi = hid * u;
+        if (R.bad) { // empty: the verification fails
+            lo = inf;
+            hi = -inf;
+        }
+        emit((double)R.q * u, lo, hi);
+        R.key = kSumTiny;
+    };
+    for (uint32_t i = 0; i < cnt; ++i) {
+        const double xi = x[i];
+        a = a + xi;
+        const int key = sum_binade(a);
+        bool stateless = key == prev && key != kSumTiny;
+        double c = 0.0;
+        if (stateless) {
+            const int e = (key >> 1) - 1023;
+            const double inv_u = __longlong_as_double((long long)(52 - e + 1023) << 52);
+            const double ax = xi < 0 ? -xi : xi;
+            if (!(ax < __longlong_as_double((long long)(e + 3 + 1023) << 52))) {
+                stateless = false;
+            } else {
+                const double q = xi * inv_u;
+                c = rint(q);
+                if (q - floor(q) == 0.5) stateless = false; // a tie: the parity of s' decides
+            }
+        }
+        if (stateless) {
+            if (R.key != key) {
+                close();
+                R = SumRun{0, 0, 0, key, false};
+            }
+            R.q += (long long)c;
+            R.qmin = R.q < R.qmin ? R.q : R.qmin;
+            R.qmax = R.q > R.qmax ? R.q : R.qmax;
+            if (R.q > (1ll << 53) || R.q < -(1ll << 53)) R.bad = true;
+        } else {
+            close();
+            emit(xi, -inf, inf);
+        }
+        prev = key;
+    }
+    close();
+}
+
+__global__ void __launch_bounds__(kSumThreads) k_sum_items(SumBufs B)
+{
+    __shared__ double xs[kSumThreads * kSumLds];
+    __shared__ double wpre[kSumThreads / 64];
+    __shared__ uint32_t wcnt[kSumThreads / 64];
+    __shared__ double a0s;
+    const int ch = blockIdx.y, m = ch / 3, j = ch % 3;
+    if (!sum_chain_active(B, ch)) return;
+    const uint32_t n = B.n[m], b = blockIdx.x;
+    if ((uint64_t)b * kSumTile >= n) {
+        if (threadIdx.x == 0) B.nitem[(uint64_t)ch * B.nblk + b] = 0u;
+        return;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const double* v = B.v[m] + j;
+    const uint64_t i0 = (uint64_t)b * kSumTile;
+    const uint32_t nb = (n - i0) < kSumTile ? (uint32_t)(n - i0) : kSumTile;
+    // the block's values into LDS (coalesced), value k of thread t at t * kSumLds + k
+#pragma unroll 8
+    for (uint32_t k = threadIdx.x; k < kSumTile; k += kSumThreads)
+        xs[(k / kSumPer) * kSumLds + k % kSumPer] = k < nb ? v[3 * (i0 + k)] : 0.0;
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (uint32_t q = 0; q < b; ++q) a += B.bsum[(uint64_t)ch * B.nblk + q];
+        a0s = a;
+    }
+    __syncthreads();
+    const double* x = xs + threadIdx.x * kSumLds;
+    const uint32_t t0 = threadIdx.x * kSumPer;
+    const uint32_t cnt = t0 < nb ? (nb - t0 < (uint32_t)kSumPer ? nb - t0 : (uint32_t)kSumPer) : 0u;
+    double t = 0.0;
+    for (uint32_t k = 0; k < cnt; ++k) t += x[k];
+    // approximate prefix at this thread's first value
+    double y = t;
+    for (int o = 1; o < 64; o <<= 1) {
+        const double z = __shfl_up(y, o, 64);
+        if (lane >= o) y += z;
+    }
+    if (lane == 63) wpre[wid] = y;
+    __syncthreads();
+    double a = a0s + (y - t);
+    for (int w = 0; w < wid; ++w) a += wpre[w];
+    uint32_t nit = 0;
+    if (cnt) sum_items_walk(x, cnt, a, [&](double, double, double) { ++nit; });
+    // this thread's offset in the block region
+    uint32_t r = nit;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t z = __shfl_up(r, o, 64);
+        if (lane >= o) r += z;
+    }
+    if (lane == 63) wcnt[wid] = r;
+    __syncthreads();
+    uint32_t off = r - nit, tot = 0;
+    for (int w = 0; w < kSumThreads / 64; ++w) {
+        if (w < wid) off += wcnt[w];
+        tot += wcnt[w];
+    }
+    const uint64_t base = ((uint64_t)ch * B.nblk + b) * kSumTile;
+    double* it = B.items + base;
+    double* lo = B.lo + base;
+    double* hi = B.hi + base;
+    if (cnt) {
+        uint32_t k = off;
+        sum_items_walk(x, cnt, a, [&](double val, double l, double h) {
+            it[k] = val;
+            lo[k] = l;
+            hi[k] = h;
+            ++k;
+        });
+    }
+    // +0.0 to the next multiple of 64 (adding +0.0 changes no partial sum)
+    const uint32_t padded = (tot + 63u) & ~63u;
+    const double inf = __longlong_as_double(0x7ff0000000000000ll);
+    for (uint32_t k = tot + threadIdx.x; k < padded; k += kSumThreads) {
+        it[k] = 0.0;
+        lo[k] = -inf;
+        hi[k] = inf;
+    }
+    if (threadIdx.x == 0) B.nitem[(uint64_t)ch * B.nblk + b] = padded;
+}
+
+// one wave per chain (on the reserved CUs): the DPP chain over the item
+// regions in order, a checkpoint after every 16 items.  A scalar cursor
+// (region, 64-item block) walks the regions ahead of the chain, so every
+// refill address is a uniform base with no dependent load; the block list is
+// written for the verification.
+__global__ void __launch_bounds__(64) k_sum_chain(SumBufs B, Params* P)
+{
+    extern __shared__ uint32_t cnt64[]; // 64-item blocks of each region
+    __builtin_amdgcn_s_setprio(3);
+    const int ch = blockIdx.x, m = ch / 3, j = ch % 3;
+    if (!sum_chain_active(B, ch)) return;
+    const uint32_t lane = threadIdx.x, c = lane & 15;
+    const uint32_t* nit = B.nitem + (uint64_t)ch * B.nblk;
+    uint32_t* list = B.list + (uint64_t)ch * B.lcap;
+    uint32_t nb = 0;
+    for (uint32_t b = 0; b < B.nblk; ++b) {
+        const uint32_t k = nit[b] / 64u;
+        if (lane == 0) cnt64[b] = k;
+        for (uint32_t q = lane; q < k; q += 64) list[nb + q] = b * (kSumTile / 64) + q;
+        nb += k;
+    }
+    __syncthreads();
+    nb = __builtin_amdgcn_readfirstlane(nb);
+    if (lane == 0) B.nb64[ch] = nb;
+    const double* items = B.items + (uint64_t)ch * B.nblk * kSumTile;
+    double* ckpt = B.ckpt + (uint64_t)ch * B.lcap;
+    // load cursor: region rb (cur 64-item blocks), block jb within it; it
+    // reads a region's count only when it enters the region and stays on the
+    // last block at the end
+    uint32_t rb = 0, jb = 0, last_blk = 0, cur = __builtin_amdgcn_readfirstlane(cnt64[0]);
+    auto next_blk = [&]() -> uint32_t {
+        while (jb >= cur && rb < B.nblk) {
+            ++rb;
+            jb = 0;
+            cur = rb < B.nblk ? __builtin_amdgcn_readfirstlane(cnt64[rb]) : 0u;
+        }
+        if (rb < B.nblk) last_blk = rb * (kSumTile / 64) + jb++;
+        return last_blk;
+    };
+    double one = 1.0, acc = 0.0;
+    if (nb > 0) {
+        double X[kSumRing][4];
+#pragma unroll
+        for (int r = 0; r < kSumRing; ++r) {
+            const double* vb = items + 64ull * next_blk();
+#pragma unroll
+            for (int q = 0; q < 4; ++q) X[r][q] = vb[16 * q + c];
+        }
+        for (uint32_t b0 = 0; b0 < nb; b0 += kSumRing) {
+#pragma unroll
+            for (int r = 0; r < kSumRing; ++r) {
+                if (b0 + r < nb) {
+                    MCGPU_SUM_ROW(acc, X[r][0], one);
+                    MCGPU_SUM_ROW(acc, X[r][1], one);
+                    MCGPU_SUM_ROW(acc, X[r][2], one);
+                    MCGPU_SUM_ROW(acc, X[r][3], one);
+                    if (lane == 0) ckpt[b0 + r] = acc;
+                }
+                const double* vb = items + 64ull * next_blk();
+#pragma unroll
+                for (int q = 0; q < 4; ++q) X[r][q] = vb[16 * q + c];
+            }
+        }
+    }
+    if (lane == 0) P->sum[m][j] = acc;
+}
+
+// one thread per 16 items: replay from the checkpoint before their 64-item
+// block (at most 48 unchecked adds), then check each run's start value in
+// the 16 against its interval, and the block's last row against the next
+// checkpoint
+__global__ void __launch_bounds__(256) k_sum_verify(SumBufs B)
+{
+    const int ch = blockIdx.y;
+    if (!sum_chain_active(B, ch)) return;
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 4 * B.nb64[ch]) return;
+    const uint32_t k = r / 4, row = r % 4;
+    const uint64_t cb = (uint64_t)ch * B.lcap;
+    const uint64_t base = (uint64_t)ch * B.nblk * kSumTile + 64ull * B.list[cb + k];
+    double v = k ? B.ckpt[cb + k - 1] : 0.0;
+    for (uint32_t i = 0; i < 16 * row; ++i) v = v + B.items[base + i];
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const uint64_t at = base + 16 * row + i;
+        const double l = B.lo[at], h = B.hi[at], x = B.items[at];
+        ok &= (l <= v) & (v <= h);
+        v = v + x;
+    }
+    if (row == 3) ok &= __double_as_longlong(v) == __double_as_longlong(B.ckpt[cb + k]);
+    if (!ok) B.fail[ch] = 1u;
 }
 
 // Batches: the 6 chains of every pair (pair p, mesh m, coordinate j is chain

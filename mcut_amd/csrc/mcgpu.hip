@@ -35,11 +35,11 @@ struct Buf {
 // slower.  B and C also get disjoint CU sets: sharing all of them, the tree
 // build's small launches queue behind the edge numbering's large hash grids
 // and the sort stretches from 0.12 to 0.25 ms.  B takes kBroadShare of the
-// CUs left after the sums, the split that balanced the three paths on the
-// 256-CU part (B 144 CUs, C 104).  A joins B before the exact pair filter and
+// CUs left after the sums (174 of 248 on the 256-CU part): it is the longest
+// of the three paths, and the edge numbering finishes well inside it.  A joins B before the exact pair filter and
 // C before the edge-face tests.
 constexpr int kSumCUs = 8;
-constexpr double kBroadShare = 0.58;
+constexpr double kBroadShare = 0.70;
 constexpr uint32_t kBatchSumWaves = 128; // persistent waves of the batched serial sums (on the reserved CUs)
 enum Stage {
     ST_SUMS,      // A: serial coordinate sums and raw boxes (k_mesh_sums alone)
@@ -63,7 +63,8 @@ const char* kStageNames[ST_COUNT] = {"coordinate_sums", "shift_params", "shift_b
 
 // event slots: stage boundaries on the three streams
 enum Ev { EV_A0, EV_S0, EV_A_SUMS, EV_A_PARAMS, EV_A_SHIFT, EV_A_JOIN, EV_A_FILTER, EV_A_PLANES, EV_A_JOIN2, EV_A_TESTS, EV_A_NARROW,
-          EV_A_END, EV_FORK, EV_B0, EV_B_SORT, EV_B_REFIT, EV_B_TRAV, EV_C0, EV_C_FILL, EV_C_TOPO, EV_COUNT };
+          EV_A_END, EV_FORK, EV_B0, EV_B_SORT, EV_B_REFIT, EV_B_TRAV, EV_C0, EV_C_FILL, EV_C_TOPO, EV_SUM_ITEMS,
+          EV_SUM_CHAIN, EV_COUNT };
 
 uint64_t next_pow2(uint64_t x)
 {
@@ -120,6 +121,8 @@ struct mcgpu_context_t {
     Buf shkeys, shcnt;
     Buf pqs, prs, pcounts; // predicate stage: per-test signs, stage counts
     Buf scan_tmp, scan_tmp_c, hist;
+    Buf sbsum, sitems, slo, shi, snitem, slist, sckpt, sflags; // coordinate sums from run items
+    uint32_t sum_nblk = 0;
 
     uint64_t cap_pairs = 0, cap_tests = 0, cap_esc = 0, cap_reg = 0, cap_big = 0;
 
@@ -342,6 +345,20 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
     CK(ensure(ctx->planes, F, &pl));
     (void)p;
     CK(ensure(ctx->scan_tmp, scan_need + 1024, &u));
+    if (!ctx->batch) { // coordinate sums from run items (k_sum_blocks .. k_sum_verify), for the chains computed here
+        const uint64_t nmax = std::max<uint64_t>(ctx->res ? 0 : ctx->nv[0], ctx->nv[1]);
+        const uint32_t nblk = (uint32_t)std::max<uint64_t>(1, (nmax + kSumTile - 1) / kSumTile);
+        const uint64_t region = 6ull * nblk * kSumTile, lcap = (uint64_t)nblk * (kSumTile / 64);
+        ctx->sum_nblk = nblk;
+        CK(ensure(ctx->sbsum, 6ull * nblk, &dp));
+        CK(ensure(ctx->sitems, region, &dp));
+        CK(ensure(ctx->slo, region, &dp));
+        CK(ensure(ctx->shi, region, &dp));
+        CK(ensure(ctx->snitem, 6ull * nblk, &u));
+        CK(ensure(ctx->slist, 6 * lcap, &u));
+        CK(ensure(ctx->sckpt, 6 * lcap, &dp));
+        CK(ensure(ctx->sflags, 16, &u));
+    }
     // the packed edge numbering scans both meshes' first-use flags as one array
     const uint64_t K2 = ctx->ncorner[0] + ctx->ncorner[1];
     CK(ensure(ctx->eflag[0], K2 + 8, &u));
@@ -396,6 +413,27 @@ mcgpu_result alloc_list_buffers(mcgpu_context ctx)
     const uint64_t need = std::max<uint64_t>(scan_tmp_elems(radix_hist_elems(sort_cap)), scan_tmp_elems(K + 8));
     CK(ensure(ctx->scan_tmp, need + 1024, &u));
     return MCGPU_SUCCESS;
+}
+
+SumBufs sum_bufs(mcgpu_context ctx)
+{
+    SumBufs B;
+    B.v[0] = ctx->res ? nullptr : ctx->dv[0];
+    B.v[1] = ctx->dv[1];
+    B.n[0] = ctx->nv[0];
+    B.n[1] = ctx->nv[1];
+    B.nblk = ctx->sum_nblk;
+    B.bsum = P_<double>(ctx->sbsum);
+    B.items = P_<double>(ctx->sitems);
+    B.lo = P_<double>(ctx->slo);
+    B.hi = P_<double>(ctx->shi);
+    B.nitem = P_<uint32_t>(ctx->snitem);
+    B.list = P_<uint32_t>(ctx->slist);
+    B.lcap = ctx->sum_nblk * (kSumTile / 64);
+    B.ckpt = P_<double>(ctx->sckpt);
+    B.fail = P_<uint32_t>(ctx->sflags);
+    B.nb64 = P_<uint32_t>(ctx->sflags) + 8;
+    return B;
 }
 
 // one packed word per edge (frontend_kernels.hpp, TopoMesh) when every vertex
@@ -663,22 +701,37 @@ void enqueue_frontend(mcgpu_context ctx)
 
     // ---- stream A, enqueued first: the serial sums start at once ----
     (void)hipEventRecord(ctx->ev[EV_FORK], s);
-    (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_FORK], 0);
-    (void)hipEventRecord(ctx->ev[EV_S0], ctx->ssum);
     const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
     Params* Pb = P_<Params>(ctx->bparams);
     const auto* brbox = (const unsigned long long(*)[2][6])ctx->brbox.p;
     if (ctx->batch) {
+        (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_FORK], 0);
+        (void)hipEventRecord(ctx->ev[EV_S0], ctx->ssum);
         const uint32_t nch = 6 * ctx->npairs;
         k_batch_sums<<<std::min<uint32_t>(nch, kBatchSumWaves), 64, 0, ctx->ssum>>>(
             ctx->dv[0], ctx->dv[1], P_<BatchPair>(ctx->bpairs), P_<uint32_t>(ctx->border), nch, C, Pb);
-    } else if (ctx->res) { // the resident source's sums are set already
-        k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(nullptr, 0, ctx->dv[1], ctx->nv[1], &C->zero, Pd);
     } else {
-        k_mesh_sums<<<6, 64, 0, ctx->ssum>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], &C->zero, Pd);
+        // run items on stream A (every CU), the chain and its check on the
+        // reserved CUs, the serial sum only for a coordinate that failed
+        (void)hipEventRecord(ctx->ev[EV_S0], s);
+        const SumBufs SB = sum_bufs(ctx);
+        const dim3 grid(SB.nblk, 6);
+        k_sum_blocks<<<grid, kSumThreads, 0, s>>>(SB);
+        k_sum_items<<<grid, kSumThreads, 0, s>>>(SB);
+        (void)hipEventRecord(ctx->ev[EV_SUM_ITEMS], s);
+        (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_SUM_ITEMS], 0);
+        k_sum_chain<<<6, 64, 4 * SB.nblk, ctx->ssum>>>(SB, Pd);
+        (void)hipEventRecord(ctx->ev[EV_SUM_CHAIN], ctx->ssum);
+        (void)hipStreamWaitEvent(s, ctx->ev[EV_SUM_CHAIN], 0);
+        k_sum_verify<<<dim3(4 * SB.nblk * (kSumTile / 64) / 256, 6), 256, 0, s>>>(SB);
+        k_mesh_sums<<<6, 64, 0, s>>>(ctx->res ? nullptr : ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], &C->zero, Pd,
+                                     SB.fail);
+        (void)hipEventRecord(ctx->ev[EV_A_SUMS], s);
     }
-    (void)hipEventRecord(ctx->ev[EV_A_SUMS], ctx->ssum);
-    (void)hipStreamWaitEvent(s, ctx->ev[EV_A_SUMS], 0);
+    if (ctx->batch) {
+        (void)hipEventRecord(ctx->ev[EV_A_SUMS], ctx->ssum);
+        (void)hipStreamWaitEvent(s, ctx->ev[EV_A_SUMS], 0);
+    }
     if (ctx->batch) {
         k_batch_shift_params<<<grid_for(ctx->npairs), kBlock, 0, s>>>(Pb, brbox, P_<BatchPair>(ctx->bpairs), ctx->npairs,
                                                                       ctx->gp_constant, absolute ? 1 : 0);
@@ -1031,7 +1084,8 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
                   &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->scan_tmp_c, &ctx->hist,
                   &ctx->re_in, &ctx->rg_in, &ctx->bpairs, &ctx->bstarts, &ctx->fpair, &ctx->vpair, &ctx->bparams,
                   &ctx->brbox, &ctx->pflags, &ctx->border, &ctx->bidx[0], &ctx->bidx[1], &ctx->rle, &ctx->rlg,
-                  &ctx->rstart, &ctx->rend, &ctx->pqs, &ctx->prs, &ctx->pcounts};
+                  &ctx->rstart, &ctx->rend, &ctx->pqs, &ctx->prs, &ctx->pcounts, &ctx->sbsum,
+                  &ctx->sitems, &ctx->slo, &ctx->shi, &ctx->snitem, &ctx->slist, &ctx->sckpt, &ctx->sflags};
     for (Buf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
@@ -1590,12 +1644,15 @@ uint64_t ctx_bytes(mcgpu_context c)
 {
     uint64_t t = 0;
     const Buf* all[] = {&c->in_v[1], &c->in_idx[1], &c->in_sz[1], &c->off[1], &c->state, &c->ps_v, &c->box,
-                        &c->hkeys[1], &c->hfirst[1], &c->cface[1], &c->eflag[1], &c->eid[1], &c->cedge[1],
+                        &c->hkeys[1], &c->hfirst[1], &c->cface[1], &c->eflag[0], &c->eid[0], &c->eflag[1],
+                        &c->eid[1], &c->cedge[1],
                         &c->etotal, &c->ev0, &c->ev1, &c->ef0, &c->ef1, &c->mcode, &c->mface, &c->mcode2,
                         &c->mface2, &c->nodes[1], &c->rawbox, &c->raw_pairs, &c->pairs, &c->phash, &c->cand,
                         &c->cand_list, &c->planes, &c->big_list, &c->big_scratch, &c->te, &c->tg, &c->esc,
                         &c->rkey, &c->rval, &c->rkey2, &c->rval2, &c->rp, &c->re, &c->rg, &c->rpts, &c->rcount,
-                        &c->re_in, &c->rg_in, &c->shkeys, &c->shcnt, &c->scan_tmp, &c->scan_tmp_c, &c->hist};
+                        &c->re_in, &c->rg_in, &c->shkeys, &c->shcnt, &c->scan_tmp, &c->scan_tmp_c, &c->hist,
+                        &c->sbsum, &c->sitems, &c->slo, &c->shi, &c->snitem, &c->slist, &c->sckpt,
+                        &c->sflags};
     for (const Buf* b : all) t += b->cap;
     return t;
 }
