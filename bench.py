@@ -34,9 +34,9 @@ HBM_PEAK_GBS = 8000.0
 
 
 # stage -> the kernel that carries it (stages of one kernel are timed exactly
-# by the events around that kernel on its stream)
+# by the events around that kernel on its stream).  The coordinate sums are
+# five kernels (run items, chain, verification, serial fallback).
 STAGE_KERNEL = {
-    "coordinate_sums": "k_mesh_sums",
     "traversal": "k_traverse_wave",
     "tests": "k_tests",
     "pair_filter": "k_filter_pairs",
@@ -52,7 +52,7 @@ def stage_bytes(name, n):
     C, CR, T, I, FC = n["pairs"], n["raw_pairs"], n["tests"], n["registry"], n["cand"]
     NODES = n["nodes"]
     table = {
-        # every coordinate read once by the serial sums
+        # every coordinate read once
         "coordinate_sums": 24 * V,
         "shift_params": 0,
         # coordinates read and shifted coordinates written; face lists read,
@@ -65,8 +65,9 @@ def stage_bytes(name, n):
         "bvh_sort": 4 * K + 24 * V + 56 * F + 4 * 16 * F,
         # leaf boxes and leaf order read, node boxes written
         "bvh_refit": 52 * F + 48 * NODES,
-        # source leaf boxes and the cut tree read once, raw pairs written
-        "traversal": 48 * F + 24 * NODES + 8 * CR,
+        # source leaves (float box and face id) and the cut tree (float node
+        # boxes, leaf face ids) read once, raw pairs written
+        "traversal": 28 * n["Fs"] + 24 * n["nodes_cut"] + 4 * (F - n["Fs"]) + 8 * CR,
         # raw pairs and both shifted boxes read; pairs, pair hash and
         # candidate faces written
         "pair_filter": CR * (8 + 96) + 16 * C + 4 * FC,
@@ -369,7 +370,7 @@ def main():
     nodes = sum((2 << max(0, int(np.ceil(np.log2(f))))) - 1 for f in (Fs, Fc))
     n = dict(V=src[0].shape[0] + cut[0].shape[0], F=Fs + Fc, K=src[1].size + cut[1].size, E=counts["edges"],
              pairs=counts["pairs"], raw_pairs=counts["raw_pairs"], tests=counts["tests"], registry=I,
-             cand=counts["cand"], nodes=nodes)
+             cand=counts["cand"], nodes=nodes, Fs=Fs, nodes_cut=(2 << max(0, int(np.ceil(np.log2(Fc))))) - 1)
     # the roofline kernel: the longest stage that one kernel carries alone
     dom = max((k for k in stages if k in STAGE_KERNEL), key=stages.get)
     dom_bytes = stage_bytes(dom, n)
