@@ -81,22 +81,36 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_tiles(uint32_t* __restrict_
     if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
-// pass 3: scan inside each tile, offset by the tile's prefix
+// pass 3: scan inside each tile, offset by the tile's prefix.  The tile goes
+// through LDS so that global loads and stores are coalesced; each thread scans
+// kScanItems consecutive elements there (stride kScanItems + 1: no bank
+// conflicts).
 __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                                                            const uint32_t* __restrict__ tile_sums, uint64_t n)
 {
-    // each thread owns kScanItems consecutive elements
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    __shared__ uint32_t t[kScanBlock * (kScanItems + 1)];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    for (int i = 0; i < kScanItems; ++i) {
+        const uint32_t k = (uint32_t)i * kScanBlock + threadIdx.x;
+        t[(k / kScanItems) * (kScanItems + 1) + k % kScanItems] = base + k < n ? in[base + k] : 0u;
+    }
+    __syncthreads();
+    uint32_t* mine = t + threadIdx.x * (kScanItems + 1);
     uint32_t v[kScanItems];
     uint32_t s = 0;
     for (int i = 0; i < kScanItems; ++i) {
-        v[i] = (base + i < n) ? in[base + i] : 0;
+        v[i] = mine[i];
         s += v[i];
     }
     uint32_t run = block_exclusive_scan(s, nullptr) + tile_sums[blockIdx.x];
     for (int i = 0; i < kScanItems; ++i) {
-        if (base + i < n) out[base + i] = run;
+        mine[i] = run;
         run += v[i];
+    }
+    __syncthreads();
+    for (int i = 0; i < kScanItems; ++i) {
+        const uint32_t k = (uint32_t)i * kScanBlock + threadIdx.x;
+        if (base + k < n) out[base + k] = t[(k / kScanItems) * (kScanItems + 1) + k % kScanItems];
     }
 }
 
