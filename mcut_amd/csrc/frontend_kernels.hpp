@@ -1080,12 +1080,19 @@ __global__ void k_morton_raw(MortonMesh M0, MortonMesh M1, uint32_t start, uint3
             c[j] = expand_bits10((uint32_t)x);
         }
         code[g] = (c[0] * 4 + c[1] * 2 + c[2]) | M.mesh_bit;
-        // single precision, rounded outward: a float box contains the double
-        // box, so the superset property carries over
+        // single precision, rounded outward, relative to the source mesh's raw
+        // lower corner (a resident source keeps its tree in that frame): far
+        // from the origin float32 would otherwise merge whole meshes into one
+        // box.  Each of the three double operations errs by at most 2^-53 of
+        // its result; pushing the bound out by 2^-48 of |x - origin| + tol
+        // covers them, so the float box still contains the exact box and the
+        // superset property carries over.
         for (int j = 0; j < 3; ++j) {
             const double t = M.widen ? P->tol[j] : 0.0;
-            o[j] = __double2float_rd(mn[j] - t);
-            o[3 + j] = __double2float_ru(mx[j] + t);
+            const double org = ord_value(M0.rbox[j]);
+            const double dlo = mn[j] - org, dhi = mx[j] - org;
+            o[j] = __double2float_rd((dlo - t) - (fabs(dlo) + t) * 3.552713678800501e-15);
+            o[3 + j] = __double2float_ru((dhi + t) + (fabs(dhi) + t) * 3.552713678800501e-15);
         }
     }
 }

@@ -891,6 +891,19 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         if (C.ov_esc) ctx->cap_esc = std::max<uint64_t>(2 * ctx->cap_esc, C.n_esc + C.n_esc / 4);
         if (C.ov_reg) ctx->cap_reg = std::max<uint64_t>(2 * ctx->cap_reg, C.n_reg + C.n_reg / 4);
         if (C.ov_big) ctx->cap_big = std::max<uint64_t>(2 * ctx->cap_big, C.n_big + C.n_big / 4);
+        // the grown lists must fit the device: a raw pair takes up to 48 bytes
+        // (raw list, pair list, pair hash of twice the capacity rounded up to a
+        // power of two), a test 10, an escalated test 4, a registry entry 100.
+        // Raw boxes rounded to float32 can overlap wholesale when coordinates
+        // sit far from the origin relative to the mesh (2^52 + a unit sphere:
+        // every face pair), and a quadratic pair count is refused here rather
+        // than allocated.
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return MCGPU_ERROR_DEVICE;
+        const long double need = 48.0L * ctx->cap_pairs + 10.0L * ctx->cap_tests + 4.0L * ctx->cap_esc +
+                                 100.0L * ctx->cap_reg;
+        if (need > 0.9L * (long double)(free_b + ctx->raw_pairs.cap + ctx->pairs.cap + ctx->phash.cap))
+            return MCGPU_ERROR_OUT_OF_MEMORY;
         return MCGPU_SUCCESS;
     }
     if (C.unsupported) return MCGPU_ERROR_UNSUPPORTED;
