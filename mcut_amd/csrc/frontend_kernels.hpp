@@ -1603,6 +1603,14 @@ __device__ __forceinline__ uint64_t pair_hash_slots(const Counters* C, uint64_t 
     return m < table ? m : table;
 }
 
+// the whole table, for a dispatch whose stream A is idle before the join: the
+// filter then sizes the hash itself (pair_hash_slots) and skips the init
+__global__ void k_pair_hash_clear(unsigned long long* __restrict__ phash, uint64_t table)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < table; i += (uint64_t)gridDim.x * blockDim.x)
+        phash[i] = kEmpty64;
+}
+
 __global__ void k_pair_hash_init(unsigned long long* __restrict__ phash, uint64_t cap, uint64_t table, Counters* C)
 {
     const uint64_t m = pair_hash_slots(C, cap, table);
@@ -1625,14 +1633,17 @@ __global__ void __launch_bounds__(kStageBlock) k_filter_pairs(const double* __re
                                                          unsigned long long* __restrict__ pairs,
                                                          unsigned long long* __restrict__ phash,
                                                          uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_list,
-                                                         Counters* C)
+                                                         uint64_t table, Counters* C)
 {
     __shared__ unsigned long long st_pair[kFilterTile];
     __shared__ uint32_t st_face[2 * kFilterTile];
     __shared__ uint32_t st_np, st_nf;
     __shared__ unsigned long long st_base;
     const uint64_t n = C->n_raw < cap ? C->n_raw : cap;
-    const uint64_t pmask = C->pmask;
+    // table > 0: the table was cleared whole before the join, and the hash's
+    // size follows the raw pair count here
+    const uint64_t pmask = table ? pair_hash_slots(C, cap, table) - 1 : C->pmask;
+    if (table && blockIdx.x == 0 && threadIdx.x == 0) C->pmask = pmask;
     for (uint64_t t0 = (uint64_t)blockIdx.x * kFilterTile; t0 < n; t0 += (uint64_t)gridDim.x * kFilterTile) {
         if (threadIdx.x == 0) st_np = st_nf = 0;
         __syncthreads();
@@ -2341,9 +2352,26 @@ __global__ void __launch_bounds__(kRegSortThreads) k_reg_tile_sort(const unsigne
         K[r] = i < cnt ? key[t0 + i] : ~0ull; // padding: all digits 0xff, after the real keys (stable)
         V[r] = i < cnt ? val[t0 + i] : 0u;
     }
+    // passes over the bits the tile's keys use (at most kbits): the padding's
+    // all-ones low bits still sort it last
+    __shared__ unsigned long long wor[kRegWaves];
+    {
+        unsigned long long o = 0;
+#pragma unroll
+        for (int r = 0; r < kRegPer; ++r)
+            if (r * kRegSortThreads + t < cnt) o |= K[r];
+        for (int q = 32; q > 0; q >>= 1) o |= __shfl_xor(o, q, 64);
+        if (lane == 0) wor[wid] = o;
+    }
+    __syncthreads();
+    unsigned long long all = 0;
+#pragma unroll
+    for (int w = 0; w < kRegWaves; ++w) all |= wor[w];
+    const int used = all ? 64 - __clzll((long long)all) : 1;
+    const int nbits = used < kbits ? used : kbits;
     // the prefix step: thread t sums digit dg over the waves of row pr
     const uint32_t dg = t & (kRadix - 1), pr = t >> kRadixBits;
-    for (int shift = 0; shift < kbits; shift += kRadixBits) {
+    for (int shift = 0; shift < nbits; shift += kRadixBits) {
         for (uint32_t q = t; q < (uint32_t)(kRegPer * kRegWaves * kRadix); q += kRegSortThreads) (&pre[0][0][0])[q] = 0;
         __syncthreads();
         uint32_t d[kRegPer], rk[kRegPer];

@@ -701,6 +701,10 @@ void enqueue_frontend(mcgpu_context ctx)
 
     // ---- stream A, enqueued first: the serial sums start at once ----
     (void)hipEventRecord(ctx->ev[EV_FORK], s);
+    // a single dispatch on its own four streams clears the whole pair hash
+    // early; batches (huge tables) and serving slots (one stream) size it
+    // after the traversal
+    const bool early_clear = ctx->own_streams && !ctx->batch;
     const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
     Params* Pb = P_<Params>(ctx->bparams);
     const auto* brbox = (const unsigned long long(*)[2][6])ctx->brbox.p;
@@ -713,6 +717,10 @@ void enqueue_frontend(mcgpu_context ctx)
     } else {
         // run items on stream A (every CU), the chain and its check on the
         // reserved CUs, the serial sum only for a coordinate that failed
+        if (early_clear) { // the pair hash, before the sum kernels (its fill would slow the sum chain beside it)
+            const uint64_t table = next_pow2(2 * ctx->cap_pairs);
+            k_pair_hash_clear<<<grid_for(table, kBlock, 2048), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash), table);
+        }
         (void)hipEventRecord(ctx->ev[EV_S0], s);
         const SumBufs SB = sum_bufs(ctx);
         const dim3 grid(SB.nblk, 6);
@@ -720,6 +728,7 @@ void enqueue_frontend(mcgpu_context ctx)
         k_sum_items<<<grid, kSumThreads, 0, s>>>(SB);
         (void)hipEventRecord(ctx->ev[EV_SUM_ITEMS], s);
         (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_SUM_ITEMS], 0);
+
         k_sum_chain<<<6, 64, 4 * SB.nblk, ctx->ssum>>>(SB, Pd);
         (void)hipEventRecord(ctx->ev[EV_SUM_CHAIN], ctx->ssum);
         (void)hipStreamWaitEvent(s, ctx->ev[EV_SUM_CHAIN], 0);
@@ -849,14 +858,16 @@ void enqueue_frontend(mcgpu_context ctx)
     (void)hipStreamWaitEvent(s, ctx->ev[EV_C_FILL], 0);
     (void)hipEventRecord(ctx->ev[EV_A_JOIN], s);
     const uint64_t table = next_pow2(2 * ctx->cap_pairs);
-    k_pair_hash_init<<<grid_for(table, kBlock, 1024), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash), ctx->cap_pairs,
-                                                                      table, C);
+    if (!early_clear)
+        k_pair_hash_init<<<grid_for(table, kBlock, 1024), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash),
+                                                                          ctx->cap_pairs, table, C);
     k_filter_pairs<<<grid_for(ctx->cap_pairs, kFilterTile, 2048), kStageBlock, 0, s>>>(P_<double>(ctx->box), P_<unsigned long long>(ctx->raw_pairs),
                                                                ctx->cap_pairs, Fs,
                                                                ctx->batch ? P_<uint32_t>(ctx->fpair) : nullptr,
                                                                P_<unsigned long long>(ctx->pairs),
                                                                P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->cand),
-                                                               P_<uint32_t>(ctx->cand_list), C);
+                                                               P_<uint32_t>(ctx->cand_list), early_clear ? table : 0,
+                                                               C);
     (void)hipEventRecord(ctx->ev[EV_A_FILTER], s);
     enqueue_narrow(ctx, S, true);
 }
@@ -1357,7 +1368,7 @@ MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_bo
         fill_u32(ctx->cand.p, 0u, F, s);
         k_filter_pairs<<<grid_for(ctx->cap_pairs, kFilterTile, 2048), kStageBlock, 0, s>>>(
             dp, P_<unsigned long long>(ctx->raw_pairs), ctx->cap_pairs, num_src, nullptr, P_<unsigned long long>(ctx->pairs),
-            P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->cand), P_<uint32_t>(ctx->cand_list), C);
+            P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->cand), P_<uint32_t>(ctx->cand_list), 0, C);
         (void)hipEventRecord(ctx->ev[EV_A_END], s);
         CK(hipStreamSynchronize(s));
         CK(hipGetLastError());
