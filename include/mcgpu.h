@@ -298,8 +298,14 @@ MCGPU_API mcgpu_result mcgpu_predicate_stage(mcgpu_context ctx, uint64_t* counts
                                              uint64_t capacity, float* device_ms);
 
 /* Per-stage device times of the last dispatch in ms (stage names via
- * mcgpu_stage_name); returns the number of stages written. */
+ * mcgpu_stage_name); returns the number of stages written, 0 when that
+ * dispatch ran without stage timing. */
 MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity);
+
+/* Record the per-stage events in this context's later dispatches (off by
+ * default: thirteen recorded events cost about 50 us a dispatch).  No
+ * counterpart in the reference; for profiling callers. */
+MCGPU_API mcgpu_result mcgpu_set_stage_timing(mcgpu_context ctx, int enable);
 MCGPU_API const char* mcgpu_stage_name(int stage);
 
 #ifdef __cplusplus

@@ -146,6 +146,8 @@ struct mcgpu_context_t {
     float stage_ms[ST_COUNT];
     bool stages_ready = false;
     int stage_mode = 0; // 0: full dispatch (three streams), 1: perturbed narrow-phase rerun
+    bool stage_timing = false; // record the per-stage events (mcgpu_set_stage_timing)
+    bool last_timed = false;   // the last dispatch recorded them
     float total_ms = 0.f;
     int sort_bits = 22;  // leaf key bits
 
@@ -296,6 +298,15 @@ Soup make_soup(mcgpu_context ctx)
 }
 
 // buffers whose size depends only on the meshes
+// a stage-boundary event: recorded only when stage timing is on.  Recorded
+// events cost host and device time (about 50 us a dispatch for thirteen of
+// them on the icosphere-7 pair); the events that order the streams and
+// EV_A0 / EV_A_END (device_ms) are always recorded.
+void stage_event(mcgpu_context ctx, int e, hipStream_t st)
+{
+    if (ctx->stage_timing) (void)hipEventRecord(ctx->ev[e], st);
+}
+
 mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
 {
     const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
@@ -458,13 +469,13 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
                                              P_<uint32_t>(ctx->big_list), ctx->cap_big, C);
     k_big_polygons<<<4, 64, 0, s>>>(S, P_<uint32_t>(ctx->big_list), C, ctx->cap_big, P_<Plane>(ctx->planes),
                                      P_<Trip>(ctx->big_scratch), C);
-    (void)hipEventRecord(ctx->ev[EV_A_PLANES], s);
+    stage_event(ctx, EV_A_PLANES, s);
     if (join_topology) (void)hipStreamWaitEvent(s, ctx->ev[EV_C_TOPO], 0);
-    (void)hipEventRecord(ctx->ev[EV_A_JOIN2], s);
+    stage_event(ctx, EV_A_JOIN2, s);
     k_tests<<<grid_for(6 * ctx->cap_pairs, kTestTile, 2048), kStageBlock, 0, s>>>(S, P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
                                                         P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->te),
                                                         P_<uint32_t>(ctx->tg), ctx->cap_tests, C);
-    (void)hipEventRecord(ctx->ev[EV_A_TESTS], s);
+    stage_event(ctx, EV_A_TESTS, s);
     TestOut TO{nullptr, nullptr, nullptr, nullptr};
     if (ctx->keep_tests)
         TO = TestOut{P_<int8_t>(ctx->tqs), P_<int8_t>(ctx->trs), P_<char>(ctx->ttype), P_<char>(ctx->tpip)};
@@ -477,7 +488,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
     k_narrow_exact<<<grid_for(ctx->cap_esc, 64, 2048), 64, 0, s>>>(S, P_<Plane>(ctx->planes), P_<uint32_t>(ctx->te),
                                                                    P_<uint32_t>(ctx->tg), P_<uint32_t>(ctx->esc),
                                                                    ctx->cap_esc, TO, R, C);
-    (void)hipEventRecord(ctx->ev[EV_A_NARROW], s);
+    stage_event(ctx, EV_A_NARROW, s);
     unsigned long long* rcount = P_<unsigned long long>(ctx->rcount);
     const uint64_t hs = next_pow2(4 * ctx->cap_reg);
     unsigned long long* hk = P_<unsigned long long>(ctx->shkeys);
@@ -529,6 +540,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
 // rejection after the dispatch.
 void enqueue_validate(mcgpu_context ctx)
 {
+    ctx->last_timed = ctx->stage_timing;
     hipStream_t s = ctx->s;
     Counters* C = &P_<DevState>(ctx->state)->C;
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
@@ -559,6 +571,7 @@ void enqueue_validate(mcgpu_context ctx)
 // mesh), each pair's broad-phase tolerance, and the cell layout.
 void enqueue_validate_batch(mcgpu_context ctx)
 {
+    ctx->last_timed = ctx->stage_timing;
     hipStream_t s = ctx->s;
     Counters* C = &P_<DevState>(ctx->state)->C;
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
@@ -639,7 +652,7 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C)
     ctx->mface_alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode) + o0, P_<uint32_t>(ctx->mface) + o0,
                                           P_<uint32_t>(ctx->mcode2) + o0, P_<uint32_t>(ctx->mface2) + o0, nullptr,
                                           F - o0, F - o0, ctx->sort_bits, rt, st);
-    (void)hipEventRecord(ctx->ev[EV_B_SORT], st);
+    stage_event(ctx, EV_B_SORT, st);
     const uint32_t* sorted = P_<uint32_t>(ctx->mface_alt ? ctx->mface2 : ctx->mface);
     Tree T[2];
     RefitTree R[2];
@@ -684,7 +697,7 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C)
             if (R[m].lin == 0) R[m].lin = -1;
         }
     }
-    (void)hipEventRecord(ctx->ev[EV_B_REFIT], st);
+    stage_event(ctx, EV_B_REFIT, st);
     const unsigned nwaves = (Fs + kWaveLeaves - 1) / kWaveLeaves;
     k_traverse_wave<<<nwaves < 65536 ? nwaves : 65536, 64, 0, st>>>(T[0], T[1], P_<unsigned long long>(ctx->raw_pairs),
                                                                     ctx->cap_pairs, C);
@@ -710,7 +723,7 @@ void enqueue_frontend(mcgpu_context ctx)
     const auto* brbox = (const unsigned long long(*)[2][6])ctx->brbox.p;
     if (ctx->batch) {
         (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_FORK], 0);
-        (void)hipEventRecord(ctx->ev[EV_S0], ctx->ssum);
+        stage_event(ctx, EV_S0, ctx->ssum);
         const uint32_t nch = 6 * ctx->npairs;
         k_batch_sums<<<std::min<uint32_t>(nch, kBatchSumWaves), 64, 0, ctx->ssum>>>(
             ctx->dv[0], ctx->dv[1], P_<BatchPair>(ctx->bpairs), P_<uint32_t>(ctx->border), nch, C, Pb);
@@ -721,7 +734,7 @@ void enqueue_frontend(mcgpu_context ctx)
             const uint64_t table = next_pow2(2 * ctx->cap_pairs);
             k_pair_hash_clear<<<grid_for(table, kBlock, 2048), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash), table);
         }
-        (void)hipEventRecord(ctx->ev[EV_S0], s);
+        stage_event(ctx, EV_S0, s);
         const SumBufs SB = sum_bufs(ctx);
         const dim3 grid(SB.nblk, 6);
         k_sum_blocks<<<grid, kSumThreads, 0, s>>>(SB);
@@ -744,22 +757,22 @@ void enqueue_frontend(mcgpu_context ctx)
     if (ctx->batch) {
         k_batch_shift_params<<<grid_for(ctx->npairs), kBlock, 0, s>>>(Pb, brbox, P_<BatchPair>(ctx->bpairs), ctx->npairs,
                                                                       ctx->gp_constant, absolute ? 1 : 0);
-        (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
+        stage_event(ctx, EV_A_PARAMS, s);
         k_batch_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1],
                                                                   P_<uint32_t>(ctx->vpair), Pb, P_<double>(ctx->ps_v));
     } else {
         k_shift_params<<<1, 1, 0, s>>>(Pd, C, ctx->nv[0], ctx->nv[1], ctx->gp_constant, absolute ? 1 : 0, 0, 0.0, 0.0, 0.0);
-        (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
+        stage_event(ctx, EV_A_PARAMS, s);
         k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], Pd,
                                                             P_<double>(ctx->ps_v), 0);
     }
     const Soup S = make_soup(ctx);
     k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, ctx->batch ? Pb : Pd, C, P_<double>(ctx->box), 0);
-    (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
+    stage_event(ctx, EV_A_SHIFT, s);
 
     // ---- stream B: raw-space trees and broad phase ----
     (void)hipStreamWaitEvent(s2, ctx->ev[EV_FORK], 0);
-    (void)hipEventRecord(ctx->ev[EV_B0], s2);
+    stage_event(ctx, EV_B0, s2);
     float* rawbox = P_<float>(ctx->rawbox);
     if (ctx->batch) {
         for (int m = 0; m < 2; ++m) {
@@ -783,7 +796,7 @@ void enqueue_frontend(mcgpu_context ctx)
 
     // ---- stream C: polygon-soup edge numbering, and the tables the filter fills ----
     (void)hipStreamWaitEvent(s3, ctx->ev[EV_FORK], 0);
-    (void)hipEventRecord(ctx->ev[EV_C0], s3);
+    stage_event(ctx, EV_C0, s3);
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
     const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
     fill_u32(ctx->cand.p, 0u, F, s3);
@@ -856,7 +869,7 @@ void enqueue_frontend(mcgpu_context ctx)
     // ---- stream A: join the broad phase, exact filter, narrow phase ----
     (void)hipStreamWaitEvent(s, ctx->ev[EV_B_TRAV], 0);
     (void)hipStreamWaitEvent(s, ctx->ev[EV_C_FILL], 0);
-    (void)hipEventRecord(ctx->ev[EV_A_JOIN], s);
+    stage_event(ctx, EV_A_JOIN, s);
     const uint64_t table = next_pow2(2 * ctx->cap_pairs);
     if (!early_clear)
         k_pair_hash_init<<<grid_for(table, kBlock, 1024), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash),
@@ -868,7 +881,7 @@ void enqueue_frontend(mcgpu_context ctx)
                                                                P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->cand),
                                                                P_<uint32_t>(ctx->cand_list), early_clear ? table : 0,
                                                                C);
-    (void)hipEventRecord(ctx->ev[EV_A_FILTER], s);
+    stage_event(ctx, EV_A_FILTER, s);
     enqueue_narrow(ctx, S, true);
 }
 
@@ -1401,15 +1414,16 @@ MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const doubl
         const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1];
         Counters* C = &P_<DevState>(ctx->state)->C;
         ctx->stage_mode = 1;
+        ctx->last_timed = ctx->stage_timing;
         (void)hipEventRecord(ctx->ev[EV_A0], s);
         k_reset_narrow<<<1, 1, 0, s>>>(C);
         k_set_perturbation<<<1, 1, 0, s>>>(&P_<DevState>(ctx->state)->P, 1, perturbation[0], perturbation[1], perturbation[2]);
-        (void)hipEventRecord(ctx->ev[EV_A_PARAMS], s);
+        stage_event(ctx, EV_A_PARAMS, s);
         k_shift_vertices<<<grid_for(3 * V), kBlock, 0, s>>>(ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1],
                                                             &P_<DevState>(ctx->state)->P, P_<double>(ctx->ps_v), 1);
-        (void)hipEventRecord(ctx->ev[EV_A_SHIFT], s);
-        (void)hipEventRecord(ctx->ev[EV_A_JOIN], s);
-        (void)hipEventRecord(ctx->ev[EV_A_FILTER], s);
+        stage_event(ctx, EV_A_SHIFT, s);
+        stage_event(ctx, EV_A_JOIN, s);
+        stage_event(ctx, EV_A_FILTER, s);
         enqueue_narrow(ctx, make_soup(ctx), false);
         bool overflow = false;
         if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
@@ -1631,11 +1645,18 @@ MCGPU_API mcgpu_result mcgpu_get_ps_vertices(mcgpu_context ctx, double* xyz, uin
 
 MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity)
 {
-    if (!ctx || !ms || !ctx->have || ctx->broad_only) return 0;
+    if (!ctx || !ms || !ctx->have || ctx->broad_only || !ctx->last_timed) return 0;
     if (!ctx->stages_ready) compute_stage_times(ctx);
     int n = std::min(capacity, (int)ST_COUNT);
     for (int i = 0; i < n; ++i) ms[i] = ctx->stage_ms[i];
     return n;
+}
+
+MCGPU_API mcgpu_result mcgpu_set_stage_timing(mcgpu_context ctx, int enable)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    ctx->stage_timing = enable != 0;
+    return MCGPU_SUCCESS;
 }
 
 MCGPU_API const char* mcgpu_stage_name(int stage)

@@ -109,6 +109,7 @@ def lib():
                                               C.POINTER(FrontendInfo), vp, C.POINTER(C.c_uint32)]
         L.mcgpu_predicate_stage.argtypes = [vp, vp, vp, vp, u64, C.POINTER(C.c_float)]
         L.mcgpu_stage_times.argtypes = [vp, vp, C.c_int]
+        L.mcgpu_set_stage_timing.argtypes = [vp, C.c_int]
         L.mcgpu_stage_name.argtypes = [C.c_int]
         L.mcgpu_stage_name.restype = C.c_char_p
         for fn in ("mcgpu_create_context", "mcgpu_release_context", "mcgpu_dispatch_frontend",
@@ -117,7 +118,7 @@ def lib():
                    "mcgpu_stage_times", "mcgpu_broad_phase", "mcgpu_dispatch_batch", "mcgpu_get_batch_status",
                    "mcgpu_get_batch_candidate_pairs", "mcgpu_server_create", "mcgpu_server_release",
                    "mcgpu_server_info", "mcgpu_server_run", "mcgpu_server_get_registry", "mcgpu_predicate_stage",
-                   "mcgpu_dispatch_enforced"):
+                   "mcgpu_dispatch_enforced", "mcgpu_set_stage_timing"):
             getattr(L, fn).restype = C.c_int
         _lib = L
     return _lib
@@ -227,7 +228,8 @@ class Result:
         return out
 
     def stage_ms(self):
-        """Device time of each pipeline stage of this dispatch, in ms (HIP events)."""
+        """Device time of each pipeline stage of this dispatch, in ms (HIP events);
+        empty unless the context had stage timing on (Context.set_stage_timing)."""
         ms = np.zeros(16, np.float32)
         n = lib().mcgpu_stage_times(self._ctx.h, _ptr(ms), 16)
         return {name: float(ms[i]) for i, name in enumerate(stage_names()[:n])}
@@ -333,6 +335,10 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_stage_timing(self, on):
+        """Record per-stage events in later dispatches (Result.stage_ms); off by default."""
+        _check(lib().mcgpu_set_stage_timing(self.h, 1 if on else 0), "mcgpu_set_stage_timing")
 
     def dispatch(self, src, cut, flags=0, gp_constant=1e-4, enforcement_count=0, keep_tests=False):
         keep = []

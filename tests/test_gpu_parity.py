@@ -260,3 +260,17 @@ def test_broad_phase_mesh_boxes(ctx):
     b = face_boxes(cut[1], ns, O["eps"])
     got = ctx.broad_phase(a, b)
     assert np.array_equal(got, O["pairs"])
+
+
+def test_stage_timing_switch(ctx):
+    """Stage events only when asked for: none by default, all twelve stages after
+    mcgpu_set_stage_timing, none again once it is off."""
+    src, cut = meshes.icosphere_pair(4)
+    assert ctx.dispatch(src, cut).stage_ms() == {}
+    ctx.set_stage_timing(True)
+    try:
+        st = ctx.dispatch(src, cut).stage_ms()
+    finally:
+        ctx.set_stage_timing(False)
+    assert len(st) == 12 and all(v >= 0.0 for v in st.values()) and st["traversal"] > 0.0
+    assert ctx.dispatch(src, cut).stage_ms() == {}
