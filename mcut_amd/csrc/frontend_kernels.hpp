@@ -1019,6 +1019,92 @@ __global__ void __launch_bounds__(256) k_raw_bbox(const double* __restrict__ v, 
     }
 }
 
+// Validation of two triangle meshes in three launches instead of nine (every
+// launch before the fork delays the broad phase): the counters reset with the
+// raw-box minima preset in one block, both raw boxes in one reduction (each
+// thread keeps one box per mesh, so a block may straddle the meshes), and both
+// meshes' faces checked with their offset tables written in one pass.
+__global__ void __launch_bounds__(256) k_reset_counters(Counters* C, int first_mesh)
+{
+    uint32_t* w = reinterpret_cast<uint32_t*>(C);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(sizeof(Counters) / 4); i += blockDim.x) w[i] = 0u;
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int m = (int)threadIdx.x / 3, j = (int)threadIdx.x % 3;
+        if (m >= first_mesh) C->rbox[m][j] = ~0ull;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_raw_bbox2(const double* __restrict__ v0, uint32_t n0, const double* __restrict__ v1,
+                                                   uint32_t n1, Counters* C)
+{
+    __shared__ double red[4][2][6];
+    double lo[2][3], hi[2][3];
+    for (int m = 0; m < 2; ++m)
+        for (int j = 0; j < 3; ++j) {
+            lo[m][j] = 1.7976931348623157e308;
+            hi[m][j] = -1.7976931348623157e308;
+        }
+    const uint64_t n = (uint64_t)n0 + n1;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int m = i >= n0;
+        const double* p = m ? v1 + 3 * (i - n0) : v0 + 3 * i;
+        for (int j = 0; j < 3; ++j) {
+            const double x = p[j];
+            lo[m][j] = x < lo[m][j] ? x : lo[m][j];
+            hi[m][j] = x > hi[m][j] ? x : hi[m][j];
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int m = 0; m < 2; ++m)
+            for (int j = 0; j < 3; ++j) {
+                const double a = __shfl_xor(lo[m][j], o, 64), b = __shfl_xor(hi[m][j], o, 64);
+                lo[m][j] = a < lo[m][j] ? a : lo[m][j];
+                hi[m][j] = b > hi[m][j] ? b : hi[m][j];
+            }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int m = 0; m < 2; ++m)
+            for (int j = 0; j < 3; ++j) {
+                red[w][m][j] = lo[m][j];
+                red[w][m][3 + j] = hi[m][j];
+            }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int m = (int)threadIdx.x / 3, j = (int)threadIdx.x % 3;
+        if (m ? n1 : n0) {
+            double a = red[0][m][j], b = red[0][m][3 + j];
+            for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+                a = red[q][m][j] < a ? red[q][m][j] : a;
+                b = red[q][m][3 + j] > b ? red[q][m][3 + j] : b;
+            }
+            atomicMin(&C->rbox[m][j], ord_key(a));
+            atomicMax(&C->rbox[m][3 + j], ord_key(b));
+        }
+    }
+}
+
+// faces of both triangle meshes (nf0 = 0 skips the source): off[f] = 3 f,
+// every vertex index below its mesh's count, no vertex twice in a face
+__global__ void k_validate_tri2(const uint32_t* __restrict__ idx0, uint32_t* __restrict__ off0, uint32_t nf0, uint32_t nv0,
+                                const uint32_t* __restrict__ idx1, uint32_t* __restrict__ off1, uint32_t nf1, uint32_t nv1,
+                                Counters* C)
+{
+    const uint64_t n = (uint64_t)nf0 + nf1;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < n; g += (uint64_t)gridDim.x * blockDim.x) {
+        const int m = g >= nf0;
+        const uint32_t f = (uint32_t)(m ? g - nf0 : g);
+        uint32_t* off = m ? off1 : off0;
+        const uint32_t nf = m ? nf1 : nf0;
+        off[f] = 3 * f;
+        if (f + 1 == nf) off[nf] = 3 * nf;
+        const uint32_t* idx = m ? idx1 : idx0;
+        const uint32_t nv = m ? nv1 : nv0;
+        const uint32_t a = idx[3 * (uint64_t)f], b = idx[3 * (uint64_t)f + 1], c = idx[3 * (uint64_t)f + 2];
+        if (a >= nv || b >= nv || c >= nv || a == b || b == c || a == c) atomicOr(&C->bad_input, 1u);
+    }
+}
+
 // 21-bit Morton codes (128 cells per axis) of raw face-box centres with the
 // mesh as bit 21, so that one sort orders the leaves of both trees, and the
 // raw face boxes the broad phase runs on.

@@ -545,6 +545,17 @@ void enqueue_validate(mcgpu_context ctx)
     Counters* C = &P_<DevState>(ctx->state)->C;
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
     (void)hipEventRecord(ctx->ev[EV_A0], s);
+    if (!ctx->dsz[0] && !ctx->dsz[1]) { // two triangle meshes: three launches (k_reset_counters)
+        const int m0 = ctx->res ? 1 : 0;
+        k_reset_counters<<<1, kBlock, 0, s>>>(C, m0);
+        if (ctx->res) k_resident_source<<<1, 1, 0, s>>>(C, &P_<DevState>(ctx->state)->P, ctx->res->dres);
+        const uint32_t n0 = m0 ? 0u : ctx->nv[0], f0 = m0 ? 0u : ctx->nf[0];
+        k_raw_bbox2<<<grid_for((uint64_t)n0 + ctx->nv[1], kBlock, 64), kBlock, 0, s>>>(m0 ? nullptr : ctx->dv[0], n0,
+                                                                                         ctx->dv[1], ctx->nv[1], C);
+        k_validate_tri2<<<grid_for((uint64_t)f0 + ctx->nf[1]), kBlock, 0, s>>>(
+            ctx->didx[0], P_<uint32_t>(ctx->off[0]), f0, ctx->nv[0], ctx->didx[1], P_<uint32_t>(ctx->off[1]), ctx->nf[1],
+            ctx->nv[1], C);
+    } else {
     fill_u32(C, 0u, sizeof(Counters) / 4, s);
     if (ctx->res) k_resident_source<<<1, 1, 0, s>>>(C, &P_<DevState>(ctx->state)->P, ctx->res->dres);
     for (int m = 0; m < 2; ++m) {
@@ -559,6 +570,7 @@ void enqueue_validate(mcgpu_context ctx)
             k_triangle_offsets<<<grid_for((uint64_t)ctx->nf[m] + 1), kBlock, 0, s>>>(off, ctx->nf[m]);
         }
         k_validate<<<grid_for(ctx->nf[m]), kBlock, 0, s>>>(ctx->didx[m], off, ctx->dsz[m], ctx->nf[m], ctx->nv[m], C);
+    }
     }
     const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
     k_tolerance<<<1, 1, 0, s>>>(&P_<DevState>(ctx->state)->P, C, ctx->nv[0], ctx->nv[1], ctx->gp_constant,
