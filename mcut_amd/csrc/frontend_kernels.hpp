@@ -1131,8 +1131,11 @@ struct MortonMesh {
 // (start = nf0 when the source's leaves are already made)
 __global__ void k_morton_raw(MortonMesh M0, MortonMesh M1, uint32_t start, uint32_t* __restrict__ code,
                              uint32_t* __restrict__ face, float* __restrict__ rawbox, const Params* __restrict__ P,
-                             const Counters* __restrict__ C)
+                             const Counters* __restrict__ C, uint32_t* __restrict__ zero, uint64_t nzero)
 {
+    // the leaf sort's group totals (radix_ghist), cleared here rather than by a launch of their own
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nzero; i += (uint64_t)gridDim.x * blockDim.x)
+        zero[i] = 0u;
     const bool aborted = dispatch_aborted(C);
     const uint32_t F = M0.nf + M1.nf;
     for (uint64_t g = start + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < F; g += (uint64_t)gridDim.x * blockDim.x) {

@@ -652,7 +652,7 @@ mcgpu_result finish_batch(mcgpu_context ctx, mcgpu_frontend_info* info)
 // (mcode, mface) and single-precision leaf boxes (rawbox) are written: one
 // stable sort orders the leaves of both trees (the mesh is the top key bit),
 // the implicit trees are refitted, and the traversal appends raw pairs.
-void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C)
+void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool ghist_zeroed)
 {
     const uint32_t Fs = ctx->nf[0];
     const uint64_t F = (uint64_t)Fs + ctx->nf[1];
@@ -663,7 +663,7 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C)
     const uint64_t o0 = ctx->res ? Fs : 0;
     ctx->mface_alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode) + o0, P_<uint32_t>(ctx->mface) + o0,
                                           P_<uint32_t>(ctx->mcode2) + o0, P_<uint32_t>(ctx->mface2) + o0, nullptr,
-                                          F - o0, F - o0, ctx->sort_bits, rt, st);
+                                          F - o0, F - o0, ctx->sort_bits, rt, st, ghist_zeroed);
     stage_event(ctx, EV_B_SORT, st);
     const uint32_t* sorted = P_<uint32_t>(ctx->mface_alt ? ctx->mface2 : ctx->mface);
     Tree T[2];
@@ -801,10 +801,13 @@ void enqueue_frontend(mcgpu_context ctx)
             M[m] = MortonMesh{ctx->dv[m], ctx->didx[m], P_<uint32_t>(ctx->off[m]), C->rbox[m], ctx->nf[m],
                               m ? (1u << 21) : 0u, m};
         const uint32_t start = ctx->res ? Fs : 0u;
+        uint64_t nzero = 0;
+        uint32_t* zero = radix_ghist(RadixTemp{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)}, F - start,
+                                     ctx->sort_bits, &nzero);
         k_morton_raw<<<grid_for(F - start), kBlock, 0, s2>>>(M[0], M[1], start, P_<uint32_t>(ctx->mcode),
-                                                             P_<uint32_t>(ctx->mface), rawbox, Pd, C);
+                                                             P_<uint32_t>(ctx->mface), rawbox, Pd, C, zero, nzero);
     }
-    enqueue_bvh_traverse(ctx, s2, C);
+    enqueue_bvh_traverse(ctx, s2, C, !ctx->batch); // k_morton_raw cleared the sort's group totals
 
     // ---- stream C: polygon-soup edge numbering, and the tables the filter fills ----
     (void)hipStreamWaitEvent(s3, ctx->ev[EV_FORK], 0);
@@ -1386,7 +1389,7 @@ MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_bo
                                                                   P_<uint32_t>(ctx->mface) + o, P_<float>(ctx->rawbox) + 6 * o,
                                                                   m ? (1u << 21) : 0u);
         }
-        enqueue_bvh_traverse(ctx, s, C);
+        enqueue_bvh_traverse(ctx, s, C, false);
         const uint64_t table = next_pow2(2 * ctx->cap_pairs);
         k_pair_hash_init<<<grid_for(table, kBlock, 1024), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash), ctx->cap_pairs,
                                                                           table, C);

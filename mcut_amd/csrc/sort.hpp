@@ -331,9 +331,23 @@ __global__ void k_zero_u32(uint32_t* __restrict__ p, uint64_t n)
 // Sorts keys[0..n) (n on device or host) by bits [0, bits) with an optional
 // u32 payload.  Ping-pongs between (keys, vals) and (keys_alt, vals_alt);
 // returns true when the sorted result ends in the *_alt buffers.
+// The group totals a sort of (cap, bits) accumulates into, which must be zero
+// when it starts: their start and count (0 when the sort takes the scan path).
+// A producer kernel on the same stream may clear them and spare the sort its
+// clearing launch (radix_sort with ghist_zeroed).
+inline uint32_t* radix_ghist(const RadixTemp& tmp, uint64_t cap, int bits, uint64_t* count)
+{
+    const uint32_t ntiles = radix_tiles(cap);
+    const int passes = (bits + kRadixBits - 1) / kRadixBits;
+    const bool grouped = ntiles <= kGroupedMaxTiles && passes <= kMaxPasses;
+    *count = grouped ? (uint64_t)kRadix * radix_groups(ntiles) * passes : 0;
+    return tmp.hist + (uint64_t)kRadix * ntiles;
+}
+
 template <typename K>
 inline bool radix_sort(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, const uint64_t* n_dev,
-                       uint64_t n_host, uint64_t cap, int bits, const RadixTemp& tmp, hipStream_t s)
+                       uint64_t n_host, uint64_t cap, int bits, const RadixTemp& tmp, hipStream_t s,
+                       bool ghist_zeroed = false)
 {
     const uint32_t ntiles = radix_tiles(cap);
     const uint32_t grid = ntiles < 2048 ? ntiles : 2048;
@@ -341,7 +355,7 @@ inline bool radix_sort(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
     const bool grouped = ntiles <= kGroupedMaxTiles && passes <= kMaxPasses;
     const uint32_t G = radix_group(ntiles), ngc = radix_groups(ntiles);
     uint32_t* ghist = tmp.hist + (uint64_t)kRadix * ntiles;
-    if (grouped) {
+    if (grouped && !ghist_zeroed) {
         const uint64_t nz = (uint64_t)kRadix * ngc * passes;
         k_zero_u32<<<(unsigned)((nz + 255) / 256 < 1024 ? (nz + 255) / 256 : 1024), 256, 0, s>>>(ghist, nz);
     }
