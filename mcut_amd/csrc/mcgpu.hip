@@ -467,8 +467,9 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
     Counters* C = &P_<DevState>(ctx->state)->C;
     k_planes<<<grid_for(F), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->cand_list), P_<Plane>(ctx->planes),
                                              P_<uint32_t>(ctx->big_list), ctx->cap_big, C);
-    k_big_polygons<<<4, 64, 0, s>>>(S, P_<uint32_t>(ctx->big_list), C, ctx->cap_big, P_<Plane>(ctx->planes),
-                                     P_<Trip>(ctx->big_scratch), C);
+    if (ctx->dsz[0] || ctx->dsz[1]) // faces of 6-16 vertices: none in two triangle meshes
+        k_big_polygons<<<4, 64, 0, s>>>(S, P_<uint32_t>(ctx->big_list), C, ctx->cap_big, P_<Plane>(ctx->planes),
+                                         P_<Trip>(ctx->big_scratch), C);
     stage_event(ctx, EV_A_PLANES, s);
     if (join_topology) (void)hipStreamWaitEvent(s, ctx->ev[EV_C_TOPO], 0);
     stage_event(ctx, EV_A_JOIN2, s);
