@@ -37,9 +37,14 @@ struct Buf {
 // and the sort stretches from 0.12 to 0.25 ms.  B takes kBroadShare of the
 // CUs left after the sums (174 of 248 on the 256-CU part): it is the longest
 // of the three paths, and the edge numbering finishes well inside it.  A joins B before the exact pair filter and
-// C before the edge-face tests.
+// C before the edge-face tests.  A batch runs on a second pair of streams
+// split kBatchBroadShare : rest: its edge numbering builds two hash tables per
+// pair over the whole batch and is as long as its broad phase (139 ms a
+// 4096-pair batch at 0.70, 113 ms at 0.58).  The pair is made on a context's
+// first batch, so a context that never batches holds four hardware queues.
 constexpr int kSumCUs = 8;
 constexpr double kBroadShare = 0.70;
+constexpr double kBatchBroadShare = 0.58;
 constexpr uint32_t kBatchSumWaves = 128; // persistent waves of the batched serial sums (on the reserved CUs)
 enum Stage {
     ST_SUMS,      // A: serial coordinate sums and raw boxes (k_mesh_sums alone)
@@ -106,6 +111,7 @@ struct Resident {
 struct mcgpu_context_t {
     int device = 0;
     hipStream_t s = nullptr, s2 = nullptr, s3 = nullptr, ssum = nullptr;
+    hipStream_t s2b = nullptr, s3b = nullptr; // B and C of batched dispatches, made by the first one
     hipEvent_t ev[EV_COUNT];
     hipError_t last_error = hipSuccess;
 
@@ -172,8 +178,8 @@ struct mcgpu_context_t {
     std::mt19937 gp_twister{gp_seed_engine()};
     std::uniform_real_distribution<double> gp_uniform{-1.0, 1.0};
 
-    // serving: a request slot of an mcgpu_server; one stream (s2, s3 and ssum
-    // are s), the source mesh is the server's resident one
+    // serving: a request slot of an mcgpu_server; one stream (s2, s3, s2b, s3b
+    // and ssum are s), the source mesh is the server's resident one
     const Resident* res = nullptr;
     bool own_streams = true;
 };
@@ -719,7 +725,7 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool g
 
 void enqueue_frontend(mcgpu_context ctx)
 {
-    hipStream_t s = ctx->s, s2 = ctx->s2, s3 = ctx->s3;
+    hipStream_t s = ctx->s, s2 = ctx->batch ? ctx->s2b : ctx->s2, s3 = ctx->batch ? ctx->s3b : ctx->s3;
     const uint32_t Fs = ctx->nf[0];
     const uint64_t F = (uint64_t)Fs + ctx->nf[1], V = (uint64_t)ctx->nv[0] + ctx->nv[1];
     Counters* C = &P_<DevState>(ctx->state)->C;
@@ -1073,6 +1079,27 @@ MCGPU_API const char* mcgpu_result_string(mcgpu_result r)
     return "unknown";
 }
 
+// Streams B and C of a dispatch: disjoint CU sets outside the first kSumCUs,
+// B taking `share` of them; a device too small to partition gets no masks.
+static hipError_t create_split_streams(int ncu, double share, hipStream_t* b, hipStream_t* c)
+{
+    const int nw = (ncu + 31) / 32;
+    std::vector<uint32_t> mb(nw, 0u), mc(nw, 0u);
+    const int nb = kSumCUs + (int)(share * (ncu - kSumCUs) + 0.5);
+    for (int i = kSumCUs; i < ncu; ++i) (i < nb ? mb : mc)[i / 32] |= 1u << (i % 32);
+    if (ncu <= 4 * kSumCUs) {
+        std::fill(mb.begin(), mb.end(), ~0u);
+        std::fill(mc.begin(), mc.end(), ~0u);
+    }
+    hipError_t e = hipExtStreamCreateWithCUMask(b, nw, mb.data());
+    if (e != hipSuccess) return e;
+    if ((e = hipExtStreamCreateWithCUMask(c, nw, mc.data())) != hipSuccess) {
+        (void)hipStreamDestroy(*b);
+        *b = nullptr;
+    }
+    return e;
+}
+
 MCGPU_API mcgpu_result mcgpu_create_context(int device, mcgpu_context* out)
 {
     if (!out) return MCGPU_ERROR_INVALID_VALUE;
@@ -1090,23 +1117,13 @@ MCGPU_API mcgpu_result mcgpu_create_context(int device, mcgpu_context* out)
     // CU masks: the first kSumCUs compute units for the serial sums, the rest
     // for the streams that run beside them
     const int ncu = prop.multiProcessorCount;
-    const int nb = kSumCUs + (int)(kBroadShare * (ncu - kSumCUs) + 0.5);
-    std::vector<uint32_t> msum((ncu + 31) / 32, 0u), mb((ncu + 31) / 32, 0u), mc((ncu + 31) / 32, 0u);
-    for (int c = 0; c < ncu; ++c) {
-        if (c < kSumCUs) msum[c / 32] |= 1u << (c % 32);
-        else if (c < nb) mb[c / 32] |= 1u << (c % 32);
-        else mc[c / 32] |= 1u << (c % 32);
-    }
-    if (ncu <= 4 * kSumCUs) { // too small a device to partition
-        std::fill(mb.begin(), mb.end(), ~0u);
-        std::fill(mc.begin(), mc.end(), ~0u);
-    }
-    if (hipExtStreamCreateWithCUMask(&ctx->ssum, (uint32_t)msum.size(), msum.data()) != hipSuccess ||
-        hipExtStreamCreateWithCUMask(&ctx->s2, (uint32_t)mb.size(), mb.data()) != hipSuccess ||
-        hipExtStreamCreateWithCUMask(&ctx->s3, (uint32_t)mc.size(), mc.data()) != hipSuccess) {
-        if (ctx->ssum) (void)hipStreamDestroy(ctx->ssum);
-        if (ctx->s2) (void)hipStreamDestroy(ctx->s2);
-        if (ctx->s3) (void)hipStreamDestroy(ctx->s3);
+    const int nw = (ncu + 31) / 32;
+    std::vector<uint32_t> msum(nw, 0u);
+    for (int c = 0; c < kSumCUs && c < ncu; ++c) msum[c / 32] |= 1u << (c % 32);
+    if (hipExtStreamCreateWithCUMask(&ctx->ssum, nw, msum.data()) != hipSuccess ||
+        create_split_streams(ncu, kBroadShare, &ctx->s2, &ctx->s3) != hipSuccess) {
+        for (hipStream_t st : {ctx->ssum, ctx->s2, ctx->s3})
+            if (st) (void)hipStreamDestroy(st);
         (void)hipStreamDestroy(ctx->s);
         delete ctx;
         return MCGPU_ERROR_DEVICE;
@@ -1122,9 +1139,8 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->s);
     if (ctx->own_streams) {
-        (void)hipStreamSynchronize(ctx->s2);
-        (void)hipStreamSynchronize(ctx->s3);
-        (void)hipStreamSynchronize(ctx->ssum);
+        for (hipStream_t st : {ctx->s2, ctx->s3, ctx->s2b, ctx->s3b, ctx->ssum})
+            if (st) (void)hipStreamSynchronize(st);
     }
     Buf* all[] = {&ctx->in_v[0], &ctx->in_v[1], &ctx->in_idx[0], &ctx->in_idx[1], &ctx->in_sz[0], &ctx->in_sz[1],
                   &ctx->off[0], &ctx->off[1], &ctx->state, &ctx->ps_v, &ctx->box, &ctx->hkeys[0],
@@ -1143,9 +1159,8 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->own_streams) {
-        (void)hipStreamDestroy(ctx->ssum);
-        (void)hipStreamDestroy(ctx->s3);
-        (void)hipStreamDestroy(ctx->s2);
+        for (hipStream_t st : {ctx->ssum, ctx->s3, ctx->s2, ctx->s3b, ctx->s2b})
+            if (st) (void)hipStreamDestroy(st);
     }
     (void)hipStreamDestroy(ctx->s);
     delete ctx;
@@ -1216,6 +1231,11 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
     const int pb = bits_for(np);
     if (pb > 28) return MCGPU_ERROR_UNSUPPORTED;
     CK(hipSetDevice(ctx->device));
+    if (ctx->own_streams && !ctx->s2b) {
+        hipDeviceProp_t prop;
+        CK(hipGetDeviceProperties(&prop, ctx->device));
+        CK(create_split_streams(prop.multiProcessorCount, kBatchBroadShare, &ctx->s2b, &ctx->s3b));
+    }
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = true;
@@ -1744,7 +1764,7 @@ mcgpu_result slot_create(int device, mcgpu_context* out)
         delete ctx;
         return MCGPU_ERROR_DEVICE;
     }
-    ctx->s2 = ctx->s3 = ctx->ssum = ctx->s;
+    ctx->s2 = ctx->s3 = ctx->s2b = ctx->s3b = ctx->ssum = ctx->s;
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventCreate(&ctx->ev[i]);
     *out = ctx;
     return MCGPU_SUCCESS;
