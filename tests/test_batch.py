@@ -140,3 +140,31 @@ def test_batch_invalid_pair_rejects_the_batch(ctxs):
     assert ei.value.code == -1
     B = bctx.dispatch_batch(pairs[:1])  # the context stays usable
     assert B.status[0] == 0
+
+
+@pytest.mark.gpu
+def test_plain_and_batch_alternate_on_one_context():
+    """A context's first batch makes the batch's own broad-phase and
+    edge-numbering streams; plain and batched dispatches then alternate on
+    the same buffers and each equals a dispatch on a fresh context."""
+    from mcut_amd import frontend
+    a, ref = frontend.Context(0), frontend.Context(0)
+    try:
+        big = meshes.icosphere_pair(5, amp=0.005)
+        small = [meshes.config1_rotated(), meshes.bench_pair(0)]
+        for rnd in range(2):
+            print(f"alternation {rnd}: plain", flush=True)
+            R = a.dispatch(*big)
+            R0 = ref.dispatch(*big)
+            assert R.status == R0.status and np.array_equal(R.pairs(), R0.pairs())
+            e, g, p = R.registry()
+            e0, g0, p0 = R0.registry()
+            assert np.array_equal(e, e0) and np.array_equal(g, g0)
+            assert np.array_equal(p.view(np.uint64), p0.view(np.uint64))
+            print(f"alternation {rnd}: batch", flush=True)
+            B = a.dispatch_batch(small + [big])
+            for i, (s, c) in enumerate(small + [big]):
+                _check_against_plain(B, i, ref.dispatch(s, c))
+    finally:
+        a.close()
+        ref.close()
