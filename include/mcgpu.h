@@ -178,6 +178,30 @@ MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_bo
                                          const double* cut_boxes, uint32_t num_cut, uint32_t input_on_device,
                                          uint64_t* num_pairs);
 
+/* Broad phase without device allocation.  No counterpart in the reference:
+ * it is mcgpu_broad_phase for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc / hipFree: every array it needs lives in
+ * a static device array of MCGPU_NOALLOC_SCRATCH_BYTES (50.1 MiB), which each
+ * device that runs it holds until the process ends.  Limits: at most
+ * MCGPU_NOALLOC_MAX_FACES faces per mesh (larger inputs return
+ * MCGPU_ERROR_INVALID_VALUE before any device work) and at most
+ * MCGPU_NOALLOC_MAX_PAIRS pairs of the raw-coordinate broad phase (more
+ * return MCGPU_ERROR_OUT_OF_MEMORY with *num_pairs = that count).
+ * src_boxes / cut_boxes: device pointers, 6 doubles per face as in
+ * mcgpu_broad_phase.  pairs: device array of `capacity` entries that receives
+ * the overlapping pairs, unsorted, each (source face << 32) | cut face; a
+ * capacity below the pair count returns MCGPU_ERROR_OUT_OF_MEMORY with
+ * *num_pairs = the count.  stream: a hipStream_t (NULL: the null stream); the
+ * call returns once the pairs are written.  All calls on one device share its
+ * array, so calls that overlap in time on one device race. */
+#define MCGPU_NOALLOC_MAX_FACES 65536u
+#define MCGPU_NOALLOC_MAX_PAIRS (1u << 20)
+#define MCGPU_NOALLOC_SCRATCH_BYTES 52566528ull
+MCGPU_API mcgpu_result mcgpu_broad_phase_noalloc(int device, const double* src_boxes, uint32_t num_src,
+                                                 const double* cut_boxes, uint32_t num_cut, uint64_t* pairs,
+                                                 uint64_t capacity, uint64_t* num_pairs, void* stream);
+
 /* Batched front end: every pair of `batch` through the front end in one
  * pass of the pipeline (each stage one launch over all pairs), with results
  * bit-identical to dispatching each pair alone.  This has no single

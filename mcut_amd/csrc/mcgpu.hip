@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <mutex>
 #include <random>
 #include <thread>
 #include <cstddef>
@@ -655,38 +656,59 @@ mcgpu_result finish_batch(mcgpu_context ctx, mcgpu_frontend_info* info)
     return MCGPU_SUCCESS;
 }
 
+// The device arrays of the trees and the raw-space traversal: leaf keys and
+// their sorted copies, single-precision leaf boxes, both trees, the sort's
+// scratch and the raw pair list.  A resident source (serving) brings its own
+// sorted leaves and refitted tree (src_*).
+struct BroadView {
+    uint32_t *mcode, *mface, *mcode2, *mface2;
+    float* rawbox;
+    float* nodes[2];
+    RadixTemp rt;
+    unsigned long long* raw_pairs;
+    uint64_t cap_pairs;
+    uint32_t nf[2];
+    int depth[2];
+    int sort_bits;
+    const float* src_rawbox;  // resident source, else null
+    const uint32_t* src_sface;
+    const float* src_nodes;
+    int src_depth;
+};
+
 // Both trees and the raw-space traversal, on stream st, once the leaf keys
 // (mcode, mface) and single-precision leaf boxes (rawbox) are written: one
 // stable sort orders the leaves of both trees (the mesh is the top key bit),
 // the implicit trees are refitted, and the traversal appends raw pairs.
-void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool ghist_zeroed)
+// Returns whether the sorted leaves ended in mface2.  b_sort / b_refit: the
+// stage events (null: none recorded).
+bool enqueue_trees_traverse(const BroadView& B, hipStream_t st, Counters* C, bool ghist_zeroed, hipEvent_t b_sort,
+                            hipEvent_t b_refit)
 {
-    const uint32_t Fs = ctx->nf[0];
-    const uint64_t F = (uint64_t)Fs + ctx->nf[1];
-    float* rawbox = P_<float>(ctx->rawbox);
-    RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
+    const uint32_t Fs = B.nf[0];
+    const uint64_t F = (uint64_t)Fs + B.nf[1];
     // a resident source keeps its own sorted leaves and tree: only the cut
     // leaves are sorted and only the cut tree is refitted
-    const uint64_t o0 = ctx->res ? Fs : 0;
-    ctx->mface_alt = radix_sort<uint32_t>(P_<uint32_t>(ctx->mcode) + o0, P_<uint32_t>(ctx->mface) + o0,
-                                          P_<uint32_t>(ctx->mcode2) + o0, P_<uint32_t>(ctx->mface2) + o0, nullptr,
-                                          F - o0, F - o0, ctx->sort_bits, rt, st, ghist_zeroed);
-    stage_event(ctx, EV_B_SORT, st);
-    const uint32_t* sorted = P_<uint32_t>(ctx->mface_alt ? ctx->mface2 : ctx->mface);
+    const bool res = B.src_nodes != nullptr;
+    const uint64_t o0 = res ? Fs : 0;
+    const bool alt = radix_sort<uint32_t>(B.mcode + o0, B.mface + o0, B.mcode2 + o0, B.mface2 + o0, nullptr, F - o0,
+                                          F - o0, B.sort_bits, B.rt, st, ghist_zeroed);
+    if (b_sort) (void)hipEventRecord(b_sort, st);
+    const uint32_t* sorted = alt ? B.mface2 : B.mface;
     Tree T[2];
     RefitTree R[2];
     for (int m = 0; m < 2; ++m) {
-        R[m].facebox = rawbox + 6 * (uint64_t)(m ? Fs : 0);
+        R[m].facebox = B.rawbox + 6 * (uint64_t)(m ? Fs : 0);
         R[m].sface = sorted + (m ? Fs : 0);
-        R[m].nodes = P_<float>(ctx->nodes[m]);
-        R[m].n = (int)ctx->nf[m];
-        R[m].D = ctx->depth[m];
-        R[m].lin = ctx->depth[m];
-        if (m == 0 && ctx->res) {
-            R[0].facebox = ctx->res->rawbox;
-            R[0].sface = ctx->res->sface;
-            R[0].nodes = const_cast<float*>(ctx->res->nodes);
-            R[0].D = ctx->res->depth;
+        R[m].nodes = B.nodes[m];
+        R[m].n = (int)B.nf[m];
+        R[m].D = B.depth[m];
+        R[m].lin = B.depth[m];
+        if (m == 0 && res) {
+            R[0].facebox = B.src_rawbox;
+            R[0].sface = B.src_sface;
+            R[0].nodes = const_cast<float*>(B.src_nodes);
+            R[0].D = B.src_depth;
             R[0].lin = -1; // already refitted
         }
         T[m].sface = R[m].sface;
@@ -716,10 +738,37 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool g
             if (R[m].lin == 0) R[m].lin = -1;
         }
     }
-    stage_event(ctx, EV_B_REFIT, st);
+    if (b_refit) (void)hipEventRecord(b_refit, st);
     const unsigned nwaves = (Fs + kWaveLeaves - 1) / kWaveLeaves;
-    k_traverse_wave<<<nwaves < 65536 ? nwaves : 65536, 64, 0, st>>>(T[0], T[1], P_<unsigned long long>(ctx->raw_pairs),
-                                                                    ctx->cap_pairs, C);
+    k_traverse_wave<<<nwaves < 65536 ? nwaves : 65536, 64, 0, st>>>(T[0], T[1], B.raw_pairs, B.cap_pairs, C);
+    return alt;
+}
+
+void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool ghist_zeroed)
+{
+    BroadView B{};
+    B.mcode = P_<uint32_t>(ctx->mcode);
+    B.mface = P_<uint32_t>(ctx->mface);
+    B.mcode2 = P_<uint32_t>(ctx->mcode2);
+    B.mface2 = P_<uint32_t>(ctx->mface2);
+    B.rawbox = P_<float>(ctx->rawbox);
+    for (int m = 0; m < 2; ++m) {
+        B.nodes[m] = P_<float>(ctx->nodes[m]);
+        B.nf[m] = ctx->nf[m];
+        B.depth[m] = ctx->depth[m];
+    }
+    B.rt = RadixTemp{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
+    B.raw_pairs = P_<unsigned long long>(ctx->raw_pairs);
+    B.cap_pairs = ctx->cap_pairs;
+    B.sort_bits = ctx->sort_bits;
+    if (ctx->res) {
+        B.src_rawbox = ctx->res->rawbox;
+        B.src_sface = ctx->res->sface;
+        B.src_nodes = ctx->res->nodes;
+        B.src_depth = ctx->res->depth;
+    }
+    ctx->mface_alt = enqueue_trees_traverse(B, st, C, ghist_zeroed, ctx->stage_timing ? ctx->ev[EV_B_SORT] : nullptr,
+                                            ctx->stage_timing ? ctx->ev[EV_B_REFIT] : nullptr);
     (void)hipEventRecord(ctx->ev[EV_B_TRAV], st);
 }
 
@@ -2078,6 +2127,174 @@ MCGPU_API mcgpu_result mcgpu_server_get_registry(mcgpu_server srv, uint32_t requ
     if (edges) std::copy(srv->reg_e[request].begin(), srv->reg_e[request].begin() + m, edges);
     if (faces) std::copy(srv->reg_g[request].begin(), srv->reg_g[request].begin() + m, faces);
     if (points) std::copy(srv->reg_p[request].begin(), srv->reg_p[request].begin() + 3 * m, points);
+    return MCGPU_SUCCESS;
+}
+
+} // extern "C"
+
+// ===========================================================================
+// Broad phase without device allocation (mcgpu_broad_phase_noalloc)
+// ===========================================================================
+// Every array of the broad phase lives in one file-scope device array,
+// `scratch`, carved into the regions below for the largest input the entry
+// point takes: MCGPU_NOALLOC_MAX_FACES faces per mesh and
+// MCGPU_NOALLOC_MAX_PAIRS raw-space pairs.  Each device holds its own copy of
+// the array (from module load until the process ends); its address is looked
+// up once per device with that device current and kept in g_scratch_ptr.
+// Every call on a device shares that device's array, and each call clears
+// what must start empty (counters, candidate-face flags, pair hash, the
+// sort's group totals): calls that overlap on one device race.
+namespace {
+
+constexpr uint64_t kNaFaces = MCGPU_NOALLOC_MAX_FACES;         // per mesh
+constexpr uint64_t kNaF = 2 * kNaFaces;                        // both meshes
+constexpr uint64_t kNaPairs = MCGPU_NOALLOC_MAX_PAIRS;         // raw-space pairs
+constexpr uint64_t kNaTable = 2 * kNaPairs;                    // pair hash slots (a power of two)
+constexpr int kNaDepth = 16;                                   // leaf level of a tree of kNaFaces leaves
+static_assert((1ull << kNaDepth) == kNaFaces, "tree depth of the largest mesh");
+static_assert((kNaTable & (kNaTable - 1)) == 0, "pair hash size is a power of two");
+
+constexpr uint64_t na_align(uint64_t x) { return (x + 255) & ~255ull; }
+
+struct NaLayout {
+    uint64_t state, box, rawbox, mcode, mface, mcode2, mface2, cand, cand_list, hist, scan, nodes0, nodes1, raw, pairs,
+        phash, total;
+};
+
+constexpr NaLayout na_layout()
+{
+    NaLayout L{};
+    uint64_t at = 0;
+    auto take = [&at](uint64_t bytes) {
+        const uint64_t o = at;
+        at = na_align(at + bytes);
+        return o;
+    };
+    L.state = take(sizeof(DevState));
+    L.box = take(48 * kNaF);
+    L.rawbox = take(24 * kNaF);
+    L.mcode = take(4 * kNaF);
+    L.mface = take(4 * kNaF);
+    L.mcode2 = take(4 * kNaF);
+    L.mface2 = take(4 * kNaF);
+    L.cand = take(4 * (kNaF + 8));
+    L.cand_list = take(4 * (kNaF + 8));
+    L.hist = take(4 * radix_hist_elems(kNaF));
+    const uint64_t scan_a = scan_tmp_elems(radix_hist_elems(kNaF)), scan_b = scan_tmp_elems(kNaF + 8);
+    L.scan = take(4 * ((scan_a > scan_b ? scan_a : scan_b) + 1024));
+    L.nodes0 = take(4 * 6 * ((2ull << kNaDepth) - 1));
+    L.nodes1 = take(4 * 6 * ((2ull << kNaDepth) - 1));
+    L.raw = take(8 * kNaPairs);
+    L.pairs = take(8 * kNaPairs);
+    L.phash = take(8 * kNaTable);
+    L.total = at;
+    return L;
+}
+
+constexpr NaLayout kNa = na_layout();
+static_assert(kNa.total == MCGPU_NOALLOC_SCRATCH_BYTES, "the header states the scratch size");
+
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNa.total];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kNaMaxDevices] = {};
+
+// the device address of `scratch` on the current device (set up once per
+// device; the array's name is a host-side stand-in, never a device address)
+unsigned char* scratch_on(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+        g_scratch_ptr[device] = (unsigned char*)ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
+} // namespace
+
+extern "C" {
+
+MCGPU_API mcgpu_result mcgpu_broad_phase_noalloc(int device, const double* src_boxes, uint32_t num_src,
+                                                 const double* cut_boxes, uint32_t num_cut, uint64_t* pairs,
+                                                 uint64_t capacity, uint64_t* num_pairs, void* stream)
+{
+    if (!src_boxes || !cut_boxes || !pairs || !num_pairs || num_src == 0 || num_cut == 0) return MCGPU_ERROR_INVALID_VALUE;
+    if (num_src > kNaFaces || num_cut > kNaFaces) return MCGPU_ERROR_INVALID_VALUE; // beyond the scratch's maximum
+    *num_pairs = 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev || device >= kNaMaxDevices)
+        return MCGPU_ERROR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return MCGPU_ERROR_DEVICE;
+    unsigned char* base = scratch_on(device);
+    if (!base) return MCGPU_ERROR_DEVICE;
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t F = (uint64_t)num_src + num_cut;
+    DevState* st = (DevState*)(base + kNa.state);
+    Counters* C = &st->C;
+    double* box = (double*)(base + kNa.box);
+    float* rawbox = (float*)(base + kNa.rawbox);
+    uint32_t* mcode = (uint32_t*)(base + kNa.mcode);
+    uint32_t* mface = (uint32_t*)(base + kNa.mface);
+    uint32_t* cand = (uint32_t*)(base + kNa.cand);
+    unsigned long long* raw = (unsigned long long*)(base + kNa.raw);
+    unsigned long long* found = (unsigned long long*)(base + kNa.pairs);
+    unsigned long long* phash = (unsigned long long*)(base + kNa.phash);
+
+    fill_u32(C, 0u, sizeof(Counters) / 4, s);
+    if (hipMemcpyAsync(box, src_boxes, 48ull * num_src, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(box + 6ull * num_src, cut_boxes, 48ull * num_cut, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return MCGPU_ERROR_DEVICE;
+    const uint32_t nf[2] = {num_src, num_cut};
+    for (int m = 0; m < 2; ++m) {
+        const uint64_t o = m ? num_src : 0;
+        fill_u64(C->rbox[m], ~0ull, 3, s);
+        k_box_bounds<<<grid_for(nf[m], kBlock, 64), kBlock, 0, s>>>(box + 6 * o, nf[m], C->rbox[m]);
+        k_morton_boxes<<<grid_for(nf[m]), kBlock, 0, s>>>(box + 6 * o, nf[m], C->rbox[m], mcode + o, mface + o,
+                                                         rawbox + 6 * o, m ? (1u << 21) : 0u);
+    }
+    BroadView B{};
+    B.mcode = mcode;
+    B.mface = mface;
+    B.mcode2 = (uint32_t*)(base + kNa.mcode2);
+    B.mface2 = (uint32_t*)(base + kNa.mface2);
+    B.rawbox = rawbox;
+    B.nodes[0] = (float*)(base + kNa.nodes0);
+    B.nodes[1] = (float*)(base + kNa.nodes1);
+    for (int m = 0; m < 2; ++m) {
+        int D = 0;
+        while ((1ull << D) < nf[m]) ++D;
+        B.nf[m] = nf[m];
+        B.depth[m] = D;
+    }
+    B.rt = RadixTemp{(uint32_t*)(base + kNa.hist), (uint32_t*)(base + kNa.scan)};
+    B.raw_pairs = raw;
+    B.cap_pairs = kNaPairs;
+    B.sort_bits = 22;
+    enqueue_trees_traverse(B, s, C, false, nullptr, nullptr);
+    k_pair_hash_init<<<grid_for(kNaTable, kBlock, 1024), kBlock, 0, s>>>(phash, kNaPairs, kNaTable, C);
+    fill_u32(cand, 0u, F, s);
+    k_filter_pairs<<<grid_for(kNaPairs, kFilterTile, 2048), kStageBlock, 0, s>>>(
+        box, raw, kNaPairs, num_src, nullptr, found, phash, cand, (uint32_t*)(base + kNa.cand_list), 0, C);
+    Counters h;
+    if (hipMemcpyAsync(&h, C, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return MCGPU_ERROR_DEVICE;
+    if (h.ov_pairs) { // more raw-space pairs than the scratch holds
+        *num_pairs = h.n_raw;
+        return MCGPU_ERROR_OUT_OF_MEMORY;
+    }
+    *num_pairs = h.n_pairs;
+    if (h.n_pairs > capacity) return MCGPU_ERROR_OUT_OF_MEMORY;
+    if (h.n_pairs && (hipMemcpyAsync(pairs, found, 8ull * h.n_pairs, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                      hipStreamSynchronize(s) != hipSuccess))
+        return MCGPU_ERROR_DEVICE;
     return MCGPU_SUCCESS;
 }
 

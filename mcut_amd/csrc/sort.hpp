@@ -172,7 +172,7 @@ inline void scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32
     k_scan_apply<<<ntiles, kScanBlock, 0, s>>>(in, out, tmp, n);
 }
 
-inline uint64_t scan_tmp_elems(uint64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+constexpr uint64_t scan_tmp_elems(uint64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
 // ---------------------------------------------------------------------------
 // Radix sort
@@ -309,15 +309,15 @@ struct RadixTemp {
 constexpr uint32_t kGroupedMaxTiles = 1024;
 constexpr int kMaxPasses = 8;
 
-inline uint32_t radix_tiles(uint64_t cap) { return (uint32_t)((cap + kSortTile - 1) / kSortTile) + 1; }
-inline uint32_t radix_group(uint32_t ntiles)
+constexpr uint32_t radix_tiles(uint64_t cap) { return (uint32_t)((cap + kSortTile - 1) / kSortTile) + 1; }
+constexpr uint32_t radix_group(uint32_t ntiles)
 {
     uint32_t g = 1;
     while (g * g < ntiles) ++g;
     return g;
 }
-inline uint32_t radix_groups(uint32_t ntiles) { return (ntiles + radix_group(ntiles) - 1) / radix_group(ntiles); }
-inline uint64_t radix_hist_elems(uint64_t cap)
+constexpr uint32_t radix_groups(uint32_t ntiles) { return (ntiles + radix_group(ntiles) - 1) / radix_group(ntiles); }
+constexpr uint64_t radix_hist_elems(uint64_t cap)
 {
     const uint32_t nt = radix_tiles(cap);
     return (uint64_t)kRadix * nt + (uint64_t)kRadix * kMaxPasses * radix_groups(nt);

@@ -96,6 +96,7 @@ def lib():
         L.mcgpu_get_ps_vertices.argtypes = [vp, vp, u64]
         L.mcgpu_registry_device.argtypes = [vp, vp, vp, vp, vp]
         L.mcgpu_broad_phase.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, C.c_uint32, C.POINTER(u64)]
+        L.mcgpu_broad_phase_noalloc.argtypes = [C.c_int, vp, C.c_uint32, vp, C.c_uint32, vp, u64, C.POINTER(u64), vp]
         L.mcgpu_dispatch_batch.argtypes = [vp, C.POINTER(Batch), C.POINTER(DispatchDesc), C.POINTER(FrontendInfo)]
         L.mcgpu_get_batch_status.argtypes = [vp, vp, vp, u64]
         L.mcgpu_get_batch_candidate_pairs.argtypes = [vp, vp, u64]
@@ -118,7 +119,7 @@ def lib():
                    "mcgpu_stage_times", "mcgpu_broad_phase", "mcgpu_dispatch_batch", "mcgpu_get_batch_status",
                    "mcgpu_get_batch_candidate_pairs", "mcgpu_server_create", "mcgpu_server_release",
                    "mcgpu_server_info", "mcgpu_server_run", "mcgpu_server_get_registry", "mcgpu_predicate_stage",
-                   "mcgpu_dispatch_enforced", "mcgpu_set_stage_timing"):
+                   "mcgpu_dispatch_enforced", "mcgpu_set_stage_timing", "mcgpu_broad_phase_noalloc"):
             getattr(L, fn).restype = C.c_int
         _lib = L
     return _lib
@@ -236,6 +237,21 @@ class Result:
 
 
 _stage_names = None
+
+
+# limits of mcgpu_broad_phase_noalloc (include/mcgpu.h)
+NOALLOC_MAX_FACES = 65536
+NOALLOC_MAX_PAIRS = 1 << 20
+
+
+def broad_phase_noalloc(device, src_boxes_ptr, num_src, cut_boxes_ptr, num_cut, pairs_ptr, capacity, stream=None):
+    """mcgpu_broad_phase_noalloc over device buffers: face boxes (6 doubles a
+    face) in, packed (source face << 32 | cut face) pairs out, unsorted; no
+    device allocation.  Returns (result code, pair count)."""
+    n = C.c_uint64(0)
+    r = lib().mcgpu_broad_phase_noalloc(device, src_boxes_ptr, num_src, cut_boxes_ptr, num_cut, pairs_ptr, capacity,
+                                        C.byref(n), stream)
+    return r, n.value
 
 
 def stage_names():
