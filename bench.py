@@ -408,6 +408,16 @@ def main():
                      "frontend_alg_bytes": sum(stage_bytes(k, n) for k in stages)},
     }
     progress(f"single pair: {ms_step:.3f} ms per step")
+    # every registry point of the GPU's own result against the exact segment /
+    # plane intersection (integer-scaled exact arithmetic)
+    from mcut_amd import exact
+    e_, g_, p_ = R.registry()
+    v0, v1, _, _ = R.edges()
+    err, straddle = exact.registry_point_errors(R.ps_vertices(), exact.triangle_soup_faces(src[1], cut[1], src[0].shape[0]),
+                                                v0, v1, e_, g_, p_)
+    line["point_error"] = {"max_rel": float(err.max()), "points": int(err.size), "all_straddle": bool(straddle.all()),
+                           "bound": 1e-12, "method": "exact rational (integer-scaled) segment/plane intersection"}
+    progress(f"single pair: max point error {err.max():.3e} over {err.size} points")
     traffic_file = os.path.join(ROOT, "profiles", "roofline_traffic.json")
     if os.path.exists(traffic_file):  # HBM bytes per launch of the roofline kernel, from rocprofv3 --pmc runs
         t = json.load(open(traffic_file))

@@ -108,7 +108,10 @@ typedef struct {
     double translation[3];
     double eps;                  /* cut-face AABB enlargement */
     float device_ms;             /* GPU time of the whole front end */
-    float reserved;
+    uint32_t partial_cut;        /* 1: the front end succeeded and an intersection lies on a cut-mesh border
+                                    edge (kernel.cpp:2646-2650): a partial cut.  With
+                                    MCGPU_DISPATCH_REQUIRE_THROUGH_CUTS the reference ends the dispatch there
+                                    with success and no cut (kernel.cpp:3269-3278) */
 } mcgpu_frontend_info;
 
 /* A batch of independent (source, cut) pairs, each dispatched as
@@ -218,6 +221,9 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
  * registry.  capacity >= num_pairs; registry_offsets needs num_pairs + 1. */
 MCGPU_API mcgpu_result mcgpu_get_batch_status(mcgpu_context ctx, int32_t* status, uint64_t* registry_offsets,
                                               uint64_t capacity);
+/* After mcgpu_dispatch_batch: each pair's mcgpu_frontend_info.partial_cut
+ * (capacity >= num_pairs). */
+MCGPU_API mcgpu_result mcgpu_get_batch_partial_cuts(mcgpu_context ctx, uint32_t* partial_cut, uint64_t capacity);
 /* After mcgpu_dispatch_batch: candidate pairs as (pair, source face, cut
  * face) triples, face ids local to the pair, sorted. */
 MCGPU_API mcgpu_result mcgpu_get_batch_candidate_pairs(mcgpu_context ctx, uint32_t* triples, uint64_t capacity);
@@ -249,7 +255,7 @@ typedef struct {
     double start_s;              /* admitted to a slot (seconds since the run began) */
     double done_s;               /* results read back */
     int32_t status;              /* mcgpu_status, or a negative mcgpu_result */
-    uint32_t reserved;
+    uint32_t partial_cut;        /* as mcgpu_frontend_info.partial_cut */
     uint64_t num_registry;
     uint64_t num_candidate_pairs;
 } mcgpu_request_result;

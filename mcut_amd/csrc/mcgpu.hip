@@ -167,6 +167,7 @@ struct mcgpu_context_t {
     std::vector<uint32_t> hstarts, horder;
     std::vector<uint32_t> hflags, hrstart, hrend;
     std::vector<int32_t> hstatus;
+    std::vector<uint32_t> hpartial;
     Buf bpairs, bstarts, fpair, vpair, bparams, brbox, pflags, border, bidx[2], rle, rlg, rstart, rend;
     const uint32_t* bin_idx[2] = {nullptr, nullptr}; // the caller's pair-local face lists
 
@@ -635,6 +636,7 @@ mcgpu_result finish_batch(mcgpu_context ctx, mcgpu_frontend_info* info)
     ctx->hrstart.resize(np);
     ctx->hrend.resize(np);
     ctx->hstatus.resize(np);
+    ctx->hpartial.resize(np);
     CK(hipMemcpy(ctx->hflags.data(), ctx->pflags.p, 4ull * np, hipMemcpyDeviceToHost));
     CK(hipMemcpy(ctx->hrstart.data(), ctx->rstart.p, 4ull * np, hipMemcpyDeviceToHost));
     CK(hipMemcpy(ctx->hrend.data(), ctx->rend.p, 4ull * np, hipMemcpyDeviceToHost));
@@ -646,6 +648,7 @@ mcgpu_result finish_batch(mcgpu_context ctx, mcgpu_frontend_info* info)
                                          ctx->hrend[p] - ctx->hrstart[p], (f & PF_PARTIAL) != 0,
                                          (f & PF_PAIR_FAIL) != 0, ctx->flags, ctx->enforcement_count);
         ctx->hstatus[p] = st;
+        ctx->hpartial[p] = (st == MCGPU_STATUS_SUCCESS && ctx->hrend[p] > ctx->hrstart[p] && (f & PF_PARTIAL)) ? 1u : 0u;
         if (first == MCGPU_STATUS_SUCCESS) first = st;
     }
     if (info) {
@@ -1030,6 +1033,7 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         }
         info->eps = ctx->P.eps;
         info->device_ms = tot;
+        info->partial_cut = (info->status == MCGPU_STATUS_SUCCESS && C.n_reg > 0 && C.partial) ? 1u : 0u;
     }
     return MCGPU_SUCCESS;
 }
@@ -1379,6 +1383,15 @@ MCGPU_API mcgpu_result mcgpu_get_batch_status(mcgpu_context ctx, int32_t* status
         at += ctx->hrend[p] - ctx->hrstart[p];
     }
     if (registry_offsets && capacity > np) registry_offsets[np] = at;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_batch_partial_cuts(mcgpu_context ctx, uint32_t* partial_cut, uint64_t capacity)
+{
+    if (!ctx || !partial_cut) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have || !ctx->batch) return MCGPU_ERROR_NO_RESULT;
+    if (capacity < ctx->npairs) return MCGPU_ERROR_INVALID_VALUE;
+    std::memcpy(partial_cut, ctx->hpartial.data(), 4ull * ctx->npairs);
     return MCGPU_SUCCESS;
 }
 
@@ -2060,6 +2073,7 @@ MCGPU_API mcgpu_result mcgpu_server_run(mcgpu_server srv, const mcgpu_mesh* cuts
                 out[i].status = (int32_t)r; // negative: an mcgpu_result
             } else {
                 out[i].status = info.status;
+                out[i].partial_cut = info.partial_cut;
                 out[i].num_registry = info.num_registry;
                 out[i].num_candidate_pairs = info.num_candidate_pairs;
                 if (keep_registries && info.num_registry) {

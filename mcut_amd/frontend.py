@@ -53,7 +53,7 @@ class FrontendInfo(C.Structure):
                 ("num_decided_f32", C.c_uint64), ("num_decided_f64", C.c_uint64),
                 ("com", C.c_double * 3),
                 ("translation", C.c_double * 3), ("eps", C.c_double), ("device_ms", C.c_float),
-                ("reserved", C.c_float)]
+                ("partial_cut", C.c_uint32)]
 
 
 class Batch(C.Structure):
@@ -68,7 +68,7 @@ class ServerLimits(C.Structure):
 
 class RequestResult(C.Structure):
     _fields_ = [("arrival_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double), ("status", C.c_int32),
-                ("reserved", C.c_uint32), ("num_registry", C.c_uint64), ("num_candidate_pairs", C.c_uint64)]
+                ("partial_cut", C.c_uint32), ("num_registry", C.c_uint64), ("num_candidate_pairs", C.c_uint64)]
 
 
 _lib = None
@@ -100,6 +100,7 @@ def lib():
         L.mcgpu_dispatch_batch.argtypes = [vp, C.POINTER(Batch), C.POINTER(DispatchDesc), C.POINTER(FrontendInfo)]
         L.mcgpu_get_batch_status.argtypes = [vp, vp, vp, u64]
         L.mcgpu_get_batch_candidate_pairs.argtypes = [vp, vp, u64]
+        L.mcgpu_get_batch_partial_cuts.argtypes = [vp, vp, u64]
         L.mcgpu_server_create.argtypes = [C.c_int, C.POINTER(Mesh), C.POINTER(DispatchDesc), C.POINTER(ServerLimits),
                                           C.POINTER(vp)]
         L.mcgpu_server_release.argtypes = [vp]
@@ -117,7 +118,7 @@ def lib():
                    "mcgpu_redispatch_perturbed", "mcgpu_get_registry", "mcgpu_get_candidate_pairs", "mcgpu_get_edges",
                    "mcgpu_get_tests", "mcgpu_get_face_planes", "mcgpu_get_ps_vertices", "mcgpu_registry_device",
                    "mcgpu_stage_times", "mcgpu_broad_phase", "mcgpu_dispatch_batch", "mcgpu_get_batch_status",
-                   "mcgpu_get_batch_candidate_pairs", "mcgpu_server_create", "mcgpu_server_release",
+                   "mcgpu_get_batch_candidate_pairs", "mcgpu_get_batch_partial_cuts", "mcgpu_server_create", "mcgpu_server_release",
                    "mcgpu_server_info", "mcgpu_server_run", "mcgpu_server_get_registry", "mcgpu_predicate_stage",
                    "mcgpu_dispatch_enforced", "mcgpu_set_stage_timing", "mcgpu_broad_phase_noalloc"):
             getattr(L, fn).restype = C.c_int
@@ -165,6 +166,7 @@ class Result:
         self.t = np.array(info.translation[:])
         self.eps = info.eps
         self.device_ms = info.device_ms
+        self.partial_cut = int(info.partial_cut)
         self._nv = nv
 
     def registry(self):
@@ -305,6 +307,9 @@ class BatchResult:
         self.reg_offsets = np.zeros(num_pairs + 1, np.uint64)
         _check(lib().mcgpu_get_batch_status(ctx.h, _ptr(self.status), _ptr(self.reg_offsets), num_pairs + 1),
                "mcgpu_get_batch_status")
+        self.partial_cut = np.zeros(num_pairs, np.uint32)
+        _check(lib().mcgpu_get_batch_partial_cuts(ctx.h, _ptr(self.partial_cut), num_pairs),
+               "mcgpu_get_batch_partial_cuts")
         self._reg = None
 
     def registry(self):
@@ -455,7 +460,7 @@ class Server:
         _check(lib().mcgpu_server_run(self.h, meshes, _ptr(arr), len(cuts), 1 if keep_registries else 0, out),
                "mcgpu_server_run")
         dt = np.dtype([("arrival_s", "f8"), ("start_s", "f8"), ("done_s", "f8"), ("status", "i4"),
-                       ("reserved", "u4"), ("num_registry", "u8"), ("num_candidate_pairs", "u8")])
+                       ("partial_cut", "u4"), ("num_registry", "u8"), ("num_candidate_pairs", "u8")])
         return np.frombuffer(bytes(out), dtype=dt).copy()
 
     def registry(self, i, n):

@@ -628,6 +628,7 @@ typedef struct {
 
 struct om_result {
     int status, stage;
+    int partial;                   /* a successful front end whose registry holds a cut-mesh border edge */
     double com[3], t[3], eps, ms;
     uint32_t Vs, Fs, Vps, Fps;
     double* ps_v;                  /* ps vertex coordinates */
@@ -1140,8 +1141,10 @@ om_result* om_frontend(const om_mesh* src, const om_mesh* cut, const om_options*
         }
         if (partial && opt->require_through_cuts) {
             R->status = OM_SUCCESS;
+            R->partial = 1;
             goto done;
         }
+        R->partial = partial; /* cleared again below if the pair check fails */
     }
     /* --- every (src face, cut face) pair needs >= 2 points, kernel.cpp:3439-3452 --- */
     {
@@ -1170,6 +1173,7 @@ om_result* om_frontend(const om_mesh* src, const om_mesh* cut, const om_options*
         free(k);
         if (bad) {
             R->status = opt->enforce_gp ? OM_GENERAL_POSITION_VIOLATION : OM_INVALID_MESH_INTERSECTION;
+            R->partial = 0;
             goto done;
         }
     }
@@ -1197,6 +1201,7 @@ void om_free(om_result* R)
 
 int om_status(const om_result* R) { return R->status; }
 int om_stage(const om_result* R) { return R->stage; }
+int om_partial(const om_result* R) { return R->partial; }
 double om_time_ms(const om_result* R) { return R->ms; }
 
 void om_shift(const om_result* R, double com[3], double t[3], double* eps)

@@ -52,6 +52,10 @@ om_result* om_frontend(const om_mesh* src, const om_mesh* cut, const om_options*
 void om_free(om_result* r);
 int om_status(const om_result* r);
 int om_stage(const om_result* r); /* 0 faces, 1 narrow phase, 2 empty registry, 3 pair check, 4 done */
+/* 1: the front end succeeded and its registry holds an intersection on a
+ * cut-mesh border edge (kernel.cpp:2646-2650): a partial cut, which ends the
+ * dispatch early under MC_DISPATCH_REQUIRE_THROUGH_CUTS (kernel.cpp:3269-3278) */
+int om_partial(const om_result* r);
 void om_shift(const om_result* r, double com[3], double t[3], double* eps);
 int64_t om_count(const om_result* r, int what);
 void om_copy_pairs(const om_result* r, uint32_t* pairs);

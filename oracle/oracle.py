@@ -42,7 +42,7 @@ def lib():
         L.om_frontend.argtypes = [C.POINTER(_Mesh), C.POINTER(_Mesh), C.POINTER(_Opt)]
         L.om_frontend.restype = C.c_void_p
         L.om_free.argtypes = [C.c_void_p]
-        for fn in ("om_status", "om_stage"):
+        for fn in ("om_status", "om_stage", "om_partial"):
             getattr(L, fn).argtypes = [C.c_void_p]
             getattr(L, fn).restype = C.c_int
         L.om_time_ms.argtypes = [C.c_void_p]
@@ -101,6 +101,7 @@ def frontend(src, cut, gp_constant=1e-4, enforce_gp=False, eps_absolute=False, e
         R = Result()
         R["status"] = L.om_status(h)
         R["stage"] = L.om_stage(h)
+        R["partial"] = L.om_partial(h)
         R["ms"] = L.om_time_ms(h)
         com, t, eps = np.zeros(3), np.zeros(3), np.zeros(1)
         L.om_shift(h, _ptr(com), _ptr(t), _ptr(eps))

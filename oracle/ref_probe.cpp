@@ -105,20 +105,36 @@ static void build_hmesh(hmesh_t& m, const double* v, const uint32_t* f, const ui
 
 } // namespace
 
-extern "C" {
-
 // --- public API, end to end ------------------------------------------------
 
-int ref_dispatch(uint32_t flags, int helpers,
-                 const double* sv, const uint32_t* sf, const uint32_t* ssz, uint32_t nsv, uint32_t nsf,
-                 const double* cv, const uint32_t* cf, const uint32_t* csz, uint32_t ncv, uint32_t ncf)
+// the kernel's reports to the debug callback, one a line: preproc.cpp:2947-2955
+// passes "<status_t name> : <reason>" before it throws, and the
+// general-position loop reports each violation (preproc.cpp:2596-2610)
+std::string g_kernel_message;
+
+static void MCAPI_PTR on_debug_message(McDebugSource source, McDebugType, unsigned int, McDebugSeverity, size_t,
+                                       const char* message, const McVoid*)
+{
+    if (source == MC_DEBUG_SOURCE_KERNEL && message)
+        g_kernel_message += std::string(message) + "\n";
+}
+
+static int dispatch_impl(uint32_t flags, int helpers, bool debug,
+                         const double* sv, const uint32_t* sf, const uint32_t* ssz, uint32_t nsv, uint32_t nsf,
+                         const double* cv, const uint32_t* cf, const uint32_t* csz, uint32_t ncv, uint32_t ncf)
 {
     g_dispatch = dispatch_result();
+    g_kernel_message.clear();
     McContext ctx = MC_NULL_HANDLE;
-    McResult r = helpers > 0 ? mcCreateContextWithHelpers(&ctx, MC_NULL_HANDLE, (uint32_t)helpers)
-                             : mcCreateContext(&ctx, MC_NULL_HANDLE);
+    const McFlags cflags = debug ? MC_DEBUG : MC_NULL_HANDLE;
+    McResult r = helpers > 0 ? mcCreateContextWithHelpers(&ctx, cflags, (uint32_t)helpers)
+                             : mcCreateContext(&ctx, cflags);
     if (r != MC_NO_ERROR)
         return -1000 - (int)r;
+    if (debug) {
+        mcDebugMessageCallback(ctx, on_debug_message, nullptr);
+        mcDebugMessageControl(ctx, MC_DEBUG_SOURCE_ALL, MC_DEBUG_TYPE_ALL, MC_DEBUG_SEVERITY_ALL, true);
+    }
     auto t0 = std::chrono::steady_clock::now();
     r = mcDispatch(ctx, flags, sv, sf, ssz, nsv, nsf, cv, cf, csz, ncv, ncf);
     auto t1 = std::chrono::steady_clock::now();
@@ -151,6 +167,29 @@ int ref_dispatch(uint32_t flags, int helpers,
     mcReleaseContext(ctx);
     return g_dispatch.status;
 }
+
+extern "C" {
+
+// --- public API, end to end ------------------------------------------------
+
+
+int ref_dispatch(uint32_t flags, int helpers,
+                 const double* sv, const uint32_t* sf, const uint32_t* ssz, uint32_t nsv, uint32_t nsf,
+                 const double* cv, const uint32_t* cf, const uint32_t* csz, uint32_t ncv, uint32_t ncf)
+{
+    return dispatch_impl(flags, helpers, false, sv, sf, ssz, nsv, nsf, cv, cf, csz, ncv, ncf);
+}
+
+// mcDispatch in a context created with MC_DEBUG and a debug callback that
+// keeps the kernel's failure report (ref_kernel_message)
+int ref_dispatch_debug(uint32_t flags,
+                       const double* sv, const uint32_t* sf, const uint32_t* ssz, uint32_t nsv, uint32_t nsf,
+                       const double* cv, const uint32_t* cf, const uint32_t* csz, uint32_t ncv, uint32_t ncf)
+{
+    return dispatch_impl(flags, 0, true, sv, sf, ssz, nsv, nsf, cv, cf, csz, ncv, ncf);
+}
+
+const char* ref_kernel_message() { return g_kernel_message.c_str(); }
 
 double ref_dispatch_ms() { return g_dispatch.ms; }
 uint32_t ref_cc_count() { return (uint32_t)g_dispatch.ccs.size(); }

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "_ref", "libref_probe.so")
 _lib = None
 
 MC_DISPATCH_VERTEX_ARRAY_DOUBLE = 1 << 1
+MC_DISPATCH_REQUIRE_THROUGH_CUTS = 1 << 2
 MC_DISPATCH_INCLUDE_VERTEX_MAP = 1 << 3
 MC_DISPATCH_FILTER_SEAM_SRCMESH = 1 << 13
 MC_DISPATCH_FILTER_SEAM_CUTMESH = 1 << 14
@@ -41,6 +42,9 @@ def lib():
         L.ref_dispatch.argtypes = [C.c_uint32, C.c_int] + mesh + mesh
         L.ref_dispatch.restype = C.c_int
         L.ref_dispatch_ms.restype = C.c_double
+        L.ref_dispatch_debug.argtypes = [C.c_uint32] + mesh + mesh
+        L.ref_dispatch_debug.restype = C.c_int
+        L.ref_kernel_message.restype = C.c_char_p
         L.ref_cc_count.restype = C.c_uint32
         L.ref_cc_sizes.argtypes = [C.c_uint32, C.POINTER(C.c_uint64)]
         L.ref_cc_sizes.restype = C.c_uint32
@@ -116,6 +120,38 @@ def dispatch(src, cut, flags, helpers=0, workdir=None):
         ccs.append(dict(type=typ, vertices=v.reshape(-1, 3), faces=f, face_sizes=fs, seam=se, vmap=vm,
                         perturbation=pv))
     return st, ms, ccs
+
+
+KERNEL_STATUS = ("INVALID_SRC_MESH", "INVALID_CUT_MESH", "INVALID_MESH_INTERSECTION", "GENERAL_POSITION_VIOLATION")
+
+
+def dispatch_status(src, cut, flags, workdir=None):
+    """mcDispatch in a debug context: (McResult, kernel status name).  The
+    reference maps several kernel statuses onto one McResult
+    (preproc.cpp:580-600); the status is read from its reports to the debug
+    callback: "<status> : <reason>" before it throws (preproc.cpp:2947-2955),
+    or a general-position violation once the perturbation attempts run out
+    (preproc.cpp:2596-2612).  "" when the dispatch succeeded."""
+    L = lib()
+    keep_s, sa = _mesh_args(src)
+    keep_c, ca = _mesh_args(cut)
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory(dir=workdir) as d:
+        os.chdir(d)
+        try:
+            st = L.ref_dispatch_debug(flags, *sa, *ca)
+        finally:
+            os.chdir(cwd)
+    lines = L.ref_kernel_message().decode().splitlines()
+    if st == 0:
+        return st, ""
+    for line in lines:
+        head = line.split(" : ")[0]
+        if head in KERNEL_STATUS:
+            return st, head
+    if "general position assumption violated!" in lines:
+        return st, "GENERAL_POSITION_VIOLATION"
+    return st, ""
 
 
 def seam_points(ccs):

@@ -16,6 +16,19 @@ Fixtures:
                      (seam vertices), perturbation vector, CC count
   ref_ps_edges.npz   polygon-soup edge tables (halfedge-0 endpoints and the
                      faces of both halfedges) built with the reference's hmesh_t
+  ref_status.npz     the kernel status the reference reports through its
+                     debug callback for cases that end in one: the 64
+                     benchmark pairs, five non-planar quads cut by a
+                     triangle (INVALID_MESH_INTERSECTION), and benchmark
+                     pair 013 with and without MC_DISPATCH_REQUIRE_THROUGH_CUTS
+                     (the partial-cut early exit of tests/source/
+                     computeSeams.cpp:356-398, which expects 2 components)
+  ref_config2.npz    the reference's broad phase on the two subdivision-7
+                     icospheres (327,680 triangles each): shift parameters,
+                     candidate-pair count and a SHA-256 of the sorted pairs
+
+Pass fixture names (functions, cases, edges, status, config2) to write only
+those.
 """
 from __future__ import annotations
 
@@ -167,11 +180,86 @@ def case(name, src, cut, flags=FLAGS, store_pairs=True):
     return d
 
 
+# five non-planar quads, each cut by one triangle: the triangle crosses the
+# quad's Newell plane inside the quad's projection once, so one (source face,
+# cut face) pair holds a single intersection point (found by a seeded search
+# with the oracle; the reference's status is what the fixture records)
+def nonplanar_quad_cases(count=5, seed=1):
+    rng = np.random.default_rng(seed)
+    out = []
+    from oracle import oracle as om
+    while len(out) < count:
+        h = rng.uniform(0.1, 0.8)
+        Q = np.array([[0, 0, 0], [1, 0, 0], [1, 1, h], [0, 1, 0]], float) + rng.normal(size=(4, 3)) * 0.01
+        c = rng.uniform(-0.2, 1.2, (3, 3))
+        c[:, 2] = rng.uniform(-0.5, 1.0, 3)
+        src = (Q, np.array([0, 1, 2, 3], np.uint32), np.array([4], np.uint32))
+        cut = (c, np.array([0, 1, 2], np.uint32), None)
+        if om.frontend(src, cut, detail=False)["status"] == om.INVALID_MESH_INTERSECTION:
+            out.append((src, cut))
+    return out
+
+
+def status_fixture():
+    d = {}
+    enforce = FLAGS | rp.MC_DISPATCH_ENFORCE_GENERAL_POSITION
+    for i in range(64):
+        s, c = meshes.bench_pair(i)
+        st, name = rp.dispatch_status(s, c, FLAGS)
+        d[f"bench{i:03d}/result"] = np.int32(st)
+        d[f"bench{i:03d}/status"] = np.array(name)
+    for k, (s, c) in enumerate(nonplanar_quad_cases()):
+        st, name = rp.dispatch_status(s, c, FLAGS)
+        st2, name2 = rp.dispatch_status(s, c, enforce)
+        d[f"quad{k}/src"] = s[0]
+        d[f"quad{k}/cut"] = c[0]
+        d[f"quad{k}/result"] = np.int32(st)
+        d[f"quad{k}/status"] = np.array(name)
+        d[f"quad{k}/enforce_status"] = np.array(name2)
+        print(f"quad{k}: {name} / enforced {name2}")
+    s, c = meshes.bench_pair(13)
+    plain = rp.MC_DISPATCH_VERTEX_ARRAY_DOUBLE
+    for tag, fl in (("through", plain | rp.MC_DISPATCH_REQUIRE_THROUGH_CUTS), ("plain", plain)):
+        st, _, ccs = rp.dispatch(s, c, fl)
+        d[f"bench013_{tag}/result"] = np.int32(st)
+        d[f"bench013_{tag}/n_cc"] = np.int32(len(ccs))
+        d[f"bench013_{tag}/n_input_cc"] = np.int32(sum(cc["type"] == rp.CC_INPUT for cc in ccs))
+        print(f"bench013 {tag}: result {st}, {len(ccs)} components")
+    np.savez_compressed(os.path.join(HERE, "ref_status.npz"), **d)
+
+
+def config2_fixture():
+    import hashlib
+    src, cut = mg.config2(level=7)
+    b = rp.broad_phase(src, cut)
+    p = b["pairs"].astype(np.uint32)
+    p = np.ascontiguousarray(p[np.lexsort((p[:, 1], p[:, 0]))])
+    d = {"com": b["com"], "t": b["t"], "eps": np.float64(b["eps"]), "n_pairs": np.int64(p.shape[0]),
+         "pairs_sha256": np.array(hashlib.sha256(p.tobytes()).hexdigest()),
+         "src_face_aabb_sha256": np.array(hashlib.sha256(np.ascontiguousarray(b["src_face_aabb"]).tobytes()).hexdigest()),
+         "cut_face_aabb_sha256": np.array(hashlib.sha256(np.ascontiguousarray(b["cut_face_aabb"]).tobytes()).hexdigest())}
+    print(f"config2: {p.shape[0]} candidate pairs")
+    np.savez_compressed(os.path.join(HERE, "ref_config2.npz"), **d)
+
+
 def main():
     if not rp.available():
         sys.exit("build the reference first: make -C oracle ref")
-    fv = function_vectors()
-    np.savez_compressed(os.path.join(HERE, "ref_functions.npz"), **fv)
+    which = set(sys.argv[1:]) or {"functions", "cases", "edges", "status", "config2"}
+    if "status" in which:
+        status_fixture()
+    if "config2" in which:
+        config2_fixture()
+    if "functions" in which:
+        fv = function_vectors()
+        np.savez_compressed(os.path.join(HERE, "ref_functions.npz"), **fv)
+    if "cases" in which:
+        cases_fixture()
+    if "edges" in which:
+        edges_fixture()
+
+
+def cases_fixture():
     cases = {}
     src, cut = mg.config1()
     cases.update(case("config1", src, cut))
@@ -188,6 +276,9 @@ def main():
         s, c = meshes.icosphere_pair(lvl, amp=amp)
         cases.update(case(f"ico{lvl}_{int(amp * 1000):03d}", s, c))
     np.savez_compressed(os.path.join(HERE, "ref_cases.npz"), **cases)
+
+
+def edges_fixture():
     edges = {}
     for name, (s, c) in [("config1_rot", meshes.config1_rotated()), ("ico3_020", meshes.icosphere_pair(3))] + \
             [(f"bench{i:03d}", meshes.bench_pair(i)) for i in range(64)]:

@@ -45,11 +45,11 @@ def _c_layout(tmp_path):
         "mcgpu_frontend_info": ["status", "gp_violation", "num_candidate_pairs", "num_candidate_faces", "num_edges",
                                 "num_tests", "num_escalated", "num_registry", "num_raw_pairs", "num_decided_f32",
                                 "num_decided_f64", "com",
-                                "translation", "eps", "device_ms", "reserved"],
+                                "translation", "eps", "device_ms", "partial_cut"],
         "mcgpu_batch": ["num_pairs", "src", "cut", "src_vertex_offsets", "src_face_offsets", "cut_vertex_offsets",
                         "cut_face_offsets"],
         "mcgpu_server_limits": ["max_cut_vertices", "max_cut_faces", "max_cut_corners", "max_slots", "hbm_budget"],
-        "mcgpu_request_result": ["arrival_s", "start_s", "done_s", "status", "reserved", "num_registry",
+        "mcgpu_request_result": ["arrival_s", "start_s", "done_s", "status", "partial_cut", "num_registry",
                                  "num_candidate_pairs"],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mcgpu.h"', "int main(void) {"]
