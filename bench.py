@@ -8,8 +8,10 @@ coordinates, synthetic data.  A step is one full front end on the GPU
 planes, narrow phase, sorted registry, status) with the input meshes already
 resident in HBM.
 
-Prints one JSON line.  The CPU baseline is the single-threaded C restatement
-of the same front end (oracle/) timed on this host on the same input.
+Prints one JSON line.  The CPU baseline is MCUT's own front end timed on this
+host on the same input (the reference compiled from its sources with its stage
+timers on, a helper thread per usable core but one, and single-threaded); the
+single-threaded C restatement of the front end (oracle/) is timed beside it.
 
 The line also carries the batched mix ("batched"): 4096 seeded sphere/box
 pairs of uneven size, one batched dispatch against one plain dispatch per
@@ -89,13 +91,42 @@ def progress(msg):
 
 
 def cpu_baseline(src, cut, registry):
+    """MCUT's own front end on this host, on the same pair: the reference
+    compiled from its sources with its stage timers on (oracle/ref_timing.py),
+    one context with a helper thread per usable core but one
+    (mcCreateContextWithHelpers, the pool is capped at the hardware
+    concurrency, tpool.h:284-288), 1 warm-up dispatch, median of 5; and the
+    same single-threaded (plain mcCreateContext: no helpers), median of 3.
+    The single-threaded C restatement (oracle/) is timed beside them."""
+    from oracle import ref_timing as rt
     from oracle import oracle as om
+    out = {}
+    cores = rt.usable_cores()
+    if rt.available():
+        helpers = max(1, cores - 1)
+        workdir = "/dev/shm" if os.path.isdir("/dev/shm") else None  # MCUT's debug meshes land in RAM
+        mt = rt.time_frontend(src, cut, helpers=helpers, runs=5, warmup=1, workdir=workdir)
+        progress(f"cpu baseline: reference {mt['median_ms']:.0f} ms with {helpers} helpers")
+        st = rt.time_frontend(src, cut, helpers=0, runs=3, warmup=1, workdir=workdir)
+        progress(f"cpu baseline: reference {st['median_ms']:.0f} ms single-threaded")
+        out = {"value": registry / (mt["median_ms"] * 1e-3), "unit": "pairs/s", "cores": cores, "kind": "reference",
+               "sample": f"MCUT's front end (sum of its own stage timers, preproc.cpp:2304 -> kernel.cpp:3229) on the "
+                         f"whole pair, mcCreateContextWithHelpers with {helpers} helpers, median of 5 after 1 warm-up "
+                         f"({mt['median_ms']:.0f} ms; min {mt['min_ms']}, max {mt['max_ms']})",
+               "cpu_model": rt.cpu_model(), "nproc": os.cpu_count(), "helpers": helpers,
+               "frontend_ms": mt["median_ms"], "runs_ms": mt["runs_ms"], "stage_ms": mt["stage_ms"],
+               "single_thread": {"value": registry / (st["median_ms"] * 1e-3), "frontend_ms": st["median_ms"],
+                                 "runs_ms": st["runs_ms"], "stage_ms": st["stage_ms"]}}
     t0 = time.perf_counter()
     O = om.frontend(src, cut, detail=False)
     dt = time.perf_counter() - t0
     assert O["counts"]["registry"] == registry, (O["counts"], registry)
-    return {"value": registry / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
-            "sample": f"the whole config front end once on the host ({dt:.2f} s, single thread)"}
+    port = {"value": registry / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": f"the C restatement of the front end once on the whole pair ({dt:.2f} s, single thread)"}
+    if not out:
+        return port
+    out["port"] = port
+    return out
 
 
 def bench_batch(args, torch, frontend, mg):
@@ -376,10 +407,15 @@ def main():
     n = dict(V=src[0].shape[0] + cut[0].shape[0], F=Fs + Fc, K=src[1].size + cut[1].size, E=counts["edges"],
              pairs=counts["pairs"], raw_pairs=counts["raw_pairs"], tests=counts["tests"], registry=I,
              cand=counts["cand"], nodes=nodes, Fs=Fs, nodes_cut=(2 << max(0, int(np.ceil(np.log2(Fc))))) - 1)
-    # the roofline kernel: the longest stage that one kernel carries alone
+    # the roofline kernel: the longest stage that one kernel carries alone,
+    # timed by the kernel's own dispatch timestamps where the library records
+    # them (the traversal), else by the events around it on its stream
     dom = max((k for k in stages if k in STAGE_KERNEL), key=stages.get)
     dom_bytes = stage_bytes(dom, n)
-    achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9 if stages[dom] > 0 else 0.0
+    kernel_ms = stages.get("traversal_kernel", 0.0) if dom == "traversal" else 0.0
+    if kernel_ms <= 0.0:
+        kernel_ms = stages[dom]
+    achieved = dom_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
     lat_sorted = sorted(lat)
 
     line = {
@@ -402,7 +438,8 @@ def main():
         "latency_ms": {"p50": lat_sorted[len(lat) // 2], "p99": lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))]},
         "counts": counts,
         "stage_ms": stages,
-        "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "stage": dom, "kernel_ms": stages[dom],
+        "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "stage": dom, "kernel_ms": kernel_ms,
+                     "stage_event_ms": stages[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "alg_bytes": dom_bytes,
                      "frontend_alg_bytes": sum(stage_bytes(k, n) for k in stages)},
@@ -428,6 +465,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline(src, cut, I)
         progress("single pair: cpu baseline done")
         line["speedup_vs_cpu_baseline"] = line["value"] / line["cpu_baseline"]["value"]
+        if "single_thread" in line["cpu_baseline"]:
+            line["speedup_vs_cpu_single_thread"] = line["value"] / line["cpu_baseline"]["single_thread"]["value"]
     ctx.close()
     del keep
     if not args.no_batch:

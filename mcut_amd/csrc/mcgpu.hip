@@ -2,6 +2,8 @@
 // the enqueue order of the front-end pipeline.
 #include "mcgpu.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -60,17 +62,19 @@ enum Stage {
     ST_TESTS,     // A: edge-face tests
     ST_NARROW,    // A: narrow phase
     ST_REGISTRY,  // A: registry sort, status
+    ST_TRAV_KERNEL, // B: the traversal kernel alone, from its own dispatch timestamps
     ST_COUNT
 };
 
 const char* kStageNames[ST_COUNT] = {"coordinate_sums", "shift_params", "shift_boxes", "topology",
                                      "bvh_sort",        "bvh_refit",    "traversal",   "pair_filter",
-                                     "face_planes",     "tests",        "narrow",      "registry"};
+                                     "face_planes",     "tests",        "narrow",      "registry",
+                                     "traversal_kernel"};
 
 // event slots: stage boundaries on the three streams
 enum Ev { EV_A0, EV_S0, EV_A_SUMS, EV_A_PARAMS, EV_A_SHIFT, EV_A_JOIN, EV_A_FILTER, EV_A_PLANES, EV_A_JOIN2, EV_A_TESTS, EV_A_NARROW,
           EV_A_END, EV_FORK, EV_B0, EV_B_SORT, EV_B_REFIT, EV_B_TRAV, EV_C0, EV_C_FILL, EV_C_TOPO, EV_SUM_ITEMS,
-          EV_SUM_CHAIN, EV_COUNT };
+          EV_SUM_CHAIN, EV_TRAV_K0, EV_TRAV_K1, EV_COUNT };
 
 uint64_t next_pow2(uint64_t x)
 {
@@ -686,7 +690,7 @@ struct BroadView {
 // Returns whether the sorted leaves ended in mface2.  b_sort / b_refit: the
 // stage events (null: none recorded).
 bool enqueue_trees_traverse(const BroadView& B, hipStream_t st, Counters* C, bool ghist_zeroed, hipEvent_t b_sort,
-                            hipEvent_t b_refit)
+                            hipEvent_t b_refit, hipEvent_t k_start = nullptr, hipEvent_t k_stop = nullptr)
 {
     const uint32_t Fs = B.nf[0];
     const uint64_t F = (uint64_t)Fs + B.nf[1];
@@ -743,7 +747,12 @@ bool enqueue_trees_traverse(const BroadView& B, hipStream_t st, Counters* C, boo
     }
     if (b_refit) (void)hipEventRecord(b_refit, st);
     const unsigned nwaves = (Fs + kWaveLeaves - 1) / kWaveLeaves;
-    k_traverse_wave<<<nwaves < 65536 ? nwaves : 65536, 64, 0, st>>>(T[0], T[1], B.raw_pairs, B.cap_pairs, C);
+    const unsigned grid = nwaves < 65536 ? nwaves : 65536;
+    if (k_start) // the launch itself records the kernel's start and end (its dispatch timestamps)
+        hipExtLaunchKernelGGL(k_traverse_wave, dim3(grid), dim3(64), 0, st, k_start, k_stop, 0, T[0], T[1], B.raw_pairs,
+                              B.cap_pairs, C);
+    else
+        k_traverse_wave<<<grid, 64, 0, st>>>(T[0], T[1], B.raw_pairs, B.cap_pairs, C);
     return alt;
 }
 
@@ -770,8 +779,10 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool g
         B.src_nodes = ctx->res->nodes;
         B.src_depth = ctx->res->depth;
     }
-    ctx->mface_alt = enqueue_trees_traverse(B, st, C, ghist_zeroed, ctx->stage_timing ? ctx->ev[EV_B_SORT] : nullptr,
-                                            ctx->stage_timing ? ctx->ev[EV_B_REFIT] : nullptr);
+    const bool t = ctx->stage_timing;
+    ctx->mface_alt = enqueue_trees_traverse(B, st, C, ghist_zeroed, t ? ctx->ev[EV_B_SORT] : nullptr,
+                                            t ? ctx->ev[EV_B_REFIT] : nullptr, t ? ctx->ev[EV_TRAV_K0] : nullptr,
+                                            t ? ctx->ev[EV_TRAV_K1] : nullptr);
     (void)hipEventRecord(ctx->ev[EV_B_TRAV], st);
 }
 
@@ -1062,6 +1073,7 @@ void compute_stage_times(mcgpu_context ctx)
     ctx->stage_ms[ST_TESTS] = el(EV_A_JOIN2, EV_A_TESTS);
     ctx->stage_ms[ST_NARROW] = el(EV_A_TESTS, EV_A_NARROW);
     ctx->stage_ms[ST_REGISTRY] = el(EV_A_NARROW, EV_A_END);
+    ctx->stage_ms[ST_TRAV_KERNEL] = full ? el(EV_TRAV_K0, EV_TRAV_K1) : 0.f;
     ctx->stages_ready = true;
 }
 

@@ -461,10 +461,10 @@ char om_point_in_polygon(const double* p, const double* P, int n, const double* 
  * triplet with non-zero orient2d of the projected polygon, in lexicographic
  * order, is sorted by std::sort with "greater |orient2d|" and the first element
  * is taken.  The winner among equal keys depends on the sort's element moves, so
- * the GCC libstdc++ introsort (16-element insertion-sort threshold, median-of-
- * three pivot, unguarded partition, final insertion sort) is replayed below.
- * Returns 0 when every triplet is collinear, -1 when the replay would fall back
- * to heap sort (depth limit exhausted) or the polygon has more than 16 vertices. */
+ * the GCC libstdc++ std::sort is replayed below: introsort with a 16-element
+ * insertion-sort threshold, median-of-three pivot moved to the front, unguarded
+ * partition, heap sort of a range whose depth limit (2 floor(log2 n)) runs out,
+ * and a final insertion sort.  Returns 0 when every triplet is collinear. */
 typedef struct {
     int i, j, k;
     double a; /* |orient2d| */
@@ -531,34 +531,95 @@ static om_trip* trip_partition(om_trip* first, om_trip* last, om_trip* pivot)
     }
 }
 
-static int trip_introsort_loop(om_trip* first, om_trip* last, int depth)
+/* the binary heap of libstdc++'s heap sort (a max-heap under trip_less):
+ * sift the hole at `hole` down to a leaf along the larger children, then
+ * push `value` up from there, no higher than `top` */
+static void trip_push_heap(om_trip* first, long hole, long top, om_trip value)
+{
+    long parent = (hole - 1) / 2;
+    while (hole > top && trip_less(first + parent, &value)) {
+        first[hole] = first[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    first[hole] = value;
+}
+
+static void trip_adjust_heap(om_trip* first, long hole, long len, om_trip value)
+{
+    const long top = hole;
+    long child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (trip_less(first + child, first + (child - 1))) child--;
+        first[hole] = first[child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        first[hole] = first[child - 1];
+        hole = child - 1;
+    }
+    trip_push_heap(first, hole, top, value);
+}
+
+/* std::__partial_sort(first, last, last): make the heap, then pop it */
+static void trip_heap_sort(om_trip* first, om_trip* last)
+{
+    const long len = last - first;
+    if (len >= 2)
+        for (long parent = (len - 2) / 2;; --parent) {
+            trip_adjust_heap(first, parent, len, first[parent]);
+            if (parent == 0) break;
+        }
+    while (last - first > 1) {
+        --last;
+        om_trip value = *last;
+        *last = *first;
+        trip_adjust_heap(first, 0, last - first, value);
+    }
+}
+
+static void trip_introsort_loop(om_trip* first, om_trip* last, int depth)
 {
     while (last - first > 16) {
-        if (depth == 0) return -1;
+        if (depth == 0) {
+            trip_heap_sort(first, last);
+            return;
+        }
         --depth;
         om_trip* mid = first + (last - first) / 2;
         trip_median_to_first(first, first + 1, mid, last - 1);
         om_trip* cut = trip_partition(first + 1, last, first);
-        if (trip_introsort_loop(cut, last, depth) < 0) return -1;
+        trip_introsort_loop(cut, last, depth);
         last = cut;
     }
-    return 0;
 }
 
-static int trip_sort(om_trip* first, om_trip* last)
+static void trip_sort(om_trip* first, om_trip* last)
 {
     long n = last - first;
-    if (n <= 0) return 0;
+    if (n <= 0) return;
     int lg = 0;
     while ((1L << (lg + 1)) <= n) ++lg;
-    if (trip_introsort_loop(first, last, 2 * lg) < 0) return -1;
+    trip_introsort_loop(first, last, 2 * lg);
     if (n > 16) {
         trip_insertion_sort(first, first + 16);
         for (om_trip* i = first + 16; i != last; ++i) trip_linear_insert(i);
     } else {
         trip_insertion_sort(first, last);
     }
-    return 0;
+}
+
+/* the replayed std::sort over caller-given keys (for tests of the replay
+ * against libstdc++ itself): order[] receives the original positions */
+void om_trip_sort_keys(const double* keys, long n, int* order)
+{
+    om_trip* T = (om_trip*)malloc(sizeof(om_trip) * (size_t)(n > 0 ? n : 1));
+    for (long i = 0; i < n; ++i) T[i].i = (int)i, T[i].j = T[i].k = 0, T[i].a = keys[i];
+    trip_sort(T, T + n);
+    for (long i = 0; i < n; ++i) order[i] = T[i].i;
+    free(T);
 }
 
 int om_best_triple(const double* P, int n, const double* normal, int k, int* ijk)
@@ -567,11 +628,12 @@ int om_best_triple(const double* P, int n, const double* normal, int k, int* ijk
         ijk[0] = 0, ijk[1] = 1, ijk[2] = 2;
         return 1;
     }
-    if (n > 16) return -1;
-    double M[6], X[2 * 16];
+    double M[6];
+    double* X = (double*)malloc(sizeof(double) * 2 * (size_t)n);
     projection_matrix(normal, k, M);
     for (int i = 0; i < n; ++i) project(M, P + 3 * i, X + 2 * i);
-    om_trip T[560];
+    const long cap = (long)n * (n - 1) * (n - 2) / 6;
+    om_trip* T = (om_trip*)malloc(sizeof(om_trip) * (size_t)cap);
     int m = 0;
     for (int a = 0; a < n; ++a)
         for (int b = a + 1; b < n; ++b)
@@ -581,9 +643,14 @@ int om_best_triple(const double* P, int n, const double* normal, int k, int* ijk
                 T[m].i = a, T[m].j = b, T[m].k = c, T[m].a = fabs(r);
                 ++m;
             }
-    if (m == 0) return 0;
-    if (trip_sort(T, T + m) < 0) return -1;
+    free(X);
+    if (m == 0) {
+        free(T);
+        return 0;
+    }
+    trip_sort(T, T + m);
     ijk[0] = T[0].i, ijk[1] = T[0].j, ijk[2] = T[0].k;
+    free(T);
     return 1;
 }
 
@@ -627,6 +694,7 @@ typedef struct {
 } om_edge;
 
 struct om_result {
+    double* pbuf;                  /* vertices of one polygon, sized for the largest face */
     int status, stage;
     int partial;                   /* a successful front end whose registry holds a cut-mesh border edge */
     double com[3], t[3], eps, ms;
@@ -1018,11 +1086,15 @@ om_result* om_frontend(const om_mesh* src, const om_mesh* cut, const om_options*
         R->plane_ijk = (int32_t*)malloc(sizeof(int32_t) * 3 * (R->ncand + 1));
         int64_t c = 0;
         int bad = -1;
+        uint32_t maxn = 3;
+        for (uint32_t f = 0; f < R->Fps; ++f)
+            if (R->face_off[f + 1] - R->face_off[f] > maxn) maxn = R->face_off[f + 1] - R->face_off[f];
+        R->pbuf = (double*)malloc(sizeof(double) * 3 * maxn); /* one polygon's vertices */
+        double* P = R->pbuf;
         for (uint32_t f = 0; f < R->Fps; ++f) {
             if (!is_cand[f]) continue;
             R->cand[c] = f;
             uint32_t n = R->face_off[f + 1] - R->face_off[f];
-            double P[3 * 64];
             for (uint32_t i = 0; i < n; ++i)
                 memcpy(P + 3 * i, R->ps_v + 3 * R->face_vtx[R->face_off[f] + i], 3 * sizeof(double));
             R->plane_k[c] = om_plane(P, (int)n, R->plane_n + 3 * c, R->plane_d + c);
@@ -1071,7 +1143,7 @@ om_result* om_frontend(const om_mesh* src, const om_mesh* cut, const om_options*
             int k = R->plane_k[lo];
             const int* ijk = R->plane_ijk + 3 * lo;
             uint32_t n = R->face_off[g + 1] - R->face_off[g];
-            double P[3 * 64];
+            double* P = R->pbuf;
             for (uint32_t j = 0; j < n; ++j)
                 memcpy(P + 3 * j, R->ps_v + 3 * R->face_vtx[R->face_off[g] + j], 3 * sizeof(double));
             const double* q = R->ps_v + 3 * R->edges[e].v0; /* source(h0) */
@@ -1194,7 +1266,7 @@ void om_free(om_result* R)
     if (!R) return;
     void* ptrs[] = {R->ps_v, R->face_off, R->face_vtx, R->face_edge, R->edges, R->aabb, R->pairs, R->tests,
                     R->qs, R->rs, R->type, R->pip, R->reg, R->reg_p, R->cand, R->plane_n, R->plane_d,
-                    R->plane_k, R->plane_ijk};
+                    R->plane_k, R->plane_ijk, R->pbuf};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
     free(R);
 }

@@ -72,6 +72,9 @@ double om_orient2d(const double* a, const double* b, const double* c);
 int om_orient3d_sign(const double* a, const double* b, const double* c, const double* d);
 int om_plane(const double* verts, int n, double* normal, double* d);
 int om_best_triple(const double* verts, int n, const double* normal, int k, int* ijk);
+/* libstdc++'s std::sort replayed over keys with the triplet ranking's
+ * comparator (greater key first): order[i] = original position of the i-th */
+void om_trip_sort_keys(const double* keys, long n, int* order);
 char om_segment_type(const double* q, const double* r, const double* verts, int n, const int* ijk, int8_t* qs,
                      int8_t* rs);
 void om_segment_point(const double* normal, double d, const double* q, const double* r, double* p);

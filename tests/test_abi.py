@@ -93,9 +93,9 @@ def test_result_strings_and_stage_names():
         s = L.mcgpu_result_string(code).decode()
         assert s and s != "unknown"
     assert L.mcgpu_result_string(7).decode() == "unknown"
-    names = [L.mcgpu_stage_name(i).decode() for i in range(12)]
-    assert names[0] == "coordinate_sums" and names[-1] == "registry" and all(names)
-    assert L.mcgpu_stage_name(12) == b"" and L.mcgpu_stage_name(-1) == b""
+    names = [L.mcgpu_stage_name(i).decode() for i in range(13)]
+    assert names[0] == "coordinate_sums" and names[11] == "registry" and names[12] == "traversal_kernel" and all(names)
+    assert L.mcgpu_stage_name(13) == b"" and L.mcgpu_stage_name(-1) == b""
 
 
 def test_null_arguments_rejected_without_a_device():
