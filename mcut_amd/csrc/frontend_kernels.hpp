@@ -1797,7 +1797,8 @@ __device__ inline bool best_triple_small(int n, VertexAt vat, const double* norm
 }
 
 __global__ void k_planes(Soup S, const uint32_t* __restrict__ cand_list, Plane* __restrict__ planes,
-                         uint32_t* __restrict__ big_list, uint64_t cap_big, Counters* C)
+                         uint32_t* __restrict__ big_list, unsigned long long* __restrict__ big_off, uint64_t cap_big,
+                         Counters* C)
 {
     const uint64_t nc = C->n_cand;
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nc; w += (uint64_t)gridDim.x * blockDim.x) {
@@ -1816,10 +1817,17 @@ __global__ void k_planes(Soup S, const uint32_t* __restrict__ cand_list, Plane* 
         } else if (fv.n > 3 && fv.n <= 5) {
             if (!best_triple_small((int)fv.n, fv, pl.n, k, pl.tri)) pl.tri[0] = 0xff;
         } else if (fv.n > 5 && fv.n <= kMaxPoly) {
+            // the projected vertices and every triplet, in the triplet scratch
             pl.big = 1;
+            const unsigned long long n = fv.n;
             const unsigned long long at = atomicAdd(&C->n_big, 1ull);
-            if (at < cap_big) big_list[at] = g;
-            else atomicOr(&C->ov_big, 1u);
+            const unsigned long long off = atomicAdd(&C->n_trip, n + n * (n - 1) * (n - 2) / 6);
+            if (at < cap_big) {
+                big_list[at] = g;
+                big_off[at] = off;
+            } else {
+                atomicOr(&C->ov_big, 1u);
+            }
         } else if (fv.n > kMaxPoly) {
             atomicOr(&C->unsupported, 1u);
             pair_flag(S, g, PF_UNSUPPORTED);
@@ -1828,137 +1836,40 @@ __global__ void k_planes(Soup S, const uint32_t* __restrict__ cand_list, Plane* 
     }
 }
 
-// Replay of GCC libstdc++'s std::sort on the triplet list (introsort with a
-// 16-element insertion-sort threshold, median-of-three pivot moved to the
-// front, unguarded partition, final insertion sort), so the first element
-// -- the triplet the reference uses -- matches for polygons of 6..16 vertices.
-struct Trip {
-    double a; // |orient2d|
-    int32_t ijk;
-};
-
-__device__ __forceinline__ bool trip_less(const Trip& x, const Trip& y) { return x.a > y.a; }
-
-__device__ inline void trip_linear_insert(Trip* last)
-{
-    Trip val = *last;
-    Trip* next = last - 1;
-    while (trip_less(val, *next)) {
-        *last = *next;
-        last = next;
-        --next;
-    }
-    *last = val;
-}
-
-__device__ inline void trip_insertion_sort(Trip* first, Trip* last)
-{
-    if (first == last) return;
-    for (Trip* i = first + 1; i != last; ++i) {
-        if (trip_less(*i, *first)) {
-            Trip val = *i;
-            for (Trip* p = i; p != first; --p) *p = *(p - 1);
-            *first = val;
-        } else {
-            trip_linear_insert(i);
-        }
-    }
-}
-
-__device__ __forceinline__ void trip_swap(Trip* x, Trip* y)
-{
-    Trip t = *x;
-    *x = *y;
-    *y = t;
-}
-
-__device__ inline int trip_sort(Trip* first, Trip* last)
-{
-    const long n = last - first;
-    if (n <= 0) return 0;
-    int lg = 0;
-    while ((1L << (lg + 1)) <= n) ++lg;
-    // iterative form of the introsort loop: the recursion on the right part
-    // happens before the loop continues on the left part
-    Trip* seg_first[64];
-    Trip* seg_last[64];
-    int seg_depth[64];
-    int top = 0;
-    seg_first[top] = first;
-    seg_last[top] = last;
-    seg_depth[top] = 2 * lg;
-    ++top;
-    while (top > 0) {
-        --top;
-        Trip* f = seg_first[top];
-        Trip* l = seg_last[top];
-        int depth = seg_depth[top];
-        while (l - f > 16) {
-            if (depth == 0) return -1;
-            --depth;
-            Trip* mid = f + (l - f) / 2;
-            Trip *a = f + 1, *b = mid, *c = l - 1;
-            if (trip_less(*a, *b)) {
-                if (trip_less(*b, *c)) trip_swap(f, b);
-                else if (trip_less(*a, *c)) trip_swap(f, c);
-                else trip_swap(f, a);
-            } else if (trip_less(*a, *c)) {
-                trip_swap(f, a);
-            } else if (trip_less(*b, *c)) {
-                trip_swap(f, c);
-            } else {
-                trip_swap(f, b);
-            }
-            Trip* lo = f + 1;
-            Trip* hi = l;
-            for (;;) {
-                while (trip_less(*lo, *f)) ++lo;
-                --hi;
-                while (trip_less(*f, *hi)) --hi;
-                if (!(lo < hi)) break;
-                trip_swap(lo, hi);
-                ++lo;
-            }
-            // recurse on [lo, l) first, then continue with [f, lo)
-            if (top >= 62) return -1;
-            // push the left part to be processed after the right part
-            seg_first[top] = f;
-            seg_last[top] = lo;
-            seg_depth[top] = depth;
-            ++top;
-            f = lo;
-        }
-    }
-    if (n > 16) {
-        trip_insertion_sort(first, first + 16);
-        for (Trip* i = first + 16; i != last; ++i) trip_linear_insert(i);
-    } else {
-        trip_insertion_sort(first, last);
-    }
-    return 0;
-}
-
-__global__ void k_big_polygons(Soup S, const uint32_t* __restrict__ big_list, const Counters* __restrict__ C,
-                               uint64_t cap_big, Plane* __restrict__ planes, Trip* __restrict__ scratch, Counters* Cw)
+// Candidate polygons of more than five vertices: one thread per polygon lists
+// its C(n, 3) triplets with non-zero orient2d in a region of the triplet
+// scratch that k_planes reserved (the n projected vertices first, as 16-byte
+// records, then the triplets) and ranks them with the replay of libstdc++'s
+// std::sort (tripsort.hpp).  A region past the scratch's capacity is left
+// undone and flagged; the host grows the scratch and runs the dispatch again.
+__global__ void k_big_polygons(Soup S, const uint32_t* __restrict__ big_list, const unsigned long long* __restrict__ big_off,
+                               const Counters* __restrict__ C, uint64_t cap_big, uint64_t cap_trip,
+                               Plane* __restrict__ planes, Trip* __restrict__ scratch, Counters* Cw)
 {
     const uint64_t nb = C->n_big < cap_big ? C->n_big : cap_big;
-    const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    Trip* T = scratch + tid * 560;
-    for (uint64_t w = tid; w < nb; w += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nb; w += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t g = big_list[w];
         const FaceView fv = face_view(S, g);
+        const uint64_t n = fv.n;
+        const uint64_t off = big_off[w];
+        if (off + n + n * (n - 1) * (n - 2) / 6 > cap_trip) {
+            atomicOr(&Cw->ov_trip, 1u);
+            continue;
+        }
         Plane pl = planes[g];
-        double M[6], X[2 * kMaxPoly];
+        double M[6];
         projection_matrix(pl.n, pl.k, M);
+        double* X = reinterpret_cast<double*>(scratch + off); // projected vertex i: X[2i], X[2i + 1]
         for (uint32_t i = 0; i < fv.n; ++i) {
             double p[3];
             fv((int)i, p);
             project2(M, p, X + 2 * i);
         }
-        int m = 0;
-        for (int a = 0; a < (int)fv.n; ++a)
-            for (int b = a + 1; b < (int)fv.n; ++b)
-                for (int c = b + 1; c < (int)fv.n; ++c) {
+        Trip* T = scratch + off + n;
+        long m = 0;
+        for (int a = 0; a < (int)n; ++a)
+            for (int b = a + 1; b < (int)n; ++b)
+                for (int c = b + 1; c < (int)n; ++c) {
                     const double r = orient2d_adaptive(X + 2 * a, X + 2 * b, X + 2 * c);
                     if (r == 0.0) continue;
                     T[m].a = fabs(r);
@@ -1967,9 +1878,8 @@ __global__ void k_big_polygons(Soup S, const uint32_t* __restrict__ big_list, co
                 }
         if (m == 0) {
             pl.tri[0] = 0xff;
-        } else if (trip_sort(T, T + m) < 0) {
-            atomicOr(&Cw->unsupported, 2u);
         } else {
+            trip_sort(T, T + m);
             pl.tri[0] = (uint8_t)(T[0].ijk & 0xff);
             pl.tri[1] = (uint8_t)((T[0].ijk >> 8) & 0xff);
             pl.tri[2] = (uint8_t)((T[0].ijk >> 16) & 0xff);

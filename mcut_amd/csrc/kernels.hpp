@@ -6,12 +6,13 @@
 #include <stdint.h>
 
 #include "geom.hpp"
+#include "tripsort.hpp"
 
 namespace mcgpu {
 
 constexpr uint64_t kEmpty64 = ~0ull;
 constexpr int kBlock = 256;
-constexpr int kMaxPoly = 16; // largest candidate polygon whose triplet choice is replayed
+constexpr int kMaxPoly = 255; // largest candidate polygon (its triplet is kept as three 8-bit vertex indices)
 
 struct Params {
     double com[3];      // shift x' = ((x - com) + t) + pert
@@ -36,13 +37,14 @@ struct Counters {
     unsigned long long n_fallback, n_lane_walks; // traversal statistics: node splits, per-lane walks
     unsigned long long n_f32, n_f64;            // tests whose two signs the FP32 / FP64 filter settled
     unsigned long long pmask; // pair hash slots - 1 (set once the raw pairs are known)
+    unsigned long long n_trip; // triplet-scratch entries reserved by the polygons of more than five vertices
     double zero;              // +0.0 (the per-dispatch reset clears it): padding the serial sums load
     unsigned int ov_pairs, ov_tests, ov_esc, ov_reg, ov_big;
-    // unsupported: bit 0 a candidate polygon of more than 16 vertices, bit 1
+    // unsupported: bit 0 a candidate polygon of more than kMaxPoly vertices, bit 1
     // coordinates whose broad-phase rounding bound is not finite
     unsigned int gp, bad_src, bad_cut, unsupported, nonmanifold, bad_input, pair_fail, partial;
     unsigned int sum_next; // batches: work-queue head of the serial sums
-    unsigned int pad;
+    unsigned int ov_trip;  // the triplet scratch overflowed
     // raw bounding box of each mesh as order keys (lo xyz, hi xyz); written
     // while the input is validated and kept by the per-dispatch counter reset
     unsigned long long rbox[2][6];

@@ -126,7 +126,7 @@ struct mcgpu_context_t {
     Buf ev0, ev1, ef0, ef1;
     Buf mcode, mface, mcode2, mface2; // Morton keys and leaf order of both meshes
     Buf nodes[2], rawbox;
-    Buf raw_pairs, pairs, phash, cand, cand_list, planes, big_list, big_scratch;
+    Buf raw_pairs, pairs, phash, cand, cand_list, planes, big_list, big_off, big_scratch;
     Buf te, tg, tqs, trs, ttype, tpip, esc;
     Buf rkey, rval, rkey2, rval2, rp, re, rg, rpts, rcount, re_in, rg_in;
     Buf shkeys, shcnt;
@@ -135,7 +135,7 @@ struct mcgpu_context_t {
     Buf sbsum, sitems, slo, shi, snitem, slist, sckpt, sflags; // coordinate sums from run items
     uint32_t sum_nblk = 0;
 
-    uint64_t cap_pairs = 0, cap_tests = 0, cap_esc = 0, cap_reg = 0, cap_big = 0;
+    uint64_t cap_pairs = 0, cap_tests = 0, cap_esc = 0, cap_reg = 0, cap_big = 0, cap_trip = 0;
 
     // state of the last dispatch
     bool have = false;
@@ -235,7 +235,8 @@ inline unsigned grid_for(uint64_t n, unsigned block = kBlock, unsigned maxg = 40
 __global__ void k_reset_narrow(Counters* C)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        C->n_tests = C->n_esc = C->n_reg = C->n_big = 0; // candidate faces are kept
+        C->n_tests = C->n_esc = C->n_reg = C->n_big = C->n_trip = 0; // candidate faces are kept
+        C->ov_trip = 0;
         C->n_f32 = C->n_f64 = 0;
         C->ov_tests = C->ov_esc = C->ov_reg = C->ov_big = 0;
         C->gp = C->bad_src = C->bad_cut = C->unsupported = C->pair_fail = C->partial = 0;
@@ -427,8 +428,9 @@ mcgpu_result alloc_list_buffers(mcgpu_context ctx)
     CK(ensure(ctx->shkeys, next_pow2(4 * ctx->cap_reg), &ull));
     CK(ensure(ctx->shcnt, next_pow2(4 * ctx->cap_reg), &u));
     CK(ensure(ctx->big_list, ctx->cap_big, &u));
+    CK(ensure(ctx->big_off, ctx->cap_big, &ull));
     Trip* tr;
-    CK(ensure(ctx->big_scratch, 256ull * 560, &tr));
+    CK(ensure(ctx->big_scratch, ctx->cap_trip, &tr));
     const uint64_t sort_cap = std::max<uint64_t>(ctx->cap_reg, (uint64_t)ctx->nf[0] + ctx->nf[1]);
     CK(ensure(ctx->hist, radix_hist_elems(sort_cap), &u));
     // the scan scratch also serves the mesh-sized scans (sized in alloc_mesh_buffers)
@@ -478,10 +480,12 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
     const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
     Counters* C = &P_<DevState>(ctx->state)->C;
     k_planes<<<grid_for(F), kBlock, 0, s>>>(S, P_<uint32_t>(ctx->cand_list), P_<Plane>(ctx->planes),
-                                             P_<uint32_t>(ctx->big_list), ctx->cap_big, C);
-    if (ctx->dsz[0] || ctx->dsz[1]) // faces of 6-16 vertices: none in two triangle meshes
-        k_big_polygons<<<4, 64, 0, s>>>(S, P_<uint32_t>(ctx->big_list), C, ctx->cap_big, P_<Plane>(ctx->planes),
-                                         P_<Trip>(ctx->big_scratch), C);
+                                             P_<uint32_t>(ctx->big_list), P_<unsigned long long>(ctx->big_off),
+                                             ctx->cap_big, C);
+    if (ctx->dsz[0] || ctx->dsz[1]) // faces of more than 5 vertices: none in two triangle meshes
+        k_big_polygons<<<grid_for(ctx->cap_big, 64, 1024), 64, 0, s>>>(
+            S, P_<uint32_t>(ctx->big_list), P_<unsigned long long>(ctx->big_off), C, ctx->cap_big, ctx->cap_trip,
+            P_<Plane>(ctx->planes), P_<Trip>(ctx->big_scratch), C);
     stage_event(ctx, EV_A_PLANES, s);
     if (join_topology) (void)hipStreamWaitEvent(s, ctx->ev[EV_C_TOPO], 0);
     stage_event(ctx, EV_A_JOIN2, s);
@@ -993,13 +997,14 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
                 C.n_fallback, C.n_lane_walks);
     if (C.bad_input || C.nonmanifold) return MCGPU_ERROR_INVALID_VALUE;
     if (C.unsupported & 2u) return MCGPU_ERROR_UNSUPPORTED;
-    if (C.ov_pairs || C.ov_tests || C.ov_esc || C.ov_reg || C.ov_big) {
+    if (C.ov_pairs || C.ov_tests || C.ov_esc || C.ov_reg || C.ov_big || C.ov_trip) {
         *overflow = true;
         if (C.ov_pairs) ctx->cap_pairs = std::max<uint64_t>(2 * ctx->cap_pairs, C.n_raw + C.n_raw / 4);
         if (C.ov_tests) ctx->cap_tests = std::max<uint64_t>(2 * ctx->cap_tests, C.n_tests + C.n_tests / 4);
         if (C.ov_esc) ctx->cap_esc = std::max<uint64_t>(2 * ctx->cap_esc, C.n_esc + C.n_esc / 4);
         if (C.ov_reg) ctx->cap_reg = std::max<uint64_t>(2 * ctx->cap_reg, C.n_reg + C.n_reg / 4);
         if (C.ov_big) ctx->cap_big = std::max<uint64_t>(2 * ctx->cap_big, C.n_big + C.n_big / 4);
+        if (C.ov_trip) ctx->cap_trip = std::max<uint64_t>(2 * ctx->cap_trip, C.n_trip + C.n_trip / 4);
         // the grown lists must fit the device: a raw pair takes up to 48 bytes
         // (raw list, pair list, pair hash of twice the capacity rounded up to a
         // power of two), a test 10, an escalated test 4, a registry entry 100.
@@ -1010,7 +1015,7 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return MCGPU_ERROR_DEVICE;
         const long double need = 48.0L * ctx->cap_pairs + 10.0L * ctx->cap_tests + 4.0L * ctx->cap_esc +
-                                 100.0L * ctx->cap_reg;
+                                 100.0L * ctx->cap_reg + 16.0L * ctx->cap_trip;
         if (need > 0.9L * (long double)(free_b + ctx->raw_pairs.cap + ctx->pairs.cap + ctx->phash.cap))
             return MCGPU_ERROR_OUT_OF_MEMORY;
         return MCGPU_SUCCESS;
@@ -1213,7 +1218,7 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
                   &ctx->eflag[1], &ctx->eid[0], &ctx->eid[1], &ctx->cedge[0], &ctx->cedge[1], &ctx->etotal, &ctx->ev0,
                   &ctx->ev1, &ctx->ef0, &ctx->ef1, &ctx->mcode, &ctx->mface, &ctx->mcode2, &ctx->mface2, &ctx->nodes[0], &ctx->nodes[1],
                   &ctx->rawbox, &ctx->raw_pairs, &ctx->pairs, &ctx->phash, &ctx->cand, &ctx->cand_list,
-                  &ctx->planes, &ctx->big_list, &ctx->big_scratch, &ctx->te, &ctx->tg, &ctx->tqs, &ctx->trs,
+                  &ctx->planes, &ctx->big_list, &ctx->big_off, &ctx->big_scratch, &ctx->te, &ctx->tg, &ctx->tqs, &ctx->trs,
                   &ctx->ttype, &ctx->tpip, &ctx->esc, &ctx->rkey, &ctx->rval, &ctx->rkey2, &ctx->rval2, &ctx->rp,
                   &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->scan_tmp_c, &ctx->hist,
                   &ctx->re_in, &ctx->rg_in, &ctx->bpairs, &ctx->bstarts, &ctx->fpair, &ctx->vpair, &ctx->bparams,
@@ -1263,6 +1268,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     if (ctx->cap_reg < (1u << 14))
         ctx->cap_reg = std::min<uint64_t>(std::max<uint64_t>(1 << 14, ctx->cap_tests / 8), kRegMergeTiles * kRegTile);
     if (ctx->cap_big < 1024) ctx->cap_big = 1024;
+    if (ctx->cap_trip < (1u << 16)) ctx->cap_trip = 1u << 16;
     for (int attempt = 0; attempt < 6; ++attempt) {
         if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
         ctx->stage_mode = 0;
@@ -1369,6 +1375,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
     if (ctx->cap_esc < (1u << 14)) ctx->cap_esc = std::max<uint64_t>(1 << 14, ctx->cap_tests / 16);
     if (ctx->cap_reg < ctx->cap_tests / 32) ctx->cap_reg = std::max<uint64_t>(1 << 14, ctx->cap_tests / 32);
     if (ctx->cap_big < 1024) ctx->cap_big = 1024;
+    if (ctx->cap_trip < (1u << 16)) ctx->cap_trip = 1u << 16;
     for (int attempt = 0; attempt < 6; ++attempt) {
         if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
         ctx->stage_mode = 0;
@@ -1803,7 +1810,7 @@ uint64_t ctx_bytes(mcgpu_context c)
                         &c->eid[1], &c->cedge[1],
                         &c->etotal, &c->ev0, &c->ev1, &c->ef0, &c->ef1, &c->mcode, &c->mface, &c->mcode2,
                         &c->mface2, &c->nodes[1], &c->rawbox, &c->raw_pairs, &c->pairs, &c->phash, &c->cand,
-                        &c->cand_list, &c->planes, &c->big_list, &c->big_scratch, &c->te, &c->tg, &c->esc,
+                        &c->cand_list, &c->planes, &c->big_list, &c->big_off, &c->big_scratch, &c->te, &c->tg, &c->esc,
                         &c->rkey, &c->rval, &c->rkey2, &c->rval2, &c->rp, &c->re, &c->rg, &c->rpts, &c->rcount,
                         &c->re_in, &c->rg_in, &c->shkeys, &c->shcnt, &c->scan_tmp, &c->scan_tmp_c, &c->hist,
                         &c->sbsum, &c->sitems, &c->slo, &c->shi, &c->snitem, &c->slist, &c->sckpt,
@@ -1876,6 +1883,7 @@ mcgpu_result slot_bind(mcgpu_server srv, mcgpu_context ctx, const mcgpu_mesh* cu
     if (ctx->cap_reg < (1u << 14))
         ctx->cap_reg = std::min<uint64_t>(std::max<uint64_t>(1 << 14, ctx->cap_tests / 8), kRegMergeTiles * kRegTile);
     if (ctx->cap_big < 1024) ctx->cap_big = 1024;
+    if (ctx->cap_trip < (1u << 16)) ctx->cap_trip = 1u << 16;
     return alloc_list_buffers(ctx);
 }
 
@@ -1999,6 +2007,7 @@ MCGPU_API mcgpu_result mcgpu_server_create(int device, const mcgpu_mesh* src, co
         c->cap_esc = std::max<uint64_t>(1 << 14, c->cap_tests / 16);
         c->cap_reg = std::min<uint64_t>(std::max<uint64_t>(1 << 14, c->cap_tests / 8), kRegMergeTiles * kRegTile);
         c->cap_big = 1024;
+        c->cap_trip = 1u << 16;
         if (alloc_list_buffers(c) != MCGPU_SUCCESS) {
             mcgpu_server_release(srv);
             return MCGPU_ERROR_OUT_OF_MEMORY;

@@ -426,19 +426,17 @@ static inline void project(const double* M, const double* x, double* out)
 /* 2-D crossing test, math.cpp:645-703 (O'Rourke) */
 static char pip2d(const double* q, const double* V2, int n)
 {
-    double W[2 * 64];
-    for (int i = 0; i < n; ++i) {
-        W[2 * i] = V2[2 * i] - q[0];
-        W[2 * i + 1] = V2[2 * i + 1] - q[1];
-    }
     int rc = 0, lc = 0;
     for (int i = 0; i < n; ++i) {
-        if (W[2 * i] == 0.0 && W[2 * i + 1] == 0.0) return 'v';
+        /* w_i = V2[i] - q, each difference one rounding as the reference forms it */
         int il = (i + n - 1) % n;
-        int rs = (W[2 * i + 1] > 0.0) != (W[2 * il + 1] > 0.0);
-        int ls = (W[2 * i + 1] < 0.0) != (W[2 * il + 1] < 0.0);
+        const double wi0 = V2[2 * i] - q[0], wi1 = V2[2 * i + 1] - q[1];
+        const double wl0 = V2[2 * il] - q[0], wl1 = V2[2 * il + 1] - q[1];
+        if (wi0 == 0.0 && wi1 == 0.0) return 'v';
+        int rs = (wi1 > 0.0) != (wl1 > 0.0);
+        int ls = (wi1 < 0.0) != (wl1 < 0.0);
         if (rs || ls) {
-            double x = (W[2 * i] * W[2 * il + 1] - W[2 * il] * W[2 * i + 1]) / (W[2 * il + 1] - W[2 * i + 1]);
+            double x = (wi0 * wl1 - wl0 * wi1) / (wl1 - wi1);
             if (rs && x > 0.0) rc++;
             if (ls && x < 0.0) lc++;
         }
@@ -450,11 +448,14 @@ static char pip2d(const double* q, const double* V2, int n)
 /* compute_point_in_polygon_test (3-D), math.cpp:851-902 */
 char om_point_in_polygon(const double* p, const double* P, int n, const double* normal, int k)
 {
-    double M[6], pp[2], V2[2 * 64];
+    double M[6], pp[2];
+    double* V2 = (double*)malloc(sizeof(double) * 2 * (size_t)n);
     projection_matrix(normal, k, M);
     project(M, p, pp);
     for (int i = 0; i < n; ++i) project(M, P + 3 * i, V2 + 2 * i);
-    return pip2d(pp, V2, n);
+    const char res = pip2d(pp, V2, n);
+    free(V2);
+    return res;
 }
 
 /* determine_three_noncollinear_vertices, math.cpp:289-389: every (i<j<k)
@@ -695,6 +696,7 @@ typedef struct {
 
 struct om_result {
     double* pbuf;                  /* vertices of one polygon, sized for the largest face */
+    uint32_t* ins;                 /* one face's vertex ids, likewise */
     int status, stage;
     int partial;                   /* a successful front end whose registry holds a cut-mesh border edge */
     double com[3], t[3], eps, ms;
@@ -934,8 +936,11 @@ om_result* om_frontend(const om_mesh* src, const om_mesh* cut, const om_options*
     R->edges = (om_edge*)malloc(sizeof(om_edge) * ncorner);
     om_hash H;
     hash_init(&H, ncorner);
-    uint32_t off = 0, sidx = 0;
-    uint32_t ins[64];
+    uint32_t off = 0, sidx = 0, maxn = 3;
+    for (uint32_t f = 0; f < Fs; ++f) maxn = face_size(src, f) > maxn ? face_size(src, f) : maxn;
+    for (uint32_t f = 0; f < Fc; ++f) maxn = face_size(cut, f) > maxn ? face_size(cut, f) : maxn;
+    R->ins = (uint32_t*)malloc(sizeof(uint32_t) * maxn); /* one face's vertex list */
+    uint32_t* ins = R->ins;
     for (uint32_t f = 0; f < Fs; ++f) {
         uint32_t n = face_size(src, f);
         R->face_off[f] = off;
@@ -1266,7 +1271,7 @@ void om_free(om_result* R)
     if (!R) return;
     void* ptrs[] = {R->ps_v, R->face_off, R->face_vtx, R->face_edge, R->edges, R->aabb, R->pairs, R->tests,
                     R->qs, R->rs, R->type, R->pip, R->reg, R->reg_p, R->cand, R->plane_n, R->plane_d,
-                    R->plane_k, R->plane_ijk, R->pbuf};
+                    R->plane_k, R->plane_ijk, R->pbuf, R->ins};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
     free(R);
 }

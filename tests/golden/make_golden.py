@@ -23,12 +23,17 @@ Fixtures:
                      pair 013 with and without MC_DISPATCH_REQUIRE_THROUGH_CUTS
                      (the partial-cut early exit of tests/source/
                      computeSeams.cpp:356-398, which expects 2 components)
+  ref_big_polygons.npz  planes and the chosen vertex triplet of polygons of
+                     17-40 vertices (regular, rounded to a grid, rectangles
+                     with collinear runs, random convex) from the reference's
+                     compute_polygon_plane_coefficients and
+                     determine_three_noncollinear_vertices
   ref_config2.npz    the reference's broad phase on the two subdivision-7
                      icospheres (327,680 triangles each): shift parameters,
                      candidate-pair count and a SHA-256 of the sorted pairs
 
-Pass fixture names (functions, cases, edges, status, config2) to write only
-those.
+Pass fixture names (functions, cases, edges, status, config2, bigpoly) to
+write only those.
 """
 from __future__ import annotations
 
@@ -228,6 +233,73 @@ def status_fixture():
     np.savez_compressed(os.path.join(HERE, "ref_status.npz"), **d)
 
 
+def big_polygons(seed=17):
+    """Polygons of 17-40 vertices in the positive quadrant, built so that many
+    triplets tie on |orient2d|: regular polygons in a random plane, regular
+    polygons rounded to a 1/64 grid in an axis plane, rectangles with evenly
+    spaced points along their sides (collinear runs, exact ties) and random
+    convex polygons."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for it in range(160):
+        n = int(rng.integers(17, 41))
+        kind = it % 4
+        ang = 2 * np.pi * np.arange(n) / n
+        if kind == 0:
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            v = np.cross(u, rng.normal(size=3))
+            v /= np.linalg.norm(v)
+            P = 2.0 + np.cos(ang)[:, None] * u + np.sin(ang)[:, None] * v
+        elif kind == 1:
+            xy = np.round(np.stack([np.cos(ang), np.sin(ang)], 1) * 64) / 64 + 2.0
+            P = np.insert(xy, int(rng.integers(0, 3)), 1.5, axis=1)
+        elif kind == 2:
+            w, h = int(rng.integers(2, 8)), int(rng.integers(2, 8))
+            per = 2 * (w + h)
+            t = np.arange(n) * per / n
+            pts = []
+            for x in t:
+                if x < w:
+                    pts.append((x, 0.0))
+                elif x < w + h:
+                    pts.append((w, x - w))
+                elif x < 2 * w + h:
+                    pts.append((w - (x - w - h), h))
+                else:
+                    pts.append((0.0, h - (x - 2 * w - h)))
+            xy = np.array(pts) / 8.0 + 1.0
+            P = np.insert(xy, int(rng.integers(0, 3)), 0.75, axis=1)
+        else:
+            a = np.sort(rng.uniform(0, 2 * np.pi, n))
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            v = np.cross(u, rng.normal(size=3))
+            v /= np.linalg.norm(v)
+            r = rng.uniform(0.7, 1.0)
+            P = 2.0 + r * (np.cos(a)[:, None] * u + np.sin(a)[:, None] * v)
+        out.append(np.ascontiguousarray(P))
+    return out
+
+
+def bigpoly_fixture():
+    polys = big_polygons()
+    d = {"poly": [], "n": [], "normal": [], "k": [], "triple": [], "ok": []}
+    for P in polys:
+        nrm, dd, k = rp.plane(P)
+        ok, ijk = rp.best_triple(P, nrm, k)
+        pad = np.zeros((40, 3))
+        pad[:P.shape[0]] = P
+        d["poly"].append(pad)
+        d["n"].append(P.shape[0])
+        d["normal"].append(nrm)
+        d["k"].append(k)
+        d["triple"].append(ijk)
+        d["ok"].append(ok)
+    np.savez_compressed(os.path.join(HERE, "ref_big_polygons.npz"), **{k: np.asarray(v) for k, v in d.items()})
+    print(f"bigpoly: {len(polys)} polygons")
+
+
 def config2_fixture():
     import hashlib
     src, cut = mg.config2(level=7)
@@ -245,7 +317,9 @@ def config2_fixture():
 def main():
     if not rp.available():
         sys.exit("build the reference first: make -C oracle ref")
-    which = set(sys.argv[1:]) or {"functions", "cases", "edges", "status", "config2"}
+    which = set(sys.argv[1:]) or {"functions", "cases", "edges", "status", "config2", "bigpoly"}
+    if "bigpoly" in which:
+        bigpoly_fixture()
     if "status" in which:
         status_fixture()
     if "config2" in which:

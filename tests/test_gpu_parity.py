@@ -263,8 +263,9 @@ def test_broad_phase_mesh_boxes(ctx):
 
 
 def test_stage_timing_switch(ctx):
-    """Stage events only when asked for: none by default, all twelve stages after
-    mcgpu_set_stage_timing, none again once it is off."""
+    """Stage events only when asked for: none by default, all twelve stages and
+    the traversal kernel's own time after mcgpu_set_stage_timing, none again
+    once it is off."""
     src, cut = meshes.icosphere_pair(4)
     assert ctx.dispatch(src, cut).stage_ms() == {}
     ctx.set_stage_timing(True)
@@ -272,5 +273,6 @@ def test_stage_timing_switch(ctx):
         st = ctx.dispatch(src, cut).stage_ms()
     finally:
         ctx.set_stage_timing(False)
-    assert len(st) == 12 and all(v >= 0.0 for v in st.values()) and st["traversal"] > 0.0
+    assert len(st) == 13 and all(v >= 0.0 for v in st.values()) and st["traversal"] > 0.0
+    assert 0.0 < st["traversal_kernel"] <= st["traversal"] * 1.5
     assert ctx.dispatch(src, cut).stage_ms() == {}
