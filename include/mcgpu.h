@@ -148,6 +148,43 @@ MCGPU_API const char* mcgpu_result_string(mcgpu_result r);
 MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
                                                const mcgpu_dispatch_desc* desc, mcgpu_frontend_info* info);
 
+/* The front-end block of dispatch() alone (kernel.cpp:1779-3229), on the
+ * inputs preproc hands it (input_t, kernel.h:125-172): the two meshes with the
+ * coordinates dispatch() holds (already shifted, and perturbed in a
+ * general-position retry; they are used as given), face lists in input order,
+ * the face AABBs (6 doubles per face, min xyz then max xyz, cut faces
+ * enlarged: source_hmesh_face_aabb_array / cut_hmesh_face_aabb_array) and the
+ * candidate pairs (`pairs`: num_pairs (source face, cut face) pairs, cut faces
+ * numbered from 0: ps_face_to_potentially_intersecting_others).  It numbers
+ * the polygon-soup edges, expands and culls the edge-face tests, computes the
+ * candidate-face planes and runs the narrow phase into the sorted registry,
+ * as mcgpu_dispatch_frontend does after its broad phase; the getters then read
+ * its results (com and translation are zero).  `topology` (optional) is the
+ * soup's edge numbering as the caller's halfedge mesh has it (hmesh_t edge ids
+ * and halfedge-0 directions, which a mesh edited in place -- preproc's
+ * floating-polygon partitioning -- no longer creates in face order); without
+ * it the edges are numbered as hmesh_t::add_face numbers them for faces added
+ * in order.  Host arrays only; desc->flags and enforcement_count select the
+ * status rules. */
+typedef struct {
+    /* per mesh, per corner of its face lists as given: the soup edge from that
+     * corner's vertex to the next one of its face */
+    const uint32_t* corner_edges[2];
+    /* per soup edge: source and target of halfedge 0 (soup vertex ids: source
+     * vertices, then cut vertices), faces of halfedges 0 and 1 (-1: none) */
+    const uint32_t* edge_v0;
+    const uint32_t* edge_v1;
+    const int32_t* edge_f0;
+    const int32_t* edge_f1;
+    uint32_t num_edges;
+    uint32_t reserved;
+} mcgpu_soup_topology;
+
+MCGPU_API mcgpu_result mcgpu_dispatch_narrow(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
+                                             const double* src_boxes, const double* cut_boxes, const uint32_t* pairs,
+                                             uint64_t num_pairs, const mcgpu_soup_topology* topology,
+                                             const mcgpu_dispatch_desc* desc, mcgpu_frontend_info* info);
+
 /* Re-runs the narrow phase with the cut mesh moved by `perturbation` (3
  * doubles), keeping the candidate pairs and cut-face AABBs of the last
  * mcgpu_dispatch_frontend, as the reference's general-position retry does

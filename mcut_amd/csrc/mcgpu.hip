@@ -790,6 +790,85 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool g
     (void)hipEventRecord(ctx->ev[EV_B_TRAV], st);
 }
 
+// Polygon-soup edge numbering on stream s3, and the candidate-face flags the
+// pair filter fills: records EV_C_FILL once the flags are clear and
+// EV_C_TOPO at the end.
+void enqueue_topology(mcgpu_context ctx, hipStream_t s3)
+{
+    const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
+    Counters* C = &P_<DevState>(ctx->state)->C;
+    stage_event(ctx, EV_C0, s3);
+    uint32_t* etotal = P_<uint32_t>(ctx->etotal);
+    const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
+    fill_u32(ctx->cand.p, 0u, F, s3);
+    (void)hipEventRecord(ctx->ev[EV_C_FILL], s3);
+    const uint32_t res_ne = ctx->res ? ctx->res->ne : 0u;
+    if (ctx->res) { // the resident source's edges occupy [0, ne) of the edge table
+        (void)hipMemcpyAsync(ctx->ev0.p, ctx->res->ev0, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
+        (void)hipMemcpyAsync(ctx->ev1.p, ctx->res->ev1, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
+        (void)hipMemcpyAsync(ctx->ef0.p, ctx->res->ef0, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
+        (void)hipMemcpyAsync(ctx->ef1.p, ctx->res->ef1, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
+    }
+    fill_u32(P_<uint32_t>(ctx->ef1) + res_ne, 0xffffffffu, K - res_ne, s3);
+    CornerMap cmap[2];
+    for (int m = 0; m < 2; ++m) {
+        cmap[m].off = P_<uint32_t>(ctx->off[m]);
+        cmap[m].cface = ctx->dsz[m] ? P_<uint32_t>(ctx->cface[m]) : nullptr;
+        if (ctx->dsz[m] && !(ctx->res && m == 0))
+            k_corner_faces<<<grid_for(ctx->nf[m]), kBlock, 0, s3>>>(cmap[m].off, ctx->nf[m], P_<uint32_t>(ctx->cface[m]));
+    }
+    // the scans of this stream use their own scratch: stream B's sort uses scan_tmp
+    uint32_t* ctmp = P_<uint32_t>(ctx->scan_tmp_c);
+    if (topology_packed(ctx)) {
+        TopoMesh TM[2];
+        for (int m = 0; m < 2; ++m)
+            TM[m] = TopoMesh{ctx->didx[m], cmap[m], P_<unsigned long long>(ctx->hkeys[m]),
+                             next_pow2(2 * ctx->ncorner[m] + 2) - 1, P_<uint32_t>(ctx->cedge[m]), ctx->ncorner[m],
+                             m ? ctx->nv[0] : 0u, m ? ctx->nf[0] : 0u};
+        const int cut_only = ctx->res ? 1 : 0;
+        const uint64_t K0 = ctx->ncorner[0], start = cut_only ? K0 : 0;
+        uint32_t* flag = P_<uint32_t>(ctx->eflag[0]);
+        uint32_t* eid = P_<uint32_t>(ctx->eid[0]);
+        const uint64_t t0 = cut_only ? 0 : TM[0].mask + 1, t1 = TM[1].mask + 1;
+        k_fill2_u64<<<grid_for(t0 + t1), kBlock, 0, s3>>>(TM[0].keys, t0, TM[1].keys, t1, kEmpty64);
+        k_edge_insert_packed<<<grid_for(K - start), kBlock, 0, s3>>>(TM[0], TM[1], start);
+        k_edge_flag_packed<<<grid_for(K - start), kBlock, 0, s3>>>(TM[0], TM[1], start, flag, C);
+        scan_exclusive(flag + start, eid + start, K - start, ctmp, etotal, s3);
+        k_store_edge_counts_packed<<<1, 1, 0, s3>>>(eid, flag, K0, etotal, cut_only, C);
+        k_edge_assign_packed<<<grid_for(K - start), kBlock, 0, s3>>>(TM[0], TM[1], start, eid, cut_only, C,
+                                                                     P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1),
+                                                                     P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
+    } else {
+    for (int m = ctx->res ? 1 : 0; m < 2; ++m) {
+        const uint64_t nk = ctx->ncorner[m];
+        const uint64_t hs = next_pow2(2 * nk + 2);
+        fill_u64(ctx->hkeys[m].p, kEmpty64, hs, s3);
+        fill_u64(ctx->hfirst[m].p, kEmpty64, hs, s3);
+        k_edge_insert<<<grid_for(nk), kBlock, 0, s3>>>(ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]),
+                                                       P_<unsigned long long>(ctx->hfirst[m]), hs - 1);
+        k_edge_flag<<<grid_for(nk), kBlock, 0, s3>>>(ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]),
+                                                     P_<unsigned long long>(ctx->hfirst[m]), hs - 1,
+                                                     P_<uint32_t>(ctx->eflag[m]), C);
+        scan_exclusive(P_<uint32_t>(ctx->eflag[m]), P_<uint32_t>(ctx->eid[m]), nk, ctmp, etotal + m, s3);
+        k_store_edge_count<<<1, 1, 0, s3>>>(etotal + m, C, m);
+    }
+    for (int m = ctx->res ? 1 : 0; m < 2; ++m) {
+        const uint64_t nk = ctx->ncorner[m];
+        const uint64_t hs = next_pow2(2 * nk + 2);
+        k_edge_assign<<<grid_for(nk), kBlock, 0, s3>>>(
+            ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]), P_<unsigned long long>(ctx->hfirst[m]),
+            hs - 1, P_<uint32_t>(ctx->eid[m]), C, m ? ctx->nv[0] : 0, m ? ctx->nf[0] : 0, P_<uint32_t>(ctx->cedge[m]),
+            P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
+    }
+    }
+    if (ctx->batch)
+        k_batch_edge_bases<<<grid_for(ctx->npairs), kBlock, 0, s3>>>(
+            P_<BatchPair>(ctx->bpairs), ctx->npairs, P_<uint32_t>(ctx->off[0]), P_<uint32_t>(ctx->off[1]), ctx->nf[0],
+            ctx->nf[1], P_<uint32_t>(ctx->eid[0]), P_<uint32_t>(ctx->eid[1]), C);
+    (void)hipEventRecord(ctx->ev[EV_C_TOPO], s3);
+
+}
+
 void enqueue_frontend(mcgpu_context ctx)
 {
     hipStream_t s = ctx->s, s2 = ctx->batch ? ctx->s2b : ctx->s2, s3 = ctx->batch ? ctx->s3b : ctx->s3;
@@ -885,75 +964,7 @@ void enqueue_frontend(mcgpu_context ctx)
 
     // ---- stream C: polygon-soup edge numbering, and the tables the filter fills ----
     (void)hipStreamWaitEvent(s3, ctx->ev[EV_FORK], 0);
-    stage_event(ctx, EV_C0, s3);
-    uint32_t* etotal = P_<uint32_t>(ctx->etotal);
-    const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
-    fill_u32(ctx->cand.p, 0u, F, s3);
-    (void)hipEventRecord(ctx->ev[EV_C_FILL], s3);
-    const uint32_t res_ne = ctx->res ? ctx->res->ne : 0u;
-    if (ctx->res) { // the resident source's edges occupy [0, ne) of the edge table
-        (void)hipMemcpyAsync(ctx->ev0.p, ctx->res->ev0, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
-        (void)hipMemcpyAsync(ctx->ev1.p, ctx->res->ev1, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
-        (void)hipMemcpyAsync(ctx->ef0.p, ctx->res->ef0, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
-        (void)hipMemcpyAsync(ctx->ef1.p, ctx->res->ef1, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
-    }
-    fill_u32(P_<uint32_t>(ctx->ef1) + res_ne, 0xffffffffu, K - res_ne, s3);
-    CornerMap cmap[2];
-    for (int m = 0; m < 2; ++m) {
-        cmap[m].off = P_<uint32_t>(ctx->off[m]);
-        cmap[m].cface = ctx->dsz[m] ? P_<uint32_t>(ctx->cface[m]) : nullptr;
-        if (ctx->dsz[m] && !(ctx->res && m == 0))
-            k_corner_faces<<<grid_for(ctx->nf[m]), kBlock, 0, s3>>>(cmap[m].off, ctx->nf[m], P_<uint32_t>(ctx->cface[m]));
-    }
-    // the scans of this stream use their own scratch: stream B's sort uses scan_tmp
-    uint32_t* ctmp = P_<uint32_t>(ctx->scan_tmp_c);
-    if (topology_packed(ctx)) {
-        TopoMesh TM[2];
-        for (int m = 0; m < 2; ++m)
-            TM[m] = TopoMesh{ctx->didx[m], cmap[m], P_<unsigned long long>(ctx->hkeys[m]),
-                             next_pow2(2 * ctx->ncorner[m] + 2) - 1, P_<uint32_t>(ctx->cedge[m]), ctx->ncorner[m],
-                             m ? ctx->nv[0] : 0u, m ? ctx->nf[0] : 0u};
-        const int cut_only = ctx->res ? 1 : 0;
-        const uint64_t K0 = ctx->ncorner[0], start = cut_only ? K0 : 0;
-        uint32_t* flag = P_<uint32_t>(ctx->eflag[0]);
-        uint32_t* eid = P_<uint32_t>(ctx->eid[0]);
-        const uint64_t t0 = cut_only ? 0 : TM[0].mask + 1, t1 = TM[1].mask + 1;
-        k_fill2_u64<<<grid_for(t0 + t1), kBlock, 0, s3>>>(TM[0].keys, t0, TM[1].keys, t1, kEmpty64);
-        k_edge_insert_packed<<<grid_for(K - start), kBlock, 0, s3>>>(TM[0], TM[1], start);
-        k_edge_flag_packed<<<grid_for(K - start), kBlock, 0, s3>>>(TM[0], TM[1], start, flag, C);
-        scan_exclusive(flag + start, eid + start, K - start, ctmp, etotal, s3);
-        k_store_edge_counts_packed<<<1, 1, 0, s3>>>(eid, flag, K0, etotal, cut_only, C);
-        k_edge_assign_packed<<<grid_for(K - start), kBlock, 0, s3>>>(TM[0], TM[1], start, eid, cut_only, C,
-                                                                     P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1),
-                                                                     P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
-    } else {
-    for (int m = ctx->res ? 1 : 0; m < 2; ++m) {
-        const uint64_t nk = ctx->ncorner[m];
-        const uint64_t hs = next_pow2(2 * nk + 2);
-        fill_u64(ctx->hkeys[m].p, kEmpty64, hs, s3);
-        fill_u64(ctx->hfirst[m].p, kEmpty64, hs, s3);
-        k_edge_insert<<<grid_for(nk), kBlock, 0, s3>>>(ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]),
-                                                       P_<unsigned long long>(ctx->hfirst[m]), hs - 1);
-        k_edge_flag<<<grid_for(nk), kBlock, 0, s3>>>(ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]),
-                                                     P_<unsigned long long>(ctx->hfirst[m]), hs - 1,
-                                                     P_<uint32_t>(ctx->eflag[m]), C);
-        scan_exclusive(P_<uint32_t>(ctx->eflag[m]), P_<uint32_t>(ctx->eid[m]), nk, ctmp, etotal + m, s3);
-        k_store_edge_count<<<1, 1, 0, s3>>>(etotal + m, C, m);
-    }
-    for (int m = ctx->res ? 1 : 0; m < 2; ++m) {
-        const uint64_t nk = ctx->ncorner[m];
-        const uint64_t hs = next_pow2(2 * nk + 2);
-        k_edge_assign<<<grid_for(nk), kBlock, 0, s3>>>(
-            ctx->didx[m], cmap[m], nk, m, P_<unsigned long long>(ctx->hkeys[m]), P_<unsigned long long>(ctx->hfirst[m]),
-            hs - 1, P_<uint32_t>(ctx->eid[m]), C, m ? ctx->nv[0] : 0, m ? ctx->nf[0] : 0, P_<uint32_t>(ctx->cedge[m]),
-            P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
-    }
-    }
-    if (ctx->batch)
-        k_batch_edge_bases<<<grid_for(ctx->npairs), kBlock, 0, s3>>>(
-            P_<BatchPair>(ctx->bpairs), ctx->npairs, P_<uint32_t>(ctx->off[0]), P_<uint32_t>(ctx->off[1]), ctx->nf[0],
-            ctx->nf[1], P_<uint32_t>(ctx->eid[0]), P_<uint32_t>(ctx->eid[1]), C);
-    (void)hipEventRecord(ctx->ev[EV_C_TOPO], s3);
+    enqueue_topology(ctx, s3);
 
     // ---- stream A: join the broad phase, exact filter, narrow phase ----
     (void)hipStreamWaitEvent(s, ctx->ev[EV_B_TRAV], 0);
@@ -2331,6 +2342,120 @@ MCGPU_API mcgpu_result mcgpu_broad_phase_noalloc(int device, const double* src_b
                       hipStreamSynchronize(s) != hipSuccess))
         return MCGPU_ERROR_DEVICE;
     return MCGPU_SUCCESS;
+}
+
+} // extern "C"
+
+extern "C" {
+
+MCGPU_API mcgpu_result mcgpu_dispatch_narrow(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
+                                             const double* src_boxes, const double* cut_boxes, const uint32_t* pairs,
+                                             uint64_t num_pairs, const mcgpu_soup_topology* topology,
+                                             const mcgpu_dispatch_desc* desc, mcgpu_frontend_info* info)
+{
+    if (!ctx || !desc || !src_boxes || !cut_boxes || (num_pairs && !pairs)) return MCGPU_ERROR_INVALID_VALUE;
+    if (topology && (!topology->corner_edges[0] || !topology->corner_edges[1] || !topology->edge_v0 ||
+                     !topology->edge_v1 || !topology->edge_f0 || !topology->edge_f1 || topology->num_edges == 0))
+        return MCGPU_ERROR_INVALID_VALUE;
+    if (desc->input_on_device) return MCGPU_ERROR_INVALID_VALUE; // host arrays, as dispatch() holds them
+    mcgpu_result r;
+    if ((r = check_mesh(src)) != MCGPU_SUCCESS || (r = check_mesh(cut)) != MCGPU_SUCCESS) return r;
+    for (uint64_t i = 0; i < num_pairs; ++i)
+        if (pairs[2 * i] >= src->num_faces || pairs[2 * i + 1] >= cut->num_faces) return MCGPU_ERROR_INVALID_VALUE;
+    CK(hipSetDevice(ctx->device));
+    ctx->have = false;
+    ctx->broad_only = false;
+    ctx->batch = false;
+    ctx->sort_bits = 22;
+    ctx->flags = desc->flags;
+    ctx->gp_constant = desc->gp_constant > 0.0 ? desc->gp_constant : 1e-4;
+    ctx->enforcement_count = desc->enforcement_count;
+    ctx->keep_tests = desc->keep_tests ? 1 : 0;
+    if ((r = upload_mesh(ctx, 0, src, false)) != MCGPU_SUCCESS) return r;
+    if ((r = upload_mesh(ctx, 1, cut, false)) != MCGPU_SUCCESS) return r;
+    const uint32_t Fs = ctx->nf[0];
+    const uint64_t F = (uint64_t)Fs + ctx->nf[1];
+    const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
+    const uint64_t NE = topology ? topology->num_edges : K;
+    if (topology) { // every corner names an edge of the table, every edge soup vertices and faces
+        const uint64_t nvs = (uint64_t)ctx->nv[0] + ctx->nv[1];
+        for (int m = 0; m < 2; ++m)
+            for (uint64_t c = 0; c < ctx->ncorner[m]; ++c)
+                if (topology->corner_edges[m][c] >= NE) return MCGPU_ERROR_INVALID_VALUE;
+        for (uint64_t e = 0; e < NE; ++e)
+            if (topology->edge_v0[e] >= nvs || topology->edge_v1[e] >= nvs || topology->edge_f0[e] >= (int64_t)F ||
+                topology->edge_f1[e] >= (int64_t)F || topology->edge_f0[e] < -1 || topology->edge_f1[e] < -1)
+                return MCGPU_ERROR_INVALID_VALUE;
+    }
+    ctx->gbits = bits_for(F + 1);
+    ctx->kbits = ctx->gbits + bits_for(std::max<uint64_t>(K, NE) + 1);
+    if (ctx->kbits > 64) return MCGPU_ERROR_UNSUPPORTED;
+    if ((r = alloc_mesh_buffers(ctx)) != MCGPU_SUCCESS) return r;
+    if (topology) {
+        uint32_t* u;
+        int32_t* i32;
+        CK(ensure(ctx->ev0, NE, &u));
+        CK(ensure(ctx->ev1, NE, &u));
+        CK(ensure(ctx->ef0, NE, &i32));
+        CK(ensure(ctx->ef1, NE, &i32));
+    }
+    const uint64_t np = std::max<uint64_t>(num_pairs, 1);
+    if (ctx->cap_pairs < np) ctx->cap_pairs = np + np / 4;
+    if (ctx->cap_pairs < (1u << 14)) ctx->cap_pairs = 1u << 14;
+    if (ctx->cap_tests < 2 * ctx->cap_pairs) ctx->cap_tests = 2 * ctx->cap_pairs;
+    if (ctx->cap_esc < (1u << 14)) ctx->cap_esc = std::max<uint64_t>(1 << 14, ctx->cap_tests / 16);
+    if (ctx->cap_reg < (1u << 14))
+        ctx->cap_reg = std::min<uint64_t>(std::max<uint64_t>(1 << 14, ctx->cap_tests / 8), kRegMergeTiles * kRegTile);
+    if (ctx->cap_big < 1024) ctx->cap_big = 1024;
+    if (ctx->cap_trip < (1u << 16)) ctx->cap_trip = 1u << 16;
+    std::vector<unsigned long long> keys(num_pairs);
+    for (uint64_t i = 0; i < num_pairs; ++i) keys[i] = ((unsigned long long)pairs[2 * i] << 32) | pairs[2 * i + 1];
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
+        hipStream_t s = ctx->s;
+        ctx->stage_mode = 1;
+        enqueue_validate(ctx); // counters, face offsets, face-list validation
+        DevState* st = P_<DevState>(ctx->state);
+        Counters* C = &st->C;
+        fill_u32(&st->P, 0u, sizeof(Params) / 4, s); // no shift: the coordinates are the soup's own
+        // the soup's coordinates, face boxes and candidate pairs as given
+        CK(hipMemcpyAsync(ctx->ps_v.p, ctx->dv[0], 24ull * ctx->nv[0], hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(P_<double>(ctx->ps_v) + 3ull * ctx->nv[0], ctx->dv[1], 24ull * ctx->nv[1],
+                          hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(ctx->box.p, src_boxes, 48ull * Fs, hipMemcpyHostToDevice, s));
+        CK(hipMemcpyAsync(P_<double>(ctx->box) + 6ull * Fs, cut_boxes, 48ull * ctx->nf[1], hipMemcpyHostToDevice, s));
+        if (num_pairs)
+            CK(hipMemcpyAsync(ctx->raw_pairs.p, keys.data(), 8ull * num_pairs, hipMemcpyHostToDevice, s));
+        fill_u64(&C->n_raw, (unsigned long long)num_pairs, 1, s);
+        if (topology) { // the caller's edge numbering
+            for (int m = 0; m < 2; ++m)
+                CK(hipMemcpyAsync(ctx->cedge[m].p, topology->corner_edges[m], 4ull * ctx->ncorner[m],
+                                  hipMemcpyHostToDevice, s));
+            CK(hipMemcpyAsync(ctx->ev0.p, topology->edge_v0, 4ull * NE, hipMemcpyHostToDevice, s));
+            CK(hipMemcpyAsync(ctx->ev1.p, topology->edge_v1, 4ull * NE, hipMemcpyHostToDevice, s));
+            CK(hipMemcpyAsync(ctx->ef0.p, topology->edge_f0, 4ull * NE, hipMemcpyHostToDevice, s));
+            CK(hipMemcpyAsync(ctx->ef1.p, topology->edge_f1, 4ull * NE, hipMemcpyHostToDevice, s));
+            fill_u64(&C->n_edges_src, (unsigned long long)NE, 1, s);
+            fill_u64(&C->n_edges_cut, 0ull, 1, s);
+            fill_u32(ctx->cand.p, 0u, F, s);
+        } else {
+            enqueue_topology(ctx, s);
+        }
+        // pair hash and candidate faces; the exact box test keeps every given pair
+        const uint64_t table = next_pow2(2 * ctx->cap_pairs);
+        k_pair_hash_init<<<grid_for(table, kBlock, 1024), kBlock, 0, s>>>(P_<unsigned long long>(ctx->phash),
+                                                                          ctx->cap_pairs, table, C);
+        k_filter_pairs<<<grid_for(ctx->cap_pairs, kFilterTile, 2048), kStageBlock, 0, s>>>(
+            P_<double>(ctx->box), P_<unsigned long long>(ctx->raw_pairs), ctx->cap_pairs, Fs, nullptr,
+            P_<unsigned long long>(ctx->pairs), P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->cand),
+            P_<uint32_t>(ctx->cand_list), 0, C);
+        stage_event(ctx, EV_A_FILTER, s);
+        enqueue_narrow(ctx, make_soup(ctx), false);
+        bool overflow = false;
+        if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
+        if (!overflow) return MCGPU_SUCCESS;
+    }
+    return MCGPU_ERROR_OUT_OF_MEMORY;
 }
 
 } // extern "C"

@@ -56,6 +56,11 @@ class FrontendInfo(C.Structure):
                 ("partial_cut", C.c_uint32)]
 
 
+class SoupTopology(C.Structure):
+    _fields_ = [("corner_edges", C.c_void_p * 2), ("edge_v0", C.c_void_p), ("edge_v1", C.c_void_p),
+                ("edge_f0", C.c_void_p), ("edge_f1", C.c_void_p), ("num_edges", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class Batch(C.Structure):
     _fields_ = [("num_pairs", C.c_uint32), ("src", Mesh), ("cut", Mesh), ("src_vertex_offsets", C.c_void_p),
                 ("src_face_offsets", C.c_void_p), ("cut_vertex_offsets", C.c_void_p), ("cut_face_offsets", C.c_void_p)]
@@ -88,6 +93,8 @@ def lib():
         L.mcgpu_dispatch_frontend.argtypes = [vp, C.POINTER(Mesh), C.POINTER(Mesh), C.POINTER(DispatchDesc),
                                               C.POINTER(FrontendInfo)]
         L.mcgpu_redispatch_perturbed.argtypes = [vp, vp, C.c_uint32, C.POINTER(FrontendInfo)]
+        L.mcgpu_dispatch_narrow.argtypes = [vp, C.POINTER(Mesh), C.POINTER(Mesh), vp, vp, vp, u64,
+                                            C.POINTER(SoupTopology), C.POINTER(DispatchDesc), C.POINTER(FrontendInfo)]
         L.mcgpu_get_registry.argtypes = [vp, vp, vp, vp, u64]
         L.mcgpu_get_candidate_pairs.argtypes = [vp, vp, u64]
         L.mcgpu_get_edges.argtypes = [vp, vp, vp, vp, vp, u64]
@@ -120,7 +127,8 @@ def lib():
                    "mcgpu_stage_times", "mcgpu_broad_phase", "mcgpu_dispatch_batch", "mcgpu_get_batch_status",
                    "mcgpu_get_batch_candidate_pairs", "mcgpu_get_batch_partial_cuts", "mcgpu_server_create", "mcgpu_server_release",
                    "mcgpu_server_info", "mcgpu_server_run", "mcgpu_server_get_registry", "mcgpu_predicate_stage",
-                   "mcgpu_dispatch_enforced", "mcgpu_set_stage_timing", "mcgpu_broad_phase_noalloc"):
+                   "mcgpu_dispatch_enforced", "mcgpu_set_stage_timing", "mcgpu_broad_phase_noalloc",
+                   "mcgpu_dispatch_narrow"):
             getattr(L, fn).restype = C.c_int
         _lib = L
     return _lib
@@ -389,6 +397,31 @@ class Context:
         _check(lib().mcgpu_dispatch_frontend(self.h, C.byref(src), C.byref(cut), C.byref(d), C.byref(info)),
                "mcgpu_dispatch_frontend")
         return Result(self, info, src.num_vertices + cut.num_vertices)
+
+    def dispatch_narrow(self, src, cut, src_boxes, cut_boxes, pairs, flags=0, enforcement_count=0, topology=None):
+        """dispatch()'s front-end block alone (kernel.cpp:1779-3229) on the
+        inputs preproc hands it: meshes with the soup's own (shifted)
+        coordinates, face AABBs (F x 6, cut faces enlarged) and candidate
+        (source face, cut face) pairs; topology (optional) = (corner edges of
+        the source mesh, of the cut mesh, edge v0, v1, f0, f1)."""
+        keep = []
+        ms, mc = _host_mesh(src, keep), _host_mesh(cut, keep)
+        a = np.ascontiguousarray(src_boxes, dtype=np.float64).reshape(-1, 6)
+        b = np.ascontiguousarray(cut_boxes, dtype=np.float64).reshape(-1, 6)
+        p = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        t = None
+        if topology is not None:
+            arrs = [np.ascontiguousarray(x, dt) for x, dt in zip(topology, (np.uint32, np.uint32, np.uint32, np.uint32,
+                                                                           np.int32, np.int32))]
+            keep += arrs
+            t = SoupTopology((C.c_void_p * 2)(_ptr(arrs[0]), _ptr(arrs[1])), _ptr(arrs[2]), _ptr(arrs[3]),
+                             _ptr(arrs[4]), _ptr(arrs[5]), arrs[2].size, 0)
+        d = DispatchDesc(flags, 1e-4, enforcement_count, 0, 0, 0)
+        info = FrontendInfo()
+        _check(lib().mcgpu_dispatch_narrow(self.h, C.byref(ms), C.byref(mc), _ptr(a), _ptr(b), _ptr(p), p.shape[0],
+                                           None if t is None else C.byref(t), C.byref(d), C.byref(info)),
+               "mcgpu_dispatch_narrow")
+        return Result(self, info, ms.num_vertices + mc.num_vertices)
 
     def broad_phase(self, src_boxes, cut_boxes):
         """Candidate (source face, cut face) pairs of two sets of face AABBs

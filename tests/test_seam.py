@@ -1,8 +1,18 @@
-"""MCUT built with its broad phase on the GPU (mcut_amd/seam: build_oibvh and
-intersectOIBVHs over libmcgpu) against MCUT compiled unchanged, through MCUT's
-public API on the same inputs: every connected component must be identical
-(type, vertex coordinates bit for bit, faces, face sizes, seam vertices), and
-a failing dispatch must fail with the same McResult."""
+"""MCUT built with parts of its front end on the GPU (mcut_amd/seam) against
+MCUT compiled unchanged, through MCUT's public API on the same inputs.
+
+* The bvh.cpp seam (build_oibvh and intersectOIBVHs over libmcgpu): every
+  connected component identical (type, vertex coordinates bit for bit, faces,
+  face sizes, seam vertices).
+* The whole front end (the bvh.cpp seam plus dispatch()'s front-end block,
+  kernel.cpp:1779-3231, over libmcgpu): the seam hands MCUT's cutting stage
+  the registry in the order MCUT's own narrow phase would have produced it,
+  so the components are identical too: first in canonical form (the same
+  components, each with the same type, the same vertex coordinates bit for
+  bit, the same faces as cycles of coordinates with their orientation, the
+  same seam vertices), then array for array.
+
+A failing dispatch must fail with the same McResult."""
 import os
 
 import numpy as np
@@ -14,6 +24,9 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libmcut_ref.so")
+
+
+GPU_LIB = os.path.join(ROOT, "mcut_amd", "seam", "_build", "libmcut_gpu.so")
 
 
 @pytest.fixture(scope="module")
@@ -65,3 +78,83 @@ def test_connected_components_match(libs, name, pair, flags):
     same(a, b)
     # the seam build ran its broad phase through libmcgpu
     assert "libmcgpu.so" in open("/proc/self/maps").read()
+
+
+def canonical(ccs):
+    """A dispatch's components as a sorted list of (type, vertices, faces,
+    seam) in which vertices are coordinate byte strings and each face is the
+    cycle of its vertices' coordinates started at its smallest one."""
+    if isinstance(ccs, int):
+        return ccs
+    out = []
+    for cc in ccs:
+        V = [v.tobytes() for v in np.ascontiguousarray(cc["vertices"])]
+        faces, at = [], 0
+        for n in cc["face_sizes"]:
+            cyc = [V[i] for i in cc["faces"][at:at + n]]
+            at += n
+            k = cyc.index(min(cyc))
+            faces.append(tuple(cyc[k:] + cyc[:k]))
+        out.append((cc["type"], tuple(sorted(V)), tuple(sorted(faces)), tuple(sorted(V[i] for i in cc["seam"]))))
+    return sorted(out)
+
+
+def gpu_cases():
+    yield from cases()
+    for i in range(64):
+        if i not in (0, 1, 2, 5, 14, 20, 33, 40):
+            yield f"bench{i:03d}", meshes.bench_pair(i), None
+    yield "icosphere3", meshes.icosphere_pair(3, amp=0.02), None
+    yield "icosphere5", meshes.icosphere_pair(5, amp=0.005), None
+
+
+@pytest.fixture(scope="module")
+def gpu_libs():
+    from mcut_amd.seam import mcut_api
+    if not os.path.exists(REF_LIB) or not os.path.exists(GPU_LIB):
+        pytest.skip("the reference build or the GPU front-end build is missing")
+    return mcut_api.Mcut(REF_LIB), mcut_api.Mcut(GPU_LIB)
+
+
+@pytest.mark.parametrize("name,pair,flags", list(gpu_cases()), ids=[c[0] for c in gpu_cases()])
+def test_whole_front_end_components_match(gpu_libs, name, pair, flags):
+    from mcut_amd.seam import mcut_api
+    ref, gpu = gpu_libs
+    if flags is None:
+        flags = mcut_api.MC_DISPATCH_FILTER_ALL
+    print(f"front-end seam case {name}", flush=True)
+    a = run(ref, pair[0], pair[1], flags)
+    b = run(gpu, pair[0], pair[1], flags)
+    assert canonical(a) == canonical(b), name
+    same(a, b)
+    assert "libmcgpu.so" in open("/proc/self/maps").read()
+
+
+def test_whole_front_end_partial_cut_exit(gpu_libs):
+    """MC_DISPATCH_REQUIRE_THROUGH_CUTS on the reference's partial-cut pair 013
+    (computeSeams.cpp:356-398): the GPU build also stops after its front end
+    with the two input components."""
+    from mcut_amd.seam import mcut_api
+    ref, gpu = gpu_libs
+    src, cut = meshes.bench_pair(13)
+    fl = mcut_api.MC_DISPATCH_REQUIRE_THROUGH_CUTS
+    a, b = run(ref, src, cut, fl), run(gpu, src, cut, fl)
+    assert len(a) == len(b) == 2
+    same(a, b)
+
+
+@pytest.mark.parametrize("name,pair", [("icosphere5", meshes.icosphere_pair(5, amp=0.005)),
+                                       ("icosphere4_rough", meshes.icosphere_pair(4, amp=0.02)),
+                                       ("bench030", meshes.bench_pair(30))], ids=["icosphere5", "icosphere4_rough",
+                                                                               "bench030"])
+def test_whole_front_end_with_helper_threads(gpu_libs, name, pair):
+    """Contexts with helper threads (mcCreateContextWithHelpers): MCUT then
+    splits its front-end loops into thread blocks, and the registry order the
+    seam rebuilds follows the same blocks."""
+    from mcut_amd.seam import mcut_api
+    ref, gpu = gpu_libs
+    print(f"threaded front-end seam case {name}", flush=True)
+    fl = mcut_api.MC_DISPATCH_FILTER_ALL
+    a = ref.dispatch(pair[0], pair[1], flags=fl, helpers=7)
+    b = gpu.dispatch(pair[0], pair[1], flags=fl, helpers=7)
+    same(a, b)
