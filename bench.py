@@ -284,38 +284,47 @@ def bench_precision(args, frontend, mg):
 
 def bench_mcdispatch(args, frontend, mg):
     """mcDispatch end to end (input meshes to connected components ready):
-    MCUT built from its own sources, unchanged, against the same MCUT with its
-    broad phase on the GPU (the bvh.cpp seam), on a pair MCUT completes:
-    icosphere-5 vs a rotated icosphere-5 (20,480 triangles each, +-0.5%
-    radial noise, offset +0.5 x).  The GPU front end alone on the same pair
-    is timed beside them."""
+    MCUT built from its own sources against the same MCUT with its whole
+    intersection front end on the GPU (mcut_amd/seam: the bvh.cpp broad phase
+    and dispatch()'s front-end block over libmcgpu; its components are
+    identical to MCUT's), both with MCUT's debug dumps switched off, on
+    displaced icosphere pairs MCUT completes (+-0.5% radial noise, rotated,
+    offset +0.5 x): median of 3 dispatches after one, with a helper thread per
+    usable core but one and single-threaded.  The GPU front end alone is
+    timed beside them."""
     from mcut_amd.seam import mcut_api
-    ref_lib = os.path.join(ROOT, "oracle", "_ref", "libmcut_ref.so")
-    if not (os.path.exists(ref_lib) and os.path.exists(mcut_api.SEAM_LIB)):
+    from oracle import ref_timing as rt
+    quiet = os.path.join(ROOT, "mcut_amd", "seam", "_build", "libmcut_quiet.so")
+    gpu_lib = os.path.join(ROOT, "mcut_amd", "seam", "_build", "libmcut_gpu.so")
+    if not (os.path.exists(quiet) and os.path.exists(gpu_lib)):
         return {"skipped": "MCUT builds not present"}
-    V, F, _ = mg.icosphere(args.mcd_level)
-    A = mg.radial_displace(V, 0.005, seed=1)
-    B = mg.radial_displace(V, 0.005, seed=2) @ mg.random_rotation(seed=3).T
-    B[:, 0] += 0.5
-    src, cut = (A, F, None), (np.ascontiguousarray(B), F.copy(), None)
-    helpers = max(1, min(os.cpu_count() or 1, 16) - 1)
-    ref, seam = mcut_api.Mcut(ref_lib), mcut_api.Mcut(mcut_api.SEAM_LIB)
-    out = {"workload": f"icosphere-{args.mcd_level} pair ({F.size // 3} tris each, +-0.5% radial noise, rotated, "
-                       f"offset +0.5x), MC_DISPATCH_FILTER_ALL",
-           "helpers": helpers}
-    r = args.mcd_repeats
-    out["reference_cpu_ms"] = min(ref.time_dispatch(src, cut, helpers=0, repeats=r))
-    out["reference_cpu_helpers_ms"] = min(ref.time_dispatch(src, cut, helpers=helpers, repeats=r))
-    progress(f"mcDispatch: reference {out['reference_cpu_ms']:.0f} ms (1 thread), "
-             f"{out['reference_cpu_helpers_ms']:.0f} ms ({helpers} helpers)")
-    out["seam_gpu_broad_phase_ms"] = min(seam.time_dispatch(src, cut, helpers=helpers, repeats=r))
-    ctx = frontend.Context(0)
-    ctx.dispatch(src, cut)
-    fe = [ctx.dispatch(src, cut).device_ms for _ in range(5)]
-    ctx.close()
-    out["gpu_frontend_device_ms"] = min(fe)
-    out["seam_vs_reference"] = out["reference_cpu_helpers_ms"] / out["seam_gpu_broad_phase_ms"]
-    progress(f"mcDispatch: seam {out['seam_gpu_broad_phase_ms']:.0f} ms, GPU front end {min(fe):.2f} ms")
+    helpers = max(1, rt.usable_cores() - 1)
+    ref, gpu = mcut_api.Mcut(quiet), mcut_api.Mcut(gpu_lib)
+    out = {"helpers": helpers, "dumps": "off (preproc.cpp:2371 switched in both builds)", "pairs": []}
+    for level in args.mcd_levels:
+        V, F, _ = mg.icosphere(level)
+        A = mg.radial_displace(V, 0.005, seed=1)
+        B = mg.radial_displace(V, 0.005, seed=2) @ mg.random_rotation(seed=3).T
+        B[:, 0] += 0.5
+        src, cut = (A, F, None), (np.ascontiguousarray(B), F.copy(), None)
+        r = args.mcd_repeats
+        row = {"workload": f"icosphere-{level} pair ({F.size // 3} tris each, +-0.5% radial noise, rotated, "
+                           f"offset +0.5x), MC_DISPATCH_FILTER_ALL"}
+        med = lambda t: float(np.median(t[1:])) if len(t) > 1 else float(t[0])
+        row["reference_cpu_ms"] = med(ref.time_dispatch(src, cut, helpers=0, repeats=r + 1))
+        row["reference_cpu_helpers_ms"] = med(ref.time_dispatch(src, cut, helpers=helpers, repeats=r + 1))
+        row["gpu_frontend_mcut_ms"] = med(gpu.time_dispatch(src, cut, helpers=0, repeats=r + 1))
+        row["gpu_frontend_mcut_helpers_ms"] = med(gpu.time_dispatch(src, cut, helpers=helpers, repeats=r + 1))
+        ctx = frontend.Context(0)
+        ctx.dispatch(src, cut)
+        row["gpu_frontend_device_ms"] = min(ctx.dispatch(src, cut).device_ms for _ in range(5))
+        ctx.close()
+        row["speedup_helpers"] = row["reference_cpu_helpers_ms"] / row["gpu_frontend_mcut_helpers_ms"]
+        row["speedup_single_thread"] = row["reference_cpu_ms"] / row["gpu_frontend_mcut_ms"]
+        progress(f"mcDispatch icosphere-{level}: MCUT {row['reference_cpu_helpers_ms']:.0f} ms ({helpers} helpers), "
+                 f"{row['reference_cpu_ms']:.0f} ms (1 thread); GPU front end {row['gpu_frontend_mcut_helpers_ms']:.0f} / "
+                 f"{row['gpu_frontend_mcut_ms']:.0f} ms")
+        out["pairs"].append(row)
     return out
 
 
@@ -335,8 +344,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (three times): the known-size kernel that "
                          "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--mcd-level", type=int, default=5)
-    ap.add_argument("--mcd-repeats", type=int, default=2)
+    ap.add_argument("--mcd-levels", type=int, nargs="+", default=[5, 6])
+    ap.add_argument("--mcd-repeats", type=int, default=3)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[10, 30, 100, 300, 1000, 3000])
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-max", type=int, default=1000)
