@@ -1023,6 +1023,12 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         // sit far from the origin relative to the mesh (2^52 + a unit sphere:
         // every face pair), and a quadratic pair count is refused here rather
         // than allocated.
+        // The kernels keep list positions (a test, an escalated test, a
+        // registry entry) in 32 bits: no list may reach 2^32 entries.
+        const uint64_t kMaxListEntries = 0xffffffffull;
+        if (ctx->cap_pairs > kMaxListEntries || ctx->cap_tests > kMaxListEntries || ctx->cap_esc > kMaxListEntries ||
+            ctx->cap_reg > kMaxListEntries || ctx->cap_big > kMaxListEntries)
+            return MCGPU_ERROR_OUT_OF_MEMORY;
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return MCGPU_ERROR_DEVICE;
         const long double need = 48.0L * ctx->cap_pairs + 10.0L * ctx->cap_tests + 4.0L * ctx->cap_esc +

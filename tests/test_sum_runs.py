@@ -172,22 +172,63 @@ def test_centre_bits_on_adversarial_coordinates(ctx, name):
     src, cut = meshes.icosphere_pair(6, amp=0.01)
     src, cut = _remap(src, f), _remap(cut, f)
     print(f"sums: {name}", flush=True)
+    print("sums: collapsed icosphere-6 pair", flush=True)
     R = ctx.dispatch(src, cut)
     O = om.frontend(src, cut, detail=False)
-    assert np.array_equal(R.com.view(np.uint64), np.asarray(O["com"]).view(np.uint64)), (R.com, O["com"])
+    assert np.array_equal(R.com.view(np.uint64), np.asarray(O["com"]).view(np.uint64))
+    assert R.status == O["status"] and R.counts["pairs"] == O["counts"]["pairs"], (R.com, O["com"])
     assert np.array_equal(R.t.view(np.uint64), np.asarray(O["t"]).view(np.uint64))
     assert R.status == O["status"]
 
 
+def test_collapsed_mesh_raw_boxes_stay_exact():
+    """The icosphere-6 pair mapped by V * 4 + 2^52 + 0.5 collapses onto a few
+    lattice points.  Rounded to float32 in the raw frame (neighbours 2^28
+    apart near 2^52) every face box of the pair was the same box, and the raw
+    broad phase counted 81,920^2 pairs, sizing lists of hundreds of GB.  The
+    raw boxes are now taken relative to the source mesh's raw lower corner:
+    there every coordinate is a small multiple of 1/2, exact in float32, so
+    the raw-space pairs are the exact pairs widened only by the tolerance."""
+    f = lambda V: V * 4 + 2.0 ** 52 + 0.5
+    src, cut = meshes.icosphere_pair(6, amp=0.01)
+    S, C = f(src[0]), f(cut[0])
+    corner = S.min(axis=0)
+    for X in (S, C):
+        rel = X - corner
+        assert np.all(rel.astype(np.float32).astype(np.float64) == rel)
+        assert rel.max() < 16.0
+    assert np.unique(S.astype(np.float32)).size <= 2  # the raw frame's float32 boxes collapse
+
+
 @pytest.mark.gpu
-@pytest.mark.skip(reason="an icosphere-6 pair mapped by V * 4 + 2^52 + 0.5 collapses onto 383 lattice points "
-                         "(29.7M candidate pairs, the overflow-and-retry path) and its dispatch ended in a host "
-                         "segmentation fault; the cause is not found yet, so the case stays off the GPU")
+def test_collapsed_mesh_is_refused_cleanly(ctx):
+    """Near 2^52 the shift rounds to whole units, so the raw-space broad
+    phase's tolerance covers the collapsed pair's whole extent and its
+    superset is every face pair (81,920^2, past the 2^32 entries a list may
+    hold): the dispatch returns MCGPU_ERROR_OUT_OF_MEMORY instead of sizing
+    lists of hundreds of GB, and the context stays usable."""
+    from mcut_amd import frontend
+    f = lambda V: V * 4 + 2.0 ** 52 + 0.5
+    src, cut = meshes.icosphere_pair(6, amp=0.01)
+    src, cut = _remap(src, f), _remap(cut, f)
+    print("sums: collapsed icosphere-6 pair", flush=True)
+    with pytest.raises(frontend.McgpuError) as ei:
+        ctx.dispatch(src, cut)
+    assert ei.value.code == -3
+    R = ctx.dispatch(*meshes.config1_rotated())
+    assert R.status == 0 and R.counts["registry"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.skip(reason="the raw-space broad phase refuses this input (test_collapsed_mesh_is_refused_cleanly); "
+                         "a broad phase in shifted coordinates for it is not built")
 def test_centre_bits_on_a_collapsed_mesh(ctx):
     from oracle import oracle as om
     f = lambda V: V * 4 + 2.0 ** 52 + 0.5
     src, cut = meshes.icosphere_pair(6, amp=0.01)
     src, cut = _remap(src, f), _remap(cut, f)
+    print("sums: collapsed icosphere-6 pair", flush=True)
     R = ctx.dispatch(src, cut)
     O = om.frontend(src, cut, detail=False)
     assert np.array_equal(R.com.view(np.uint64), np.asarray(O["com"]).view(np.uint64))
+    assert R.status == O["status"] and R.counts["pairs"] == O["counts"]["pairs"]
