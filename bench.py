@@ -158,6 +158,14 @@ def bench_batch(args, torch, frontend, mg):
     B = bctx.dispatch_batch_device(np_, ms, mc, offs)  # warm-up: sizes the buffers
     torch.cuda.synchronize()
     progress(f"batched mix: warm-up batch done ({B.counts})")
+    tuning = None
+    if args.shape:
+        bctx.set_tuning(*args.shape)
+        tuning = dict(wave_leaves=args.shape[0], tree_stride=args.shape[1], queues=args.shape[2],
+                      chunk=args.shape[3], forced=True)
+    elif not args.no_autotune:  # the traversal shape for this batch class, before the timed batches
+        tuning = bctx.autotune()
+        progress(f"batched mix: autotuned {tuning}")
     bt = []
     for _ in range(args.batch_steps):
         t1 = time.perf_counter()
@@ -168,6 +176,7 @@ def bench_batch(args, torch, frontend, mg):
     bstages = bctx.dispatch_batch_device(np_, ms, mc, offs).stage_ms()
     bctx.set_stage_timing(False)
     progress(f"batched mix: batch {batch_s * 1e3:.1f} ms")
+    B = bctx.dispatch_batch_device(np_, ms, mc, offs)  # the per-pair outputs, read before the next dispatch
     reg_b = np.diff(B.reg_offsets.astype(np.int64))
     status_b = B.status.copy()
     counts_b = dict(B.counts)
@@ -204,7 +213,7 @@ def bench_batch(args, torch, frontend, mg):
             "batch_ms": batch_s * 1e3, "batch_device_ms": device_ms_b, "plain_loop_ms": plain_s * 1e3,
             "batched_vs_plain": plain_s / batch_s, "batch_pairs_per_s": I / batch_s,
             "plain_pairs_per_s": I / plain_s, "counts": counts_b, "stage_ms": bstages,
-            "generate_s": gen_s}
+            "autotune": tuning, "generate_s": gen_s}
 
 
 def bench_serve(args, frontend, mg):
@@ -341,6 +350,11 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-mcdispatch", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the default traversal shape instead of the autotuned one")
+    ap.add_argument("--shape", type=int, nargs=4, metavar=("LEAVES", "STRIDE", "QUEUES", "CHUNK"),
+                    help="time this traversal shape (source leaves per block, cut levels per step, 0 or 8 work "
+                         "queues, blocks per pop) instead of tuning one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (three times): the known-size kernel that "
                          "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -385,6 +399,14 @@ def main():
     for _ in range(args.warmup):
         R = ctx.dispatch_device(ms, mc)
     torch.cuda.synchronize()
+    tuning = None
+    if args.shape:  # a fixed traversal shape (profiling runs: no tuner launches in the trace)
+        ctx.set_tuning(*args.shape)
+        R = ctx.dispatch_device(ms, mc)
+        tuning = dict(ctx.tuning(), forced=True)
+    elif not args.no_autotune:  # the traversal shape for this workload class, before the timed steps
+        tuning = ctx.autotune()
+        progress(f"single pair: autotuned {tuning}")
     lat = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -447,6 +469,7 @@ def main():
         "latency_ms": {"p50": lat_sorted[len(lat) // 2], "p99": lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))]},
         "counts": counts,
         "stage_ms": stages,
+        "autotune": tuning,
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "stage": dom, "kernel_ms": kernel_ms,
                      "stage_event_ms": stages[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -375,6 +375,41 @@ MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity);
 MCGPU_API mcgpu_result mcgpu_set_stage_timing(mcgpu_context ctx, int enable);
 MCGPU_API const char* mcgpu_stage_name(int stage);
 
+/* Launch shape of the broad-phase traversal of a dispatch.  No counterpart
+ * in the reference, whose broad phase is a recursive CPU walk
+ * (bvh.cpp:638-783): these are the knobs of the GPU traversal that the
+ * autotuner picks per workload class (single pairs or batches, by the bit
+ * lengths of the two face counts). */
+typedef struct mcgpu_tuning {
+    uint32_t wave_leaves;       /* source leaves per traversal block: 8, 16 or 32 */
+    uint32_t tree_stride;       /* cut-tree levels per breadth-first step: 2, 3 or 4 */
+    uint32_t queues;            /* 0: one wave per block; 8: persistent waves over 8 per-XCD work queues */
+    uint32_t chunk;             /* blocks a persistent wave takes per queue pop */
+    uint32_t broad_share_pct;   /* percent of the CUs outside the sums' reservation that the
+                                   broad-phase stream holds (reported; 0 in mcgpu_set_tuning) */
+    uint32_t candidates;        /* traversal shapes the autotuner timed (0: defaults, not tuned) */
+    float traversal_ms;         /* kernel time of the chosen traversal on the tuned dispatch */
+    float default_traversal_ms; /* kernel time of the default traversal on it */
+    float dispatch_ms;          /* device time of the dispatch with the chosen shape */
+    float default_dispatch_ms;  /* device time of the dispatch with the default shape */
+} mcgpu_tuning;
+
+/* Tunes the traversal shape for the workload class of the last successful
+ * mcgpu_dispatch_frontend or mcgpu_dispatch_batch (its inputs must still be
+ * valid): 27 shapes timed alone on its trees, three runs each, each required
+ * to find the same raw pairs; then the dispatch re-run twice with the default
+ * and twice with the chosen shape for the report.  Later dispatches of the
+ * class use the choice.  The last dispatch's outputs stay valid. */
+MCGPU_API mcgpu_result mcgpu_autotune(mcgpu_context ctx, mcgpu_tuning* out);
+/* enable != 0: the first dispatch of each untuned class tunes it (above) */
+MCGPU_API mcgpu_result mcgpu_set_autotune(mcgpu_context ctx, int enable);
+/* Forces a traversal shape for every later dispatch (t = NULL: back to the
+ * tuned or default choice).  MCGPU_ERROR_INVALID_VALUE for a shape outside
+ * the lists above or a nonzero broad_share_pct. */
+MCGPU_API mcgpu_result mcgpu_set_tuning(mcgpu_context ctx, const mcgpu_tuning* t);
+/* the shape the last dispatch ran with (and its tuning report) */
+MCGPU_API mcgpu_result mcgpu_get_tuning(mcgpu_context ctx, mcgpu_tuning* out);
+
 #ifdef __cplusplus
 }
 #endif

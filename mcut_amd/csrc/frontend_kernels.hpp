@@ -1441,15 +1441,16 @@ struct Tree {
 };
 
 // Wavefront-cooperative BVH x BVH traversal.  A 64-lane wave owns
-// kWaveLeaves Morton-consecutive source leaves; their union is the source-tree
-// node at depth D_s - log2(kWaveLeaves).  The wave walks the cut tree breadth
+// 2^LL Morton-consecutive source leaves (a source block; kWaveLeaves by
+// default, the autotuner may choose 8 or 32); their union is the source-tree
+// node at depth D_s - LL.  The wave walks the cut tree breadth
 // first with that node's box, all lanes testing children in parallel and
 // compacting the survivors into an LDS frontier with ballots; at the leaf
 // level every lane owning a source face tests it against the surviving cut
 // leaves, staged through LDS.  A walk starts at cut level kStartLevel with all
 // of its nodes, tested in one memory round trip, instead of descending the
 // nearly empty levels above one round trip each, and then descends
-// kTravStride levels per step.  16 source leaves per wave (not 32) keep the
+// STRIDE levels per step (kTravStride by default).  16 source leaves per wave (not 32) keep the
 // union box tight; with the 3-level steps the traversal of the icosphere-7
 // pair went from 0.23 to 0.17 ms.
 // A frontier wider than kSplitFrontier means the node's leaves are spread
@@ -1463,6 +1464,7 @@ struct Tree {
 // is latency-bound, so its LDS (8 KB a wave) and registers (at most 128, four
 // waves per SIMD) are sized for occupancy: against 14 KB and 129 registers
 // the traversal went from 0.171 to 0.157 ms.
+constexpr unsigned long long kMaxListEntries = 0xffffffffull; // list positions are 32-bit
 constexpr int kFrontierCap = 512;
 constexpr int kSplitFrontier = 256;
 constexpr int kWaveLeavesLog = 4;
@@ -1498,17 +1500,26 @@ __device__ __forceinline__ void per_lane_walk(const float* q, uint32_t s, const 
     const long long budget = 2ll << D;
     int l = 0;
     uint32_t k = 0;
+    // a walk that starts with the list already full only counts its pairs
+    // (one atomic at its end instead of one per pair); the host sizes the
+    // next attempt, or refuses, from the count
+    const bool counting = __hip_atomic_load(&C->n_raw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cap;
+    unsigned long long counted = 0;
     for (long long it = 0; it < budget; ++it) {
         const float* nb = cut.nodes + 6 * ((uint64_t)(1u << l) - 1 + k);
         float b[6];
         for (int j = 0; j < 6; ++j) b[j] = nb[j];
         if (boxes_overlap(q, b)) {
             if (l == D) {
-                const uint32_t c = cut.sface[k];
-                const uint64_t key = ((uint64_t)s << 32) | c;
-                const unsigned long long at = wave_append(&C->n_raw);
-                if (at < cap) pairs[at] = key;
-                else atomicOr(&C->ov_pairs, 1u);
+                if (counting) {
+                    ++counted;
+                } else {
+                    const uint32_t c = cut.sface[k];
+                    const uint64_t key = ((uint64_t)s << 32) | c;
+                    const unsigned long long at = wave_append(&C->n_raw);
+                    if (at < cap) pairs[at] = key;
+                    else atomicOr(&C->ov_pairs, 1u);
+                }
             } else {
                 ++l;
                 k <<= 1;
@@ -1522,10 +1533,29 @@ __device__ __forceinline__ void per_lane_walk(const float* q, uint32_t s, const 
         if (l == 0) break;
         ++k;
     }
+    if (counted) {
+        atomicAdd(&C->n_raw, counted);
+        atomicOr(&C->ov_pairs, 1u);
+    }
 }
 
+// How the source blocks (work items of LL-level source nodes) reach the waves.
+// queues == 0: one wave per block, grid-stride (the hardware dispatcher hands
+// out the waves).  queues == Q > 0: a persistent grid; the blocks, in Morton
+// order, are cut into Q contiguous ranges, each with a head counter
+// (Counters::trav_q), and a wave takes `chunk` blocks at a time from the range
+// of queue blockIdx % Q, then from the others in turn.  Workgroups are handed
+// to the XCDs round robin, so with Q = 8 a range of Morton-adjacent blocks,
+// and the part of the cut tree they reach, stays on one XCD's L2 as long as
+// its waves find work there.
+struct TravQueue {
+    int queues;
+    int chunk;
+};
+
+template <int LL, int STRIDE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_traverse_wave(Tree src, Tree cut, unsigned long long* __restrict__ pairs,
-                                                      uint64_t cap, Counters* C)
+                                                      uint64_t cap, Counters* C, TravQueue tq)
 {
     __shared__ uint32_t front[2][kFrontierCap];
     __shared__ float lbox[64][6];
@@ -1535,12 +1565,46 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     const int lane = threadIdx.x;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int Ds = src.D;
-    const int lw = Ds >= kWaveLeavesLog ? Ds - kWaveLeavesLog : 0; // level of a wave's source node
-    const int nwaves = Ds >= kWaveLeavesLog ? (src.n + kWaveLeaves - 1) / kWaveLeaves : 1;
+    const int lw = Ds >= LL ? Ds - LL : 0; // level of a wave's source node
+    const uint32_t nwaves = Ds >= LL ? (uint32_t)((src.n + (1 << LL) - 1) >> LL) : 1u;
     const int s0 = cut.D < kStartLevel ? cut.D : kStartLevel;
     const uint32_t leaf_base = (uint32_t)(1u << cut.D) - 1;
     unsigned long long splits = 0, lane_walks = 0;
-    for (int w = blockIdx.x; w < nwaves; w += gridDim.x) {
+    const int nq = tq.queues;
+    const int home = nq ? (int)(blockIdx.x % (unsigned)nq) : 0;
+    int qr = 0;                                  // queues tried so far (persistent)
+    uint32_t w = nq ? 0u : blockIdx.x, wend = 0; // current block; end of the taken chunk
+    for (;;) {
+        if (nq) {
+            if (w >= wend) { // take the next chunk: home queue first, then the others
+                bool got = false;
+                while (qr < nq) {
+                    const int qi = (home + qr) % nq;
+                    const uint32_t lo = (uint32_t)((uint64_t)nwaves * qi / nq);
+                    const uint32_t hi = (uint32_t)((uint64_t)nwaves * (qi + 1) / nq);
+                    uint32_t g = 0;
+                    if (lane == 0) g = atomicAdd(&C->trav_q[qi], (uint32_t)tq.chunk);
+                    g = __builtin_amdgcn_readfirstlane(g);
+                    if (g < hi - lo) {
+                        w = lo + g;
+                        wend = (hi - w) < (uint32_t)tq.chunk ? hi : w + (uint32_t)tq.chunk;
+                        got = true;
+                        break;
+                    }
+                    ++qr;
+                }
+                if (!got) break;
+            }
+        } else if (w >= nwaves) {
+            break;
+        }
+        // past 2^32 raw pairs the host refuses the dispatch whatever the
+        // exact count: the remaining blocks are not walked
+        {
+            unsigned long long seen = 0;
+            if (lane == 0) seen = __hip_atomic_load(&C->n_raw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__shfl(seen, 0, 64) > kMaxListEntries) break;
+        }
         int l = lw;
         uint32_t k = (uint32_t)w;
         for (;;) {
@@ -1594,7 +1658,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             __syncthreads();
             while (nf > 0 && level < cut.D && !overflow) {
                 int nn = 0;
-                const int step = cut.D - level < kTravStride ? cut.D - level : kTravStride;
+                const int step = cut.D - level < STRIDE ? cut.D - level : STRIDE;
                 const int nchild = nf << step;
                 const uint32_t base_next = (uint32_t)(1u << (level + step)) - 1;
                 for (int c0 = 0; c0 < nchild; c0 += 64 * kTravILP) {
@@ -1670,6 +1734,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             if (l == lw) break;
             ++k;
         }
+        w += nq ? 1u : gridDim.x;
     }
     if (sn > 0) flush_pairs(stage, sn, pairs, cap, C);
     if (lane == 0 && splits) atomicAdd(&C->n_fallback, splits); // rare: statistics only

@@ -45,6 +45,7 @@ struct Counters {
     unsigned int gp, bad_src, bad_cut, unsupported, nonmanifold, bad_input, pair_fail, partial;
     unsigned int sum_next; // batches: work-queue head of the serial sums
     unsigned int ov_trip;  // the triplet scratch overflowed
+    unsigned int trav_q[8]; // persistent traversal: head of each work queue (blocks taken)
     // raw bounding box of each mesh as order keys (lo xyz, hi xyz); written
     // while the input is validated and kept by the per-dispatch counter reset
     unsigned long long rbox[2][6];

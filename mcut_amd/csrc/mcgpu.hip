@@ -41,7 +41,7 @@ struct Buf {
 // CUs left after the sums (174 of 248 on the 256-CU part): it is the longest
 // of the three paths, and the edge numbering finishes well inside it.  A joins B before the exact pair filter and
 // C before the edge-face tests.  A batch runs on a second pair of streams
-// split kBatchBroadShare : rest: its edge numbering builds two hash tables per
+// kBatchBroadShare : rest: its edge numbering builds two hash tables per
 // pair over the whole batch and is as long as its broad phase (139 ms a
 // 4096-pair batch at 0.70, 113 ms at 0.58).  The pair is made on a context's
 // first batch, so a context that never batches holds four hardware queues.
@@ -75,6 +75,30 @@ const char* kStageNames[ST_COUNT] = {"coordinate_sums", "shift_params", "shift_b
 enum Ev { EV_A0, EV_S0, EV_A_SUMS, EV_A_PARAMS, EV_A_SHIFT, EV_A_JOIN, EV_A_FILTER, EV_A_PLANES, EV_A_JOIN2, EV_A_TESTS, EV_A_NARROW,
           EV_A_END, EV_FORK, EV_B0, EV_B_SORT, EV_B_REFIT, EV_B_TRAV, EV_C0, EV_C_FILL, EV_C_TOPO, EV_SUM_ITEMS,
           EV_SUM_CHAIN, EV_TRAV_K0, EV_TRAV_K1, EV_COUNT };
+
+// Launch shape of the traversal kernel, chosen per workload class by the
+// autotuner (mcgpu_autotune) or set by the caller (mcgpu_set_tuning).
+struct TravConfig {
+    int ll = kWaveLeavesLog;  // source leaves per block: 2^ll
+    int stride = kTravStride; // cut-tree levels per breadth-first step
+    int queues = 0;           // 0: one wave per block; 8: persistent waves over XCD work queues
+    int chunk = 1;            // blocks a persistent wave takes per queue pop
+};
+constexpr int kTravWavesPerCU = 16; // persistent grid: four waves per SIMD (the kernel's occupancy)
+
+// the traversal configurations the autotuner times
+constexpr int kTuneLL[] = {3, 4, 5};
+constexpr int kTuneStride[] = {2, 3, 4};
+constexpr int kTuneQueue[][2] = {{0, 1}, {8, 1}, {8, 4}}; // (queues, chunk)
+
+// one autotuned workload class: single pairs or batches, by the bit lengths of
+// the two face counts
+struct TuneEntry {
+    bool batch;
+    int bs, bc;
+    TravConfig trav;
+    mcgpu_tuning report;
+};
 
 uint64_t next_pow2(uint64_t x)
 {
@@ -188,6 +212,17 @@ struct mcgpu_context_t {
     // and ssum are s), the source mesh is the server's resident one
     const Resident* res = nullptr;
     bool own_streams = true;
+
+    // traversal launch shape of the current dispatch; the autotuned classes;
+    // a caller's override
+    int ncu = 0;                       // compute units of the device
+    TravConfig trav;
+    bool autotune = false;
+    std::vector<TuneEntry> tuned;
+    bool forced = false;
+    TravConfig forced_trav;
+    mcgpu_tuning used{};               // what the last dispatch ran with
+    Buf tune_c;                        // counters of the tuner's traversal runs
 };
 
 #define CK(x)                                                                                                          \
@@ -685,7 +720,79 @@ struct BroadView {
     const uint32_t* src_sface;
     const float* src_nodes;
     int src_depth;
+    TravConfig trav; // traversal launch shape
+    int cus;         // compute units of the stream the traversal runs on
 };
+
+template <int LL, int ST>
+void launch_trav_t(dim3 grid, hipStream_t st, hipEvent_t a, hipEvent_t b, const Tree& src, const Tree& cut,
+                   unsigned long long* pairs, uint64_t cap, Counters* C, TravQueue q)
+{
+    if (a) // the launch itself records the kernel's start and end (its dispatch timestamps)
+        hipExtLaunchKernelGGL(k_traverse_wave<LL, ST>, grid, dim3(64), 0, st, a, b, 0, src, cut, pairs, cap, C, q);
+    else
+        k_traverse_wave<LL, ST><<<grid, 64, 0, st>>>(src, cut, pairs, cap, C, q);
+}
+
+// The traversal of src against cut with launch shape t on stream st, whose
+// mask holds `cus` compute units.  One wave per source block, or a persistent
+// grid of kTravWavesPerCU waves per CU over the work queues (whose heads the
+// per-dispatch counter reset cleared).
+void launch_traversal(const TravConfig& t, int cus, const Tree& src, const Tree& cut, unsigned long long* pairs,
+                      uint64_t cap, Counters* C, hipStream_t st, hipEvent_t a = nullptr, hipEvent_t b = nullptr)
+{
+    const uint64_t nblocks = src.D >= t.ll ? (((uint64_t)src.n + (1u << t.ll) - 1) >> t.ll) : 1;
+    const TravQueue q{t.queues, t.chunk < 1 ? 1 : t.chunk};
+    uint64_t g = q.queues ? std::min<uint64_t>((nblocks + q.chunk - 1) / q.chunk,
+                                               (uint64_t)std::max(cus, 1) * kTravWavesPerCU)
+                          : std::min<uint64_t>(nblocks, 65536);
+    const dim3 grid((unsigned)std::max<uint64_t>(g, 1));
+    switch (t.ll * 16 + t.stride) {
+    case 3 * 16 + 2: launch_trav_t<3, 2>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    case 3 * 16 + 3: launch_trav_t<3, 3>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    case 3 * 16 + 4: launch_trav_t<3, 4>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    case 4 * 16 + 2: launch_trav_t<4, 2>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    case 4 * 16 + 4: launch_trav_t<4, 4>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    case 5 * 16 + 2: launch_trav_t<5, 2>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    case 5 * 16 + 3: launch_trav_t<5, 3>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    case 5 * 16 + 4: launch_trav_t<5, 4>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    default: launch_trav_t<kWaveLeavesLog, kTravStride>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    }
+}
+
+bool trav_config_valid(const TravConfig& t)
+{
+    return t.ll >= 3 && t.ll <= 5 && t.stride >= 2 && t.stride <= 4 && (t.queues == 0 || t.queues == 8) &&
+           t.chunk >= 1 && t.chunk <= 64;
+}
+
+// The two trees of a broad view once its leaves are sorted (into mface2 when
+// `alt`): refit state R and traversal view T.
+void trees_of(const BroadView& B, bool alt, Tree* T, RefitTree* R)
+{
+    const uint32_t Fs = B.nf[0];
+    const uint32_t* sorted = alt ? B.mface2 : B.mface;
+    for (int m = 0; m < 2; ++m) {
+        R[m].facebox = B.rawbox + 6 * (uint64_t)(m ? Fs : 0);
+        R[m].sface = sorted + (m ? Fs : 0);
+        R[m].nodes = B.nodes[m];
+        R[m].n = (int)B.nf[m];
+        R[m].D = B.depth[m];
+        R[m].lin = B.depth[m];
+        if (m == 0 && B.src_nodes) { // a resident source: already refitted
+            R[0].facebox = B.src_rawbox;
+            R[0].sface = B.src_sface;
+            R[0].nodes = const_cast<float*>(B.src_nodes);
+            R[0].D = B.src_depth;
+            R[0].lin = -1;
+        }
+        T[m].sface = R[m].sface;
+        T[m].nodes = R[m].nodes;
+        T[m].facebox = R[m].facebox;
+        T[m].n = R[m].n;
+        T[m].D = R[m].D;
+    }
+}
 
 // Both trees and the raw-space traversal, on stream st, once the leaf keys
 // (mcode, mface) and single-precision leaf boxes (rawbox) are written: one
@@ -705,29 +812,9 @@ bool enqueue_trees_traverse(const BroadView& B, hipStream_t st, Counters* C, boo
     const bool alt = radix_sort<uint32_t>(B.mcode + o0, B.mface + o0, B.mcode2 + o0, B.mface2 + o0, nullptr, F - o0,
                                           F - o0, B.sort_bits, B.rt, st, ghist_zeroed);
     if (b_sort) (void)hipEventRecord(b_sort, st);
-    const uint32_t* sorted = alt ? B.mface2 : B.mface;
     Tree T[2];
     RefitTree R[2];
-    for (int m = 0; m < 2; ++m) {
-        R[m].facebox = B.rawbox + 6 * (uint64_t)(m ? Fs : 0);
-        R[m].sface = sorted + (m ? Fs : 0);
-        R[m].nodes = B.nodes[m];
-        R[m].n = (int)B.nf[m];
-        R[m].D = B.depth[m];
-        R[m].lin = B.depth[m];
-        if (m == 0 && res) {
-            R[0].facebox = B.src_rawbox;
-            R[0].sface = B.src_sface;
-            R[0].nodes = const_cast<float*>(B.src_nodes);
-            R[0].D = B.src_depth;
-            R[0].lin = -1; // already refitted
-        }
-        T[m].sface = R[m].sface;
-        T[m].nodes = R[m].nodes;
-        T[m].facebox = R[m].facebox;
-        T[m].n = R[m].n;
-        T[m].D = R[m].D;
-    }
+    trees_of(B, alt, T, R);
     // each round lifts both trees by up to kRefitLevels levels; a tree of
     // 2^19 leaves needs two rounds
     for (;;) {
@@ -750,17 +837,26 @@ bool enqueue_trees_traverse(const BroadView& B, hipStream_t st, Counters* C, boo
         }
     }
     if (b_refit) (void)hipEventRecord(b_refit, st);
-    const unsigned nwaves = (Fs + kWaveLeaves - 1) / kWaveLeaves;
-    const unsigned grid = nwaves < 65536 ? nwaves : 65536;
-    if (k_start) // the launch itself records the kernel's start and end (its dispatch timestamps)
-        hipExtLaunchKernelGGL(k_traverse_wave, dim3(grid), dim3(64), 0, st, k_start, k_stop, 0, T[0], T[1], B.raw_pairs,
-                              B.cap_pairs, C);
-    else
-        k_traverse_wave<<<grid, 64, 0, st>>>(T[0], T[1], B.raw_pairs, B.cap_pairs, C);
+    launch_traversal(B.trav, B.cus, T[0], T[1], B.raw_pairs, B.cap_pairs, C, st, k_start, k_stop);
     return alt;
 }
 
-void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool ghist_zeroed)
+// compute units of a broad-phase stream that takes pct percent of the CUs
+// outside the sums' reservation (the rounding of create_split_streams)
+int cus_of_share(int ncu, int pct)
+{
+    if (ncu <= 4 * kSumCUs) return ncu;
+    return (int)(pct / 100.0 * (ncu - kSumCUs) + 0.5);
+}
+
+int cus_of(mcgpu_context ctx, bool batch)
+{
+    if (!ctx->own_streams) return ctx->ncu;
+    return cus_of_share(ctx->ncu, (int)((batch ? kBatchBroadShare : kBroadShare) * 100.0 + 0.5));
+}
+
+// the broad view of a context's last (or current) dispatch
+BroadView broad_view(mcgpu_context ctx)
 {
     BroadView B{};
     B.mcode = P_<uint32_t>(ctx->mcode);
@@ -783,6 +879,14 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool g
         B.src_nodes = ctx->res->nodes;
         B.src_depth = ctx->res->depth;
     }
+    B.trav = ctx->trav;
+    B.cus = cus_of(ctx, ctx->batch);
+    return B;
+}
+
+void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool ghist_zeroed)
+{
+    const BroadView B = broad_view(ctx);
     const bool t = ctx->stage_timing;
     ctx->mface_alt = enqueue_trees_traverse(B, st, C, ghist_zeroed, t ? ctx->ev[EV_B_SORT] : nullptr,
                                             t ? ctx->ev[EV_B_REFIT] : nullptr, t ? ctx->ev[EV_TRAV_K0] : nullptr,
@@ -1010,6 +1114,14 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
     if (C.unsupported & 2u) return MCGPU_ERROR_UNSUPPORTED;
     if (C.ov_pairs || C.ov_tests || C.ov_esc || C.ov_reg || C.ov_big || C.ov_trip) {
         *overflow = true;
+        // a refusal leaves the context's capacities as they were
+        const uint64_t was[6] = {ctx->cap_pairs, ctx->cap_tests, ctx->cap_esc, ctx->cap_reg, ctx->cap_big,
+                                 ctx->cap_trip};
+        auto refuse = [&]() {
+            ctx->cap_pairs = was[0], ctx->cap_tests = was[1], ctx->cap_esc = was[2];
+            ctx->cap_reg = was[3], ctx->cap_big = was[4], ctx->cap_trip = was[5];
+            return MCGPU_ERROR_OUT_OF_MEMORY;
+        };
         if (C.ov_pairs) ctx->cap_pairs = std::max<uint64_t>(2 * ctx->cap_pairs, C.n_raw + C.n_raw / 4);
         if (C.ov_tests) ctx->cap_tests = std::max<uint64_t>(2 * ctx->cap_tests, C.n_tests + C.n_tests / 4);
         if (C.ov_esc) ctx->cap_esc = std::max<uint64_t>(2 * ctx->cap_esc, C.n_esc + C.n_esc / 4);
@@ -1025,16 +1137,15 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         // than allocated.
         // The kernels keep list positions (a test, an escalated test, a
         // registry entry) in 32 bits: no list may reach 2^32 entries.
-        const uint64_t kMaxListEntries = 0xffffffffull;
         if (ctx->cap_pairs > kMaxListEntries || ctx->cap_tests > kMaxListEntries || ctx->cap_esc > kMaxListEntries ||
             ctx->cap_reg > kMaxListEntries || ctx->cap_big > kMaxListEntries)
-            return MCGPU_ERROR_OUT_OF_MEMORY;
+            return refuse();
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return MCGPU_ERROR_DEVICE;
         const long double need = 48.0L * ctx->cap_pairs + 10.0L * ctx->cap_tests + 4.0L * ctx->cap_esc +
                                  100.0L * ctx->cap_reg + 16.0L * ctx->cap_trip;
         if (need > 0.9L * (long double)(free_b + ctx->raw_pairs.cap + ctx->pairs.cap + ctx->phash.cap))
-            return MCGPU_ERROR_OUT_OF_MEMORY;
+            return refuse();
         return MCGPU_SUCCESS;
     }
     if (C.unsupported) return MCGPU_ERROR_UNSUPPORTED;
@@ -1187,6 +1298,188 @@ static hipError_t create_split_streams(int ncu, double share, hipStream_t* b, hi
     return e;
 }
 
+// percent of the CUs outside the sums' reservation that the dispatch's
+// broad-phase stream holds (100: one stream for everything)
+static int split_pct(mcgpu_context ctx)
+{
+    if (!ctx->own_streams) return 100;
+    return (int)((ctx->batch ? kBatchBroadShare : kBroadShare) * 100.0 + 0.5);
+}
+
+static mcgpu_tuning tuning_report(const TravConfig& t, int pct)
+{
+    mcgpu_tuning r{};
+    r.wave_leaves = 1u << t.ll;
+    r.tree_stride = (uint32_t)t.stride;
+    r.queues = (uint32_t)t.queues;
+    r.chunk = (uint32_t)t.chunk;
+    r.broad_share_pct = (uint32_t)pct;
+    return r;
+}
+
+// The traversal shape of the dispatch about to run (sizes set): the
+// caller's override, else the autotuned entry of its workload class, else the
+// default.  Batches get their stream pair on the context's first batch.
+static mcgpu_result apply_tuning(mcgpu_context ctx)
+{
+    if (ctx->batch && ctx->own_streams && !ctx->s2b)
+        CK(create_split_streams(ctx->ncu, kBatchBroadShare, &ctx->s2b, &ctx->s3b));
+    TravConfig t;
+    const TuneEntry* hit = nullptr;
+    if (ctx->forced) {
+        t = ctx->forced_trav;
+    } else {
+        const int bs = bits_for((uint64_t)ctx->nf[0] + 1), bc = bits_for((uint64_t)ctx->nf[1] + 1);
+        for (const TuneEntry& e : ctx->tuned)
+            if (e.batch == ctx->batch && e.bs == bs && e.bc == bc) {
+                hit = &e;
+                t = e.trav;
+                break;
+            }
+    }
+    ctx->trav = t;
+    ctx->used = hit ? hit->report : tuning_report(t, split_pct(ctx));
+    return MCGPU_SUCCESS;
+}
+
+// The last dispatch once more with the same inputs and capacities (which it
+// sized): its device time in *ms.
+static mcgpu_result rerun_dispatch(mcgpu_context ctx, float* ms)
+{
+    if (ctx->batch) enqueue_validate_batch(ctx);
+    else enqueue_validate(ctx);
+    ctx->stage_mode = 0;
+    enqueue_frontend(ctx);
+    bool overflow = false;
+    mcgpu_result r = finish(ctx, nullptr, &overflow);
+    if (r != MCGPU_SUCCESS) return r;
+    if (overflow) return MCGPU_ERROR_DEVICE; // the same input cannot outgrow the lists it sized
+    if (ctx->batch && (r = finish_batch(ctx, nullptr)) != MCGPU_SUCCESS) return r;
+    *ms = ctx->total_ms;
+    return MCGPU_SUCCESS;
+}
+
+// The traversal of the last dispatch's trees with launch shape t, `reps`
+// times on its broad-phase stream, into the tuner's own counters: the median
+// kernel time (from the launch's dispatch timestamps) in *ms, and whether
+// every run found the dispatch's raw pair count with no overflow.
+static mcgpu_result time_traversal(mcgpu_context ctx, const TravConfig& t, int reps, float* ms, bool* same)
+{
+    BroadView B = broad_view(ctx);
+    B.trav = t;
+    Tree T[2];
+    RefitTree R[2];
+    trees_of(B, ctx->mface_alt, T, R);
+    Counters* C;
+    CK(ensure(ctx->tune_c, 1, &C));
+    hipStream_t st = ctx->batch ? ctx->s2b : ctx->s2;
+    hipEvent_t e[2];
+    CK(hipEventCreate(&e[0]));
+    if (hipEventCreate(&e[1]) != hipSuccess) {
+        (void)hipEventDestroy(e[0]);
+        return MCGPU_ERROR_DEVICE;
+    }
+    std::vector<float> v;
+    mcgpu_result r = MCGPU_SUCCESS;
+    *same = true;
+    for (int i = 0; i < reps && r == MCGPU_SUCCESS; ++i) {
+        fill_u32(C, 0u, sizeof(Counters) / 4, st);
+        launch_traversal(t, B.cus, T[0], T[1], B.raw_pairs, B.cap_pairs, C, st, e[0], e[1]);
+        Counters h;
+        float x = 0.f;
+        if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess ||
+            hipEventElapsedTime(&x, e[0], e[1]) != hipSuccess ||
+            hipMemcpy(&h, C, sizeof(Counters), hipMemcpyDeviceToHost) != hipSuccess) {
+            r = MCGPU_ERROR_DEVICE;
+            break;
+        }
+        if (h.n_raw != ctx->C.n_raw || h.ov_pairs) *same = false;
+        v.push_back(x);
+    }
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+    if (r != MCGPU_SUCCESS) return r;
+    std::sort(v.begin(), v.end());
+    *ms = v[v.size() / 2];
+    return MCGPU_SUCCESS;
+}
+
+// The autotuner for the last dispatch's workload class: every traversal
+// launch shape of the candidate lists (leaves per block, levels per step,
+// queue mode) is timed alone on the dispatch's own trees (a median of three)
+// and must find the same raw pairs; the fastest wins.  The whole dispatch is
+// then re-run with the default and with the chosen shape (best of two device
+// times each) for the report.  The choice is kept for the class (single pair
+// or batch, bit lengths of the two face counts) and used by its later
+// dispatches.  The CU split between the broad-phase and edge-numbering
+// streams stays at its defaults: the streams of a context keep their masks
+// for its lifetime.
+static mcgpu_result autotune(mcgpu_context ctx, mcgpu_tuning* out)
+{
+    const TravConfig def{};
+    float best = 3.4e38f, def_trav = 0.f;
+    TravConfig bt = def;
+    uint32_t n = 0;
+    mcgpu_result r;
+    for (int ll : kTuneLL)
+        for (int st : kTuneStride)
+            for (const auto& q : kTuneQueue) {
+                TravConfig t;
+                t.ll = ll;
+                t.stride = st;
+                t.queues = q[0];
+                t.chunk = q[1];
+                float ms = 0.f;
+                bool same = true;
+                if ((r = time_traversal(ctx, t, 3, &ms, &same)) != MCGPU_SUCCESS) return r;
+                if (!same) return MCGPU_ERROR_DEVICE; // every shape finds the same pairs
+                ++n;
+                if (t.ll == def.ll && t.stride == def.stride && t.queues == def.queues) def_trav = ms;
+                if (ms < best) {
+                    best = ms;
+                    bt = t;
+                }
+            }
+    auto timed = [&](const TravConfig& t, float* ms) -> mcgpu_result {
+        ctx->trav = t;
+        float a = 0.f, b = 0.f;
+        mcgpu_result e = rerun_dispatch(ctx, &a);
+        if (e == MCGPU_SUCCESS) e = rerun_dispatch(ctx, &b);
+        *ms = std::min(a, b);
+        return e;
+    };
+    float def_disp = 0.f, disp = 0.f;
+    if ((r = timed(def, &def_disp)) != MCGPU_SUCCESS || (r = timed(bt, &disp)) != MCGPU_SUCCESS) return r;
+    TuneEntry e;
+    e.batch = ctx->batch;
+    e.bs = bits_for((uint64_t)ctx->nf[0] + 1);
+    e.bc = bits_for((uint64_t)ctx->nf[1] + 1);
+    e.trav = bt;
+    e.report = tuning_report(bt, split_pct(ctx));
+    e.report.candidates = n;
+    e.report.traversal_ms = best;
+    e.report.default_traversal_ms = def_trav;
+    e.report.dispatch_ms = disp;
+    e.report.default_dispatch_ms = def_disp;
+    bool put = false;
+    for (TuneEntry& x : ctx->tuned)
+        if (x.batch == e.batch && x.bs == e.bs && x.bc == e.bc) {
+            x = e;
+            put = true;
+        }
+    if (!put) ctx->tuned.push_back(e);
+    ctx->used = e.report;
+    if (out) *out = e.report;
+    return MCGPU_SUCCESS;
+}
+
+// after a dispatch that succeeded: tune its class once when the context asks
+static mcgpu_result autotune_first(mcgpu_context ctx)
+{
+    if (!ctx->autotune || ctx->forced || !ctx->own_streams || ctx->used.candidates) return MCGPU_SUCCESS;
+    return autotune(ctx, nullptr);
+}
+
 MCGPU_API mcgpu_result mcgpu_create_context(int device, mcgpu_context* out)
 {
     if (!out) return MCGPU_ERROR_INVALID_VALUE;
@@ -1204,6 +1497,7 @@ MCGPU_API mcgpu_result mcgpu_create_context(int device, mcgpu_context* out)
     // CU masks: the first kSumCUs compute units for the serial sums, the rest
     // for the streams that run beside them
     const int ncu = prop.multiProcessorCount;
+    ctx->ncu = ncu;
     const int nw = (ncu + 31) / 32;
     std::vector<uint32_t> msum(nw, 0u);
     for (int c = 0; c < kSumCUs && c < ncu; ++c) msum[c / 32] |= 1u << (c % 32);
@@ -1241,7 +1535,8 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
                   &ctx->re_in, &ctx->rg_in, &ctx->bpairs, &ctx->bstarts, &ctx->fpair, &ctx->vpair, &ctx->bparams,
                   &ctx->brbox, &ctx->pflags, &ctx->border, &ctx->bidx[0], &ctx->bidx[1], &ctx->rle, &ctx->rlg,
                   &ctx->rstart, &ctx->rend, &ctx->pqs, &ctx->prs, &ctx->pcounts, &ctx->sbsum,
-                  &ctx->sitems, &ctx->slo, &ctx->shi, &ctx->snitem, &ctx->slist, &ctx->sckpt, &ctx->sflags};
+                  &ctx->sitems, &ctx->slo, &ctx->shi, &ctx->snitem, &ctx->slist, &ctx->sckpt, &ctx->sflags,
+                  &ctx->tune_c};
     for (Buf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
@@ -1277,6 +1572,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     ctx->kbits = ctx->gbits + bits_for(ctx->ncorner[0] + ctx->ncorner[1] + 1);
     if (ctx->kbits > 64) return MCGPU_ERROR_UNSUPPORTED;
     if ((r = alloc_mesh_buffers(ctx)) != MCGPU_SUCCESS) return r;
+    if ((r = apply_tuning(ctx)) != MCGPU_SUCCESS) return r;
     if (ctx->cap_pairs < 4 * F) ctx->cap_pairs = std::max<uint64_t>(1 << 14, 4 * F);
     if (ctx->cap_tests < 2 * ctx->cap_pairs) ctx->cap_tests = 2 * ctx->cap_pairs;
     if (ctx->cap_esc < (1u << 14)) ctx->cap_esc = std::max<uint64_t>(1 << 14, ctx->cap_tests / 16);
@@ -1293,7 +1589,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
         enqueue_frontend(ctx);
         bool overflow = false;
         if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
-        if (!overflow) return MCGPU_SUCCESS;
+        if (!overflow) return autotune_first(ctx);
     }
     return MCGPU_ERROR_OUT_OF_MEMORY;
 }
@@ -1319,11 +1615,6 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
     const int pb = bits_for(np);
     if (pb > 28) return MCGPU_ERROR_UNSUPPORTED;
     CK(hipSetDevice(ctx->device));
-    if (ctx->own_streams && !ctx->s2b) {
-        hipDeviceProp_t prop;
-        CK(hipGetDeviceProperties(&prop, ctx->device));
-        CK(create_split_streams(prop.multiProcessorCount, kBatchBroadShare, &ctx->s2b, &ctx->s3b));
-    }
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = true;
@@ -1364,6 +1655,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
     ctx->kbits = ctx->gbits + bits_for(ctx->ncorner[0] + ctx->ncorner[1] + 1);
     if (ctx->kbits > 64) return MCGPU_ERROR_UNSUPPORTED;
     if ((r = alloc_mesh_buffers(ctx)) != MCGPU_SUCCESS) return r;
+    if ((r = apply_tuning(ctx)) != MCGPU_SUCCESS) return r;
     {
         BatchPair* bp;
         uint32_t* u;
@@ -1400,7 +1692,10 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
         enqueue_frontend(ctx);
         bool overflow = false;
         if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
-        if (!overflow) return finish_batch(ctx, info);
+        if (!overflow) {
+            if ((r = finish_batch(ctx, info)) != MCGPU_SUCCESS) return r;
+            return autotune_first(ctx);
+        }
     }
     return MCGPU_ERROR_OUT_OF_MEMORY;
 }
@@ -1461,6 +1756,7 @@ MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_bo
     ctx->broad_only = true;
     ctx->batch = false;
     ctx->sort_bits = 22;
+    ctx->trav = ctx->forced ? ctx->forced_trav : TravConfig{};
     ctx->nf[0] = num_src;
     ctx->nf[1] = num_cut;
     ctx->nv[0] = ctx->nv[1] = 0;
@@ -1793,6 +2089,49 @@ MCGPU_API mcgpu_result mcgpu_set_stage_timing(mcgpu_context ctx, int enable)
     return MCGPU_SUCCESS;
 }
 
+MCGPU_API mcgpu_result mcgpu_autotune(mcgpu_context ctx, mcgpu_tuning* out)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have || ctx->broad_only || ctx->stage_mode != 0 || ctx->res) return MCGPU_ERROR_NO_RESULT;
+    CK(hipSetDevice(ctx->device));
+    return autotune(ctx, out);
+}
+
+MCGPU_API mcgpu_result mcgpu_set_autotune(mcgpu_context ctx, int enable)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    ctx->autotune = enable != 0;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_set_tuning(mcgpu_context ctx, const mcgpu_tuning* t)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!t) {
+        ctx->forced = false;
+        return MCGPU_SUCCESS;
+    }
+    TravConfig c;
+    int ll = 0;
+    while (ll < 8 && (1u << ll) < t->wave_leaves) ++ll;
+    c.ll = ll;
+    c.stride = (int)t->tree_stride;
+    c.queues = (int)t->queues;
+    c.chunk = (int)t->chunk;
+    if ((1u << ll) != t->wave_leaves || !trav_config_valid(c) || t->broad_share_pct != 0)
+        return MCGPU_ERROR_INVALID_VALUE;
+    ctx->forced = true;
+    ctx->forced_trav = c;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_tuning(mcgpu_context ctx, mcgpu_tuning* out)
+{
+    if (!ctx || !out) return MCGPU_ERROR_INVALID_VALUE;
+    *out = ctx->used;
+    return MCGPU_SUCCESS;
+}
+
 MCGPU_API const char* mcgpu_stage_name(int stage)
 {
     return (stage >= 0 && stage < ST_COUNT) ? kStageNames[stage] : "";
@@ -1863,6 +2202,7 @@ mcgpu_result slot_create(int device, mcgpu_context* out)
         return MCGPU_ERROR_DEVICE;
     }
     ctx->s2 = ctx->s3 = ctx->s2b = ctx->s3b = ctx->ssum = ctx->s;
+    (void)hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, device);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventCreate(&ctx->ev[i]);
     *out = ctx;
     return MCGPU_SUCCESS;

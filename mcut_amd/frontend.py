@@ -79,6 +79,17 @@ class RequestResult(C.Structure):
 _lib = None
 
 
+class Tuning(C.Structure):
+    """mcgpu_tuning: the traversal's launch shape and the CU split."""
+    _fields_ = [("wave_leaves", C.c_uint32), ("tree_stride", C.c_uint32), ("queues", C.c_uint32),
+                ("chunk", C.c_uint32), ("broad_share_pct", C.c_uint32), ("candidates", C.c_uint32),
+                ("traversal_ms", C.c_float), ("default_traversal_ms", C.c_float), ("dispatch_ms", C.c_float),
+                ("default_dispatch_ms", C.c_float)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -120,6 +131,10 @@ def lib():
         L.mcgpu_stage_times.argtypes = [vp, vp, C.c_int]
         L.mcgpu_set_stage_timing.argtypes = [vp, C.c_int]
         L.mcgpu_stage_name.argtypes = [C.c_int]
+        L.mcgpu_autotune.argtypes = [vp, C.POINTER(Tuning)]
+        L.mcgpu_set_autotune.argtypes = [vp, C.c_int]
+        L.mcgpu_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.mcgpu_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.mcgpu_stage_name.restype = C.c_char_p
         for fn in ("mcgpu_create_context", "mcgpu_release_context", "mcgpu_dispatch_frontend",
                    "mcgpu_redispatch_perturbed", "mcgpu_get_registry", "mcgpu_get_candidate_pairs", "mcgpu_get_edges",
@@ -364,6 +379,30 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def autotune(self):
+        """Tunes the traversal shape for the last dispatch's workload class
+        (re-running it); returns the report as a dict."""
+        t = Tuning()
+        _check(lib().mcgpu_autotune(self.h, C.byref(t)), "mcgpu_autotune")
+        return t.as_dict()
+
+    def set_autotune(self, on):
+        _check(lib().mcgpu_set_autotune(self.h, 1 if on else 0), "mcgpu_set_autotune")
+
+    def set_tuning(self, wave_leaves=None, tree_stride=3, queues=0, chunk=1):
+        """Forces a traversal shape (wave_leaves=None: back to tuned/default)."""
+        if wave_leaves is None:
+            _check(lib().mcgpu_set_tuning(self.h, None), "mcgpu_set_tuning")
+            return
+        t = Tuning(wave_leaves=wave_leaves, tree_stride=tree_stride, queues=queues, chunk=chunk)
+        _check(lib().mcgpu_set_tuning(self.h, C.byref(t)), "mcgpu_set_tuning")
+
+    def tuning(self):
+        """The shape and split the last dispatch ran with."""
+        t = Tuning()
+        _check(lib().mcgpu_get_tuning(self.h, C.byref(t)), "mcgpu_get_tuning")
+        return t.as_dict()
 
     def set_stage_timing(self, on):
         """Record per-stage events in later dispatches (Result.stage_ms); off by default."""

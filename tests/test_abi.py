@@ -51,6 +51,8 @@ def _c_layout(tmp_path):
         "mcgpu_server_limits": ["max_cut_vertices", "max_cut_faces", "max_cut_corners", "max_slots", "hbm_budget"],
         "mcgpu_request_result": ["arrival_s", "start_s", "done_s", "status", "partial_cut", "num_registry",
                                  "num_candidate_pairs"],
+        "mcgpu_tuning": ["wave_leaves", "tree_stride", "queues", "chunk", "broad_share_pct", "candidates",
+                         "traversal_ms", "default_traversal_ms", "dispatch_ms", "default_dispatch_ms"],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mcgpu.h"', "int main(void) {"]
     for s, fs in fields.items():
@@ -75,7 +77,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     from mcut_amd import frontend
     py = {"mcgpu_mesh": frontend.Mesh, "mcgpu_dispatch_desc": frontend.DispatchDesc,
           "mcgpu_frontend_info": frontend.FrontendInfo, "mcgpu_batch": frontend.Batch,
-          "mcgpu_server_limits": frontend.ServerLimits, "mcgpu_request_result": frontend.RequestResult}
+          "mcgpu_server_limits": frontend.ServerLimits, "mcgpu_request_result": frontend.RequestResult,
+          "mcgpu_tuning": frontend.Tuning}
     fields, res = _c_layout(tmp_path)
     for s, fs in fields.items():
         T = py[s]
@@ -107,6 +110,10 @@ def test_null_arguments_rejected_without_a_device():
     assert L.mcgpu_redispatch_perturbed(None, None, 0, None) == -1
     assert L.mcgpu_get_registry(None, None, None, None, 0) == -1
     assert L.mcgpu_stage_times(None, None, 0) == 0
+    assert L.mcgpu_autotune(None, None) == -1
+    assert L.mcgpu_set_autotune(None, 1) == -1
+    assert L.mcgpu_set_tuning(None, None) == -1
+    assert L.mcgpu_get_tuning(None, None) == -1
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="a GPU is present")
