@@ -179,6 +179,21 @@ def bench_batch(args, torch, frontend, mg):
     B = bctx.dispatch_batch_device(np_, ms, mc, offs)  # the per-pair outputs, read before the next dispatch
     reg_b = np.diff(B.reg_offsets.astype(np.int64))
     status_b = B.status.copy()
+    e_b, g_b, p_b = (x.copy() for x in B.registry())
+    # the same batch replayed from HIP graphs
+    bctx.set_graphs(True)
+    G = bctx.dispatch_batch_device(np_, ms, mc, offs)  # records the program, then replays it
+    gt = []
+    for _ in range(args.batch_steps):
+        t1 = time.perf_counter()
+        G = bctx.dispatch_batch_device(np_, ms, mc, offs)
+        gt.append(time.perf_counter() - t1)
+    ge, gg, gp = G.registry()
+    graph_b = {"batch_ms": min(gt) * 1e3, "device_ms": G.device_ms, **bctx.graph_stats(),
+               "same_as_eager": bool(np.array_equal(G.status, status_b) and np.array_equal(ge, e_b)
+                                     and np.array_equal(gg, g_b) and np.array_equal(gp.view(np.uint64),
+                                                                                    p_b.view(np.uint64)))}
+    progress(f"batched mix: graph replay {graph_b['batch_ms']:.1f} ms")
     counts_b = dict(B.counts)
     device_ms_b = B.device_ms
     bctx.close()
@@ -213,7 +228,7 @@ def bench_batch(args, torch, frontend, mg):
             "batch_ms": batch_s * 1e3, "batch_device_ms": device_ms_b, "plain_loop_ms": plain_s * 1e3,
             "batched_vs_plain": plain_s / batch_s, "batch_pairs_per_s": I / batch_s,
             "plain_pairs_per_s": I / plain_s, "counts": counts_b, "stage_ms": bstages,
-            "autotune": tuning, "generate_s": gen_s}
+            "autotune": tuning, "graph_replay": graph_b, "generate_s": gen_s}
 
 
 def bench_serve(args, frontend, mg):
@@ -421,6 +436,34 @@ def main():
     assert R.status == 0, R.status_name
     device_ms = R.device_ms
 
+    # the same dispatches replayed from HIP graphs (mcgpu_set_graphs): the
+    # shape is recorded once, then every step is a replay of its program
+    gctx = frontend.Context(0)
+    if tuning is not None and not tuning.get("forced"):
+        gctx.set_tuning(tuning["wave_leaves"], tuning["tree_stride"], tuning["queues"], tuning["chunk"])
+    elif args.shape:
+        gctx.set_tuning(*args.shape)
+    gctx.set_graphs(True)
+    for _ in range(max(2, args.warmup)):
+        G = gctx.dispatch_device(ms, mc)
+    torch.cuda.synchronize()
+    glat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        G = gctx.dispatch_device(ms, mc)
+        glat.append((time.perf_counter() - s0) * 1e3)
+    torch.cuda.synchronize()
+    g_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ge, gg, gp = G.registry()
+    re_, rg_, rp_ = R.registry()
+    graph_line = {"ms_per_step": g_ms, "device_ms": G.device_ms, "p50_ms": sorted(glat)[len(glat) // 2],
+                  "value": G.counts["registry"] / (g_ms * 1e-3), **gctx.graph_stats(),
+                  "same_as_eager": bool(G.status == R.status and np.array_equal(ge, re_) and np.array_equal(gg, rg_)
+                                        and np.array_equal(gp.view(np.uint64), rp_.view(np.uint64)))}
+    gctx.close()
+    progress(f"single pair: graph replay {g_ms:.3f} ms per step ({graph_line['graph_launches']} graphs)")
+
     # per-stage device times (HIP events on the library's streams), gathered
     # from further dispatches outside the timed loop: the timed dispatches
     # record no stage events (each costs time)
@@ -470,6 +513,7 @@ def main():
         "counts": counts,
         "stage_ms": stages,
         "autotune": tuning,
+        "graph_replay": graph_line,
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "stage": dom, "kernel_ms": kernel_ms,
                      "stage_event_ms": stages[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

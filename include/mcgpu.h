@@ -375,6 +375,21 @@ MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity);
 MCGPU_API mcgpu_result mcgpu_set_stage_timing(mcgpu_context ctx, int enable);
 MCGPU_API const char* mcgpu_stage_name(int stage);
 
+/* Graph replay of dispatches.  No counterpart in the reference (a CPU
+ * pipeline); for callers that dispatch the same shape repeatedly (the same
+ * sizes and input buffers, new coordinates or connectivity behind them).
+ * enable != 0: mcgpu_dispatch_frontend and mcgpu_dispatch_batch record their
+ * launches once per dispatch shape as HIP graphs, one per stream segment
+ * between the cross-stream event records and waits, and replay them (each on
+ * the stream, and CU mask, it was captured from) while the shape stays the
+ * same; a different shape (sizes, capacities, flags, traversal shape, buffer
+ * addresses) records anew.  Dispatches with stage timing on run eagerly.
+ * MCGPU_ERROR_UNSUPPORTED on a serving slot's context. */
+MCGPU_API mcgpu_result mcgpu_set_graphs(mcgpu_context ctx, int enable);
+/* programs recorded and replayed so far, and the graphs of the current one */
+MCGPU_API mcgpu_result mcgpu_graph_stats(mcgpu_context ctx, uint64_t* recordings, uint64_t* replays,
+                                         uint32_t* graph_launches);
+
 /* Launch shape of the broad-phase traversal of a dispatch.  No counterpart
  * in the reference, whose broad phase is a recursive CPU walk
  * (bvh.cpp:638-783): these are the knobs of the GPU traversal that the

@@ -91,6 +91,14 @@ constexpr int kTuneLL[] = {3, 4, 5};
 constexpr int kTuneStride[] = {2, 3, 4};
 constexpr int kTuneQueue[][2] = {{0, 1}, {8, 1}, {8, 4}}; // (queues, chunk)
 
+// one step of a dispatch's replay program (mcgpu_set_graphs)
+struct GraphOp {
+    enum Kind { LAUNCH, RECORD, WAIT } kind;
+    hipStream_t st;
+    int ev;              // RECORD / WAIT: the context's event slot
+    hipGraphExec_t exec; // LAUNCH
+};
+
 // one autotuned workload class: single pairs or batches, by the bit lengths of
 // the two face counts
 struct TuneEntry {
@@ -223,6 +231,15 @@ struct mcgpu_context_t {
     TravConfig forced_trav;
     mcgpu_tuning used{};               // what the last dispatch ran with
     Buf tune_c;                        // counters of the tuner's traversal runs
+
+    // graph replay (mcgpu_set_graphs): the recorded program and the
+    // dispatch shape it was recorded for
+    bool graphs = false;
+    bool recording = false;
+    hipError_t rec_error = hipSuccess;
+    std::vector<GraphOp> prog;
+    std::vector<uint64_t> prog_key;
+    uint64_t replays = 0, recordings = 0;
 };
 
 #define CK(x)                                                                                                          \
@@ -509,6 +526,55 @@ bool topology_packed(mcgpu_context ctx)
 // the part of the pipeline that runs again for a perturbed cut mesh (whose
 // candidate pairs and faces are those of the unperturbed dispatch):
 // candidate-face planes, tests, narrow phase, registry, status
+// ---------------------------------------------------------------------------
+// Graph replay of a dispatch (mcgpu_set_graphs).  The enqueue functions order
+// their four streams only through ev_record / ev_wait.  When a context
+// records, every stream is under stream capture and each of those calls cuts
+// its stream's capture: the kernels captured since the stream's previous cut
+// become one instantiated graph.  The dispatch is then a program of graph
+// launches, event records and event waits in the order the eager enqueue
+// issues them, each graph launched on the stream it was captured from, so its
+// kernels keep that stream's CU mask; replaying the program is the dispatch.
+// ---------------------------------------------------------------------------
+
+void graph_cut(mcgpu_context ctx, hipStream_t st)
+{
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (e == hipSuccess && g) {
+        size_t n = 0;
+        e = hipGraphGetNodes(g, nullptr, &n);
+        if (e == hipSuccess && n) {
+            hipGraphExec_t ex = nullptr;
+            e = hipGraphInstantiateWithFlags(&ex, g, 0);
+            if (e == hipSuccess) ctx->prog.push_back(GraphOp{GraphOp::LAUNCH, st, -1, ex});
+        }
+        (void)hipGraphDestroy(g);
+    }
+    if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess && ctx->rec_error == hipSuccess) ctx->rec_error = e;
+}
+
+void ev_record(mcgpu_context ctx, int e, hipStream_t st)
+{
+    if (!ctx->recording) {
+        (void)hipEventRecord(ctx->ev[e], st);
+        return;
+    }
+    graph_cut(ctx, st);
+    ctx->prog.push_back(GraphOp{GraphOp::RECORD, st, e, nullptr});
+}
+
+void ev_wait(mcgpu_context ctx, hipStream_t st, int e)
+{
+    if (!ctx->recording) {
+        (void)hipStreamWaitEvent(st, ctx->ev[e], 0);
+        return;
+    }
+    graph_cut(ctx, st);
+    ctx->prog.push_back(GraphOp{GraphOp::WAIT, st, e, nullptr});
+}
+
 void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
 {
     hipStream_t s = ctx->s;
@@ -522,7 +588,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
             S, P_<uint32_t>(ctx->big_list), P_<unsigned long long>(ctx->big_off), C, ctx->cap_big, ctx->cap_trip,
             P_<Plane>(ctx->planes), P_<Trip>(ctx->big_scratch), C);
     stage_event(ctx, EV_A_PLANES, s);
-    if (join_topology) (void)hipStreamWaitEvent(s, ctx->ev[EV_C_TOPO], 0);
+    if (join_topology) ev_wait(ctx, s, EV_C_TOPO);
     stage_event(ctx, EV_A_JOIN2, s);
     k_tests<<<grid_for(6 * ctx->cap_pairs, kTestTile, 2048), kStageBlock, 0, s>>>(S, P_<unsigned long long>(ctx->pairs), ctx->cap_pairs,
                                                         P_<unsigned long long>(ctx->phash), P_<uint32_t>(ctx->te),
@@ -581,7 +647,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
             P_<BatchPair>(ctx->bpairs), C, P_<uint32_t>(ctx->rle), P_<uint32_t>(ctx->rlg), P_<uint32_t>(ctx->rstart),
             P_<uint32_t>(ctx->rend));
     }
-    (void)hipEventRecord(ctx->ev[EV_A_END], s);
+    ev_record(ctx, EV_A_END, s);
 }
 
 // Per-dispatch reset, raw bounding boxes, face offsets, input validation and
@@ -596,7 +662,7 @@ void enqueue_validate(mcgpu_context ctx)
     hipStream_t s = ctx->s;
     Counters* C = &P_<DevState>(ctx->state)->C;
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
-    (void)hipEventRecord(ctx->ev[EV_A0], s);
+    ev_record(ctx, EV_A0, s);
     if (!ctx->dsz[0] && !ctx->dsz[1]) { // two triangle meshes: three launches (k_reset_counters)
         const int m0 = ctx->res ? 1 : 0;
         k_reset_counters<<<1, kBlock, 0, s>>>(C, m0);
@@ -646,7 +712,7 @@ void enqueue_validate_batch(mcgpu_context ctx)
     const BatchPair* bp = P_<BatchPair>(ctx->bpairs);
     auto* brbox = (unsigned long long(*)[2][6])ctx->brbox.p;
     Params* Pb = P_<Params>(ctx->bparams);
-    (void)hipEventRecord(ctx->ev[EV_A0], s);
+    ev_record(ctx, EV_A0, s);
     fill_u32(C, 0u, sizeof(Counters) / 4, s);
     fill_u32(ctx->pflags.p, 0u, np, s);
     k_segment_ids<<<grid_for(ctx->nv[0]), kBlock, 0, s>>>(st, np, ctx->nv[0], vp);
@@ -891,7 +957,7 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool g
     ctx->mface_alt = enqueue_trees_traverse(B, st, C, ghist_zeroed, t ? ctx->ev[EV_B_SORT] : nullptr,
                                             t ? ctx->ev[EV_B_REFIT] : nullptr, t ? ctx->ev[EV_TRAV_K0] : nullptr,
                                             t ? ctx->ev[EV_TRAV_K1] : nullptr);
-    (void)hipEventRecord(ctx->ev[EV_B_TRAV], st);
+    ev_record(ctx, EV_B_TRAV, st);
 }
 
 // Polygon-soup edge numbering on stream s3, and the candidate-face flags the
@@ -905,7 +971,7 @@ void enqueue_topology(mcgpu_context ctx, hipStream_t s3)
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
     const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
     fill_u32(ctx->cand.p, 0u, F, s3);
-    (void)hipEventRecord(ctx->ev[EV_C_FILL], s3);
+    ev_record(ctx, EV_C_FILL, s3);
     const uint32_t res_ne = ctx->res ? ctx->res->ne : 0u;
     if (ctx->res) { // the resident source's edges occupy [0, ne) of the edge table
         (void)hipMemcpyAsync(ctx->ev0.p, ctx->res->ev0, 4ull * res_ne, hipMemcpyDeviceToDevice, s3);
@@ -969,7 +1035,7 @@ void enqueue_topology(mcgpu_context ctx, hipStream_t s3)
         k_batch_edge_bases<<<grid_for(ctx->npairs), kBlock, 0, s3>>>(
             P_<BatchPair>(ctx->bpairs), ctx->npairs, P_<uint32_t>(ctx->off[0]), P_<uint32_t>(ctx->off[1]), ctx->nf[0],
             ctx->nf[1], P_<uint32_t>(ctx->eid[0]), P_<uint32_t>(ctx->eid[1]), C);
-    (void)hipEventRecord(ctx->ev[EV_C_TOPO], s3);
+    ev_record(ctx, EV_C_TOPO, s3);
 
 }
 
@@ -982,7 +1048,7 @@ void enqueue_frontend(mcgpu_context ctx)
     Params* Pd = &P_<DevState>(ctx->state)->P;
 
     // ---- stream A, enqueued first: the serial sums start at once ----
-    (void)hipEventRecord(ctx->ev[EV_FORK], s);
+    ev_record(ctx, EV_FORK, s);
     // a single dispatch on its own four streams clears the whole pair hash
     // early; batches (huge tables) and serving slots (one stream) size it
     // after the traversal
@@ -991,7 +1057,7 @@ void enqueue_frontend(mcgpu_context ctx)
     Params* Pb = P_<Params>(ctx->bparams);
     const auto* brbox = (const unsigned long long(*)[2][6])ctx->brbox.p;
     if (ctx->batch) {
-        (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_FORK], 0);
+        ev_wait(ctx, ctx->ssum, EV_FORK);
         stage_event(ctx, EV_S0, ctx->ssum);
         const uint32_t nch = 6 * ctx->npairs;
         k_batch_sums<<<std::min<uint32_t>(nch, kBatchSumWaves), 64, 0, ctx->ssum>>>(
@@ -1008,20 +1074,20 @@ void enqueue_frontend(mcgpu_context ctx)
         const dim3 grid(SB.nblk, 6);
         k_sum_blocks<<<grid, kSumThreads, 0, s>>>(SB);
         k_sum_items<<<grid, kSumThreads, 0, s>>>(SB);
-        (void)hipEventRecord(ctx->ev[EV_SUM_ITEMS], s);
-        (void)hipStreamWaitEvent(ctx->ssum, ctx->ev[EV_SUM_ITEMS], 0);
+        ev_record(ctx, EV_SUM_ITEMS, s);
+        ev_wait(ctx, ctx->ssum, EV_SUM_ITEMS);
 
         k_sum_chain<<<6, 64, 4 * SB.nblk, ctx->ssum>>>(SB, Pd);
-        (void)hipEventRecord(ctx->ev[EV_SUM_CHAIN], ctx->ssum);
-        (void)hipStreamWaitEvent(s, ctx->ev[EV_SUM_CHAIN], 0);
+        ev_record(ctx, EV_SUM_CHAIN, ctx->ssum);
+        ev_wait(ctx, s, EV_SUM_CHAIN);
         k_sum_verify<<<dim3(4 * SB.nblk * (kSumTile / 64) / 256, 6), 256, 0, s>>>(SB);
         k_mesh_sums<<<6, 64, 0, s>>>(ctx->res ? nullptr : ctx->dv[0], ctx->nv[0], ctx->dv[1], ctx->nv[1], &C->zero, Pd,
                                      SB.fail);
-        (void)hipEventRecord(ctx->ev[EV_A_SUMS], s);
+        ev_record(ctx, EV_A_SUMS, s);
     }
     if (ctx->batch) {
-        (void)hipEventRecord(ctx->ev[EV_A_SUMS], ctx->ssum);
-        (void)hipStreamWaitEvent(s, ctx->ev[EV_A_SUMS], 0);
+        ev_record(ctx, EV_A_SUMS, ctx->ssum);
+        ev_wait(ctx, s, EV_A_SUMS);
     }
     if (ctx->batch) {
         k_batch_shift_params<<<grid_for(ctx->npairs), kBlock, 0, s>>>(Pb, brbox, P_<BatchPair>(ctx->bpairs), ctx->npairs,
@@ -1040,7 +1106,7 @@ void enqueue_frontend(mcgpu_context ctx)
     stage_event(ctx, EV_A_SHIFT, s);
 
     // ---- stream B: raw-space trees and broad phase ----
-    (void)hipStreamWaitEvent(s2, ctx->ev[EV_FORK], 0);
+    ev_wait(ctx, s2, EV_FORK);
     stage_event(ctx, EV_B0, s2);
     float* rawbox = P_<float>(ctx->rawbox);
     if (ctx->batch) {
@@ -1067,12 +1133,12 @@ void enqueue_frontend(mcgpu_context ctx)
     enqueue_bvh_traverse(ctx, s2, C, !ctx->batch); // k_morton_raw cleared the sort's group totals
 
     // ---- stream C: polygon-soup edge numbering, and the tables the filter fills ----
-    (void)hipStreamWaitEvent(s3, ctx->ev[EV_FORK], 0);
+    ev_wait(ctx, s3, EV_FORK);
     enqueue_topology(ctx, s3);
 
     // ---- stream A: join the broad phase, exact filter, narrow phase ----
-    (void)hipStreamWaitEvent(s, ctx->ev[EV_B_TRAV], 0);
-    (void)hipStreamWaitEvent(s, ctx->ev[EV_C_FILL], 0);
+    ev_wait(ctx, s, EV_B_TRAV);
+    ev_wait(ctx, s, EV_C_FILL);
     stage_event(ctx, EV_A_JOIN, s);
     const uint64_t table = next_pow2(2 * ctx->cap_pairs);
     if (!early_clear)
@@ -1208,6 +1274,107 @@ void compute_stage_times(mcgpu_context ctx)
     ctx->stage_ms[ST_REGISTRY] = el(EV_A_NARROW, EV_A_END);
     ctx->stage_ms[ST_TRAV_KERNEL] = full ? el(EV_TRAV_K0, EV_TRAV_K1) : 0.f;
     ctx->stages_ready = true;
+}
+
+// every device buffer a context owns
+std::vector<Buf*> context_bufs(mcgpu_context ctx)
+{
+    return {&ctx->in_v[0], &ctx->in_v[1], &ctx->in_idx[0], &ctx->in_idx[1], &ctx->in_sz[0], &ctx->in_sz[1],
+                  &ctx->off[0], &ctx->off[1], &ctx->state, &ctx->ps_v, &ctx->box, &ctx->hkeys[0],
+                  &ctx->hkeys[1], &ctx->hfirst[0], &ctx->hfirst[1], &ctx->cface[0], &ctx->cface[1], &ctx->eflag[0],
+                  &ctx->eflag[1], &ctx->eid[0], &ctx->eid[1], &ctx->cedge[0], &ctx->cedge[1], &ctx->etotal, &ctx->ev0,
+                  &ctx->ev1, &ctx->ef0, &ctx->ef1, &ctx->mcode, &ctx->mface, &ctx->mcode2, &ctx->mface2, &ctx->nodes[0], &ctx->nodes[1],
+                  &ctx->rawbox, &ctx->raw_pairs, &ctx->pairs, &ctx->phash, &ctx->cand, &ctx->cand_list,
+                  &ctx->planes, &ctx->big_list, &ctx->big_off, &ctx->big_scratch, &ctx->te, &ctx->tg, &ctx->tqs, &ctx->trs,
+                  &ctx->ttype, &ctx->tpip, &ctx->esc, &ctx->rkey, &ctx->rval, &ctx->rkey2, &ctx->rval2, &ctx->rp,
+                  &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->scan_tmp_c, &ctx->hist,
+                  &ctx->re_in, &ctx->rg_in, &ctx->bpairs, &ctx->bstarts, &ctx->fpair, &ctx->vpair, &ctx->bparams,
+                  &ctx->brbox, &ctx->pflags, &ctx->border, &ctx->bidx[0], &ctx->bidx[1], &ctx->rle, &ctx->rlg,
+                  &ctx->rstart, &ctx->rend, &ctx->pqs, &ctx->prs, &ctx->pcounts, &ctx->sbsum,
+                  &ctx->sitems, &ctx->slo, &ctx->shi, &ctx->snitem, &ctx->slist, &ctx->sckpt, &ctx->sflags,
+                  &ctx->tune_c};
+}
+
+// The shape of the dispatch about to be enqueued: everything its kernels'
+// launch parameters depend on (sizes, capacities, flags, the traversal shape,
+// the streams and the address of every buffer).  A recorded program replays
+// only for the same shape; the data behind the addresses may differ.
+std::vector<uint64_t> graph_key(mcgpu_context ctx)
+{
+    std::vector<uint64_t> k;
+    auto put = [&k](uint64_t x) { k.push_back(x); };
+    auto ptr = [&k](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
+    put(ctx->batch), put(ctx->npairs), put(ctx->batch_mb), put(ctx->sort_bits), put(ctx->gbits), put(ctx->kbits);
+    for (int m = 0; m < 2; ++m) {
+        put(ctx->nv[m]), put(ctx->nf[m]), put(ctx->ncorner[m]), put(ctx->depth[m]);
+        ptr(ctx->dv[m]), ptr(ctx->didx[m]), ptr(ctx->dsz[m]), ptr(ctx->bin_idx[m]);
+    }
+    put(ctx->cap_pairs), put(ctx->cap_tests), put(ctx->cap_esc), put(ctx->cap_reg), put(ctx->cap_big),
+        put(ctx->cap_trip), put(ctx->sum_nblk);
+    uint64_t gp;
+    std::memcpy(&gp, &ctx->gp_constant, 8);
+    put(ctx->flags), put(gp), put(ctx->enforcement_count), put(ctx->keep_tests);
+    put(ctx->trav.ll), put(ctx->trav.stride), put(ctx->trav.queues), put(ctx->trav.chunk);
+    ptr(ctx->s), ptr(ctx->s2), ptr(ctx->s3), ptr(ctx->s2b), ptr(ctx->s3b), ptr(ctx->ssum);
+    for (Buf* b : context_bufs(ctx)) ptr(b->p);
+    return k;
+}
+
+void graph_drop(mcgpu_context ctx)
+{
+    for (const GraphOp& op : ctx->prog)
+        if (op.exec) (void)hipGraphExecDestroy(op.exec);
+    ctx->prog.clear();
+    ctx->prog_key.clear();
+}
+
+// Enqueues the validation and front end of a single pair or a batch: eagerly,
+// or (graphs on) by replaying the context's program, recorded first when the
+// dispatch's shape differs from the one it was recorded for.
+mcgpu_result enqueue_dispatch(mcgpu_context ctx)
+{
+    if (!ctx->graphs || ctx->stage_timing || !ctx->own_streams) {
+        if (ctx->batch) enqueue_validate_batch(ctx);
+        else enqueue_validate(ctx);
+        enqueue_frontend(ctx);
+        return MCGPU_SUCCESS;
+    }
+    std::vector<uint64_t> key = graph_key(ctx);
+    if (key != ctx->prog_key || ctx->prog.empty()) {
+        graph_drop(ctx);
+        hipStream_t sts[4] = {ctx->s, ctx->batch ? ctx->s2b : ctx->s2, ctx->batch ? ctx->s3b : ctx->s3, ctx->ssum};
+        ctx->rec_error = hipSuccess;
+        for (hipStream_t st : sts) CK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        ctx->recording = true;
+        if (ctx->batch) enqueue_validate_batch(ctx);
+        else enqueue_validate(ctx);
+        enqueue_frontend(ctx);
+        for (hipStream_t st : sts) { // the streams' last segments, then the capture ends
+            graph_cut(ctx, st);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g); // empty: every stream ends on an event record
+            if (e != hipSuccess && ctx->rec_error == hipSuccess) ctx->rec_error = e;
+        }
+        ctx->recording = false;
+        const hipError_t le = hipGetLastError();
+        if (ctx->rec_error != hipSuccess || le != hipSuccess) {
+            ctx->last_error = ctx->rec_error != hipSuccess ? ctx->rec_error : le;
+            graph_drop(ctx);
+            return MCGPU_ERROR_DEVICE;
+        }
+        ctx->prog_key.swap(key);
+        ++ctx->recordings;
+    }
+    for (const GraphOp& op : ctx->prog) {
+        switch (op.kind) {
+        case GraphOp::LAUNCH: CK(hipGraphLaunch(op.exec, op.st)); break;
+        case GraphOp::RECORD: CK(hipEventRecord(ctx->ev[op.ev], op.st)); break;
+        case GraphOp::WAIT: CK(hipStreamWaitEvent(op.st, ctx->ev[op.ev], 0)); break;
+        }
+    }
+    ++ctx->replays;
+    return MCGPU_SUCCESS;
 }
 
 mcgpu_result upload_mesh(mcgpu_context ctx, int m, const mcgpu_mesh* mesh, bool on_device)
@@ -1346,12 +1513,11 @@ static mcgpu_result apply_tuning(mcgpu_context ctx)
 // sized): its device time in *ms.
 static mcgpu_result rerun_dispatch(mcgpu_context ctx, float* ms)
 {
-    if (ctx->batch) enqueue_validate_batch(ctx);
-    else enqueue_validate(ctx);
     ctx->stage_mode = 0;
-    enqueue_frontend(ctx);
+    mcgpu_result r = enqueue_dispatch(ctx);
+    if (r != MCGPU_SUCCESS) return r;
     bool overflow = false;
-    mcgpu_result r = finish(ctx, nullptr, &overflow);
+    r = finish(ctx, nullptr, &overflow);
     if (r != MCGPU_SUCCESS) return r;
     if (overflow) return MCGPU_ERROR_DEVICE; // the same input cannot outgrow the lists it sized
     if (ctx->batch && (r = finish_batch(ctx, nullptr)) != MCGPU_SUCCESS) return r;
@@ -1523,21 +1689,9 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
         for (hipStream_t st : {ctx->s2, ctx->s3, ctx->s2b, ctx->s3b, ctx->ssum})
             if (st) (void)hipStreamSynchronize(st);
     }
-    Buf* all[] = {&ctx->in_v[0], &ctx->in_v[1], &ctx->in_idx[0], &ctx->in_idx[1], &ctx->in_sz[0], &ctx->in_sz[1],
-                  &ctx->off[0], &ctx->off[1], &ctx->state, &ctx->ps_v, &ctx->box, &ctx->hkeys[0],
-                  &ctx->hkeys[1], &ctx->hfirst[0], &ctx->hfirst[1], &ctx->cface[0], &ctx->cface[1], &ctx->eflag[0],
-                  &ctx->eflag[1], &ctx->eid[0], &ctx->eid[1], &ctx->cedge[0], &ctx->cedge[1], &ctx->etotal, &ctx->ev0,
-                  &ctx->ev1, &ctx->ef0, &ctx->ef1, &ctx->mcode, &ctx->mface, &ctx->mcode2, &ctx->mface2, &ctx->nodes[0], &ctx->nodes[1],
-                  &ctx->rawbox, &ctx->raw_pairs, &ctx->pairs, &ctx->phash, &ctx->cand, &ctx->cand_list,
-                  &ctx->planes, &ctx->big_list, &ctx->big_off, &ctx->big_scratch, &ctx->te, &ctx->tg, &ctx->tqs, &ctx->trs,
-                  &ctx->ttype, &ctx->tpip, &ctx->esc, &ctx->rkey, &ctx->rval, &ctx->rkey2, &ctx->rval2, &ctx->rp,
-                  &ctx->re, &ctx->rg, &ctx->rpts, &ctx->rcount, &ctx->shkeys, &ctx->shcnt, &ctx->scan_tmp, &ctx->scan_tmp_c, &ctx->hist,
-                  &ctx->re_in, &ctx->rg_in, &ctx->bpairs, &ctx->bstarts, &ctx->fpair, &ctx->vpair, &ctx->bparams,
-                  &ctx->brbox, &ctx->pflags, &ctx->border, &ctx->bidx[0], &ctx->bidx[1], &ctx->rle, &ctx->rlg,
-                  &ctx->rstart, &ctx->rend, &ctx->pqs, &ctx->prs, &ctx->pcounts, &ctx->sbsum,
-                  &ctx->sitems, &ctx->slo, &ctx->shi, &ctx->snitem, &ctx->slist, &ctx->sckpt, &ctx->sflags,
-                  &ctx->tune_c};
-    for (Buf* b : all)
+    for (const GraphOp& op : ctx->prog)
+        if (op.exec) (void)hipGraphExecDestroy(op.exec);
+    for (Buf* b : context_bufs(ctx))
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->own_streams) {
@@ -1585,8 +1739,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     for (int attempt = 0; attempt < 6; ++attempt) {
         if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
         ctx->stage_mode = 0;
-        enqueue_validate(ctx);
-        enqueue_frontend(ctx);
+        if ((r = enqueue_dispatch(ctx)) != MCGPU_SUCCESS) return r;
         bool overflow = false;
         if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
         if (!overflow) return autotune_first(ctx);
@@ -1688,8 +1841,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
     for (int attempt = 0; attempt < 6; ++attempt) {
         if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
         ctx->stage_mode = 0;
-        enqueue_validate_batch(ctx);
-        enqueue_frontend(ctx);
+        if ((r = enqueue_dispatch(ctx)) != MCGPU_SUCCESS) return r;
         bool overflow = false;
         if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
         if (!overflow) {
@@ -2095,6 +2247,30 @@ MCGPU_API mcgpu_result mcgpu_autotune(mcgpu_context ctx, mcgpu_tuning* out)
     if (!ctx->have || ctx->broad_only || ctx->stage_mode != 0 || ctx->res) return MCGPU_ERROR_NO_RESULT;
     CK(hipSetDevice(ctx->device));
     return autotune(ctx, out);
+}
+
+MCGPU_API mcgpu_result mcgpu_set_graphs(mcgpu_context ctx, int enable)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->own_streams) return MCGPU_ERROR_UNSUPPORTED;
+    ctx->graphs = enable != 0;
+    if (!ctx->graphs) {
+        CK(hipSetDevice(ctx->device));
+        graph_drop(ctx);
+    }
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_graph_stats(mcgpu_context ctx, uint64_t* recordings, uint64_t* replays,
+                                         uint32_t* graph_launches)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    uint32_t n = 0;
+    for (const GraphOp& op : ctx->prog) n += op.kind == GraphOp::LAUNCH;
+    if (recordings) *recordings = ctx->recordings;
+    if (replays) *replays = ctx->replays;
+    if (graph_launches) *graph_launches = n;
+    return MCGPU_SUCCESS;
 }
 
 MCGPU_API mcgpu_result mcgpu_set_autotune(mcgpu_context ctx, int enable)

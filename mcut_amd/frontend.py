@@ -132,6 +132,8 @@ def lib():
         L.mcgpu_set_stage_timing.argtypes = [vp, C.c_int]
         L.mcgpu_stage_name.argtypes = [C.c_int]
         L.mcgpu_autotune.argtypes = [vp, C.POINTER(Tuning)]
+        L.mcgpu_set_graphs.argtypes = [vp, C.c_int]
+        L.mcgpu_graph_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(C.c_uint32)]
         L.mcgpu_set_autotune.argtypes = [vp, C.c_int]
         L.mcgpu_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.mcgpu_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
@@ -386,6 +388,15 @@ class Context:
         t = Tuning()
         _check(lib().mcgpu_autotune(self.h, C.byref(t)), "mcgpu_autotune")
         return t.as_dict()
+
+    def set_graphs(self, on):
+        """Record each dispatch shape once as HIP graphs and replay them."""
+        _check(lib().mcgpu_set_graphs(self.h, 1 if on else 0), "mcgpu_set_graphs")
+
+    def graph_stats(self):
+        r, p, n = C.c_uint64(0), C.c_uint64(0), C.c_uint32(0)
+        _check(lib().mcgpu_graph_stats(self.h, C.byref(r), C.byref(p), C.byref(n)), "mcgpu_graph_stats")
+        return dict(recordings=r.value, replays=p.value, graph_launches=n.value)
 
     def set_autotune(self, on):
         _check(lib().mcgpu_set_autotune(self.h, 1 if on else 0), "mcgpu_set_autotune")

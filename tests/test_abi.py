@@ -114,6 +114,8 @@ def test_null_arguments_rejected_without_a_device():
     assert L.mcgpu_set_autotune(None, 1) == -1
     assert L.mcgpu_set_tuning(None, None) == -1
     assert L.mcgpu_get_tuning(None, None) == -1
+    assert L.mcgpu_set_graphs(None, 1) == -1
+    assert L.mcgpu_graph_stats(None, None, None, None) == -1
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="a GPU is present")
