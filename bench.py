@@ -155,6 +155,8 @@ def bench_batch(args, torch, frontend, mg):
     offs = [P["src_vo"], P["src_fo"], P["cut_vo"], P["cut_fo"]]
     np_ = P["num_pairs"]
     bctx = frontend.Context(0)
+    if args.batch_split:
+        bctx.set_batch_split(args.batch_split)
     B = bctx.dispatch_batch_device(np_, ms, mc, offs)  # warm-up: sizes the buffers
     torch.cuda.synchronize()
     progress(f"batched mix: warm-up batch done ({B.counts})")
@@ -198,6 +200,14 @@ def bench_batch(args, torch, frontend, mg):
     device_ms_b = B.device_ms
     bctx.close()
 
+    I = int(reg_b.sum())
+    out = {"workload": f"{np_} pairs: geodesic sphere (logU 200-200k tris, +-2% radial noise) vs rotated "
+                       f"tessellated box (logU 12-12k tris), seed 5",
+           "faces": nfs + nfc, "registry": I, "pairs_not_success": int(np.count_nonzero(status_b)),
+           "batch_ms": batch_s * 1e3, "batch_device_ms": device_ms_b, "batch_pairs_per_s": I / batch_s,
+           "counts": counts_b, "stage_ms": bstages, "autotune": tuning, "graph_replay": graph_b, "generate_s": gen_s}
+    if args.no_plain_loop:  # profiling runs: the batch alone
+        return out
     pctx = frontend.Context(0)
 
     def plain(p):
@@ -221,14 +231,8 @@ def bench_batch(args, torch, frontend, mg):
     plain_s = time.perf_counter() - t1
     pctx.close()
     assert np.array_equal(reg_b, reg_p) and np.array_equal(status_b, status_p), "batch and plain results differ"
-    I = int(reg_b.sum())
-    return {"workload": f"{np_} pairs: geodesic sphere (logU 200-200k tris, +-2% radial noise) vs rotated "
-                        f"tessellated box (logU 12-12k tris), seed 5",
-            "faces": nfs + nfc, "registry": I, "pairs_not_success": int(np.count_nonzero(status_b)),
-            "batch_ms": batch_s * 1e3, "batch_device_ms": device_ms_b, "plain_loop_ms": plain_s * 1e3,
-            "batched_vs_plain": plain_s / batch_s, "batch_pairs_per_s": I / batch_s,
-            "plain_pairs_per_s": I / plain_s, "counts": counts_b, "stage_ms": bstages,
-            "autotune": tuning, "graph_replay": graph_b, "generate_s": gen_s}
+    out.update({"plain_loop_ms": plain_s * 1e3, "batched_vs_plain": plain_s / batch_s, "plain_pairs_per_s": I / plain_s})
+    return out
 
 
 def bench_serve(args, frontend, mg):
@@ -352,43 +356,8 @@ def bench_mcdispatch(args, frontend, mg):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--level", type=int, default=7)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch-pairs", type=int, default=4096)
-    ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-precision", action="store_true")
-    ap.add_argument("--no-mcdispatch", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="time the default traversal shape instead of the autotuned one")
-    ap.add_argument("--shape", type=int, nargs=4, metavar=("LEAVES", "STRIDE", "QUEUES", "CHUNK"),
-                    help="time this traversal shape (source leaves per block, cut levels per step, 0 or 8 work "
-                         "queues, blocks per pop) instead of tuning one")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a 256 MiB device-to-device copy (three times): the known-size kernel that "
-                         "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--mcd-levels", type=int, nargs="+", default=[5, 6])
-    ap.add_argument("--mcd-repeats", type=int, default=3)
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[10, 30, 100, 300, 1000, 3000])
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
-    ap.add_argument("--serve-max", type=int, default=1000)
-    ap.add_argument("--serve-pool", type=int, default=256)
-    ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-budget-gb", type=int, default=16)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X (use --gpus 1)", file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-    from mcut_amd import frontend, meshgen as mg
-
+def bench_single(args, torch, frontend, mg):
+    """The headline: K front ends of the icosphere-7 pair, inputs resident in HBM."""
     src, cut = mg.config2(level=args.level)
     dev = torch.device("cuda:0")
     keep = []
@@ -545,6 +514,57 @@ def main():
             line["speedup_vs_cpu_single_thread"] = line["value"] / line["cpu_baseline"]["single_thread"]["value"]
     ctx.close()
     del keep
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--level", type=int, default=7)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-pairs", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--batch-split", type=int, default=0,
+                    help="percent of the CUs beside the sums' for the batch's broad-phase stream (0: default)")
+    ap.add_argument("--no-plain-loop", action="store_true",
+                    help="skip the per-pair plain dispatches of the batched mix (profiling runs)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-pair measurement (profiling runs of the other workloads)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-mcdispatch", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the default traversal shape instead of the autotuned one")
+    ap.add_argument("--shape", type=int, nargs=4, metavar=("LEAVES", "STRIDE", "QUEUES", "CHUNK"),
+                    help="time this traversal shape (source leaves per block, cut levels per step, 0 or 8 work "
+                         "queues, blocks per pop) instead of tuning one")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (three times): the known-size kernel that "
+                         "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--mcd-levels", type=int, nargs="+", default=[5, 6])
+    ap.add_argument("--mcd-repeats", type=int, default=3)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[10, 30, 100, 300, 1000, 3000])
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-max", type=int, default=1000)
+    ap.add_argument("--serve-pool", type=int, default=256)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mcut_amd import frontend, meshgen as mg
+
+    if args.no_single:  # profiling runs of the other workloads: no headline
+        line = {"metric": "intersecting edge-face pairs/sec (front end, one mcDispatch pair)", "value": None,
+                "unit": "pairs/s", "n_gpus": 1, "skipped": "single pair (--no-single)"}
+    else:
+        line = bench_single(args, torch, frontend, mg)
     if not args.no_batch:
         line["batched"] = bench_batch(args, torch, frontend, mg)
     if not args.no_serve:

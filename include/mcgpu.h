@@ -416,6 +416,12 @@ typedef struct mcgpu_tuning {
  * and twice with the chosen shape for the report.  Later dispatches of the
  * class use the choice.  The last dispatch's outputs stay valid. */
 MCGPU_API mcgpu_result mcgpu_autotune(mcgpu_context ctx, mcgpu_tuning* out);
+/* The CU split of a context's batch streams: broad_share_pct percent (10-90)
+ * of the CUs outside the sums' reservation for the broad phase, the rest for
+ * the edge numbering (default 58).  The streams are made with their masks by
+ * the context's first batch, so this is accepted only before it
+ * (MCGPU_ERROR_INVALID_VALUE after). */
+MCGPU_API mcgpu_result mcgpu_set_batch_split(mcgpu_context ctx, uint32_t broad_share_pct);
 /* enable != 0: the first dispatch of each untuned class tunes it (above) */
 MCGPU_API mcgpu_result mcgpu_set_autotune(mcgpu_context ctx, int enable);
 /* Forces a traversal shape for every later dispatch (t = NULL: back to the

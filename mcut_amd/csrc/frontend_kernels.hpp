@@ -938,6 +938,109 @@ __global__ void k_edge_assign_packed(TopoMesh M0, TopoMesh M1, uint64_t start, c
     }
 }
 
+// Batches: the packed edge table with one region per (pair, mesh).  Every
+// corner's edge lies within its pair, so the words of pair p's edges live in a
+// region of 2 K_p slots at 2 c_p of its part's table (c_p: the pair's first
+// corner in the part, K_p its corner count) and hold pair-local vertex ids and
+// insertion ranks (21 and 22 bits; the host takes this path only when every
+// pair fits).  A pair's corners are consecutive, so the waves of a launch
+// touch the regions of a few pairs at a time and stay in cache, where one
+// table hashed over the whole batch misses on every probe.  Flags, scans and
+// edge ids are the part-wide ones of the two-table path: the numbering does
+// not change.
+struct BatchTopo {
+    const uint32_t* idx;        // face vertex ids of the part (ids of the concatenated part)
+    CornerMap cm;
+    unsigned long long* keys;   // 2 nk slots
+    uint32_t* cedge;
+    uint64_t nk;
+    const uint32_t* fpair;      // soup face -> pair
+    const BatchPair* bp;
+    const uint32_t* off;        // the part's face -> first corner (nf + 1 entries)
+    uint32_t fofs;              // soup id of the part's first face (0 or Fs)
+    int m;                      // 0 source part, 1 cut part
+};
+
+// the region of face f's pair: first vertex id, first corner and corner count
+__device__ __forceinline__ void batch_region(const BatchTopo& T, uint32_t f, uint32_t& vb, uint64_t& cb, uint64_t& kp)
+{
+    const BatchPair b = T.bp[T.fpair[T.fofs + f]];
+    const uint32_t f0 = T.m ? b.fc0 : b.fs0, nfp = T.m ? b.nfc : b.nfs;
+    vb = T.m ? b.vc0 : b.vs0;
+    cb = T.off[f0];
+    kp = T.off[f0 + nfp] - cb;
+}
+
+__global__ void k_edge_insert_batch(BatchTopo T)
+{
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < T.nk; c += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t f, b, n, i;
+        T.cm(c, f, b, n, i);
+        uint32_t vb;
+        uint64_t cb, kp;
+        batch_region(T, f, vb, cb, kp);
+        const uint32_t a = T.idx[c] - vb, d = T.idx[b + (i + 1 == n ? 0 : i + 1)] - vb;
+        const uint64_t ek = packed_edge(a, d);
+        const uint32_t rank = corner_rank(b, i, n, T.m) - (uint32_t)cb;
+        const unsigned long long w = (ek << kPackedRankBits) | rank;
+        const uint64_t base = 2 * cb, size = 2 * kp;
+        uint64_t h = base + __umul64hi(hash64(ek), size);
+        for (uint64_t probe = 0; probe < size; ++probe) {
+            const unsigned long long prev = atomicCAS(&T.keys[h], kEmpty64, w);
+            if (prev == kEmpty64) break;
+            if ((prev >> kPackedRankBits) == ek) {
+                if ((prev & kPackedRankMask) > rank) atomicMin(&T.keys[h], w);
+                break;
+            }
+            h = (h + 1 == base + size) ? base : h + 1;
+        }
+        T.cedge[c] = (uint32_t)h;
+    }
+}
+
+// flag[rank] = 1 at an edge's first use (part-local corner ranks)
+__global__ void k_edge_flag_batch(BatchTopo T, uint32_t* __restrict__ flag, Counters* C)
+{
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < T.nk; c += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t f, b, n, i;
+        T.cm(c, f, b, n, i);
+        uint32_t vb;
+        uint64_t cb, kp;
+        batch_region(T, f, vb, cb, kp);
+        const uint32_t a = T.idx[c];
+        const uint32_t rank = corner_rank(b, i, n, T.m);
+        const uint32_t r0 = (uint32_t)cb + (uint32_t)(T.keys[T.cedge[c]] & kPackedRankMask);
+        flag[rank] = (r0 == rank) ? 1u : 0u;
+        if (r0 != rank && T.idx[rank_corner(T.cm, r0, T.m)] == a) atomicOr(&C->nonmanifold, 1u); // used twice in one direction
+    }
+}
+
+__global__ void k_edge_assign_batch(BatchTopo T, const uint32_t* __restrict__ eid_by_rank, Counters* C, uint32_t vofs,
+                                    uint32_t* __restrict__ ev0, uint32_t* __restrict__ ev1, int32_t* __restrict__ ef0,
+                                    int32_t* __restrict__ ef1)
+{
+    const uint32_t ebase = T.m ? (uint32_t)C->n_edges_src : 0u;
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < T.nk; c += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t f, b, n, i;
+        T.cm(c, f, b, n, i);
+        uint32_t vb;
+        uint64_t cb, kp;
+        batch_region(T, f, vb, cb, kp);
+        const uint32_t a = T.idx[c], d = T.idx[b + (i + 1 == n ? 0 : i + 1)];
+        const uint32_t rank = corner_rank(b, i, n, T.m);
+        const uint32_t r0 = (uint32_t)cb + (uint32_t)(T.keys[T.cedge[c]] & kPackedRankMask);
+        const uint32_t e = ebase + eid_by_rank[r0];
+        T.cedge[c] = e;
+        if (r0 == rank) {
+            ev0[e] = vofs + a;
+            ev1[e] = vofs + d;
+            ef0[e] = (int32_t)(T.fofs + f);
+        } else if (atomicExch(&ef1[e], (int32_t)(T.fofs + f)) != -1) {
+            atomicOr(&C->nonmanifold, 4u);
+        }
+    }
+}
+
 // ===========================================================================
 // 4. face AABBs (bvh.cpp:242-313) and the LBVH over them
 // ===========================================================================

@@ -41,13 +41,14 @@ struct Buf {
 // CUs left after the sums (174 of 248 on the 256-CU part): it is the longest
 // of the three paths, and the edge numbering finishes well inside it.  A joins B before the exact pair filter and
 // C before the edge-face tests.  A batch runs on a second pair of streams
-// kBatchBroadShare : rest: its edge numbering builds two hash tables per
-// pair over the whole batch and is as long as its broad phase (139 ms a
-// 4096-pair batch at 0.70, 113 ms at 0.58).  The pair is made on a context's
-// first batch, so a context that never batches holds four hardware queues.
+// kBatchBroadShare : rest (mcgpu_set_batch_split before the first batch):
+// its edge numbering runs over every pair's corners and is its longest path
+// (a 4096-pair batch: 80.2 ms at 0.46, 81.7 ms at 0.40, 85.8 ms at 0.52,
+// 86.1 ms at 0.58).  The pair is made on a context's first batch, so a
+// context that never batches holds four hardware queues.
 constexpr int kSumCUs = 8;
 constexpr double kBroadShare = 0.70;
-constexpr double kBatchBroadShare = 0.58;
+constexpr double kBatchBroadShare = 0.46;
 constexpr uint32_t kBatchSumWaves = 128; // persistent waves of the batched serial sums (on the reserved CUs)
 enum Stage {
     ST_SUMS,      // A: serial coordinate sums and raw boxes (k_mesh_sums alone)
@@ -199,6 +200,8 @@ struct mcgpu_context_t {
     bool batch = false;
     uint32_t npairs = 0;
     int batch_mb = 7;                    // Morton bits per axis within a pair
+    bool batch_packed = false;           // every pair's vertex ids and corner ranks fit the packed edge word
+    int batch_share_pct = (int)(kBatchBroadShare * 100.0 + 0.5); // split of the batch streams (made on the first batch)
     std::vector<BatchPair> hpairs;       // pair ranges (host copy, edge ranges read back)
     std::vector<uint32_t> hstarts, horder;
     std::vector<uint32_t> hflags, hrstart, hrend;
@@ -394,12 +397,16 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
     CK(ensure(ctx->ef1, K, &i32));
     CK(ensure(ctx->etotal, 2, &u));
     uint64_t scan_need = scan_tmp_elems(K + 8);
+    // the single pair's packed table and the two-table path hash over a power
+    // of two; a packed batch takes 2 slots per corner (one region per pair)
+    // and has no first-use table
+    const bool packed_batch = ctx->batch && ctx->batch_packed;
     for (int m = 0; m < 2; ++m) {
         const uint64_t nk = ctx->ncorner[m], nf = ctx->nf[m];
-        const uint64_t hs = next_pow2(2 * nk + 2);
+        const uint64_t hs = packed_batch ? 2 * nk + 8 : next_pow2(2 * nk + 2);
         CK(ensure(ctx->off[m], nf + 1, &u));
         CK(ensure(ctx->hkeys[m], hs, &ull));
-        CK(ensure(ctx->hfirst[m], hs, &ull));
+        if (!packed_batch) CK(ensure(ctx->hfirst[m], hs, &ull));
         CK(ensure(ctx->cface[m], nk, &u));
         CK(ensure(ctx->eflag[m], nk, &u));
         CK(ensure(ctx->eid[m], nk, &u));
@@ -918,7 +925,7 @@ int cus_of_share(int ncu, int pct)
 int cus_of(mcgpu_context ctx, bool batch)
 {
     if (!ctx->own_streams) return ctx->ncu;
-    return cus_of_share(ctx->ncu, (int)((batch ? kBatchBroadShare : kBroadShare) * 100.0 + 0.5));
+    return cus_of_share(ctx->ncu, batch ? ctx->batch_share_pct : (int)(kBroadShare * 100.0 + 0.5));
 }
 
 // the broad view of a context's last (or current) dispatch
@@ -1008,6 +1015,22 @@ void enqueue_topology(mcgpu_context ctx, hipStream_t s3)
         k_edge_assign_packed<<<grid_for(K - start), kBlock, 0, s3>>>(TM[0], TM[1], start, eid, cut_only, C,
                                                                      P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1),
                                                                      P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
+    } else if (ctx->batch && ctx->batch_packed) {
+        BatchTopo T[2];
+        for (int m = 0; m < 2; ++m) {
+            T[m] = BatchTopo{ctx->didx[m], cmap[m], P_<unsigned long long>(ctx->hkeys[m]), P_<uint32_t>(ctx->cedge[m]),
+                             ctx->ncorner[m], P_<uint32_t>(ctx->fpair), P_<BatchPair>(ctx->bpairs),
+                             P_<uint32_t>(ctx->off[m]), m ? ctx->nf[0] : 0u, m};
+            fill_u64(T[m].keys, kEmpty64, 2 * ctx->ncorner[m], s3);
+            k_edge_insert_batch<<<grid_for(ctx->ncorner[m]), kBlock, 0, s3>>>(T[m]);
+            k_edge_flag_batch<<<grid_for(ctx->ncorner[m]), kBlock, 0, s3>>>(T[m], P_<uint32_t>(ctx->eflag[m]), C);
+            scan_exclusive(P_<uint32_t>(ctx->eflag[m]), P_<uint32_t>(ctx->eid[m]), ctx->ncorner[m], ctmp, etotal + m, s3);
+            k_store_edge_count<<<1, 1, 0, s3>>>(etotal + m, C, m);
+        }
+        for (int m = 0; m < 2; ++m)
+            k_edge_assign_batch<<<grid_for(ctx->ncorner[m]), kBlock, 0, s3>>>(
+                T[m], P_<uint32_t>(ctx->eid[m]), C, m ? ctx->nv[0] : 0u, P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1),
+                P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
     } else {
     for (int m = ctx->res ? 1 : 0; m < 2; ++m) {
         const uint64_t nk = ctx->ncorner[m];
@@ -1304,7 +1327,8 @@ std::vector<uint64_t> graph_key(mcgpu_context ctx)
     std::vector<uint64_t> k;
     auto put = [&k](uint64_t x) { k.push_back(x); };
     auto ptr = [&k](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
-    put(ctx->batch), put(ctx->npairs), put(ctx->batch_mb), put(ctx->sort_bits), put(ctx->gbits), put(ctx->kbits);
+    put(ctx->batch), put(ctx->npairs), put(ctx->batch_mb), put(ctx->batch_packed), put(ctx->sort_bits),
+        put(ctx->gbits), put(ctx->kbits);
     for (int m = 0; m < 2; ++m) {
         put(ctx->nv[m]), put(ctx->nf[m]), put(ctx->ncorner[m]), put(ctx->depth[m]);
         ptr(ctx->dv[m]), ptr(ctx->didx[m]), ptr(ctx->dsz[m]), ptr(ctx->bin_idx[m]);
@@ -1470,7 +1494,7 @@ static hipError_t create_split_streams(int ncu, double share, hipStream_t* b, hi
 static int split_pct(mcgpu_context ctx)
 {
     if (!ctx->own_streams) return 100;
-    return (int)((ctx->batch ? kBatchBroadShare : kBroadShare) * 100.0 + 0.5);
+    return ctx->batch ? ctx->batch_share_pct : (int)(kBroadShare * 100.0 + 0.5);
 }
 
 static mcgpu_tuning tuning_report(const TravConfig& t, int pct)
@@ -1490,7 +1514,7 @@ static mcgpu_tuning tuning_report(const TravConfig& t, int pct)
 static mcgpu_result apply_tuning(mcgpu_context ctx)
 {
     if (ctx->batch && ctx->own_streams && !ctx->s2b)
-        CK(create_split_streams(ctx->ncu, kBatchBroadShare, &ctx->s2b, &ctx->s3b));
+        CK(create_split_streams(ctx->ncu, ctx->batch_share_pct / 100.0, &ctx->s2b, &ctx->s3b));
     TravConfig t;
     const TuneEntry* hit = nullptr;
     if (ctx->forced) {
@@ -1797,6 +1821,29 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
         ctx->hstarts[2ull * np + p] = b.fs0;
         ctx->hstarts[3ull * np + p] = b.fc0;
         for (int j = 0; j < 6; ++j) len[6ull * p + j] = j < 3 ? b.nvs : b.nvc;
+    }
+    // the packed edge numbering with one table region per pair, when every
+    // pair's vertex ids and corner ranks fit its word (corner counts are
+    // known on the host for triangle meshes and host face sizes)
+    ctx->batch_packed = true;
+    for (int m = 0; m < 2 && ctx->batch_packed; ++m) {
+        const bool tri = mesh[m]->face_sizes == nullptr;
+        if (!tri && desc->input_on_device) {
+            ctx->batch_packed = false;
+            break;
+        }
+        for (uint32_t p = 0; p < np; ++p) {
+            uint64_t k = 0;
+            if (tri) {
+                k = 3ull * (fo[m][p + 1] - fo[m][p]);
+            } else {
+                for (uint32_t f = fo[m][p]; f < fo[m][p + 1]; ++f) k += mesh[m]->face_sizes[f];
+            }
+            if (vo[m][p + 1] - vo[m][p] >= (1u << kPackedVertexBits) || k >= (1ull << kPackedRankBits)) {
+                ctx->batch_packed = false;
+                break;
+            }
+        }
     }
     // serial-sum chains, longest first
     ctx->horder.resize(6ull * np);
@@ -2270,6 +2317,15 @@ MCGPU_API mcgpu_result mcgpu_graph_stats(mcgpu_context ctx, uint64_t* recordings
     if (recordings) *recordings = ctx->recordings;
     if (replays) *replays = ctx->replays;
     if (graph_launches) *graph_launches = n;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_set_batch_split(mcgpu_context ctx, uint32_t broad_share_pct)
+{
+    if (!ctx || broad_share_pct < 10 || broad_share_pct > 90) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->own_streams) return MCGPU_ERROR_UNSUPPORTED;
+    if (ctx->s2b) return MCGPU_ERROR_INVALID_VALUE; // the batch streams exist: their masks are fixed
+    ctx->batch_share_pct = (int)broad_share_pct;
     return MCGPU_SUCCESS;
 }
 

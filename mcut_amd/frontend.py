@@ -133,6 +133,7 @@ def lib():
         L.mcgpu_stage_name.argtypes = [C.c_int]
         L.mcgpu_autotune.argtypes = [vp, C.POINTER(Tuning)]
         L.mcgpu_set_graphs.argtypes = [vp, C.c_int]
+        L.mcgpu_set_batch_split.argtypes = [vp, C.c_uint32]
         L.mcgpu_graph_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(C.c_uint32)]
         L.mcgpu_set_autotune.argtypes = [vp, C.c_int]
         L.mcgpu_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
@@ -397,6 +398,10 @@ class Context:
         r, p, n = C.c_uint64(0), C.c_uint64(0), C.c_uint32(0)
         _check(lib().mcgpu_graph_stats(self.h, C.byref(r), C.byref(p), C.byref(n)), "mcgpu_graph_stats")
         return dict(recordings=r.value, replays=p.value, graph_launches=n.value)
+
+    def set_batch_split(self, broad_share_pct):
+        """CU split of the batch streams; only before the context's first batch."""
+        _check(lib().mcgpu_set_batch_split(self.h, broad_share_pct), "mcgpu_set_batch_split")
 
     def set_autotune(self, on):
         _check(lib().mcgpu_set_autotune(self.h, 1 if on else 0), "mcgpu_set_autotune")

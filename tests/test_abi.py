@@ -115,6 +115,7 @@ def test_null_arguments_rejected_without_a_device():
     assert L.mcgpu_set_tuning(None, None) == -1
     assert L.mcgpu_get_tuning(None, None) == -1
     assert L.mcgpu_set_graphs(None, 1) == -1
+    assert L.mcgpu_set_batch_split(None, 50) == -1
     assert L.mcgpu_graph_stats(None, None, None, None) == -1
 
 

@@ -168,3 +168,44 @@ def test_plain_and_batch_alternate_on_one_context():
     finally:
         a.close()
         ref.close()
+
+
+@pytest.mark.gpu
+def test_device_polygon_batch_matches_host_batch(ctxs):
+    """Two edge numberings of a batch: host input (corner counts known on the
+    host: one packed table region per pair) and device input with face sizes
+    on the device (the two-table hash over the whole batch) give the same
+    per-pair outputs."""
+    from mcut_amd import frontend
+    from test_zz_no_alloc import Hip
+    bctx, pctx = ctxs
+    pairs = [meshes.bench_pair(i) for i in (0, 1, 2, 5, 14, 20, 33, 40)] + [meshes.config1_rotated()]
+    pairs += mg.config4_pairs(6, seed=29, tri_range=(200, 5_000), box_range=(12, 1_000))
+    H = bctx.dispatch_batch(pairs)
+    ref = [tuple(np.ascontiguousarray(x).copy() for x in H.pair_registry(i)) for i in range(len(pairs))]
+    ref_status = H.status.copy()
+    P = frontend.pack_batch(pairs)
+    hip = Hip()  # device buffers from the HIP runtime libmcgpu links
+    bufs = []
+
+    def up(side):
+        V, F, S = P[side]
+        if S is None:
+            S = np.full(F.size // 3, 3, np.uint32)
+        ptrs = [hip.upload(np.ascontiguousarray(a)) for a in (np.asarray(V, np.float64), np.asarray(F, np.uint32),
+                                                             np.asarray(S, np.uint32))]
+        bufs.extend(ptrs)
+        return frontend.device_mesh(ptrs[0], ptrs[1], ptrs[2], V.shape[0], S.size)
+
+    try:
+        ms, mc = up("src"), up("cut")
+        D = bctx.dispatch_batch_device(P["num_pairs"], ms, mc, [P["src_vo"], P["src_fo"], P["cut_vo"], P["cut_fo"]])
+        assert np.array_equal(D.status, ref_status)
+        for i in range(len(pairs)):
+            got = D.pair_registry(i)
+            for x, y in zip(ref[i], got):
+                assert np.array_equal(np.ascontiguousarray(x).view(np.uint8),
+                                      np.ascontiguousarray(y).view(np.uint8)), i
+    finally:
+        for p in bufs:
+            hip.free(p)
