@@ -375,6 +375,35 @@ MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity);
 MCGPU_API mcgpu_result mcgpu_set_stage_timing(mcgpu_context ctx, int enable);
 MCGPU_API const char* mcgpu_stage_name(int stage);
 
+/* Cut-path edges of the last successful mcgpu_dispatch_frontend or
+ * mcgpu_dispatch_narrow (kernel.cpp:3427-3615, the first consumer of the
+ * registry): the edges between intersection points along which each
+ * (source face, cut face) pair cuts, in the order MCUT creates them.
+ * order: NULL (registry order) or a permutation of the registry giving the
+ * caller's intersection-point order (order[k] = registry index of point k;
+ * mcDispatch numbers its points in its own order); points and edges are then
+ * named by rank k.  short_group = 1 when a group holds a single point: MCUT
+ * then stops the dispatch with a general-position violation when it enforces
+ * general position, else with an invalid mesh intersection
+ * (kernel.cpp:3445-3459). */
+typedef struct mcgpu_cutpath_info {
+    uint64_t num_edges;      /* cut-path edges made */
+    uint64_t num_groups;     /* (source face, cut face) groups */
+    uint64_t num_candidates; /* consecutive point pairs examined */
+    uint32_t short_group;    /* a group of one point */
+    uint32_t unsupported;    /* a midpoint test named a face without a plane */
+    float device_ms;
+    float reserved;
+} mcgpu_cutpath_info;
+MCGPU_API mcgpu_result mcgpu_cutpath_edges(mcgpu_context ctx, const uint32_t* order, mcgpu_cutpath_info* info);
+/* The edges in creation order: source and target point (ranks), the two faces
+ * the edge is filed under (a two-point group's source and cut face, else the
+ * two faces whose area holds its midpoint; polygon-soup face ids), the group
+ * (in (source face, cut face) order) and kind (0: group of two points, 1:
+ * consecutive points of a longer group).  Any output may be NULL. */
+MCGPU_API mcgpu_result mcgpu_get_cutpath_edges(mcgpu_context ctx, uint32_t* src, uint32_t* tgt, uint32_t* faces,
+                                               uint32_t* groups, uint8_t* kinds, uint64_t capacity);
+
 /* Graph replay of dispatches.  No counterpart in the reference (a CPU
  * pipeline); for callers that dispatch the same shape repeatedly (the same
  * sizes and input buffers, new coordinates or connectivity behind them).

@@ -2897,4 +2897,285 @@ __global__ void k_batch_localize(const uint32_t* __restrict__ re, const uint32_t
     }
 }
 
+// ===========================================================================
+// 7. cut-path edges (kernel.cpp:3427-3615): the edges between intersection
+// points that cut each (source face, cut face) pair.
+//
+// The registry's points are grouped by (source face, cut face) pair as the
+// narrow phase files them (kernel.cpp:2616-2631): a point on a cut-mesh edge
+// joins the groups of the source face it crosses with each face of its edge,
+// a point on a source-mesh edge the groups of each face of its edge with the
+// cut face; groups are visited in (source face, cut face) order
+// (std::map<pair<fd_t>, ...>) and hold their points in registry order.  A
+// group of two points gives the edge (first, second); a longer group is sorted
+// along its line (linear_projection_sort, kernel.cpp:1496-1531: projections on
+// the unit vector from the second point to the first, libstdc++'s std::sort
+// ascending) and each consecutive pair becomes an edge when its midpoint lies
+// inside exactly two of the faces both points' registry entries name
+// (kernel.cpp:3527-3583).  An edge that an earlier group already made is not
+// made again.  A group of one point stops the dispatch (kernel.cpp:3445-3459).
+// ===========================================================================
+
+constexpr unsigned long long kCutNone = ~0ull;
+
+// device counts of the cut-path stage (the scans write their totals here)
+struct CutCounts {
+    uint32_t groups, items, candidates, edges;
+    uint32_t short_group, unsupported;
+};
+
+// a point's two group keys (source face << fbits | cut face); kCutNone for none
+__global__ void k_cut_items(uint64_t n, const uint32_t* __restrict__ order, const uint32_t* __restrict__ re,
+                            const uint32_t* __restrict__ rg, const int32_t* __restrict__ ef0,
+                            const int32_t* __restrict__ ef1, uint32_t Fs, int fbits,
+                            unsigned long long* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = order ? order[k] : (uint32_t)k;
+        const uint32_t e = re[i];
+        const unsigned long long g = rg[i];
+        const int32_t f0 = ef0[e], f1 = ef1[e];
+        const int32_t ef = f0 >= 0 ? f0 : f1;
+        unsigned long long a, b = kCutNone;
+        if ((uint32_t)ef >= Fs) { // a cut-mesh edge through source face g
+            a = (g << fbits) | (uint32_t)ef;
+            const int32_t other = ef == f0 ? f1 : -1;
+            if (other >= 0) b = (g << fbits) | (uint32_t)other;
+        } else { // a source-mesh edge through cut face g
+            a = ((unsigned long long)ef << fbits) | g;
+            const int32_t other = ef == f0 ? f1 : f0;
+            if (other >= 0) b = ((unsigned long long)other << fbits) | g;
+        }
+        keys[2 * k] = a;
+        keys[2 * k + 1] = b;
+        vals[2 * k] = vals[2 * k + 1] = (uint32_t)k;
+    }
+}
+
+// first item of each group (flag) and, once, the number of grouped items
+__global__ void k_cut_flags(const unsigned long long* __restrict__ keys, uint64_t n2, uint32_t* __restrict__ flag,
+                            CutCounts* cc)
+{
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n2; j += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long key = keys[j];
+        flag[j] = (key != kCutNone && (j == 0 || keys[j - 1] != key)) ? 1u : 0u;
+        if (key != kCutNone && (j + 1 == n2 || keys[j + 1] == kCutNone)) cc->items = (uint32_t)(j + 1);
+    }
+}
+
+__global__ void k_cut_starts(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ gid, uint64_t n2,
+                             uint32_t* __restrict__ gstart)
+{
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n2; j += (uint64_t)gridDim.x * blockDim.x)
+        if (flag[j]) gstart[gid[j]] = (uint32_t)j;
+}
+
+// candidates per group (consecutive point pairs); groups of one point flagged
+__global__ void k_cut_sizes(const uint32_t* __restrict__ gstart, CutCounts* cc, uint32_t* __restrict__ ncand)
+{
+    const uint32_t G = cc->groups;
+    const uint64_t items = cc->items;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < G; g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t len = (g + 1 < G ? gstart[g + 1] : items) - gstart[g];
+        ncand[g] = len >= 2 ? (uint32_t)(len - 1) : 0u;
+        if (len < 2) atomicOr(&cc->short_group, 1u);
+    }
+}
+
+struct CutCands {
+    uint32_t *src, *tgt;     // points (registry ranks in the caller's order)
+    uint32_t *fa, *fb;       // faces the edge is filed under: the group's pair, or the two containing faces
+    uint32_t* group;         // group index (map order)
+    uint8_t* kind;           // 0: group of two points; 1: consecutive points of a longer group (midpoint test)
+    uint8_t* accepted;
+};
+
+// One thread per group: its candidates at coff[g].  Longer groups are sorted
+// along their line in the triplet scratch (the group's item range).
+__global__ void k_cut_groups(const uint32_t* __restrict__ gstart, const CutCounts* __restrict__ cc,
+                             const unsigned long long* __restrict__ keys,
+                             const uint32_t* __restrict__ vals, const uint32_t* __restrict__ order,
+                             const double* __restrict__ rpts, const uint32_t* __restrict__ coff, int fbits,
+                             Trip* __restrict__ scratch, CutCands out)
+{
+    const uint32_t G = cc->groups;
+    const uint64_t items = cc->items;
+    const unsigned long long fmask = (1ull << fbits) - 1;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < G; g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s0 = gstart[g];
+        const uint64_t len = (g + 1 < G ? gstart[g + 1] : items) - s0;
+        if (len < 2) continue;
+        const unsigned long long key = keys[s0];
+        const uint32_t sm = (uint32_t)(key >> fbits), cm = (uint32_t)(key & fmask);
+        const uint32_t c0 = coff[g];
+        if (len == 2) {
+            out.src[c0] = vals[s0];
+            out.tgt[c0] = vals[s0 + 1];
+            out.fa[c0] = sm;
+            out.fb[c0] = cm;
+            out.group[c0] = (uint32_t)g;
+            out.kind[c0] = 0;
+            continue;
+        }
+        // linear_projection_sort: origin = first point, unit vector from the
+        // second point to the first, projections of (origin - point) on it
+        double o[3], d[3], u[3];
+        {
+            const uint32_t i0 = order ? order[vals[s0]] : vals[s0];
+            const uint32_t i1 = order ? order[vals[s0 + 1]] : vals[s0 + 1];
+            for (int j = 0; j < 3; ++j) {
+                o[j] = rpts[3 * (uint64_t)i0 + j];
+                d[j] = rpts[3 * (uint64_t)i1 + j];
+            }
+            double v[3];
+            for (int j = 0; j < 3; ++j) v[j] = o[j] - d[j];
+            double sq = 0.0;
+            for (int j = 0; j < 3; ++j) sq = sq + v[j] * v[j];
+            const double len_v = sqrt(sq);
+            for (int j = 0; j < 3; ++j) u[j] = v[j] / len_v;
+        }
+        Trip* T = scratch + s0;
+        for (uint64_t q = 0; q < len; ++q) {
+            const uint32_t k = vals[s0 + q];
+            const uint32_t i = order ? order[k] : k;
+            double w[3];
+            for (int j = 0; j < 3; ++j) w[j] = o[j] - rpts[3 * (uint64_t)i + j];
+            double pr = 0.0;
+            for (int j = 0; j < 3; ++j) pr = pr + w[j] * u[j];
+            // the replayed sort orders by descending key: negated projections
+            // ascend, and negation keeps every comparison's outcome
+            T[q].a = -pr;
+            T[q].ijk = (int32_t)k;
+        }
+        trip_sort(T, T + len);
+        for (uint64_t q = 1; q < len; ++q) {
+            const uint32_t c = c0 + (uint32_t)(q - 1);
+            out.src[c] = (uint32_t)T[q - 1].ijk;
+            out.tgt[c] = (uint32_t)T[q].ijk;
+            out.fa[c] = sm;
+            out.fb[c] = cm;
+            out.group[c] = (uint32_t)g;
+            out.kind[c] = 1;
+        }
+    }
+}
+
+// the faces a registry entry names: its face, then its edge's two faces
+// (ps_get_ivtx_registry_entry_faces, kernel.cpp:1407-1415; -1: none)
+__device__ __forceinline__ void entry_faces(uint32_t i, const uint32_t* re, const uint32_t* rg, const int32_t* ef0,
+                                            const int32_t* ef1, int32_t* f)
+{
+    const uint32_t e = re[i];
+    f[0] = (int32_t)rg[i];
+    f[1] = ef0[e];
+    f[2] = ef1[e];
+}
+
+// Midpoint tests of the candidates of longer groups (kernel.cpp:3527-3583):
+// the faces both points name (in the first point's order, none skipped but
+// "no face"), each tested with its plane and vertices; accepted when the
+// midpoint lies inside exactly two, which the edge is filed under.
+__global__ void k_cut_midpoints(const CutCounts* __restrict__ cc, CutCands cand, const uint32_t* __restrict__ order,
+                                const uint32_t* __restrict__ re, const uint32_t* __restrict__ rg,
+                                const int32_t* __restrict__ ef0, const int32_t* __restrict__ ef1,
+                                const double* __restrict__ rpts, const Plane* __restrict__ planes,
+                                const uint32_t* __restrict__ cand_face, Soup S, CutCounts* ccw)
+{
+    const uint64_t n = cc->candidates;
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < n; c += (uint64_t)gridDim.x * blockDim.x) {
+        if (cand.kind[c] == 0) {
+            cand.accepted[c] = 1;
+            continue;
+        }
+        const uint32_t a = order ? order[cand.src[c]] : cand.src[c];
+        const uint32_t b = order ? order[cand.tgt[c]] : cand.tgt[c];
+        int32_t fa[3], fb[3];
+        entry_faces(a, re, rg, ef0, ef1, fa);
+        entry_faces(b, re, rg, ef0, ef1, fb);
+        double mid[3];
+        for (int j = 0; j < 3; ++j) mid[j] = (rpts[3 * (uint64_t)b + j] + rpts[3 * (uint64_t)a + j]) * 0.5;
+        int inside = 0;
+        uint32_t in_face[3] = {0, 0, 0};
+        for (int x = 0; x < 3; ++x) {
+            const int32_t f = fa[x];
+            if (f < 0 || !(f == fb[0] || f == fb[1] || f == fb[2])) continue;
+            if (!cand_face[f]) { // a face without a plane: the reference would find none either
+                atomicOr(&ccw->unsupported, 1u);
+                continue;
+            }
+            const Plane pl = planes[f];
+            const FaceView fv = face_view(S, (uint32_t)f);
+            if (point_in_polygon(mid, (int)fv.n, fv, pl.n, pl.k) == 'i') {
+                if (inside < 3) in_face[inside] = (uint32_t)f;
+                ++inside;
+            }
+        }
+        cand.accepted[c] = inside == 2 ? 1 : 0;
+        if (inside == 2) {
+            cand.fa[c] = in_face[0];
+            cand.fb[c] = in_face[1];
+        }
+    }
+}
+
+// an edge's first accepted candidate (candidates are numbered in group order)
+__global__ void k_cut_first(const CutCounts* __restrict__ cc, CutCands cand, unsigned long long* __restrict__ table,
+                            uint64_t mask)
+{
+    const uint64_t n = cc->candidates;
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < n; c += (uint64_t)gridDim.x * blockDim.x) {
+        if (!cand.accepted[c]) continue;
+        const uint32_t a = cand.src[c], b = cand.tgt[c];
+        const unsigned long long key = a < b ? ((unsigned long long)a << 32 | b) : ((unsigned long long)b << 32 | a);
+        // slots hold (edge key, first candidate) in two words: the key by CAS, the candidate by atomicMin
+        uint64_t h = hash64(key) & mask;
+        for (uint64_t probe = 0; probe <= mask; ++probe) {
+            const unsigned long long prev = atomicCAS(&table[2 * h], kCutNone, key);
+            if (prev == kCutNone || prev == key) {
+                atomicMin(&table[2 * h + 1], (unsigned long long)c);
+                break;
+            }
+            h = (h + 1) & mask;
+        }
+    }
+}
+
+__global__ void k_cut_keep(const CutCounts* __restrict__ cc, CutCands cand, const unsigned long long* __restrict__ table,
+                           uint64_t mask, uint32_t* __restrict__ keep)
+{
+    const uint64_t n = cc->candidates;
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < n; c += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t k = 0;
+        if (cand.accepted[c]) {
+            const uint32_t a = cand.src[c], b = cand.tgt[c];
+            const unsigned long long key = a < b ? ((unsigned long long)a << 32 | b) : ((unsigned long long)b << 32 | a);
+            uint64_t h = hash64(key) & mask;
+            while (table[2 * h] != key) h = (h + 1) & mask; // inserted by k_cut_first
+            k = table[2 * h + 1] == c ? 1u : 0u;
+        }
+        keep[c] = k;
+    }
+}
+
+struct CutEdges {
+    uint32_t *src, *tgt, *fa, *fb, *group;
+    uint8_t* kind;
+};
+
+__global__ void k_cut_emit(const CutCounts* __restrict__ cc, CutCands cand, const uint32_t* __restrict__ keep,
+                           const uint32_t* __restrict__ pos, CutEdges out)
+{
+    const uint64_t n = cc->candidates;
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < n; c += (uint64_t)gridDim.x * blockDim.x) {
+        if (!keep[c]) continue;
+        const uint32_t q = pos[c];
+        out.src[q] = cand.src[c];
+        out.tgt[q] = cand.tgt[c];
+        out.fa[q] = cand.fa[c];
+        out.fb[q] = cand.fb[c];
+        out.group[q] = cand.group[c];
+        out.kind[q] = cand.kind[c];
+    }
+}
+
 } // namespace mcgpu

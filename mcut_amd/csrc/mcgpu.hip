@@ -243,6 +243,13 @@ struct mcgpu_context_t {
     std::vector<GraphOp> prog;
     std::vector<uint64_t> prog_key;
     uint64_t replays = 0, recordings = 0;
+
+    // cut-path edges of the last dispatch (mcgpu_cutpath_edges)
+    Buf cp_order, cp_keys, cp_vals, cp_keys2, cp_vals2, cp_flag, cp_gid, cp_gstart, cp_ncand, cp_coff, cp_trip;
+    Buf cp_src, cp_tgt, cp_fa, cp_fb, cp_group, cp_kind, cp_acc, cp_table, cp_keep, cp_pos;
+    Buf cp_osrc, cp_otgt, cp_ofa, cp_ofb, cp_ogroup, cp_okind, cp_cnt;
+    bool cp_have = false;
+    CutCounts cp_counts{};
 };
 
 #define CK(x)                                                                                                          \
@@ -1180,6 +1187,7 @@ void enqueue_frontend(mcgpu_context ctx)
 
 mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow)
 {
+    ctx->cp_have = false; // cut-path edges belong to the dispatch they were made from
     // stream A joins the other three before its last kernel, so it finishes last
     CK(hipStreamSynchronize(ctx->s));
     CK(hipGetLastError());
@@ -1315,7 +1323,11 @@ std::vector<Buf*> context_bufs(mcgpu_context ctx)
                   &ctx->brbox, &ctx->pflags, &ctx->border, &ctx->bidx[0], &ctx->bidx[1], &ctx->rle, &ctx->rlg,
                   &ctx->rstart, &ctx->rend, &ctx->pqs, &ctx->prs, &ctx->pcounts, &ctx->sbsum,
                   &ctx->sitems, &ctx->slo, &ctx->shi, &ctx->snitem, &ctx->slist, &ctx->sckpt, &ctx->sflags,
-                  &ctx->tune_c};
+                  &ctx->tune_c, &ctx->cp_order, &ctx->cp_keys, &ctx->cp_vals, &ctx->cp_keys2, &ctx->cp_vals2,
+                  &ctx->cp_flag, &ctx->cp_gid, &ctx->cp_gstart, &ctx->cp_ncand, &ctx->cp_coff, &ctx->cp_trip,
+                  &ctx->cp_src, &ctx->cp_tgt, &ctx->cp_fa, &ctx->cp_fb, &ctx->cp_group, &ctx->cp_kind, &ctx->cp_acc,
+                  &ctx->cp_table, &ctx->cp_keep, &ctx->cp_pos, &ctx->cp_osrc, &ctx->cp_otgt, &ctx->cp_ofa,
+                  &ctx->cp_ofb, &ctx->cp_ogroup, &ctx->cp_okind, &ctx->cp_cnt};
 }
 
 // The shape of the dispatch about to be enqueued: everything its kernels'
@@ -2294,6 +2306,146 @@ MCGPU_API mcgpu_result mcgpu_autotune(mcgpu_context ctx, mcgpu_tuning* out)
     if (!ctx->have || ctx->broad_only || ctx->stage_mode != 0 || ctx->res) return MCGPU_ERROR_NO_RESULT;
     CK(hipSetDevice(ctx->device));
     return autotune(ctx, out);
+}
+
+MCGPU_API mcgpu_result mcgpu_cutpath_edges(mcgpu_context ctx, const uint32_t* order, mcgpu_cutpath_info* info)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->have || ctx->broad_only || ctx->batch || ctx->res) return MCGPU_ERROR_NO_RESULT;
+    CK(hipSetDevice(ctx->device));
+    ctx->cp_have = false;
+    const uint64_t n = ctx->C.n_reg, n2 = 2 * n;
+    const uint64_t F = (uint64_t)ctx->nf[0] + ctx->nf[1];
+    const int fbits = bits_for(F + 1);
+    if (n == 0 || 2 * fbits > 62 || n2 > 0xffffffffull) {
+        if (n == 0) {
+            ctx->cp_counts = CutCounts{};
+            ctx->cp_have = true;
+            if (info) std::memset(info, 0, sizeof(*info));
+            return MCGPU_SUCCESS;
+        }
+        return MCGPU_ERROR_UNSUPPORTED;
+    }
+    hipStream_t s = ctx->s;
+    uint32_t *ord = nullptr, *u;
+    unsigned long long* ull;
+    uint8_t* b8;
+    Trip* tp;
+    CutCounts* cc;
+    if (order) { // the caller's registry order: rank k -> registry index, a permutation
+        std::vector<uint8_t> seen(n, 0);
+        for (uint64_t k = 0; k < n; ++k) {
+            if (order[k] >= n || seen[order[k]]) return MCGPU_ERROR_INVALID_VALUE;
+            seen[order[k]] = 1;
+        }
+        CK(ensure(ctx->cp_order, n, &ord));
+        CK(hipMemcpyAsync(ord, order, 4 * n, hipMemcpyHostToDevice, s));
+    }
+    CK(ensure(ctx->cp_keys, n2, &ull));
+    CK(ensure(ctx->cp_keys2, n2, &ull));
+    for (Buf* b : {&ctx->cp_vals, &ctx->cp_vals2, &ctx->cp_flag, &ctx->cp_gid, &ctx->cp_gstart, &ctx->cp_ncand,
+                   &ctx->cp_coff, &ctx->cp_src, &ctx->cp_tgt, &ctx->cp_fa, &ctx->cp_fb, &ctx->cp_group, &ctx->cp_keep,
+                   &ctx->cp_pos, &ctx->cp_osrc, &ctx->cp_otgt, &ctx->cp_ofa, &ctx->cp_ofb, &ctx->cp_ogroup})
+        CK(ensure(*b, n2 + 8, &u));
+    for (Buf* b : {&ctx->cp_kind, &ctx->cp_acc, &ctx->cp_okind}) CK(ensure(*b, n2 + 8, &b8));
+    CK(ensure(ctx->cp_trip, n2 + 8, &tp));
+    const uint64_t tslots = next_pow2(2 * n2 + 2);
+    CK(ensure(ctx->cp_table, 2 * tslots, &ull));
+    CK(ensure(ctx->cp_cnt, 1, &cc));
+    CK(ensure(ctx->hist, radix_hist_elems(n2), &u));
+    CK(ensure(ctx->scan_tmp, std::max<uint64_t>(scan_tmp_elems(n2 + 8), scan_tmp_elems(radix_hist_elems(n2))) + 1024, &u));
+    fill_u32(cc, 0u, sizeof(CutCounts) / 4, s);
+    const uint32_t* re = P_<uint32_t>(ctx->re);
+    const uint32_t* rg = P_<uint32_t>(ctx->rg);
+    const double* rp = P_<double>(ctx->rpts);
+    const int32_t* ef0 = P_<int32_t>(ctx->ef0);
+    const int32_t* ef1 = P_<int32_t>(ctx->ef1);
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    (void)hipEventRecord(e0, s);
+    // groups: both keys of every point, stably sorted by (source face, cut face)
+    k_cut_items<<<grid_for(n), kBlock, 0, s>>>(n, ord, re, rg, ef0, ef1, ctx->nf[0], fbits, P_<unsigned long long>(ctx->cp_keys),
+                                             P_<uint32_t>(ctx->cp_vals));
+    RadixTemp rt{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)};
+    const bool alt = radix_sort<unsigned long long>(P_<unsigned long long>(ctx->cp_keys), P_<uint32_t>(ctx->cp_vals),
+                                                    P_<unsigned long long>(ctx->cp_keys2), P_<uint32_t>(ctx->cp_vals2),
+                                                    nullptr, n2, n2, 2 * fbits, rt, s);
+    const unsigned long long* keys = P_<unsigned long long>(alt ? ctx->cp_keys2 : ctx->cp_keys);
+    const uint32_t* vals = P_<uint32_t>(alt ? ctx->cp_vals2 : ctx->cp_vals);
+    uint32_t* flag = P_<uint32_t>(ctx->cp_flag);
+    uint32_t* gid = P_<uint32_t>(ctx->cp_gid);
+    uint32_t* gstart = P_<uint32_t>(ctx->cp_gstart);
+    uint32_t* ncand = P_<uint32_t>(ctx->cp_ncand);
+    uint32_t* coff = P_<uint32_t>(ctx->cp_coff);
+    uint32_t* stmp = P_<uint32_t>(ctx->scan_tmp);
+    k_cut_flags<<<grid_for(n2), kBlock, 0, s>>>(keys, n2, flag, cc);
+    scan_exclusive(flag, gid, n2, stmp, &cc->groups, s);
+    k_cut_starts<<<grid_for(n2), kBlock, 0, s>>>(flag, gid, n2, gstart);
+    fill_u32(ncand, 0u, n2, s);
+    k_cut_sizes<<<grid_for(n2), kBlock, 0, s>>>(gstart, cc, ncand);
+    scan_exclusive(ncand, coff, n2, stmp, &cc->candidates, s);
+    CutCands cd{P_<uint32_t>(ctx->cp_src), P_<uint32_t>(ctx->cp_tgt), P_<uint32_t>(ctx->cp_fa), P_<uint32_t>(ctx->cp_fb),
+                P_<uint32_t>(ctx->cp_group), P_<uint8_t>(ctx->cp_kind), P_<uint8_t>(ctx->cp_acc)};
+    k_cut_groups<<<grid_for(n2, 64), 64, 0, s>>>(gstart, cc, keys, vals, ord, rp, coff, fbits, tp, cd);
+    const Soup S = make_soup(ctx);
+    k_cut_midpoints<<<grid_for(n2), kBlock, 0, s>>>(cc, cd, ord, re, rg, ef0, ef1, rp, P_<Plane>(ctx->planes),
+                                                   P_<uint32_t>(ctx->cand), S, cc);
+    // first accepted candidate of each edge, then the kept ones compacted in candidate order
+    unsigned long long* table = P_<unsigned long long>(ctx->cp_table);
+    fill_u64(table, kCutNone, 2 * tslots, s);
+    k_cut_first<<<grid_for(n2), kBlock, 0, s>>>(cc, cd, table, tslots - 1);
+    uint32_t* keep = P_<uint32_t>(ctx->cp_keep);
+    uint32_t* pos = P_<uint32_t>(ctx->cp_pos);
+    fill_u32(keep, 0u, n2, s);
+    k_cut_keep<<<grid_for(n2), kBlock, 0, s>>>(cc, cd, table, tslots - 1, keep);
+    scan_exclusive(keep, pos, n2, stmp, &cc->edges, s);
+    CutEdges oe{P_<uint32_t>(ctx->cp_osrc), P_<uint32_t>(ctx->cp_otgt), P_<uint32_t>(ctx->cp_ofa), P_<uint32_t>(ctx->cp_ofb),
+                P_<uint32_t>(ctx->cp_ogroup), P_<uint8_t>(ctx->cp_okind)};
+    k_cut_emit<<<grid_for(n2), kBlock, 0, s>>>(cc, cd, keep, pos, oe);
+    (void)hipEventRecord(e1, s);
+    hipError_t err = hipStreamSynchronize(s);
+    if (err == hipSuccess) err = hipGetLastError();
+    float ms = 0.f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    if (err == hipSuccess) err = hipMemcpy(&ctx->cp_counts, cc, sizeof(CutCounts), hipMemcpyDeviceToHost);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    CK(err);
+    ctx->cp_have = true;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->num_edges = ctx->cp_counts.edges;
+        info->num_groups = ctx->cp_counts.groups;
+        info->num_candidates = ctx->cp_counts.candidates;
+        info->short_group = ctx->cp_counts.short_group;
+        info->unsupported = ctx->cp_counts.unsupported;
+        info->device_ms = ms;
+    }
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API mcgpu_result mcgpu_get_cutpath_edges(mcgpu_context ctx, uint32_t* src, uint32_t* tgt, uint32_t* faces,
+                                               uint32_t* groups, uint8_t* kinds, uint64_t capacity)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->cp_have) return MCGPU_ERROR_NO_RESULT;
+    const uint64_t n = std::min<uint64_t>(ctx->cp_counts.edges, capacity);
+    if (n == 0) return MCGPU_SUCCESS;
+    if (src) CK(hipMemcpy(src, ctx->cp_osrc.p, 4 * n, hipMemcpyDeviceToHost));
+    if (tgt) CK(hipMemcpy(tgt, ctx->cp_otgt.p, 4 * n, hipMemcpyDeviceToHost));
+    if (groups) CK(hipMemcpy(groups, ctx->cp_ogroup.p, 4 * n, hipMemcpyDeviceToHost));
+    if (kinds) CK(hipMemcpy(kinds, ctx->cp_okind.p, n, hipMemcpyDeviceToHost));
+    if (faces) {
+        std::vector<uint32_t> a(n), b(n);
+        CK(hipMemcpy(a.data(), ctx->cp_ofa.p, 4 * n, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(b.data(), ctx->cp_ofb.p, 4 * n, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < n; ++i) {
+            faces[2 * i] = a[i];
+            faces[2 * i + 1] = b[i];
+        }
+    }
+    return MCGPU_SUCCESS;
 }
 
 MCGPU_API mcgpu_result mcgpu_set_graphs(mcgpu_context ctx, int enable)

@@ -79,6 +79,13 @@ class RequestResult(C.Structure):
 _lib = None
 
 
+class CutpathInfo(C.Structure):
+    """mcgpu_cutpath_info"""
+    _fields_ = [("num_edges", C.c_uint64), ("num_groups", C.c_uint64), ("num_candidates", C.c_uint64),
+                ("short_group", C.c_uint32), ("unsupported", C.c_uint32), ("device_ms", C.c_float),
+                ("reserved", C.c_float)]
+
+
 class Tuning(C.Structure):
     """mcgpu_tuning: the traversal's launch shape and the CU split."""
     _fields_ = [("wave_leaves", C.c_uint32), ("tree_stride", C.c_uint32), ("queues", C.c_uint32),
@@ -132,6 +139,8 @@ def lib():
         L.mcgpu_set_stage_timing.argtypes = [vp, C.c_int]
         L.mcgpu_stage_name.argtypes = [C.c_int]
         L.mcgpu_autotune.argtypes = [vp, C.POINTER(Tuning)]
+        L.mcgpu_cutpath_edges.argtypes = [vp, vp, C.POINTER(CutpathInfo)]
+        L.mcgpu_get_cutpath_edges.argtypes = [vp, vp, vp, vp, vp, vp, u64]
         L.mcgpu_set_graphs.argtypes = [vp, C.c_int]
         L.mcgpu_set_batch_split.argtypes = [vp, C.c_uint32]
         L.mcgpu_graph_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(C.c_uint32)]
@@ -255,6 +264,23 @@ class Result:
         if signs:
             out["qsign"], out["rsign"] = qs, rs
         return out
+
+    def cutpath(self, order=None):
+        """Cut-path edges of this dispatch (mcgpu_cutpath_edges): order = None
+        (registry order) or order[k] = registry index of point k.  Returns the
+        edges (src, tgt point ranks, faces (n, 2), groups, kinds) and the info."""
+        info = CutpathInfo()
+        o = None if order is None else np.ascontiguousarray(order, np.uint32)
+        _check(lib().mcgpu_cutpath_edges(self._ctx.h, None if o is None else _ptr(o), C.byref(info)),
+               "mcgpu_cutpath_edges")
+        n = info.num_edges
+        src, tgt, grp = np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        faces, kinds = np.zeros((n, 2), np.uint32), np.zeros(n, np.uint8)
+        _check(lib().mcgpu_get_cutpath_edges(self._ctx.h, _ptr(src), _ptr(tgt), _ptr(faces), _ptr(grp), _ptr(kinds), n),
+               "mcgpu_get_cutpath_edges")
+        return dict(src=src, tgt=tgt, faces=faces, groups=grp, kinds=kinds, short_group=info.short_group,
+                    unsupported=info.unsupported, num_groups=info.num_groups, num_candidates=info.num_candidates,
+                    device_ms=info.device_ms)
 
     def stage_ms(self):
         """Device time of each pipeline stage of this dispatch, in ms (HIP events);

@@ -33,6 +33,12 @@ mcgpu_context seam_context()
     return g_ctx;
 }
 
+McgpuSeamCutpath& seam_cutpath()
+{
+    thread_local McgpuSeamCutpath cp;
+    return cp;
+}
+
 void check(mcgpu_result r, const char* what)
 {
     if (r != MCGPU_SUCCESS) throw std::runtime_error(std::string(what) + ": " + mcgpu_result_string(r));
@@ -104,6 +110,7 @@ bool mcgpu_seam_frontend(const input_t& input, output_t& output, const hmesh_t& 
                          std::unordered_map<fd_t, std::vector<vd_t>>& face_ivertices,
                          bool& partial_cut)
 {
+    seam_cutpath() = McgpuSeamCutpath(); // nothing from an earlier dispatch
     hmesh_t& ps = const_cast<hmesh_t&>(ps_const); // is_border(edge) is not a const member
     logger_t& lg = output.logger;
     const int sm_vtx_cnt = sm.number_of_vertices();
@@ -163,9 +170,10 @@ bool mcgpu_seam_frontend(const input_t& input, output_t& output, const hmesh_t& 
     std::vector<uint32_t> re, rf, pf, v0;
     std::vector<double> rp, pn, pd;
     std::vector<int32_t> pk;
+    // the context is held until the cut-path edges of this dispatch are read
+    std::unique_lock<std::mutex> lock(g_mutex);
+    mcgpu_context ctx = seam_context();
     {
-        std::lock_guard<std::mutex> lock(g_mutex);
-        mcgpu_context ctx = seam_context();
         check(mcgpu_dispatch_narrow(ctx, &ms, &mc, sbox.data(), cbox.data(), pairs.data(), pairs.size() / 2, &topo, &desc,
                                     &info),
               "mcgpu_dispatch_narrow");
@@ -268,6 +276,29 @@ bool mcgpu_seam_frontend(const input_t& input, output_t& output, const hmesh_t& 
         if (order.size() != re.size()) throw std::logic_error("libmcgpu seam: a registry edge outside the tested edges");
     }
 
+    // the cut-path edges of this registry in MCUT's point order (kernel.cpp:3427-3615),
+    // applied to m0 by the copy of dispatch() (cutpath_block.inc)
+    {
+        McgpuSeamCutpath& cp = seam_cutpath();
+        cp = McgpuSeamCutpath();
+        if (!order.empty()) {
+            std::vector<uint32_t> ord(order.begin(), order.end());
+            mcgpu_cutpath_info ci;
+            check(mcgpu_cutpath_edges(ctx, ord.data(), &ci), "mcgpu_cutpath_edges");
+            if (ci.unsupported) throw std::runtime_error("libmcgpu seam: a cut-path midpoint test names a face without a plane");
+            cp.short_group = ci.short_group != 0;
+            const size_t ne = ci.num_edges;
+            cp.src.resize(ne), cp.tgt.resize(ne), cp.kind.resize(ne);
+            std::vector<uint32_t> faces(2 * ne);
+            if (ne)
+                check(mcgpu_get_cutpath_edges(ctx, cp.src.data(), cp.tgt.data(), faces.data(), nullptr, cp.kind.data(), ne),
+                      "mcgpu_get_cutpath_edges");
+            cp.fa.resize(ne), cp.fb.resize(ne);
+            for (size_t i = 0; i < ne; ++i) cp.fa[i] = faces[2 * i], cp.fb[i] = faces[2 * i + 1];
+        }
+    }
+    lock.unlock();
+
     // the registry and its side maps (kernel.cpp:2603-2654): intersection
     // point k (in that order) is vertex ps_vtx_cnt + k of m0
     for (size_t k = 0; k < order.size(); ++k) {
@@ -310,3 +341,5 @@ bool mcgpu_seam_frontend(const input_t& input, output_t& output, const hmesh_t& 
     }
     return false;
 }
+
+const McgpuSeamCutpath& mcgpu_seam_cutpath_result() { return seam_cutpath(); }

@@ -4,6 +4,7 @@
 // "Calculate polygon intersection points" through the empty-registry check).
 #pragma once
 
+#include <cstdint>
 #include <map>
 #include <unordered_map>
 #include <utility>
@@ -37,3 +38,16 @@ bool mcgpu_seam_frontend(const input_t& input, output_t& output, const hmesh_t& 
                          std::map<pair<fd_t>, std::vector<vd_t>>& cutpath_edges,
                          std::unordered_map<fd_t, std::vector<vd_t>>& face_ivertices,
                          bool& partial_cut);
+
+// The cut-path edges libmcgpu made for the last mcgpu_seam_frontend call of
+// this thread (kernel.cpp:3427-3615), in creation order: points as ranks of
+// the registry (vertex ps_vtx_cnt + rank of m0), the two faces each edge is
+// filed under, and its kind (0: a group of two points, whose faces' neighbours
+// are updated too; 1: consecutive points of a longer group).  short_group: a
+// group held a single point.
+struct McgpuSeamCutpath {
+    bool short_group = false;
+    std::vector<uint32_t> src, tgt, fa, fb;
+    std::vector<uint8_t> kind;
+};
+const McgpuSeamCutpath& mcgpu_seam_cutpath_result();

@@ -9,6 +9,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SEAM_LIB = os.path.join(HERE, "_build", "libmcut_seam.so")
+GPU_LIB = os.path.join(HERE, "_build", "libmcut_gpu.so")      # front end and cut-path edges on libmcgpu
+PROBE_LIB = os.path.join(HERE, "_build", "libmcut_probe.so")  # MCUT unchanged, cut-path edges recorded
 
 # mcut.h:202-208, 367-441, 213-224, 292-306
 MC_NO_ERROR = 0
@@ -50,6 +52,27 @@ class Mcut:
                   "mcGetConnectedComponentData", "mcReleaseConnectedComponents", "mcReleaseContext"):
             getattr(L, f).restype = C.c_int
         self.L = L
+
+    def cutpath_record(self):
+        """The cut-path edges of this library's last dispatch that made them
+        (builds with the cut-path recorder: libmcut_gpu.so, libmcut_probe.so):
+        the registry in the dispatch's point order (edge, face), the edges as
+        (source, target) point ranks in creation order, and every face's edge
+        list as (face, edge) rows, faces ascending."""
+        L = self.L
+        if not hasattr(L, "mcut_probe_cutpath_sizes"):
+            raise McutError(-1, "this build records no cut-path edges")
+        u64 = C.c_uint64
+        L.mcut_probe_cutpath_sizes.argtypes = [C.POINTER(u64)] * 3
+        L.mcut_probe_cutpath_sizes.restype = u64
+        L.mcut_probe_cutpath_copy.argtypes = [C.c_void_p] * 5
+        npt, ne, nfe = u64(0), u64(0), u64(0)
+        count = L.mcut_probe_cutpath_sizes(C.byref(npt), C.byref(ne), C.byref(nfe))
+        re, rf = np.zeros(npt.value, np.uint32), np.zeros(npt.value, np.uint32)
+        src, tgt = np.zeros(ne.value, np.uint32), np.zeros(ne.value, np.uint32)
+        fe = np.zeros((nfe.value, 2), np.uint32)
+        L.mcut_probe_cutpath_copy(re.ctypes.data, rf.ctypes.data, src.ctypes.data, tgt.ctypes.data, fe.ctypes.data)
+        return dict(dispatches=count, reg_edge=re, reg_face=rf, src=src, tgt=tgt, face_edges=fe)
 
     def _ok(self, code, what):
         if code != MC_NO_ERROR:

@@ -1351,3 +1351,164 @@ void om_copy_ps_vertices(const om_result* R, double* xyz)
 {
     memcpy(xyz, R->ps_v, sizeof(double) * 3 * R->Vps);
 }
+
+/* ======================================================================
+ * Cut-path edges, kernel.cpp:3427-3615 (grouping: kernel.cpp:2616-2631;
+ * linear_projection_sort: kernel.cpp:1496-1531; registry faces:
+ * ps_get_ivtx_registry_entry_faces, kernel.cpp:1407-1415)
+ * ====================================================================== */
+
+typedef struct {
+    uint32_t sm, cm, k;
+} om_item;
+
+static int cmp_item(const void* a, const void* b)
+{
+    const om_item* x = (const om_item*)a;
+    const om_item* y = (const om_item*)b;
+    if (x->sm != y->sm) return x->sm < y->sm ? -1 : 1;
+    if (x->cm != y->cm) return x->cm < y->cm ? -1 : 1;
+    return x->k < y->k ? -1 : (x->k > y->k);
+}
+
+static int64_t plane_slot(const om_result* R, uint32_t g)
+{
+    int64_t lo = 0, hi = R->ncand - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (R->cand[mid] < g) lo = mid + 1;
+        else hi = mid;
+    }
+    return (R->ncand > 0 && R->cand[lo] == g) ? lo : -1;
+}
+
+/* The cut-path edges of R's registry taken in `order` (order[k] = registry
+ * index of point k; NULL: registry order).  Writes up to `cap` edges (source
+ * and target ranks, the two faces each is filed under, its group and kind)
+ * and returns their count; *short_group = 1 when a group holds one point
+ * (the dispatch stops there and no edge is meaningful), -1 on a midpoint
+ * test against a face without a plane. */
+int64_t om_cutpath(const om_result* R, const uint32_t* order, uint32_t* src, uint32_t* tgt, uint32_t* faces,
+                   uint32_t* groups, uint8_t* kinds, int64_t cap, int* short_group)
+{
+    const int64_t n = R->nreg;
+    *short_group = 0;
+    om_item* it = (om_item*)malloc(sizeof(om_item) * (size_t)(2 * n + 1));
+    int64_t ni = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        const uint64_t i = order ? order[k] : (uint64_t)k;
+        const uint32_t e = (uint32_t)(R->reg[i] >> 32), g = (uint32_t)R->reg[i];
+        const int32_t f0 = R->edges[e].f0, f1 = R->edges[e].f1;
+        const int32_t ef = f0 >= 0 ? f0 : f1;
+        if ((uint32_t)ef >= R->Fs) { /* a cut-mesh edge through source face g */
+            it[ni++] = (om_item){g, (uint32_t)ef, (uint32_t)k};
+            const int32_t other = ef == f0 ? f1 : -1;
+            if (other >= 0) it[ni++] = (om_item){g, (uint32_t)other, (uint32_t)k};
+        } else { /* a source-mesh edge through cut face g */
+            it[ni++] = (om_item){(uint32_t)ef, g, (uint32_t)k};
+            const int32_t other = ef == f0 ? f1 : f0;
+            if (other >= 0) it[ni++] = (om_item){(uint32_t)other, g, (uint32_t)k};
+        }
+    }
+    qsort(it, (size_t)ni, sizeof(om_item), cmp_item); /* std::map order; points in registry order */
+    /* made edges: a hash on the (min, max) rank pair */
+    om_hash made;
+    hash_init(&made, (uint64_t)ni + 1);
+    int64_t ne = 0;
+    uint32_t group = 0;
+    double* proj = (double*)malloc(sizeof(double) * (size_t)(ni + 1));
+    int* ord = (int*)malloc(sizeof(int) * (size_t)(ni + 1));
+    uint32_t* pts = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(ni + 1));
+    for (int64_t a = 0; a < ni; ++group) {
+        int64_t b = a;
+        while (b < ni && it[b].sm == it[a].sm && it[b].cm == it[a].cm) ++b;
+        const int64_t len = b - a;
+        const uint32_t sm = it[a].sm, cm = it[a].cm;
+        if (len < 2) {
+            *short_group = 1;
+            break;
+        }
+        for (int64_t q = 0; q < len; ++q) pts[q] = it[a + q].k;
+        if (len > 2) { /* linear_projection_sort */
+            const uint32_t i0 = order ? order[pts[0]] : pts[0];
+            const uint32_t i1 = order ? order[pts[1]] : pts[1];
+            const double* o = R->reg_p + 3 * (size_t)i0;
+            const double* d = R->reg_p + 3 * (size_t)i1;
+            double v[3], u[3], sq = 0.0;
+            for (int j = 0; j < 3; ++j) v[j] = o[j] - d[j];
+            for (int j = 0; j < 3; ++j) sq = sq + v[j] * v[j];
+            const double lv = sqrt(sq);
+            for (int j = 0; j < 3; ++j) u[j] = v[j] / lv;
+            for (int64_t q = 0; q < len; ++q) {
+                const uint32_t iq = order ? order[pts[q]] : pts[q];
+                double w[3], pr = 0.0;
+                for (int j = 0; j < 3; ++j) w[j] = o[j] - R->reg_p[3 * (size_t)iq + j];
+                for (int j = 0; j < 3; ++j) pr = pr + w[j] * u[j];
+                proj[q] = -pr; /* the replayed std::sort orders descending keys: negated, ascending */
+            }
+            om_trip_sort_keys(proj, (long)len, ord);
+            uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)len);
+            for (int64_t q = 0; q < len; ++q) tmp[q] = pts[ord[q]];
+            memcpy(pts, tmp, sizeof(uint32_t) * (size_t)len);
+            free(tmp);
+        }
+        for (int64_t q = 1; q < len; ++q) {
+            const uint32_t s_ = pts[q - 1], t_ = pts[q];
+            const uint64_t key = s_ < t_ ? ((uint64_t)s_ << 32 | t_) : ((uint64_t)t_ << 32 | s_);
+            int found = 0;
+            uint32_t* slot = hash_slot(&made, key, &found);
+            if (!found) *slot = 0;     /* a pair seen for the first time: not made yet */
+            if (*slot) continue;       /* m0.halfedge(src, tgt) exists */
+            uint32_t fa = sm, fb = cm;
+            if (len > 2) {
+                const uint32_t ia = order ? order[s_] : s_, ib = order ? order[t_] : t_;
+                int32_t FA[3], FB[3];
+                FA[0] = (int32_t)(uint32_t)R->reg[ia], FA[1] = R->edges[R->reg[ia] >> 32].f0, FA[2] = R->edges[R->reg[ia] >> 32].f1;
+                FB[0] = (int32_t)(uint32_t)R->reg[ib], FB[1] = R->edges[R->reg[ib] >> 32].f0, FB[2] = R->edges[R->reg[ib] >> 32].f1;
+                double mid[3];
+                for (int j = 0; j < 3; ++j) mid[j] = (R->reg_p[3 * (size_t)ib + j] + R->reg_p[3 * (size_t)ia + j]) * 0.5;
+                int inside = 0;
+                uint32_t in_face[3] = {0, 0, 0};
+                for (int x = 0; x < 3; ++x) {
+                    const int32_t f = FA[x];
+                    if (f < 0 || !(f == FB[0] || f == FB[1] || f == FB[2])) continue;
+                    const int64_t ps = plane_slot(R, (uint32_t)f);
+                    if (ps < 0) {
+                        *short_group = -1;
+                        continue;
+                    }
+                    const uint32_t nv = R->face_off[f + 1] - R->face_off[f];
+                    double* P = (double*)malloc(sizeof(double) * 3 * nv);
+                    for (uint32_t j = 0; j < nv; ++j)
+                        memcpy(P + 3 * j, R->ps_v + 3 * (size_t)R->face_vtx[R->face_off[f] + j], 3 * sizeof(double));
+                    const char res = om_point_in_polygon(mid, P, (int)nv, R->plane_n + 3 * ps, R->plane_k[ps]);
+                    free(P);
+                    if (res == 'i') {
+                        if (inside < 3) in_face[inside] = (uint32_t)f;
+                        ++inside;
+                    }
+                }
+                if (inside != 2) continue;
+                fa = in_face[0];
+                fb = in_face[1];
+            }
+            *slot = 1; /* the edge is made */
+            if (ne < cap) {
+                if (src) src[ne] = s_;
+                if (tgt) tgt[ne] = t_;
+                if (faces) faces[2 * ne] = fa, faces[2 * ne + 1] = fb;
+                if (groups) groups[ne] = group;
+                if (kinds) kinds[ne] = len > 2 ? 1 : 0;
+            }
+            ++ne;
+        }
+        a = b;
+    }
+    free(made.key);
+    free(made.val);
+    free(proj);
+    free(ord);
+    free(pts);
+    free(it);
+    return ne;
+}

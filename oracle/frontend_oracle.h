@@ -79,6 +79,9 @@ char om_segment_type(const double* q, const double* r, const double* verts, int 
                      int8_t* rs);
 void om_segment_point(const double* normal, double d, const double* q, const double* r, double* p);
 char om_point_in_polygon(const double* p, const double* verts, int n, const double* normal, int k);
+/* cut-path edges of the registry in the given order (kernel.cpp:3427-3615) */
+int64_t om_cutpath(const om_result* r, const uint32_t* order, uint32_t* src, uint32_t* tgt, uint32_t* faces,
+                   uint32_t* groups, uint8_t* kinds, int64_t cap, int* short_group);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,9 @@ def lib():
         L.om_point_in_polygon.restype = C.c_char
         L.om_segment_point.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
         L.om_best_triple.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+        L.om_cutpath.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_int64, C.POINTER(C.c_int)]
+        L.om_cutpath.restype = C.c_int64
         L.om_best_triple.restype = C.c_int
         _lib = L
     return _lib
@@ -90,7 +93,10 @@ class Result(dict):
 
 
 def frontend(src, cut, gp_constant=1e-4, enforce_gp=False, eps_absolute=False, enforcement_count=0,
-             require_through_cuts=False, perturbation=None, detail=True):
+             require_through_cuts=False, perturbation=None, detail=True, cutpath=None):
+    """The oracle's front end.  cutpath: None (skip), True (the cut-path
+    edges of the registry in its own order) or an order array (order[k] =
+    registry index of point k): R["cutpath"] holds them (om_cutpath)."""
     L = lib()
     keep = []
     ms, mc = _mesh(src, keep), _mesh(cut, keep)
@@ -143,6 +149,20 @@ def frontend(src, cut, gp_constant=1e-4, enforce_gp=False, eps_absolute=False, e
             psv = np.zeros((nv, 3))
             L.om_copy_ps_vertices(h, _ptr(psv))
             R["ps_vertices"] = psv
+        if cutpath is not None:
+            nr = n["registry"]
+            order = None
+            if cutpath is not True:
+                order = np.ascontiguousarray(cutpath, np.uint32)
+                assert order.size == nr
+            cap = 2 * nr + 1
+            cs, ct, cg = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+            cf, ck = np.zeros((cap, 2), np.uint32), np.zeros(cap, np.uint8)
+            sg = C.c_int(0)
+            ne = L.om_cutpath(h, None if order is None else _ptr(order), _ptr(cs), _ptr(ct), _ptr(cf), _ptr(cg),
+                              _ptr(ck), cap, C.byref(sg))
+            R["cutpath"] = dict(src=cs[:ne], tgt=ct[:ne], faces=cf[:ne], groups=cg[:ne], kinds=ck[:ne],
+                                short_group=sg.value)
         return R
     finally:
         L.om_free(h)

@@ -51,6 +51,8 @@ def _c_layout(tmp_path):
         "mcgpu_server_limits": ["max_cut_vertices", "max_cut_faces", "max_cut_corners", "max_slots", "hbm_budget"],
         "mcgpu_request_result": ["arrival_s", "start_s", "done_s", "status", "partial_cut", "num_registry",
                                  "num_candidate_pairs"],
+        "mcgpu_cutpath_info": ["num_edges", "num_groups", "num_candidates", "short_group", "unsupported",
+                               "device_ms", "reserved"],
         "mcgpu_tuning": ["wave_leaves", "tree_stride", "queues", "chunk", "broad_share_pct", "candidates",
                          "traversal_ms", "default_traversal_ms", "dispatch_ms", "default_dispatch_ms"],
     }
@@ -78,7 +80,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     py = {"mcgpu_mesh": frontend.Mesh, "mcgpu_dispatch_desc": frontend.DispatchDesc,
           "mcgpu_frontend_info": frontend.FrontendInfo, "mcgpu_batch": frontend.Batch,
           "mcgpu_server_limits": frontend.ServerLimits, "mcgpu_request_result": frontend.RequestResult,
-          "mcgpu_tuning": frontend.Tuning}
+          "mcgpu_tuning": frontend.Tuning, "mcgpu_cutpath_info": frontend.CutpathInfo}
     fields, res = _c_layout(tmp_path)
     for s, fs in fields.items():
         T = py[s]
@@ -116,6 +118,8 @@ def test_null_arguments_rejected_without_a_device():
     assert L.mcgpu_get_tuning(None, None) == -1
     assert L.mcgpu_set_graphs(None, 1) == -1
     assert L.mcgpu_set_batch_split(None, 50) == -1
+    assert L.mcgpu_cutpath_edges(None, None, None) == -1
+    assert L.mcgpu_get_cutpath_edges(None, None, None, None, None, None, 0) == -1
     assert L.mcgpu_graph_stats(None, None, None, None) == -1
 
 

@@ -5,7 +5,8 @@ MCUT compiled unchanged, through MCUT's public API on the same inputs.
   connected component identical (type, vertex coordinates bit for bit, faces,
   face sizes, seam vertices).
 * The whole front end (the bvh.cpp seam plus dispatch()'s front-end block,
-  kernel.cpp:1779-3231, over libmcgpu): the seam hands MCUT's cutting stage
+  kernel.cpp:1779-3231, and its cut-path edge loop, kernel.cpp:3427-3615,
+  over libmcgpu): the seam hands MCUT's cutting stage
   the registry in the order MCUT's own narrow phase would have produced it,
   so the components are identical too: first in canonical form (the same
   components, each with the same type, the same vertex coordinates bit for
