@@ -1041,6 +1041,66 @@ __global__ void k_edge_assign_batch(BatchTopo T, const uint32_t* __restrict__ ei
     }
 }
 
+// Single-component check of each input mesh (check_input_mesh,
+// preproc.cpp:505-578 with find_connected_components, kernel.cpp:235-364):
+// the reference walks the vertex graph breadth first from every unvisited
+// vertex, so every vertex, a vertex no face uses included, belongs to exactly
+// one component, and a mesh of more than one is refused (MC_INVALID_VALUE).
+// Here a union-find over the soup's edges: every root links to the smaller of
+// two roots by compare-and-swap, finds halve their paths, and the roots left
+// are counted per mesh.
+__global__ void k_cc_init(uint32_t* __restrict__ parent, uint64_t n)
+{
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        parent[v] = (uint32_t)v;
+}
+
+__device__ __forceinline__ uint32_t cc_find(uint32_t* parent, uint32_t v)
+{
+    for (;;) {
+        const uint32_t p = __hip_atomic_load(&parent[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p == v) return v;
+        const uint32_t gp = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (gp != p) atomicCAS(&parent[v], p, gp); // path halving; losing the race is harmless
+        v = p;
+    }
+}
+
+// cut_only: the source's edges are skipped (a resident source, checked once)
+__global__ void k_cc_union(const uint32_t* __restrict__ ev0, const uint32_t* __restrict__ ev1, const Counters* C,
+                           int cut_only, uint32_t* parent)
+{
+    const uint64_t ne = C->n_edges_src + C->n_edges_cut;
+    for (uint64_t e = (cut_only ? C->n_edges_src : 0) + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < ne;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t a = ev0[e], b = ev1[e];
+        for (;;) {
+            a = cc_find(parent, a);
+            b = cc_find(parent, b);
+            if (a == b) break;
+            const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+            if (atomicCAS(&parent[hi], hi, lo) == hi) break; // hi was still a root: linked
+        }
+    }
+}
+
+// roots of [0, nv0) and [nv0, n) (a root's parent is itself once every union
+// is done); vertices below `start` are not counted
+__global__ void k_cc_count(const uint32_t* __restrict__ parent, uint64_t start, uint64_t nv0, uint64_t n, Counters* C)
+{
+    unsigned c0 = 0, c1 = 0;
+    for (uint64_t v = start + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        if (parent[v] == (uint32_t)v) (v < nv0 ? c0 : c1)++;
+    for (int o = 32; o > 0; o >>= 1) {
+        c0 += __shfl_xor(c0, o, 64);
+        c1 += __shfl_xor(c1, o, 64);
+    }
+    if (__lane_id() == 0) {
+        if (c0) atomicAdd(&C->ncc[0], c0);
+        if (c1) atomicAdd(&C->ncc[1], c1);
+    }
+}
+
 // ===========================================================================
 // 4. face AABBs (bvh.cpp:242-313) and the LBVH over them
 // ===========================================================================

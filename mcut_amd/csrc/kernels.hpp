@@ -46,6 +46,7 @@ struct Counters {
     unsigned int sum_next; // batches: work-queue head of the serial sums
     unsigned int ov_trip;  // the triplet scratch overflowed
     unsigned int trav_q[8]; // persistent traversal: head of each work queue (blocks taken)
+    unsigned int ncc[2];    // connected components of each mesh's vertex graph (all pairs of a batch)
     // raw bounding box of each mesh as order keys (lo xyz, hi xyz); written
     // while the input is validated and kept by the per-dispatch counter reset
     unsigned long long rbox[2][6];

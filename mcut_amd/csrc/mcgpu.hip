@@ -201,6 +201,8 @@ struct mcgpu_context_t {
     uint32_t npairs = 0;
     int batch_mb = 7;                    // Morton bits per axis within a pair
     bool batch_packed = false;           // every pair's vertex ids and corner ranks fit the packed edge word
+    bool cc_check = false;               // count each mesh's connected components (check_input_mesh)
+    Buf cc_parent;                       // union-find parents of the soup's vertices
     int batch_share_pct = (int)(kBatchBroadShare * 100.0 + 0.5); // split of the batch streams (made on the first batch)
     std::vector<BatchPair> hpairs;       // pair ranges (host copy, edge ranges read back)
     std::vector<uint32_t> hstarts, horder;
@@ -453,6 +455,7 @@ mcgpu_result alloc_mesh_buffers(mcgpu_context ctx)
     const uint64_t K2 = ctx->ncorner[0] + ctx->ncorner[1];
     CK(ensure(ctx->eflag[0], K2 + 8, &u));
     CK(ensure(ctx->eid[0], K2 + 8, &u));
+    if (ctx->cc_check) CK(ensure(ctx->cc_parent, (uint64_t)ctx->nv[0] + ctx->nv[1] + 8, &u));
     CK(ensure(ctx->scan_tmp_c, scan_tmp_elems(K2 + 8) + 1024, &u));
     return MCGPU_SUCCESS;
 }
@@ -1061,6 +1064,15 @@ void enqueue_topology(mcgpu_context ctx, hipStream_t s3)
             P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
     }
     }
+    if (ctx->cc_check) { // one connected component per input mesh (per pair in a batch)
+        // a resident source was checked once: only the cut mesh
+        const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1], v0 = ctx->res ? ctx->nv[0] : 0;
+        uint32_t* parent = P_<uint32_t>(ctx->cc_parent);
+        k_cc_init<<<grid_for(V), kBlock, 0, s3>>>(parent, V);
+        k_cc_union<<<grid_for(K), kBlock, 0, s3>>>(P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), C, ctx->res ? 1 : 0,
+                                                   parent);
+        k_cc_count<<<grid_for(V - v0), kBlock, 0, s3>>>(parent, v0, ctx->nv[0], V, C);
+    }
     if (ctx->batch)
         k_batch_edge_bases<<<grid_for(ctx->npairs), kBlock, 0, s3>>>(
             P_<BatchPair>(ctx->bpairs), ctx->npairs, P_<uint32_t>(ctx->off[0]), P_<uint32_t>(ctx->off[1]), ctx->nf[0],
@@ -1208,6 +1220,10 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
                 (unsigned long long)ctx->cap_tests, (unsigned long long)ctx->cap_esc, (unsigned long long)ctx->cap_reg,
                 C.n_fallback, C.n_lane_walks);
     if (C.bad_input || C.nonmanifold) return MCGPU_ERROR_INVALID_VALUE;
+    if (ctx->cc_check) { // check_input_mesh: one component per mesh (per pair of a batch)
+        const uint32_t want = ctx->batch ? ctx->npairs : 1u;
+        if ((!ctx->res && C.ncc[0] != want) || C.ncc[1] != want) return MCGPU_ERROR_INVALID_VALUE;
+    }
     if (C.unsupported & 2u) return MCGPU_ERROR_UNSUPPORTED;
     if (C.ov_pairs || C.ov_tests || C.ov_esc || C.ov_reg || C.ov_big || C.ov_trip) {
         *overflow = true;
@@ -1323,7 +1339,7 @@ std::vector<Buf*> context_bufs(mcgpu_context ctx)
                   &ctx->brbox, &ctx->pflags, &ctx->border, &ctx->bidx[0], &ctx->bidx[1], &ctx->rle, &ctx->rlg,
                   &ctx->rstart, &ctx->rend, &ctx->pqs, &ctx->prs, &ctx->pcounts, &ctx->sbsum,
                   &ctx->sitems, &ctx->slo, &ctx->shi, &ctx->snitem, &ctx->slist, &ctx->sckpt, &ctx->sflags,
-                  &ctx->tune_c, &ctx->cp_order, &ctx->cp_keys, &ctx->cp_vals, &ctx->cp_keys2, &ctx->cp_vals2,
+                  &ctx->tune_c, &ctx->cc_parent, &ctx->cp_order, &ctx->cp_keys, &ctx->cp_vals, &ctx->cp_keys2, &ctx->cp_vals2,
                   &ctx->cp_flag, &ctx->cp_gid, &ctx->cp_gstart, &ctx->cp_ncand, &ctx->cp_coff, &ctx->cp_trip,
                   &ctx->cp_src, &ctx->cp_tgt, &ctx->cp_fa, &ctx->cp_fb, &ctx->cp_group, &ctx->cp_kind, &ctx->cp_acc,
                   &ctx->cp_table, &ctx->cp_keep, &ctx->cp_pos, &ctx->cp_osrc, &ctx->cp_otgt, &ctx->cp_ofa,
@@ -1339,7 +1355,7 @@ std::vector<uint64_t> graph_key(mcgpu_context ctx)
     std::vector<uint64_t> k;
     auto put = [&k](uint64_t x) { k.push_back(x); };
     auto ptr = [&k](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
-    put(ctx->batch), put(ctx->npairs), put(ctx->batch_mb), put(ctx->batch_packed), put(ctx->sort_bits),
+    put(ctx->batch), put(ctx->npairs), put(ctx->batch_mb), put(ctx->batch_packed), put(ctx->cc_check), put(ctx->sort_bits),
         put(ctx->gbits), put(ctx->kbits);
     for (int m = 0; m < 2; ++m) {
         put(ctx->nv[m]), put(ctx->nf[m]), put(ctx->ncorner[m]), put(ctx->depth[m]);
@@ -1750,6 +1766,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = false;
+    ctx->cc_check = true;
     ctx->sort_bits = 22;
     ctx->flags = desc->flags;
     ctx->gp_constant = desc->gp_constant;
@@ -1807,6 +1824,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = true;
+    ctx->cc_check = true;
     ctx->npairs = np;
     ctx->batch_mb = std::min(7, (31 - pb) / 3);
     ctx->sort_bits = 32;
@@ -1966,6 +1984,7 @@ MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_bo
     ctx->have = false;
     ctx->broad_only = true;
     ctx->batch = false;
+    ctx->cc_check = false;
     ctx->sort_bits = 22;
     ctx->trav = ctx->forced ? ctx->forced_trav : TravConfig{};
     ctx->nf[0] = num_src;
@@ -2600,6 +2619,7 @@ mcgpu_result slot_bind(mcgpu_server srv, mcgpu_context ctx, const mcgpu_mesh* cu
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = false;
+    ctx->cc_check = true;
     ctx->sort_bits = 22;
     ctx->flags = srv->desc.flags;
     ctx->gp_constant = srv->desc.gp_constant;
@@ -3096,6 +3116,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_narrow(mcgpu_context ctx, const mcgpu_mesh
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = false;
+    ctx->cc_check = false;
     ctx->sort_bits = 22;
     ctx->flags = desc->flags;
     ctx->gp_constant = desc->gp_constant > 0.0 ? desc->gp_constant : 1e-4;

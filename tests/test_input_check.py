@@ -1,0 +1,120 @@
+"""Single-component check of the input meshes (check_input_mesh,
+preproc.cpp:505-578; find_connected_components, kernel.cpp:235-364): MCUT
+refuses a source or cut mesh whose vertex graph has more than one connected
+component -- two separate shells, or a vertex no face uses -- with
+MC_INVALID_VALUE, and the GPU front end refuses the same inputs with
+MCGPU_ERROR_INVALID_VALUE.  The rule is pinned on MCUT itself (its unchanged
+build, CPU) and the GPU is held to it on the same inputs."""
+import os
+
+import numpy as np
+import pytest
+
+import meshes
+from mcut_amd import meshgen as mg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libmcut_ref.so")
+MC_INVALID_VALUE = -(1 << 2)  # mcut.h:205
+
+
+def two_shells(m, shift):
+    V, F, S = m
+    V = np.asarray(V, np.float64)
+    F = np.asarray(F, np.uint32)
+    V2 = np.concatenate([V, V + np.asarray(shift, np.float64)])
+    F2 = np.concatenate([F, F + V.shape[0]])
+    S2 = None if S is None else np.concatenate([S, S])
+    return V2, F2, S2
+
+
+def loose_vertex(m):
+    V, F, S = m
+    V = np.concatenate([np.asarray(V, np.float64), [[9.0, 9.0, 9.0]]])
+    return V, np.asarray(F, np.uint32), S
+
+
+def cases():
+    src, cut = meshes.config1_rotated()
+    yield "source_two_shells", two_shells(src, (5.0, 0.0, 0.0)), cut, True
+    yield "cut_two_shells", src, two_shells(cut, (0.0, 7.0, 0.0)), True
+    yield "source_loose_vertex", loose_vertex(src), cut, True
+    yield "cut_loose_vertex", src, loose_vertex(cut), True
+    yield "one_shell_each", src, cut, False
+
+
+@pytest.fixture(scope="module")
+def ref():
+    from mcut_amd.seam import mcut_api
+    if not os.path.exists(REF_LIB):
+        pytest.skip("the reference build is missing")
+    return mcut_api.Mcut(REF_LIB)
+
+
+@pytest.mark.parametrize("name,src,cut,refused", list(cases()), ids=[c[0] for c in cases()])
+def test_reference_refuses_more_than_one_component(ref, name, src, cut, refused):
+    from mcut_amd.seam import mcut_api
+    try:
+        ref.dispatch(src, cut)
+        code = 0
+    except mcut_api.McutError as e:
+        code = e.code
+    assert (code == MC_INVALID_VALUE) == refused, (name, code)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,src,cut,refused", list(cases()), ids=[c[0] for c in cases()])
+def test_gpu_refuses_more_than_one_component(name, src, cut, refused):
+    from mcut_amd import frontend
+    ctx = frontend.Context(0)
+    try:
+        print(f"input check: {name}", flush=True)
+        if refused:
+            with pytest.raises(frontend.McgpuError) as ei:
+                ctx.dispatch(src, cut)
+            assert ei.value.code == -1
+        R = ctx.dispatch(*meshes.config1_rotated())  # the context stays usable
+        assert R.status == 0 and R.counts["registry"] == 134
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_batch_with_a_two_shell_pair_is_refused():
+    from mcut_amd import frontend
+    src, cut = meshes.config1_rotated()
+    good = [meshes.bench_pair(0), (src, cut)]
+    ctx = frontend.Context(0)
+    try:
+        with pytest.raises(frontend.McgpuError) as ei:
+            ctx.dispatch_batch(good + [(two_shells(src, (5.0, 0.0, 0.0)), cut)])
+        assert ei.value.code == -1
+        B = ctx.dispatch_batch(good)
+        assert list(B.status) == [0, 0]
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_accepts_every_benchmark_pair_mcut_accepts(ref):
+    """No false refusals: the 64 benchmark pairs (polygons, open sheets) and
+    the seeded mixes pass the GPU check exactly when MCUT's passes."""
+    from mcut_amd import frontend
+    from mcut_amd.seam import mcut_api
+    ctx = frontend.Context(0)
+    try:
+        for i in range(64):
+            src, cut = meshes.bench_pair(i)
+            try:
+                ref.dispatch(src, cut)
+                mc = 0
+            except mcut_api.McutError as e:
+                mc = e.code
+            try:
+                ctx.dispatch(src, cut)
+                g = 0
+            except frontend.McgpuError as e:
+                g = e.code
+            assert (mc == MC_INVALID_VALUE) == (g == -1), (i, mc, g)
+    finally:
+        ctx.close()
