@@ -433,6 +433,13 @@ def bench_single(args, torch, frontend, mg):
     gctx.close()
     progress(f"single pair: graph replay {g_ms:.3f} ms per step ({graph_line['graph_launches']} graphs)")
 
+    # the optional single-component check (MCGPU_DISPATCH_CHECK_COMPONENTS):
+    # device time of a dispatch that also counts each mesh's components
+    cc_ms = []
+    for _ in range(3):
+        cc_ms.append(ctx.dispatch_device(ms, mc, flags=frontend.CHECK_COMPONENTS).device_ms)
+    R = ctx.dispatch_device(ms, mc)  # the timed dispatches' outputs again, for the checks below
+
     # per-stage device times (HIP events on the library's streams), gathered
     # from further dispatches outside the timed loop: the timed dispatches
     # record no stage events (each costs time)
@@ -483,6 +490,7 @@ def bench_single(args, torch, frontend, mg):
         "stage_ms": stages,
         "autotune": tuning,
         "graph_replay": graph_line,
+        "component_check_device_ms": min(cc_ms),
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "stage": dom, "kernel_ms": kernel_ms,
                      "stage_event_ms": stages[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -70,6 +70,12 @@ typedef enum {
 #define MCGPU_DISPATCH_REQUIRE_THROUGH_CUTS (1u << 2)
 #define MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION (1u << 15)
 #define MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE (1u << 16)
+/* libmcgpu's own: refuse (MCGPU_ERROR_INVALID_VALUE) an input mesh whose
+ * vertex graph has more than one connected component, as MCUT's
+ * check_input_mesh refuses it with MC_INVALID_VALUE (preproc.cpp:505-578).
+ * MCUT makes that check on the host before its front end, so a caller inside
+ * MCUT has it already; a standalone caller asks for it with this flag. */
+#define MCGPU_DISPATCH_CHECK_COMPONENTS (1u << 30)
 
 /* A mesh as mcDispatch takes it: xyz doubles per vertex, concatenated face
  * vertex indices, and face sizes (NULL = every face is a triangle,

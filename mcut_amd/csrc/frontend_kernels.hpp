@@ -1055,14 +1055,18 @@ __global__ void k_cc_init(uint32_t* __restrict__ parent, uint64_t n)
         parent[v] = (uint32_t)v;
 }
 
+// Parents are read with device-coherent loads (a link made on another XCD is
+// seen), so a failed link always finds the new root; path halving writes with
+// plain stores, only ever to a vertex that is no longer a root (which no link
+// touches) and only an ancestor of it.
 __device__ __forceinline__ uint32_t cc_find(uint32_t* parent, uint32_t v)
 {
     for (;;) {
         const uint32_t p = __hip_atomic_load(&parent[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (p == v) return v;
         const uint32_t gp = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (gp != p) atomicCAS(&parent[v], p, gp); // path halving; losing the race is harmless
-        v = p;
+        if (gp != p) parent[v] = gp;
+        v = gp;
     }
 }
 

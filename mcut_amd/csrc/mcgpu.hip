@@ -75,7 +75,7 @@ const char* kStageNames[ST_COUNT] = {"coordinate_sums", "shift_params", "shift_b
 // event slots: stage boundaries on the three streams
 enum Ev { EV_A0, EV_S0, EV_A_SUMS, EV_A_PARAMS, EV_A_SHIFT, EV_A_JOIN, EV_A_FILTER, EV_A_PLANES, EV_A_JOIN2, EV_A_TESTS, EV_A_NARROW,
           EV_A_END, EV_FORK, EV_B0, EV_B_SORT, EV_B_REFIT, EV_B_TRAV, EV_C0, EV_C_FILL, EV_C_TOPO, EV_SUM_ITEMS,
-          EV_SUM_CHAIN, EV_TRAV_K0, EV_TRAV_K1, EV_COUNT };
+          EV_SUM_CHAIN, EV_TRAV_K0, EV_TRAV_K1, EV_B_CC, EV_COUNT };
 
 // Launch shape of the traversal kernel, chosen per workload class by the
 // autotuner (mcgpu_autotune) or set by the caller (mcgpu_set_tuning).
@@ -664,6 +664,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
             P_<BatchPair>(ctx->bpairs), C, P_<uint32_t>(ctx->rle), P_<uint32_t>(ctx->rlg), P_<uint32_t>(ctx->rstart),
             P_<uint32_t>(ctx->rend));
     }
+    if (ctx->cc_check && ctx->stage_mode == 0) ev_wait(ctx, s, EV_B_CC); // the component counts are in
     ev_record(ctx, EV_A_END, s);
 }
 
@@ -1064,15 +1065,6 @@ void enqueue_topology(mcgpu_context ctx, hipStream_t s3)
             P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), P_<int32_t>(ctx->ef0), P_<int32_t>(ctx->ef1));
     }
     }
-    if (ctx->cc_check) { // one connected component per input mesh (per pair in a batch)
-        // a resident source was checked once: only the cut mesh
-        const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1], v0 = ctx->res ? ctx->nv[0] : 0;
-        uint32_t* parent = P_<uint32_t>(ctx->cc_parent);
-        k_cc_init<<<grid_for(V), kBlock, 0, s3>>>(parent, V);
-        k_cc_union<<<grid_for(K), kBlock, 0, s3>>>(P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), C, ctx->res ? 1 : 0,
-                                                   parent);
-        k_cc_count<<<grid_for(V - v0), kBlock, 0, s3>>>(parent, v0, ctx->nv[0], V, C);
-    }
     if (ctx->batch)
         k_batch_edge_bases<<<grid_for(ctx->npairs), kBlock, 0, s3>>>(
             P_<BatchPair>(ctx->bpairs), ctx->npairs, P_<uint32_t>(ctx->off[0]), P_<uint32_t>(ctx->off[1]), ctx->nf[0],
@@ -1177,6 +1169,22 @@ void enqueue_frontend(mcgpu_context ctx)
     // ---- stream C: polygon-soup edge numbering, and the tables the filter fills ----
     ev_wait(ctx, s3, EV_FORK);
     enqueue_topology(ctx, s3);
+
+    // ---- stream B, once its traversal and stream C's edge table are done:
+    // one connected component per input mesh (per pair in a batch), in the
+    // time B would otherwise idle while A filters and tests ----
+    if (ctx->cc_check) {
+        ev_wait(ctx, s2, EV_C_TOPO);
+        // a resident source was checked once: only the cut mesh
+        const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1], v0 = ctx->res ? ctx->nv[0] : 0;
+        const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
+        uint32_t* parent = P_<uint32_t>(ctx->cc_parent);
+        k_cc_init<<<grid_for(V), kBlock, 0, s2>>>(parent, V);
+        k_cc_union<<<grid_for(K), kBlock, 0, s2>>>(P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), C, ctx->res ? 1 : 0,
+                                                   parent);
+        k_cc_count<<<grid_for(V - v0), kBlock, 0, s2>>>(parent, v0, ctx->nv[0], V, C);
+        ev_record(ctx, EV_B_CC, s2);
+    }
 
     // ---- stream A: join the broad phase, exact filter, narrow phase ----
     ev_wait(ctx, s, EV_B_TRAV);
@@ -1766,7 +1774,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = false;
-    ctx->cc_check = true;
+    ctx->cc_check = (desc->flags & MCGPU_DISPATCH_CHECK_COMPONENTS) != 0;
     ctx->sort_bits = 22;
     ctx->flags = desc->flags;
     ctx->gp_constant = desc->gp_constant;
@@ -1824,7 +1832,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = true;
-    ctx->cc_check = true;
+    ctx->cc_check = (desc->flags & MCGPU_DISPATCH_CHECK_COMPONENTS) != 0;
     ctx->npairs = np;
     ctx->batch_mb = std::min(7, (31 - pb) / 3);
     ctx->sort_bits = 32;
@@ -2619,7 +2627,7 @@ mcgpu_result slot_bind(mcgpu_server srv, mcgpu_context ctx, const mcgpu_mesh* cu
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = false;
-    ctx->cc_check = true;
+    ctx->cc_check = (srv->desc.flags & MCGPU_DISPATCH_CHECK_COMPONENTS) != 0;
     ctx->sort_bits = 22;
     ctx->flags = srv->desc.flags;
     ctx->gp_constant = srv->desc.gp_constant;

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmcgpu.so")
 REQUIRE_THROUGH_CUTS = 1 << 2
 ENFORCE_GENERAL_POSITION = 1 << 15
 ENFORCE_GENERAL_POSITION_ABSOLUTE = 1 << 16
+CHECK_COMPONENTS = 1 << 30  # libmcgpu's own: refuse meshes of more than one connected component
 
 STATUS = {0: "SUCCESS", -1: "INVALID_SRC_MESH", -2: "INVALID_CUT_MESH", -3: "INVALID_MESH_INTERSECTION",
           -4: "GENERAL_POSITION_VIOLATION"}

@@ -2,9 +2,11 @@
 preproc.cpp:505-578; find_connected_components, kernel.cpp:235-364): MCUT
 refuses a source or cut mesh whose vertex graph has more than one connected
 component -- two separate shells, or a vertex no face uses -- with
-MC_INVALID_VALUE, and the GPU front end refuses the same inputs with
+MC_INVALID_VALUE, and the GPU front end, asked to check
+(MCGPU_DISPATCH_CHECK_COMPONENTS), refuses the same inputs with
 MCGPU_ERROR_INVALID_VALUE.  The rule is pinned on MCUT itself (its unchanged
-build, CPU) and the GPU is held to it on the same inputs."""
+build, CPU) and the GPU is held to it on the same inputs; without the flag the
+GPU does not check (MCUT checks on the host before its front end)."""
 import os
 
 import numpy as np
@@ -71,9 +73,12 @@ def test_gpu_refuses_more_than_one_component(name, src, cut, refused):
         print(f"input check: {name}", flush=True)
         if refused:
             with pytest.raises(frontend.McgpuError) as ei:
-                ctx.dispatch(src, cut)
+                ctx.dispatch(src, cut, flags=frontend.CHECK_COMPONENTS)
             assert ei.value.code == -1
-        R = ctx.dispatch(*meshes.config1_rotated())  # the context stays usable
+            ctx.dispatch(src, cut)  # unchecked: accepted
+        else:
+            assert ctx.dispatch(src, cut, flags=frontend.CHECK_COMPONENTS).status == 0
+        R = ctx.dispatch(*meshes.config1_rotated(), flags=frontend.CHECK_COMPONENTS)  # the context stays usable
         assert R.status == 0 and R.counts["registry"] == 134
     finally:
         ctx.close()
@@ -87,9 +92,9 @@ def test_gpu_batch_with_a_two_shell_pair_is_refused():
     ctx = frontend.Context(0)
     try:
         with pytest.raises(frontend.McgpuError) as ei:
-            ctx.dispatch_batch(good + [(two_shells(src, (5.0, 0.0, 0.0)), cut)])
+            ctx.dispatch_batch(good + [(two_shells(src, (5.0, 0.0, 0.0)), cut)], flags=frontend.CHECK_COMPONENTS)
         assert ei.value.code == -1
-        B = ctx.dispatch_batch(good)
+        B = ctx.dispatch_batch(good, flags=frontend.CHECK_COMPONENTS)
         assert list(B.status) == [0, 0]
     finally:
         ctx.close()
@@ -111,7 +116,7 @@ def test_gpu_accepts_every_benchmark_pair_mcut_accepts(ref):
             except mcut_api.McutError as e:
                 mc = e.code
             try:
-                ctx.dispatch(src, cut)
+                ctx.dispatch(src, cut, flags=frontend.CHECK_COMPONENTS)
                 g = 0
             except frontend.McgpuError as e:
                 g = e.code
