@@ -1046,52 +1046,62 @@ __global__ void k_edge_assign_batch(BatchTopo T, const uint32_t* __restrict__ ei
 // the reference walks the vertex graph breadth first from every unvisited
 // vertex, so every vertex, a vertex no face uses included, belongs to exactly
 // one component, and a mesh of more than one is refused (MC_INVALID_VALUE).
-// Here a union-find over the soup's edges: every root links to the smaller of
-// two roots by compare-and-swap, finds halve their paths, and the roots left
-// are counted per mesh.
+// Here hooking and pointer jumping in rounds (Shiloach-Vishkin style), each
+// kernel reading what the previous one wrote: every edge whose ends have
+// different roots points the larger root at the smaller (a plain store;
+// when several edges hook one root, one of them wins and the others cross
+// again next round), then every vertex jumps to its root.  A round without a
+// hook ends the check, and the roots left are counted per mesh.  Every round
+// with a crossing edge removes a root, so rounds continue (the host enqueues
+// more) until none crosses.
 __global__ void k_cc_init(uint32_t* __restrict__ parent, uint64_t n)
 {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
         parent[v] = (uint32_t)v;
 }
 
-// Parents are read with device-coherent loads (a link made on another XCD is
-// seen), so a failed link always finds the new root; path halving writes with
-// plain stores, only ever to a vertex that is no longer a root (which no link
-// touches) and only an ancestor of it.
-__device__ __forceinline__ uint32_t cc_find(uint32_t* parent, uint32_t v)
-{
-    for (;;) {
-        const uint32_t p = __hip_atomic_load(&parent[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (p == v) return v;
-        const uint32_t gp = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (gp != p) parent[v] = gp;
-        v = gp;
-    }
-}
-
 // cut_only: the source's edges are skipped (a resident source, checked once)
-__global__ void k_cc_union(const uint32_t* __restrict__ ev0, const uint32_t* __restrict__ ev1, const Counters* C,
-                           int cut_only, uint32_t* parent)
+__global__ void k_cc_hook(const uint32_t* __restrict__ ev0, const uint32_t* __restrict__ ev1, Counters* C,
+                          int cut_only, uint32_t* parent)
 {
+    if (C->cc_done) return;
     const uint64_t ne = C->n_edges_src + C->n_edges_cut;
+    bool hooked = false;
     for (uint64_t e = (cut_only ? C->n_edges_src : 0) + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < ne;
          e += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t a = ev0[e], b = ev1[e];
-        for (;;) {
-            a = cc_find(parent, a);
-            b = cc_find(parent, b);
-            if (a == b) break;
-            const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
-            if (atomicCAS(&parent[hi], hi, lo) == hi) break; // hi was still a root: linked
+        const uint32_t pu = parent[ev0[e]], pv = parent[ev1[e]];
+        if (pu != pv) {
+            parent[pu > pv ? pu : pv] = pu > pv ? pv : pu;
+            hooked = true;
         }
+    }
+    if (__any(hooked) && __lane_id() == 0) C->cc_changed = 1u;
+}
+
+__global__ void k_cc_jump(uint32_t* parent, uint64_t start, uint64_t n, const Counters* C)
+{
+    if (C->cc_done) return;
+    for (uint64_t v = start + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t x = parent[v], y = parent[x];
+        while (x != y) {
+            x = y;
+            y = parent[x];
+        }
+        parent[v] = x;
     }
 }
 
-// roots of [0, nv0) and [nv0, n) (a root's parent is itself once every union
-// is done); vertices below `start` are not counted
+__global__ void k_cc_round_end(Counters* C)
+{
+    if (C->cc_done) return;
+    if (!C->cc_changed) C->cc_done = 1u;
+    C->cc_changed = 0u;
+}
+
+// roots of [start, nv0) and [nv0, n), once the rounds are done
 __global__ void k_cc_count(const uint32_t* __restrict__ parent, uint64_t start, uint64_t nv0, uint64_t n, Counters* C)
 {
+    if (!C->cc_done) return;
     unsigned c0 = 0, c1 = 0;
     for (uint64_t v = start + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
         if (parent[v] == (uint32_t)v) (v < nv0 ? c0 : c1)++;

@@ -47,6 +47,7 @@ struct Counters {
     unsigned int ov_trip;  // the triplet scratch overflowed
     unsigned int trav_q[8]; // persistent traversal: head of each work queue (blocks taken)
     unsigned int ncc[2];    // connected components of each mesh's vertex graph (all pairs of a batch)
+    unsigned int cc_changed, cc_done; // component check: a hook this round; no hook in the last round
     // raw bounding box of each mesh as order keys (lo xyz, hi xyz); written
     // while the input is validated and kept by the per-dispatch counter reset
     unsigned long long rbox[2][6];

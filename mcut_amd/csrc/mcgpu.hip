@@ -1073,6 +1073,24 @@ void enqueue_topology(mcgpu_context ctx, hipStream_t s3)
 
 }
 
+// Rounds of the component check (hook, jump, round end) and the root count,
+// on stream st; a resident source was checked once, so only the cut mesh.
+constexpr int kCcRounds = 6;
+void enqueue_cc_rounds(mcgpu_context ctx, hipStream_t st)
+{
+    Counters* C = &P_<DevState>(ctx->state)->C;
+    const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1], v0 = ctx->res ? ctx->nv[0] : 0;
+    const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
+    uint32_t* parent = P_<uint32_t>(ctx->cc_parent);
+    for (int r = 0; r < kCcRounds; ++r) {
+        k_cc_hook<<<grid_for(K), kBlock, 0, st>>>(P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), C, ctx->res ? 1 : 0,
+                                                  parent);
+        k_cc_jump<<<grid_for(V - v0), kBlock, 0, st>>>(parent, v0, V, C);
+        k_cc_round_end<<<1, 1, 0, st>>>(C);
+    }
+    k_cc_count<<<grid_for(V - v0), kBlock, 0, st>>>(parent, v0, ctx->nv[0], V, C);
+}
+
 void enqueue_frontend(mcgpu_context ctx)
 {
     hipStream_t s = ctx->s, s2 = ctx->batch ? ctx->s2b : ctx->s2, s3 = ctx->batch ? ctx->s3b : ctx->s3;
@@ -1175,14 +1193,9 @@ void enqueue_frontend(mcgpu_context ctx)
     // time B would otherwise idle while A filters and tests ----
     if (ctx->cc_check) {
         ev_wait(ctx, s2, EV_C_TOPO);
-        // a resident source was checked once: only the cut mesh
-        const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1], v0 = ctx->res ? ctx->nv[0] : 0;
-        const uint64_t K = ctx->ncorner[0] + ctx->ncorner[1];
         uint32_t* parent = P_<uint32_t>(ctx->cc_parent);
-        k_cc_init<<<grid_for(V), kBlock, 0, s2>>>(parent, V);
-        k_cc_union<<<grid_for(K), kBlock, 0, s2>>>(P_<uint32_t>(ctx->ev0), P_<uint32_t>(ctx->ev1), C, ctx->res ? 1 : 0,
-                                                   parent);
-        k_cc_count<<<grid_for(V - v0), kBlock, 0, s2>>>(parent, v0, ctx->nv[0], V, C);
+        k_cc_init<<<grid_for((uint64_t)ctx->nv[0] + ctx->nv[1]), kBlock, 0, s2>>>(parent, (uint64_t)ctx->nv[0] + ctx->nv[1]);
+        enqueue_cc_rounds(ctx, s2);
         ev_record(ctx, EV_B_CC, s2);
     }
 
@@ -1229,6 +1242,13 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
                 C.n_fallback, C.n_lane_walks);
     if (C.bad_input || C.nonmanifold) return MCGPU_ERROR_INVALID_VALUE;
     if (ctx->cc_check) { // check_input_mesh: one component per mesh (per pair of a batch)
+        // rounds until no edge crosses two components (each round removes a root)
+        for (uint64_t more = 0; !ctx->C.cc_done; ++more) {
+            if (more > ctx->nv[0] + (uint64_t)ctx->nv[1]) return MCGPU_ERROR_DEVICE;
+            enqueue_cc_rounds(ctx, ctx->s);
+            CK(hipStreamSynchronize(ctx->s));
+            CK(hipMemcpy(&ctx->C, &P_<DevState>(ctx->state)->C, sizeof(Counters), hipMemcpyDeviceToHost));
+        }
         const uint32_t want = ctx->batch ? ctx->npairs : 1u;
         if ((!ctx->res && C.ncc[0] != want) || C.ncc[1] != want) return MCGPU_ERROR_INVALID_VALUE;
     }
