@@ -381,6 +381,26 @@ MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity);
 MCGPU_API mcgpu_result mcgpu_set_stage_timing(mcgpu_context ctx, int enable);
 MCGPU_API const char* mcgpu_stage_name(int stage);
 
+/* The dispatch of mcgpu_dispatch_frontend, enqueued without waiting for it
+ * (mcEnqueueDispatch's contract, frontend.cpp:690-706: the call returns once
+ * the work is queued and the caller waits for its event).  stream (a
+ * hipStream_t, or NULL): the dispatch starts after the work queued so far on
+ * it, so device inputs written there are ready.  Host (and device) inputs
+ * must stay valid until mcgpu_dispatch_wait returns: a dispatch whose lists
+ * overflow is re-run from them there.  Until then, the context takes no other
+ * dispatch (MCGPU_ERROR_INVALID_VALUE) and its getters have no result. */
+MCGPU_API mcgpu_result mcgpu_dispatch_frontend_async(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
+                                                     const mcgpu_dispatch_desc* desc, void* stream);
+/* 1 when the enqueued dispatch has finished on the device (or none is
+ * pending), 0 while it runs; never blocks.  A list that overflowed is only
+ * seen, and the dispatch re-run with grown lists, by mcgpu_dispatch_wait. */
+MCGPU_API int mcgpu_dispatch_query(mcgpu_context ctx);
+/* Waits for the enqueued dispatch and completes it as mcgpu_dispatch_frontend
+ * would (info filled, results readable).  stream (or NULL): made to wait for
+ * the dispatch's end, so work queued there afterwards sees the registry.
+ * MCGPU_ERROR_NO_RESULT when nothing was enqueued. */
+MCGPU_API mcgpu_result mcgpu_dispatch_wait(mcgpu_context ctx, mcgpu_frontend_info* info, void* stream);
+
 /* Cut-path edges of the last successful mcgpu_dispatch_frontend or
  * mcgpu_dispatch_narrow (kernel.cpp:3427-3615, the first consumer of the
  * registry): the edges between intersection points along which each

@@ -202,6 +202,8 @@ struct mcgpu_context_t {
     int batch_mb = 7;                    // Morton bits per axis within a pair
     bool batch_packed = false;           // every pair's vertex ids and corner ranks fit the packed edge word
     bool cc_check = false;               // count each mesh's connected components (check_input_mesh)
+    bool pending = false;                // an enqueued dispatch (mcgpu_dispatch_frontend_async) not yet waited for
+    hipEvent_t ev_user = nullptr;        // recorded on the caller's stream before an enqueued dispatch
     Buf cc_parent;                       // union-find parents of the soup's vertices
     int batch_share_pct = (int)(kBatchBroadShare * 100.0 + 0.5); // split of the batch streams (made on the first batch)
     std::vector<BatchPair> hpairs;       // pair ranges (host copy, edge ranges read back)
@@ -1774,6 +1776,7 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
     for (Buf* b : context_bufs(ctx))
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
+    if (ctx->ev_user) (void)hipEventDestroy(ctx->ev_user);
     if (ctx->own_streams) {
         for (hipStream_t st : {ctx->ssum, ctx->s3, ctx->s2, ctx->s3b, ctx->s2b})
             if (st) (void)hipStreamDestroy(st);
@@ -1783,10 +1786,13 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
     return MCGPU_SUCCESS;
 }
 
-MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
-                                               const mcgpu_dispatch_desc* desc, mcgpu_frontend_info* info)
+// A single pair's dispatch up to its first enqueue: checks, uploads, buffers,
+// the traversal shape and the list capacities.
+static mcgpu_result frontend_prepare(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
+                                     const mcgpu_dispatch_desc* desc)
 {
     if (!ctx || !desc) return MCGPU_ERROR_INVALID_VALUE;
+    if (ctx->pending) return MCGPU_ERROR_INVALID_VALUE; // an enqueued dispatch has not been waited for
     mcgpu_result r;
     if ((r = check_mesh(src)) != MCGPU_SUCCESS || (r = check_mesh(cut)) != MCGPU_SUCCESS) return r;
     if (!(desc->gp_constant > 0.0)) return MCGPU_ERROR_INVALID_VALUE; // frontend.cpp:391-397
@@ -1817,20 +1823,75 @@ MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_me
         ctx->cap_reg = std::min<uint64_t>(std::max<uint64_t>(1 << 14, ctx->cap_tests / 8), kRegMergeTiles * kRegTile);
     if (ctx->cap_big < 1024) ctx->cap_big = 1024;
     if (ctx->cap_trip < (1u << 16)) ctx->cap_trip = 1u << 16;
-    for (int attempt = 0; attempt < 6; ++attempt) {
-        if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
-        ctx->stage_mode = 0;
-        if ((r = enqueue_dispatch(ctx)) != MCGPU_SUCCESS) return r;
+    return MCGPU_SUCCESS;
+}
+
+// Completes an enqueued single-pair dispatch: waits, reads its counters, and
+// re-runs it with grown lists while one overflowed.
+static mcgpu_result frontend_complete(mcgpu_context ctx, mcgpu_frontend_info* info, int attempt)
+{
+    mcgpu_result r;
+    for (;; ++attempt) {
         bool overflow = false;
         if ((r = finish(ctx, info, &overflow)) != MCGPU_SUCCESS) return r;
         if (!overflow) return autotune_first(ctx);
+        if (attempt + 1 >= 6) return MCGPU_ERROR_OUT_OF_MEMORY;
+        if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
+        ctx->stage_mode = 0;
+        if ((r = enqueue_dispatch(ctx)) != MCGPU_SUCCESS) return r;
     }
-    return MCGPU_ERROR_OUT_OF_MEMORY;
+}
+
+MCGPU_API mcgpu_result mcgpu_dispatch_frontend(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
+                                               const mcgpu_dispatch_desc* desc, mcgpu_frontend_info* info)
+{
+    mcgpu_result r;
+    if ((r = frontend_prepare(ctx, src, cut, desc)) != MCGPU_SUCCESS) return r;
+    if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
+    ctx->stage_mode = 0;
+    if ((r = enqueue_dispatch(ctx)) != MCGPU_SUCCESS) return r;
+    return frontend_complete(ctx, info, 0);
+}
+
+MCGPU_API mcgpu_result mcgpu_dispatch_frontend_async(mcgpu_context ctx, const mcgpu_mesh* src, const mcgpu_mesh* cut,
+                                                     const mcgpu_dispatch_desc* desc, void* stream)
+{
+    mcgpu_result r;
+    if ((r = frontend_prepare(ctx, src, cut, desc)) != MCGPU_SUCCESS) return r;
+    if ((r = alloc_list_buffers(ctx)) != MCGPU_SUCCESS) return r;
+    if (stream) { // the dispatch starts after the work queued so far on the caller's stream
+        if (!ctx->ev_user) CK(hipEventCreateWithFlags(&ctx->ev_user, hipEventDisableTiming));
+        CK(hipEventRecord(ctx->ev_user, (hipStream_t)stream));
+        CK(hipStreamWaitEvent(ctx->s, ctx->ev_user, 0));
+    }
+    ctx->stage_mode = 0;
+    if ((r = enqueue_dispatch(ctx)) != MCGPU_SUCCESS) return r;
+    ctx->pending = true;
+    return MCGPU_SUCCESS;
+}
+
+MCGPU_API int mcgpu_dispatch_query(mcgpu_context ctx)
+{
+    if (!ctx || !ctx->pending) return 1;
+    return hipEventQuery(ctx->ev[EV_A_END]) == hipSuccess ? 1 : 0;
+}
+
+MCGPU_API mcgpu_result mcgpu_dispatch_wait(mcgpu_context ctx, mcgpu_frontend_info* info, void* stream)
+{
+    if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
+    if (!ctx->pending) return MCGPU_ERROR_NO_RESULT;
+    CK(hipSetDevice(ctx->device));
+    ctx->pending = false;
+    const mcgpu_result r = frontend_complete(ctx, info, 0);
+    if (r == MCGPU_SUCCESS && stream) // work queued on the caller's stream from now on sees the results
+        CK(hipStreamWaitEvent((hipStream_t)stream, ctx->ev[EV_A_END], 0));
+    return r;
 }
 
 MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch* batch, const mcgpu_dispatch_desc* desc,
                                             mcgpu_frontend_info* info)
 {
+    if (ctx && ctx->pending) return MCGPU_ERROR_INVALID_VALUE; // an enqueued dispatch not yet waited for
     if (!ctx || !batch || !desc || batch->num_pairs == 0) return MCGPU_ERROR_INVALID_VALUE;
     const uint32_t np = batch->num_pairs;
     const uint32_t* vo[2] = {batch->src_vertex_offsets, batch->cut_vertex_offsets};
@@ -2007,6 +2068,7 @@ MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_bo
                                          const double* cut_boxes, uint32_t num_cut, uint32_t input_on_device,
                                          uint64_t* num_pairs)
 {
+    if (ctx && ctx->pending) return MCGPU_ERROR_INVALID_VALUE; // an enqueued dispatch not yet waited for
     if (!ctx || !src_boxes || !cut_boxes || num_src == 0 || num_cut == 0) return MCGPU_ERROR_INVALID_VALUE;
     CK(hipSetDevice(ctx->device));
     ctx->have = false;
@@ -2091,6 +2153,7 @@ MCGPU_API mcgpu_result mcgpu_broad_phase(mcgpu_context ctx, const double* src_bo
 MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const double* perturbation,
                                                   uint32_t enforcement_count, mcgpu_frontend_info* info)
 {
+    if (ctx && ctx->pending) return MCGPU_ERROR_INVALID_VALUE; // an enqueued dispatch not yet waited for
     if (!ctx || !perturbation) return MCGPU_ERROR_INVALID_VALUE;
     if (!ctx->have || ctx->broad_only || ctx->batch) return MCGPU_ERROR_NO_RESULT;
     CK(hipSetDevice(ctx->device));
@@ -2124,6 +2187,7 @@ MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const doubl
 MCGPU_API mcgpu_result mcgpu_predicate_stage(mcgpu_context ctx, uint64_t* counts, int8_t* qsign, int8_t* rsign,
                                              uint64_t capacity, float* device_ms)
 {
+    if (ctx && ctx->pending) return MCGPU_ERROR_INVALID_VALUE; // an enqueued dispatch not yet waited for
     if (!ctx || !counts) return MCGPU_ERROR_INVALID_VALUE;
     if (!ctx->have || ctx->broad_only || ctx->batch) return MCGPU_ERROR_NO_RESULT;
     CK(hipSetDevice(ctx->device));
@@ -2179,6 +2243,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_enforced(mcgpu_context ctx, const mcgpu_me
                                                const mcgpu_dispatch_desc* desc, uint32_t attempts,
                                                mcgpu_frontend_info* info, double* perturbation, uint32_t* perturbations)
 {
+    if (ctx && ctx->pending) return MCGPU_ERROR_INVALID_VALUE; // an enqueued dispatch not yet waited for
     if (!ctx || !desc || !info) return MCGPU_ERROR_INVALID_VALUE;
     mcgpu_dispatch_desc d = *desc;
     d.enforcement_count = 0;
@@ -2349,6 +2414,7 @@ MCGPU_API mcgpu_result mcgpu_set_stage_timing(mcgpu_context ctx, int enable)
 
 MCGPU_API mcgpu_result mcgpu_autotune(mcgpu_context ctx, mcgpu_tuning* out)
 {
+    if (ctx && ctx->pending) return MCGPU_ERROR_INVALID_VALUE; // an enqueued dispatch not yet waited for
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
     if (!ctx->have || ctx->broad_only || ctx->stage_mode != 0 || ctx->res) return MCGPU_ERROR_NO_RESULT;
     CK(hipSetDevice(ctx->device));
@@ -2357,6 +2423,7 @@ MCGPU_API mcgpu_result mcgpu_autotune(mcgpu_context ctx, mcgpu_tuning* out)
 
 MCGPU_API mcgpu_result mcgpu_cutpath_edges(mcgpu_context ctx, const uint32_t* order, mcgpu_cutpath_info* info)
 {
+    if (ctx && ctx->pending) return MCGPU_ERROR_INVALID_VALUE; // an enqueued dispatch not yet waited for
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
     if (!ctx->have || ctx->broad_only || ctx->batch || ctx->res) return MCGPU_ERROR_NO_RESULT;
     CK(hipSetDevice(ctx->device));
@@ -2497,6 +2564,7 @@ MCGPU_API mcgpu_result mcgpu_get_cutpath_edges(mcgpu_context ctx, uint32_t* src,
 
 MCGPU_API mcgpu_result mcgpu_set_graphs(mcgpu_context ctx, int enable)
 {
+    if (ctx && ctx->pending) return MCGPU_ERROR_INVALID_VALUE; // an enqueued dispatch not yet waited for
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
     if (!ctx->own_streams) return MCGPU_ERROR_UNSUPPORTED;
     ctx->graphs = enable != 0;
@@ -3131,6 +3199,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_narrow(mcgpu_context ctx, const mcgpu_mesh
                                              uint64_t num_pairs, const mcgpu_soup_topology* topology,
                                              const mcgpu_dispatch_desc* desc, mcgpu_frontend_info* info)
 {
+    if (ctx && ctx->pending) return MCGPU_ERROR_INVALID_VALUE; // an enqueued dispatch not yet waited for
     if (!ctx || !desc || !src_boxes || !cut_boxes || (num_pairs && !pairs)) return MCGPU_ERROR_INVALID_VALUE;
     if (topology && (!topology->corner_edges[0] || !topology->corner_edges[1] || !topology->edge_v0 ||
                      !topology->edge_v1 || !topology->edge_f0 || !topology->edge_f1 || topology->num_edges == 0))

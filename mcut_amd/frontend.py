@@ -141,6 +141,9 @@ def lib():
         L.mcgpu_stage_name.argtypes = [C.c_int]
         L.mcgpu_autotune.argtypes = [vp, C.POINTER(Tuning)]
         L.mcgpu_cutpath_edges.argtypes = [vp, vp, C.POINTER(CutpathInfo)]
+        L.mcgpu_dispatch_frontend_async.argtypes = [vp, C.POINTER(Mesh), C.POINTER(Mesh), C.POINTER(DispatchDesc), vp]
+        L.mcgpu_dispatch_query.argtypes = [vp]
+        L.mcgpu_dispatch_wait.argtypes = [vp, C.POINTER(FrontendInfo), vp]
         L.mcgpu_get_cutpath_edges.argtypes = [vp, vp, vp, vp, vp, vp, u64]
         L.mcgpu_set_graphs.argtypes = [vp, C.c_int]
         L.mcgpu_set_batch_split.argtypes = [vp, C.c_uint32]
@@ -472,6 +475,27 @@ class Context:
         _check(lib().mcgpu_dispatch_enforced(self.h, C.byref(ms), C.byref(mc), C.byref(d), attempts, C.byref(info),
                                              _ptr(p), C.byref(k)), "mcgpu_dispatch_enforced")
         return Result(self, info, ms.num_vertices + mc.num_vertices), p, k.value
+
+    def dispatch_async(self, src, cut, flags=0, gp_constant=1e-4, stream=None):
+        """Enqueues a dispatch of host meshes and returns at once; wait() ends it."""
+        keep = []
+        ms, mc = _host_mesh(src, keep), _host_mesh(cut, keep)
+        d = DispatchDesc(flags, gp_constant, 0, 0, 0, 0)
+        _check(lib().mcgpu_dispatch_frontend_async(self.h, C.byref(ms), C.byref(mc), C.byref(d), stream),
+               "mcgpu_dispatch_frontend_async")
+        self._pending_nv = ms.num_vertices + mc.num_vertices
+        self._pending_keep = keep  # the host arrays stay alive until the wait
+
+    def query(self):
+        return bool(lib().mcgpu_dispatch_query(self.h))
+
+    def wait(self, stream=None):
+        info = FrontendInfo()
+        try:
+            _check(lib().mcgpu_dispatch_wait(self.h, C.byref(info), stream), "mcgpu_dispatch_wait")
+        finally:
+            self._pending_keep = None
+        return Result(self, info, self._pending_nv)
 
     def dispatch_device(self, src: Mesh, cut: Mesh, flags=0, gp_constant=1e-4, enforcement_count=0, keep_tests=False):
         d = DispatchDesc(flags, gp_constant, enforcement_count, 1, 1 if keep_tests else 0, 0)

@@ -119,6 +119,9 @@ def test_null_arguments_rejected_without_a_device():
     assert L.mcgpu_set_graphs(None, 1) == -1
     assert L.mcgpu_set_batch_split(None, 50) == -1
     assert L.mcgpu_cutpath_edges(None, None, None) == -1
+    assert L.mcgpu_dispatch_frontend_async(None, None, None, None, None) == -1
+    assert L.mcgpu_dispatch_wait(None, None, None) == -1
+    assert L.mcgpu_dispatch_query(None) == 1
     assert L.mcgpu_get_cutpath_edges(None, None, None, None, None, None, 0) == -1
     assert L.mcgpu_graph_stats(None, None, None, None) == -1
 
