@@ -383,23 +383,36 @@ def bench_single(args, torch, frontend, mg):
     for _ in range(args.warmup):
         R = ctx.dispatch_device(ms, mc)
     torch.cuda.synchronize()
+    # The timed steps run the library's default traversal shape (32 source
+    # leaves a block, four tree levels a step: the autotuner's usual choice on
+    # this pair), so the kernel profiled under profiles/ is the one timed; the
+    # autotuner's own choice is reported beside it, from a context of its own.
     tuning = None
-    if args.shape:  # a fixed traversal shape (profiling runs: no tuner launches in the trace)
+    if args.shape:  # another fixed shape
         ctx.set_tuning(*args.shape)
         R = ctx.dispatch_device(ms, mc)
-        tuning = dict(ctx.tuning(), forced=True)
-    elif not args.no_autotune:  # the traversal shape for this workload class, before the timed steps
-        tuning = ctx.autotune()
-        progress(f"single pair: autotuned {tuning}")
+    elif not args.no_autotune:
+        tctx = frontend.Context(0)
+        tctx.dispatch_device(ms, mc)
+        tuning = tctx.autotune()
+        tctx.close()
+        progress(f"single pair: autotuner's choice {tuning}")
+    shape_used = ctx.tuning()
     lat = []
+    # the timed steps record the traversal kernel's own dispatch timestamps
+    # (two events; the roofline kernel time is their mean over these steps)
+    ctx.set_stage_timing("kernel")
+    trav_ms = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
         R = ctx.dispatch_device(ms, mc)
         lat.append((time.perf_counter() - s0) * 1e3)
+        trav_ms.append(R.stage_ms()["traversal_kernel"])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    ctx.set_stage_timing(False)
     ms_step = elapsed * 1e3 / args.steps
     I = R.counts["registry"]
     assert R.status == 0, R.status_name
@@ -408,9 +421,7 @@ def bench_single(args, torch, frontend, mg):
     # the same dispatches replayed from HIP graphs (mcgpu_set_graphs): the
     # shape is recorded once, then every step is a replay of its program
     gctx = frontend.Context(0)
-    if tuning is not None and not tuning.get("forced"):
-        gctx.set_tuning(tuning["wave_leaves"], tuning["tree_stride"], tuning["queues"], tuning["chunk"])
-    elif args.shape:
+    if args.shape:
         gctx.set_tuning(*args.shape)
     gctx.set_graphs(True)
     for _ in range(max(2, args.warmup)):
@@ -432,6 +443,27 @@ def bench_single(args, torch, frontend, mg):
                                         and np.array_equal(gp.view(np.uint64), rp_.view(np.uint64)))}
     gctx.close()
     progress(f"single pair: graph replay {g_ms:.3f} ms per step ({graph_line['graph_launches']} graphs)")
+
+    # what a drop-in pays beyond the resident-input steps: host arrays in
+    # (copied over PCIe each dispatch) and the registry back to the host
+    hctx = frontend.Context(0)
+    if args.shape:
+        hctx.set_tuning(*args.shape)
+    for _ in range(2):
+        H = hctx.dispatch(src, cut)
+        H.registry()
+    ht = []
+    for _ in range(args.steps):
+        h0 = time.perf_counter()
+        H = hctx.dispatch(src, cut)
+        he, hg, hp = H.registry()
+        ht.append((time.perf_counter() - h0) * 1e3)
+    hctx.close()
+    h_ms = sorted(ht)[len(ht) // 2]
+    host_line = {"ms_per_step_p50": h_ms, "value": H.counts["registry"] / (h_ms * 1e-3),
+                 "input_bytes": int(sum(np.asarray(m[0]).nbytes + np.asarray(m[1]).nbytes for m in (src, cut))),
+                 "registry_bytes": int(he.nbytes + hg.nbytes + hp.nbytes)}
+    progress(f"single pair: host arrays in, registry out: {h_ms:.3f} ms per step")
 
     # the optional single-component check (MCGPU_DISPATCH_CHECK_COMPONENTS):
     # device time of a dispatch that also counts each mesh's components
@@ -462,7 +494,8 @@ def bench_single(args, torch, frontend, mg):
     # them (the traversal), else by the events around it on its stream
     dom = max((k for k in stages if k in STAGE_KERNEL), key=stages.get)
     dom_bytes = stage_bytes(dom, n)
-    kernel_ms = stages.get("traversal_kernel", 0.0) if dom == "traversal" else 0.0
+    # the traversal's time: its dispatch timestamps in the timed steps
+    kernel_ms = float(np.mean(trav_ms)) if dom == "traversal" else 0.0
     if kernel_ms <= 0.0:
         kernel_ms = stages[dom]
     achieved = dom_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
@@ -488,9 +521,11 @@ def bench_single(args, torch, frontend, mg):
         "latency_ms": {"p50": lat_sorted[len(lat) // 2], "p99": lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))]},
         "counts": counts,
         "stage_ms": stages,
+        "traversal_shape": {k: shape_used[k] for k in ("wave_leaves", "tree_stride", "queues", "chunk")},
         "autotune": tuning,
         "graph_replay": graph_line,
         "component_check_device_ms": min(cc_ms),
+        "host_arrays_and_readback": host_line,
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "stage": dom, "kernel_ms": kernel_ms,
                      "stage_event_ms": stages[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -511,7 +546,8 @@ def bench_single(args, torch, frontend, mg):
     traffic_file = os.path.join(ROOT, "profiles", "roofline_traffic.json")
     if os.path.exists(traffic_file):  # HBM bytes per launch of the roofline kernel, from rocprofv3 --pmc runs
         t = json.load(open(traffic_file))
-        if t.get("kernel") == STAGE_KERNEL[dom] and t.get("workload") == line["config"]["workload"]:
+        if (t.get("kernel") == STAGE_KERNEL[dom] and t.get("workload") == line["config"]["workload"]
+                and t.get("shape") == line["traversal_shape"]):
             line["roofline"]["traffic"] = t["bytes_per_launch"]
             line["roofline"]["traffic_source"] = t["source"]
     if not args.no_cpu_baseline:

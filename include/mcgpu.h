@@ -376,8 +376,10 @@ MCGPU_API mcgpu_result mcgpu_predicate_stage(mcgpu_context ctx, uint64_t* counts
 MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity);
 
 /* Record the per-stage events in this context's later dispatches (off by
- * default: thirteen recorded events cost about 50 us a dispatch).  No
- * counterpart in the reference; for profiling callers. */
+ * default: thirteen recorded events cost about 50 us a dispatch).  enable = 2
+ * records only the traversal kernel's own dispatch timestamps (stage
+ * "traversal_kernel"; the other stages read 0), cheap enough for timed runs.
+ * No counterpart in the reference; for profiling callers. */
 MCGPU_API mcgpu_result mcgpu_set_stage_timing(mcgpu_context ctx, int enable);
 MCGPU_API const char* mcgpu_stage_name(int stage);
 

@@ -1627,9 +1627,10 @@ struct Tree {
 // leaves, staged through LDS.  A walk starts at cut level kStartLevel with all
 // of its nodes, tested in one memory round trip, instead of descending the
 // nearly empty levels above one round trip each, and then descends
-// STRIDE levels per step (kTravStride by default).  16 source leaves per wave (not 32) keep the
-// union box tight; with the 3-level steps the traversal of the icosphere-7
-// pair went from 0.23 to 0.17 ms.
+// STRIDE levels per step (kTravStride by default).  The defaults, 32 leaves
+// a block and four levels a step, are what the autotuner chose on the
+// icosphere-7 pair (0.151 ms against 0.202 ms for 16 leaves and three levels;
+// one level a step and 32 leaves took 0.23 ms).
 // A frontier wider than kSplitFrontier means the node's leaves are spread
 // out (the Morton curve jumped): the wave then splits the source node and
 // walks its two children in turn, depth first and without a stack (the next
@@ -1644,13 +1645,14 @@ struct Tree {
 constexpr unsigned long long kMaxListEntries = 0xffffffffull; // list positions are 32-bit
 constexpr int kFrontierCap = 512;
 constexpr int kSplitFrontier = 256;
-constexpr int kWaveLeavesLog = 4;
+constexpr int kWaveLeavesLog = 5;
 constexpr int kWaveLeaves = 1 << kWaveLeavesLog;
 constexpr int kStartLevel = 8; // 256 nodes: kTravILP per lane
 // levels descended per breadth-first step: a node's box contains its
-// descendants', so testing the 8 great-grandchildren of a frontier node is
-// exact (only less pruned) and takes a third of the dependent round trips
-constexpr int kTravStride = 3;
+// descendants', so testing the 16 descendants four levels down of a frontier
+// node is exact (only less pruned) and takes a quarter of the dependent round
+// trips
+constexpr int kTravStride = 4;
 constexpr int kTravILP = 4;
 constexpr int kTravStage = 256;
 
@@ -1785,13 +1787,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         int l = lw;
         uint32_t k = (uint32_t)w;
         for (;;) {
-            // source node (l, k): leaves [k << span, (k + 1) << span), one per lane
+            // source node (l, k): leaves [k << span, (k + 1) << span), one per
+            // lane of each of the 64 >> span lane groups (span <= LL <= 5);
+            // the first group's lane is the leaf's own (valid), the copies let
+            // the leaf phase test 64 >> span cut leaves at a time
             const int span = Ds - l;
-            const uint32_t i = (k << span) + (uint32_t)lane;
-            const bool valid = lane < (1 << span) && i < (uint32_t)src.n;
+            const uint32_t mask = (1u << span) - 1;
+            const uint32_t i = (k << span) + ((uint32_t)lane & mask);
+            const bool rep = i < (uint32_t)src.n;
+            const bool valid = lane < (1 << span) && rep;
             float q[6];
             uint32_t s = 0;
-            if (valid) {
+            if (rep) {
                 s = src.sface[i];
                 for (int j = 0; j < 6; ++j) q[j] = src.facebox[6 * (uint64_t)s + j];
             } else {
@@ -1887,14 +1894,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                     }
                     __syncthreads();
                     const int cnt = (nf - t0) < 64 ? (nf - t0) : 64;
-                    for (int u = 0; u < cnt; ++u) {
+                    const int groups = 64 >> span, grp = lane >> span;
+                    for (int u0 = 0; u0 < cnt; u0 += groups) {
                         if (sn > kTravStage - 64) {
                             flush_pairs(stage, sn, pairs, cap, C);
                             sn = 0;
                         }
-                        float b[6];
-                        for (int j = 0; j < 6; ++j) b[j] = lbox[u][j];
-                        const bool hit = valid && boxes_overlap(q, b);
+                        const int u = u0 + grp; // this lane group's cut leaf
+                        bool hit = false;
+                        if (u < cnt) {
+                            float b[6];
+                            for (int j = 0; j < 6; ++j) b[j] = lbox[u][j];
+                            hit = rep && boxes_overlap(q, b);
+                        }
                         const uint64_t m = __ballot(hit);
                         if (hit) stage[sn + __popcll(m & lt)] = ((uint64_t)s << 32) | lface[u];
                         sn += __popcll(m);

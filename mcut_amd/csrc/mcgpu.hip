@@ -191,7 +191,9 @@ struct mcgpu_context_t {
     bool stages_ready = false;
     int stage_mode = 0; // 0: full dispatch (three streams), 1: perturbed narrow-phase rerun
     bool stage_timing = false; // record the per-stage events (mcgpu_set_stage_timing)
+    bool kernel_timing = false; // record the traversal kernel's dispatch timestamps only (mcgpu_set_stage_timing 2)
     bool last_timed = false;   // the last dispatch recorded them
+    bool last_kernel_only = false; // ... only the traversal kernel's timestamps
     float total_ms = 0.f;
     int sort_bits = 22;  // leaf key bits
 
@@ -678,7 +680,8 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
 // rejection after the dispatch.
 void enqueue_validate(mcgpu_context ctx)
 {
-    ctx->last_timed = ctx->stage_timing;
+    ctx->last_timed = ctx->stage_timing || ctx->kernel_timing;
+    ctx->last_kernel_only = !ctx->stage_timing && ctx->kernel_timing;
     hipStream_t s = ctx->s;
     Counters* C = &P_<DevState>(ctx->state)->C;
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
@@ -721,7 +724,8 @@ void enqueue_validate(mcgpu_context ctx)
 // mesh), each pair's broad-phase tolerance, and the cell layout.
 void enqueue_validate_batch(mcgpu_context ctx)
 {
-    ctx->last_timed = ctx->stage_timing;
+    ctx->last_timed = ctx->stage_timing || ctx->kernel_timing;
+    ctx->last_kernel_only = !ctx->stage_timing && ctx->kernel_timing;
     hipStream_t s = ctx->s;
     Counters* C = &P_<DevState>(ctx->state)->C;
     uint32_t* etotal = P_<uint32_t>(ctx->etotal);
@@ -838,11 +842,12 @@ void launch_traversal(const TravConfig& t, int cus, const Tree& src, const Tree&
     case 3 * 16 + 3: launch_trav_t<3, 3>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
     case 3 * 16 + 4: launch_trav_t<3, 4>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
     case 4 * 16 + 2: launch_trav_t<4, 2>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    case 4 * 16 + 3: launch_trav_t<4, 3>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
     case 4 * 16 + 4: launch_trav_t<4, 4>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
     case 5 * 16 + 2: launch_trav_t<5, 2>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
     case 5 * 16 + 3: launch_trav_t<5, 3>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
-    case 5 * 16 + 4: launch_trav_t<5, 4>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
-    default: launch_trav_t<kWaveLeavesLog, kTravStride>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
+    default: launch_trav_t<5, 4>(grid, st, a, b, src, cut, pairs, cap, C, q); break; // the default shape
+    static_assert(kWaveLeavesLog == 5 && kTravStride == 4, "the switch's default is the default shape");
     }
 }
 
@@ -974,9 +979,10 @@ void enqueue_bvh_traverse(mcgpu_context ctx, hipStream_t st, Counters* C, bool g
 {
     const BroadView B = broad_view(ctx);
     const bool t = ctx->stage_timing;
+    const bool kt = t || ctx->kernel_timing;
     ctx->mface_alt = enqueue_trees_traverse(B, st, C, ghist_zeroed, t ? ctx->ev[EV_B_SORT] : nullptr,
-                                            t ? ctx->ev[EV_B_REFIT] : nullptr, t ? ctx->ev[EV_TRAV_K0] : nullptr,
-                                            t ? ctx->ev[EV_TRAV_K1] : nullptr);
+                                            t ? ctx->ev[EV_B_REFIT] : nullptr, kt ? ctx->ev[EV_TRAV_K0] : nullptr,
+                                            kt ? ctx->ev[EV_TRAV_K1] : nullptr);
     ev_record(ctx, EV_B_TRAV, st);
 }
 
@@ -1334,6 +1340,12 @@ void compute_stage_times(mcgpu_context ctx)
         if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) ms = 0.f;
         return ms;
     };
+    if (ctx->last_kernel_only) { // only the traversal kernel's timestamps were recorded
+        for (int i = 0; i < ST_COUNT; ++i) ctx->stage_ms[i] = 0.f;
+        ctx->stage_ms[ST_TRAV_KERNEL] = ctx->stage_mode == 0 ? el(EV_TRAV_K0, EV_TRAV_K1) : 0.f;
+        ctx->stages_ready = true;
+        return;
+    }
     // stages of streams B and C run only in a full dispatch; A's stages after a
     // join start at the event recorded once the join completed
     const bool full = ctx->stage_mode == 0;
@@ -1415,7 +1427,7 @@ void graph_drop(mcgpu_context ctx)
 // dispatch's shape differs from the one it was recorded for.
 mcgpu_result enqueue_dispatch(mcgpu_context ctx)
 {
-    if (!ctx->graphs || ctx->stage_timing || !ctx->own_streams) {
+    if (!ctx->graphs || ctx->stage_timing || ctx->kernel_timing || !ctx->own_streams) {
         if (ctx->batch) enqueue_validate_batch(ctx);
         else enqueue_validate(ctx);
         enqueue_frontend(ctx);
@@ -2165,7 +2177,8 @@ MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const doubl
         const uint64_t V = (uint64_t)ctx->nv[0] + ctx->nv[1];
         Counters* C = &P_<DevState>(ctx->state)->C;
         ctx->stage_mode = 1;
-        ctx->last_timed = ctx->stage_timing;
+        ctx->last_timed = ctx->stage_timing || ctx->kernel_timing;
+    ctx->last_kernel_only = !ctx->stage_timing && ctx->kernel_timing;
         (void)hipEventRecord(ctx->ev[EV_A0], s);
         k_reset_narrow<<<1, 1, 0, s>>>(C);
         k_set_perturbation<<<1, 1, 0, s>>>(&P_<DevState>(ctx->state)->P, 1, perturbation[0], perturbation[1], perturbation[2]);
@@ -2408,7 +2421,9 @@ MCGPU_API int mcgpu_stage_times(mcgpu_context ctx, float* ms, int capacity)
 MCGPU_API mcgpu_result mcgpu_set_stage_timing(mcgpu_context ctx, int enable)
 {
     if (!ctx) return MCGPU_ERROR_INVALID_VALUE;
-    ctx->stage_timing = enable != 0;
+    if (enable < 0 || enable > 2) return MCGPU_ERROR_INVALID_VALUE;
+    ctx->stage_timing = enable == 1;
+    ctx->kernel_timing = enable == 2;
     return MCGPU_SUCCESS;
 }
 

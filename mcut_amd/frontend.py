@@ -451,8 +451,10 @@ class Context:
         return t.as_dict()
 
     def set_stage_timing(self, on):
-        """Record per-stage events in later dispatches (Result.stage_ms); off by default."""
-        _check(lib().mcgpu_set_stage_timing(self.h, 1 if on else 0), "mcgpu_set_stage_timing")
+        """Record per-stage events in later dispatches (Result.stage_ms); off by
+        default.  on = "kernel": only the traversal kernel's timestamps."""
+        mode = 2 if on == "kernel" else (1 if on else 0)
+        _check(lib().mcgpu_set_stage_timing(self.h, mode), "mcgpu_set_stage_timing")
 
     def dispatch(self, src, cut, flags=0, gp_constant=1e-4, enforcement_count=0, keep_tests=False):
         keep = []

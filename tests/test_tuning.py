@@ -41,7 +41,7 @@ def test_every_shape_gives_the_same_dispatch(ctx, name):
     print(f"tuning: shapes on {name}", flush=True)
     ctx.set_tuning(None)
     ref = snapshot(ctx.dispatch(*pair))
-    assert ctx.tuning()["wave_leaves"] == 16 and ctx.tuning()["queues"] == 0
+    assert ctx.tuning()["wave_leaves"] == 32 and ctx.tuning()["tree_stride"] == 4 and ctx.tuning()["queues"] == 0
     try:
         for ll, st, q, ch in SHAPES:
             ctx.set_tuning(wave_leaves=ll, tree_stride=st, queues=q, chunk=ch)
