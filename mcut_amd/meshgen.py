@@ -435,6 +435,67 @@ def read_off(path: str):
     return V, F, (None if np.all(S == 3) else S)
 
 
+def read_obj(path: str):
+    """Read the vertices and faces of an OBJ file (``v x y z``, ``f a b c ...``
+    with 1-based or negative indices, ``a/t/n`` forms allowed; other records
+    ignored); returns (V, F, S) like read_off."""
+    V, F, S = [], [], []
+    with open(path, "r") as fh:
+        for line in fh:
+            t = line.split("#", 1)[0].split()
+            if not t:
+                continue
+            if t[0] == "v":
+                V.append([float(t[1]), float(t[2]), float(t[3])])
+            elif t[0] == "f":
+                ids = []
+                for w in t[1:]:
+                    k = int(w.split("/", 1)[0])
+                    ids.append(k - 1 if k > 0 else len(V) + k)
+                F.extend(ids)
+                S.append(len(ids))
+    V = np.asarray(V, dtype=np.float64).reshape(-1, 3)
+    S = np.asarray(S, dtype=np.uint32)
+    F = np.asarray(F, dtype=np.uint32)
+    return V, F, (None if np.all(S == 3) else S)
+
+
+def _faces(F, S):
+    F = np.asarray(F)
+    if S is None:
+        return F.reshape(-1, 3).tolist()
+    out, at = [], 0
+    for k in np.asarray(S):
+        out.append(F[at:at + int(k)].tolist())
+        at += int(k)
+    return out
+
+
+def write_off(path: str, V, F, S=None):
+    """Write (V, F, S) as OFF the way hmesh.cpp:1268-1340 lays it out: the
+    header, counts with 0 edges, one vertex a line, then each face's size and
+    vertex ids.  Coordinates are written with 17 significant digits, so
+    read_off gives back the same doubles."""
+    V = np.asarray(V, dtype=np.float64).reshape(-1, 3)
+    faces = _faces(F, S)
+    with open(path, "w") as fh:
+        fh.write("OFF\n%d %d 0\n" % (V.shape[0], len(faces)))
+        for x, y, z in V:
+            fh.write("%.17g %.17g %.17g\n" % (x, y, z))
+        for f in faces:
+            fh.write("%d %s\n" % (len(f), " ".join(str(i) for i in f)))
+
+
+def write_obj(path: str, V, F, S=None):
+    """Write (V, F, S) as OBJ (1-based face indices, 17 significant digits)."""
+    V = np.asarray(V, dtype=np.float64).reshape(-1, 3)
+    with open(path, "w") as fh:
+        for x, y, z in V:
+            fh.write("v %.17g %.17g %.17g\n" % (x, y, z))
+        for f in _faces(F, S):
+            fh.write("f %s\n" % " ".join(str(i + 1) for i in f))
+
+
 def newell_min_sq(V: np.ndarray, F: np.ndarray, S=None) -> float:
     """Smallest squared Newell-normal length over the faces (plain float64)."""
     if S is None:

@@ -185,3 +185,20 @@ class Mcut:
         finally:
             os.chdir(cwd)
             L.mcReleaseContext(ctx)
+
+
+def write_components(ccs, directory, fmt="off", prefix="cc"):
+    """Writes connected components (Mcut.dispatch's output: the arrays of
+    mcGetConnectedComponentData's VERTEX_DOUBLE / FACE / FACE_SIZE queries,
+    frontend.cpp:2006-2400) as OFF or OBJ files, one per component; returns
+    the paths."""
+    from mcut_amd import meshgen as mg
+    write = {"off": mg.write_off, "obj": mg.write_obj}[fmt]
+    paths = []
+    for i, cc in enumerate(ccs):
+        path = os.path.join(directory, f"{prefix}{i}.{fmt}")
+        sizes = cc["face_sizes"]
+        write(path, cc["vertices"], cc["faces"], None if np.all(sizes == 3) else sizes)
+        paths.append(path)
+    return paths
+
