@@ -384,8 +384,8 @@ def bench_single(args, torch, frontend, mg):
         R = ctx.dispatch_device(ms, mc)
     torch.cuda.synchronize()
     # The timed steps run the library's default traversal shape (32 source
-    # leaves a block, four tree levels a step: the autotuner's usual choice on
-    # this pair), so the kernel profiled under profiles/ is the one timed; the
+    # leaves a block, four tree levels a step, each XCD's blocks in runs of 16),
+    # so the kernel profiled under profiles/ is the one timed; the
     # autotuner's own choice is reported beside it, from a context of its own.
     tuning = None
     if args.shape:  # another fixed shape
@@ -583,8 +583,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the default traversal shape instead of the autotuned one")
     ap.add_argument("--shape", type=int, nargs=4, metavar=("LEAVES", "STRIDE", "QUEUES", "CHUNK"),
-                    help="time this traversal shape (source leaves per block, cut levels per step, 0 or 8 work "
-                         "queues, blocks per pop) instead of tuning one")
+                    help="time this traversal shape (source leaves per block, cut levels per step, 0 (Morton order), "
+                         "1 (XCD-contiguous) or 8 work queues, blocks per pop) instead of tuning one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (three times): the known-size kernel that "
                          "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")

@@ -455,8 +455,11 @@ MCGPU_API mcgpu_result mcgpu_graph_stats(mcgpu_context ctx, uint64_t* recordings
 typedef struct mcgpu_tuning {
     uint32_t wave_leaves;       /* source leaves per traversal block: 8, 16 or 32 */
     uint32_t tree_stride;       /* cut-tree levels per breadth-first step: 2, 3 or 4 */
-    uint32_t queues;            /* 0: one wave per block; 8: persistent waves over 8 per-XCD work queues */
-    uint32_t chunk;             /* blocks a persistent wave takes per queue pop */
+    uint32_t queues;            /* 0: one wave per block; 1: the same, each XCD's blocks in runs of
+                                   `chunk` Morton-consecutive ones; 8: persistent waves over 8 per-XCD
+                                   work queues */
+    uint32_t chunk;             /* blocks a persistent wave takes per queue pop; the run length with
+                                   queues 1 */
     uint32_t broad_share_pct;   /* percent of the CUs outside the sums' reservation that the
                                    broad-phase stream holds (reported; 0 in mcgpu_set_tuning) */
     uint32_t candidates;        /* traversal shapes the autotuner timed (0: defaults, not tuned) */
@@ -468,7 +471,7 @@ typedef struct mcgpu_tuning {
 
 /* Tunes the traversal shape for the workload class of the last successful
  * mcgpu_dispatch_frontend or mcgpu_dispatch_batch (its inputs must still be
- * valid): 27 shapes timed alone on its trees, three runs each, each required
+ * valid): 36 shapes timed alone on its trees, three runs each, each required
  * to find the same raw pairs; then the dispatch re-run twice with the default
  * and twice with the chosen shape for the report.  Later dispatches of the
  * class use the choice.  The last dispatch's outputs stay valid. */

@@ -82,15 +82,16 @@ enum Ev { EV_A0, EV_S0, EV_A_SUMS, EV_A_PARAMS, EV_A_SHIFT, EV_A_JOIN, EV_A_FILT
 struct TravConfig {
     int ll = kWaveLeavesLog;  // source leaves per block: 2^ll
     int stride = kTravStride; // cut-tree levels per breadth-first step
-    int queues = 0;           // 0: one wave per block; 8: persistent waves over XCD work queues
-    int chunk = 1;            // blocks a persistent wave takes per queue pop
+    int queues = kTravQueues; // 0: one wave per block; 1: the same in per-XCD runs of `chunk`; 8: persistent
+                              // waves over XCD work queues
+    int chunk = kTravRun;     // blocks a persistent wave takes per queue pop; with queues 1 the run length
 };
 constexpr int kTravWavesPerCU = 16; // persistent grid: four waves per SIMD (the kernel's occupancy)
 
 // the traversal configurations the autotuner times
 constexpr int kTuneLL[] = {3, 4, 5};
 constexpr int kTuneStride[] = {2, 3, 4};
-constexpr int kTuneQueue[][2] = {{0, 1}, {8, 1}, {8, 4}}; // (queues, chunk)
+constexpr int kTuneQueue[][2] = {{0, 1}, {1, 16}, {8, 1}, {8, 4}}; // (queues, chunk)
 
 // one step of a dispatch's replay program (mcgpu_set_graphs)
 struct GraphOp {
@@ -833,9 +834,13 @@ void launch_traversal(const TravConfig& t, int cus, const Tree& src, const Tree&
 {
     const uint64_t nblocks = src.D >= t.ll ? (((uint64_t)src.n + (1u << t.ll) - 1) >> t.ll) : 1;
     const TravQueue q{t.queues, t.chunk < 1 ? 1 : t.chunk};
-    uint64_t g = q.queues ? std::min<uint64_t>((nblocks + q.chunk - 1) / q.chunk,
+    uint64_t g = q.queues > 1 ? std::min<uint64_t>((nblocks + q.chunk - 1) / q.chunk,
                                                (uint64_t)std::max(cus, 1) * kTravWavesPerCU)
                           : std::min<uint64_t>(nblocks, 65536);
+    if (q.queues == 1) { // whole runs of q.chunk blocks on every XCD; the surplus workgroups find no block
+        const uint64_t unit = (uint64_t)kXcds * q.chunk;
+        g = (g + unit - 1) / unit * unit;
+    }
     const dim3 grid((unsigned)std::max<uint64_t>(g, 1));
     switch (t.ll * 16 + t.stride) {
     case 3 * 16 + 2: launch_trav_t<3, 2>(grid, st, a, b, src, cut, pairs, cap, C, q); break;
@@ -853,7 +858,7 @@ void launch_traversal(const TravConfig& t, int cus, const Tree& src, const Tree&
 
 bool trav_config_valid(const TravConfig& t)
 {
-    return t.ll >= 3 && t.ll <= 5 && t.stride >= 2 && t.stride <= 4 && (t.queues == 0 || t.queues == 8) &&
+    return t.ll >= 3 && t.ll <= 5 && t.stride >= 2 && t.stride <= 4 && (t.queues == 0 || t.queues == 1 || t.queues == 8) &&
            t.chunk >= 1 && t.chunk <= 64;
 }
 
@@ -879,7 +884,6 @@ void trees_of(const BroadView& B, bool alt, Tree* T, RefitTree* R)
         }
         T[m].sface = R[m].sface;
         T[m].nodes = R[m].nodes;
-        T[m].facebox = R[m].facebox;
         T[m].n = R[m].n;
         T[m].D = R[m].D;
     }
@@ -1694,7 +1698,7 @@ static mcgpu_result autotune(mcgpu_context ctx, mcgpu_tuning* out)
                 if ((r = time_traversal(ctx, t, 3, &ms, &same)) != MCGPU_SUCCESS) return r;
                 if (!same) return MCGPU_ERROR_DEVICE; // every shape finds the same pairs
                 ++n;
-                if (t.ll == def.ll && t.stride == def.stride && t.queues == def.queues) def_trav = ms;
+                if (t.ll == def.ll && t.stride == def.stride && t.queues == def.queues && t.chunk == def.chunk) def_trav = ms;
                 if (ms < best) {
                     best = ms;
                     bt = t;

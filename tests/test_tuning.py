@@ -1,7 +1,8 @@
 """Traversal launch shapes and the autotuner (mcgpu_set_tuning,
 mcgpu_autotune): every shape -- 8, 16 or 32 source leaves per block, 2, 3 or
-4 cut-tree levels per step, one wave per block or a persistent grid over the
-eight per-XCD work queues -- must leave every output of a dispatch unchanged:
+4 cut-tree levels per step, one wave per block (in Morton order or with each
+XCD's blocks one contiguous Morton range) or a persistent grid over the eight
+per-XCD work queues -- must leave every output of a dispatch unchanged:
 status, counts, candidate pairs, registry edges, faces and point bits.  The
 default shape is held to the oracle by test_gpu_parity.py."""
 import numpy as np
@@ -12,7 +13,10 @@ from mcut_amd import meshgen as mg
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(ll, st, q, ch) for ll in (8, 16, 32) for st in (2, 3, 4) for q, ch in ((0, 1), (8, 1), (8, 4))]
+# the autotuner's candidates, and the shapes held to the default's outputs
+# (those and runs of one block per XCD)
+TUNED = [(ll, st, q, ch) for ll in (8, 16, 32) for st in (2, 3, 4) for q, ch in ((0, 1), (1, 16), (8, 1), (8, 4))]
+SHAPES = TUNED + [(ll, st, 1, 1) for ll in (8, 16, 32) for st in (2, 3, 4)]
 
 
 @pytest.fixture(scope="module")
@@ -41,7 +45,8 @@ def test_every_shape_gives_the_same_dispatch(ctx, name):
     print(f"tuning: shapes on {name}", flush=True)
     ctx.set_tuning(None)
     ref = snapshot(ctx.dispatch(*pair))
-    assert ctx.tuning()["wave_leaves"] == 32 and ctx.tuning()["tree_stride"] == 4 and ctx.tuning()["queues"] == 0
+    t = ctx.tuning()
+    assert (t["wave_leaves"], t["tree_stride"], t["queues"], t["chunk"]) == (32, 4, 1, 16)
     try:
         for ll, st, q, ch in SHAPES:
             ctx.set_tuning(wave_leaves=ll, tree_stride=st, queues=q, chunk=ch)
@@ -89,13 +94,13 @@ def test_autotune_single_pair(ctx):
     print("tuning: autotune icosphere-5", flush=True)
     ref = snapshot(ctx.dispatch(*pair))
     rep = ctx.autotune()
-    assert rep["candidates"] == len(SHAPES)
-    assert rep["wave_leaves"] in (8, 16, 32) and rep["tree_stride"] in (2, 3, 4) and rep["queues"] in (0, 8)
+    assert rep["candidates"] == len(TUNED)
+    assert rep["wave_leaves"] in (8, 16, 32) and rep["tree_stride"] in (2, 3, 4) and rep["queues"] in (0, 1, 8)
     assert 0 < rep["traversal_ms"] <= rep["default_traversal_ms"] and rep["dispatch_ms"] > 0
     # the class's later dispatches run with the choice, unchanged in result
     R = ctx.dispatch(*pair)
     t = ctx.tuning()
-    assert t["candidates"] == len(SHAPES) and t["wave_leaves"] == rep["wave_leaves"]
+    assert t["candidates"] == len(TUNED) and t["wave_leaves"] == rep["wave_leaves"]
     assert t["broad_share_pct"] == rep["broad_share_pct"]
     same(ref, snapshot(R))
 
@@ -123,7 +128,7 @@ def test_autotune_batch():
 
         check(c.dispatch_batch(pairs))
         t = c.tuning()
-        assert t["candidates"] == len(SHAPES)
+        assert t["candidates"] == len(TUNED)
         check(c.dispatch_batch(pairs))
         assert c.tuning() == t
     finally:
