@@ -215,7 +215,12 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_hist(const K* __restrict__
     }
 }
 
-// stable scatter by digit; offs = exclusive scan of hist (digit-major)
+// stable scatter by digit; offs = exclusive scan of hist (digit-major), or
+// with group totals the raw tile histograms.  A tile is ranked digit by digit
+// as before, but its keys are first placed in tile order by digit in LDS and
+// then written out in that order, so the lanes of a wave write runs of
+// consecutive addresses (one run per digit of the tile) instead of one
+// scattered element each.
 template <typename K, bool HasVal>
 __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
                                                               K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
@@ -226,6 +231,9 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restric
     __shared__ uint32_t wcount[kSortBlock / 64][kRadix];
     __shared__ uint32_t running[kRadix];
     __shared__ uint32_t gbase[kRadix];
+    __shared__ uint32_t lstart[kRadix]; // first tile-local slot of each digit
+    __shared__ K lkey[kSortTile];
+    __shared__ uint32_t lval[HasVal ? kSortTile : 1];
     const uint64_t n = read_count(n_dev, n_host);
     const uint32_t nt = (uint32_t)((n + kSortTile - 1) / kSortTile);
     (void)ntiles;
@@ -245,7 +253,9 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restric
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint64_t base = (uint64_t)t * kSortTile;
         if (base >= n) break; // tiles are visited in increasing order
+        const uint32_t cnt = (uint32_t)((n - base) < (uint64_t)kSortTile ? (n - base) : kSortTile);
         running[threadIdx.x] = 0;
+        lstart[threadIdx.x] = 0;
         if (ghist) {
             const uint32_t g = t / G;
             const uint32_t* gd = ghist + (uint64_t)threadIdx.x * ngc;
@@ -258,14 +268,30 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restric
             gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * nt + t];
         }
         for (int w = 0; w < kSortBlock / 64; ++w) wcount[w][threadIdx.x] = 0;
-        __syncthreads();
+        K key[kSortItems];
+        uint32_t val[kSortItems];
+#pragma unroll
         for (int j = 0; j < kSortItems; ++j) {
-            const uint64_t idx = base + (uint64_t)j * kSortBlock + threadIdx.x;
-            const bool valid = idx < n;
-            K key = valid ? keys_in[idx] : (K)0;
-            uint32_t val = 0;
-            if (HasVal && valid) val = vals_in[idx];
-            const uint32_t d = digit_of(key, shift);
+            const uint32_t li = (uint32_t)j * kSortBlock + threadIdx.x;
+            key[j] = li < cnt ? keys_in[base + li] : (K)0;
+            val[j] = (HasVal && li < cnt) ? vals_in[base + li] : 0u;
+        }
+        __syncthreads();
+        // the tile's digit counts, then their exclusive scan
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j)
+            if ((uint32_t)j * kSortBlock + threadIdx.x < cnt) atomicAdd(&lstart[digit_of(key[j], shift)], 1u);
+        __syncthreads();
+        {
+            const uint32_t c = lstart[threadIdx.x];
+            __syncthreads();
+            lstart[threadIdx.x] = block_exclusive_scan(c, nullptr);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            const bool valid = (uint32_t)j * kSortBlock + threadIdx.x < cnt;
+            const uint32_t d = digit_of(key[j], shift);
             // lanes of this wave holding the same digit
             uint64_t peers = __ballot(valid);
             for (int b = 0; b < kRadixBits; ++b) {
@@ -279,9 +305,8 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restric
             if (valid) {
                 uint32_t pos = running[d] + rank;
                 for (int w = 0; w < wid; ++w) pos += wcount[w][d];
-                const uint32_t dst = gbase[d] + pos;
-                keys_out[dst] = key;
-                if (HasVal) vals_out[dst] = val;
+                lkey[lstart[d] + pos] = key[j];
+                if (HasVal) lval[lstart[d] + pos] = val[j];
             }
             __syncthreads();
             {
@@ -294,6 +319,15 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restric
             }
             __syncthreads();
         }
+        // the tile in digit order: consecutive slots of one digit go to consecutive addresses
+        for (uint32_t i = threadIdx.x; i < cnt; i += kSortBlock) {
+            const K k = lkey[i];
+            const uint32_t d = digit_of(k, shift);
+            const uint64_t dst = (uint64_t)gbase[d] + (i - lstart[d]);
+            keys_out[dst] = k;
+            if (HasVal) vals_out[dst] = lval[i];
+        }
+        __syncthreads();
     }
 }
 
