@@ -808,9 +808,12 @@ __global__ void k_edge_assign(const uint32_t* __restrict__ idx, CornerMap cm, ui
 // key and first-use tables: 53 us instead of 103 us for the 983,040 corners of
 // an icosphere-7.  Both meshes go through one launch per pass: corners
 // [0, K0) are the source's, [K0, K0 + K1) the cut mesh's (is_cut = 1).  The
-// insert leaves each corner's slot in its cedge entry; the flag and assign
-// passes read it instead of probing, and the assign pass overwrites it with
-// the edge id.  Edge ids come from one exclusive scan over both meshes'
+// insert leaves each corner's slot in its cedge entry; the flag pass reads
+// it instead of probing and replaces it with the edge's first rank, which the
+// assign pass then reads in corner order (the table word is a gather at a
+// hashed slot that brings a whole line; the first rank's eid entry lies near
+// the corner's own rank) before it overwrites the entry with the edge id.
+// Edge ids come from one exclusive scan over both meshes'
 // first-use flags, so a cut edge's id already counts the source's edges
 // (with a resident source only the cut mesh is scanned, after its ne edges).
 constexpr uint32_t kPackedVertexBits = 21, kPackedRankBits = 22;
@@ -890,6 +893,7 @@ __global__ void k_edge_flag_packed(TopoMesh M0, TopoMesh M1, uint64_t start, uin
         const uint32_t a = M.idx[c];
         const uint32_t rank = corner_rank(b, i, n, m);
         const uint32_t r0 = (uint32_t)(M.keys[M.cedge[c]] & kPackedRankMask);
+        M.cedge[c] = r0; // the assign pass reads the edge's first rank here, not through the table
         flag[(m ? M0.nk : 0) + rank] = (r0 == rank) ? 1u : 0u;
         if (r0 != rank && M.idx[rank_corner(M.cm, r0, m)] == a) atomicOr(&C->nonmanifold, 1u); // used twice in one direction
     }
@@ -925,7 +929,7 @@ __global__ void k_edge_assign_packed(TopoMesh M0, TopoMesh M1, uint64_t start, c
         M.cm(c, f, b, n, i);
         const uint32_t a = M.idx[c], d = M.idx[b + (i + 1 == n ? 0 : i + 1)];
         const uint32_t rank = corner_rank(b, i, n, m);
-        const uint32_t r0 = (uint32_t)(M.keys[M.cedge[c]] & kPackedRankMask);
+        const uint32_t r0 = M.cedge[c]; // the edge's first rank (flag pass)
         const uint32_t e = (m ? eb1 : 0u) + eid[(m ? M0.nk : 0) + r0];
         M.cedge[c] = e;
         if (r0 == rank) {
@@ -1010,6 +1014,7 @@ __global__ void k_edge_flag_batch(BatchTopo T, uint32_t* __restrict__ flag, Coun
         const uint32_t a = T.idx[c];
         const uint32_t rank = corner_rank(b, i, n, T.m);
         const uint32_t r0 = (uint32_t)cb + (uint32_t)(T.keys[T.cedge[c]] & kPackedRankMask);
+        T.cedge[c] = r0; // the assign pass reads the edge's first rank here, not through the table
         flag[rank] = (r0 == rank) ? 1u : 0u;
         if (r0 != rank && T.idx[rank_corner(T.cm, r0, T.m)] == a) atomicOr(&C->nonmanifold, 1u); // used twice in one direction
     }
@@ -1028,7 +1033,7 @@ __global__ void k_edge_assign_batch(BatchTopo T, const uint32_t* __restrict__ ei
         batch_region(T, f, vb, cb, kp);
         const uint32_t a = T.idx[c], d = T.idx[b + (i + 1 == n ? 0 : i + 1)];
         const uint32_t rank = corner_rank(b, i, n, T.m);
-        const uint32_t r0 = (uint32_t)cb + (uint32_t)(T.keys[T.cedge[c]] & kPackedRankMask);
+        const uint32_t r0 = T.cedge[c]; // the edge's first rank (flag pass)
         const uint32_t e = ebase + eid_by_rank[r0];
         T.cedge[c] = e;
         if (r0 == rank) {
