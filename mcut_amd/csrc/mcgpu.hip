@@ -1237,7 +1237,8 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
     CK(hipStreamSynchronize(ctx->s));
     CK(hipGetLastError());
     DevState hs;
-    CK(hipMemcpy(&hs, ctx->state.p, sizeof(DevState), hipMemcpyDeviceToHost));
+    CK(hipMemcpyAsync(&hs, ctx->state.p, sizeof(DevState), hipMemcpyDeviceToHost, ctx->s));
+    CK(hipStreamSynchronize(ctx->s));
     ctx->C = hs.C;
     ctx->P = hs.P;
     const Counters& C = ctx->C;
@@ -1259,7 +1260,8 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
             if (more > ctx->nv[0] + (uint64_t)ctx->nv[1]) return MCGPU_ERROR_DEVICE;
             enqueue_cc_rounds(ctx, ctx->s);
             CK(hipStreamSynchronize(ctx->s));
-            CK(hipMemcpy(&ctx->C, &P_<DevState>(ctx->state)->C, sizeof(Counters), hipMemcpyDeviceToHost));
+            CK(hipMemcpyAsync(&ctx->C, &P_<DevState>(ctx->state)->C, sizeof(Counters), hipMemcpyDeviceToHost, ctx->s));
+            CK(hipStreamSynchronize(ctx->s));
         }
         const uint32_t want = ctx->batch ? ctx->npairs : 1u;
         if ((!ctx->res && C.ncc[0] != want) || C.ncc[1] != want) return MCGPU_ERROR_INVALID_VALUE;
@@ -2297,9 +2299,11 @@ MCGPU_API mcgpu_result mcgpu_get_registry(mcgpu_context ctx, uint32_t* edges, ui
     const uint64_t n = std::min<uint64_t>(ctx->C.n_reg, capacity);
     const Buf& be = ctx->batch ? ctx->rle : ctx->re; // batches: pair-local ids, pairs in order
     const Buf& bg = ctx->batch ? ctx->rlg : ctx->rg;
-    if (edges) CK(hipMemcpy(edges, be.p, 4 * n, hipMemcpyDeviceToHost));
-    if (faces) CK(hipMemcpy(faces, bg.p, 4 * n, hipMemcpyDeviceToHost));
-    if (points) CK(hipMemcpy(points, ctx->rpts.p, 24 * n, hipMemcpyDeviceToHost));
+    // ordered on the context's own stream (a server slot's stream synchronises with the null stream)
+    if (edges) CK(hipMemcpyAsync(edges, be.p, 4 * n, hipMemcpyDeviceToHost, ctx->s));
+    if (faces) CK(hipMemcpyAsync(faces, bg.p, 4 * n, hipMemcpyDeviceToHost, ctx->s));
+    if (points) CK(hipMemcpyAsync(points, ctx->rpts.p, 24 * n, hipMemcpyDeviceToHost, ctx->s));
+    CK(hipStreamSynchronize(ctx->s));
     return MCGPU_SUCCESS;
 }
 
@@ -2715,12 +2719,27 @@ mcgpu_result slot_create(int device, mcgpu_context* out)
     mcgpu_context ctx = new mcgpu_context_t();
     ctx->device = device;
     ctx->own_streams = false;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking) != hipSuccess) {
+    // A slot's stream carries a CU mask (every CU), which gives it a hardware
+    // queue of its own: plain streams share the process's few queues (four
+    // by default), and in a program that had made and released other
+    // contexts first, the slots' plain streams shared them, so that slots
+    // ran one after another (1,900 requests/s served at 3,000 offered; a
+    // fresh process, or sixteen queues, served all 3,000).  Such a stream
+    // synchronises with the null stream, so a slot's copies are ordered on
+    // its own stream (finish, mcgpu_get_registry).
+    int ncu = 0;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1) {
+        delete ctx;
+        return MCGPU_ERROR_DEVICE;
+    }
+    std::vector<uint32_t> all((ncu + 31) / 32, ~0u);
+    if (hipExtStreamCreateWithCUMask(&ctx->s, (uint32_t)all.size(), all.data()) != hipSuccess) {
         delete ctx;
         return MCGPU_ERROR_DEVICE;
     }
     ctx->s2 = ctx->s3 = ctx->s2b = ctx->s3b = ctx->ssum = ctx->s;
-    (void)hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, device);
+    ctx->ncu = ncu;
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventCreate(&ctx->ev[i]);
     *out = ctx;
     return MCGPU_SUCCESS;
