@@ -209,3 +209,114 @@ def test_device_polygon_batch_matches_host_batch(ctxs):
     finally:
         for p in bufs:
             hip.free(p)
+
+
+# ---------------------------------------------------------------------------
+# Non-manifold soups in a batch.  The batch numbers each pair's edges in a
+# table region of its own, with corners handled in blocks of consecutive
+# corners; these soups put the extra corners of a shared edge thousands of
+# corners away from the first two (another 2,048-corner block), once as a
+# second closed sphere glued on by an edge (four corners, two each way) and
+# once as a fin placed before or after a sphere (three corners, two the same
+# way).  MCUT refuses such a soup (add_face returns null_face,
+# hmesh.cpp:610-627, and mcDispatch throws std::invalid_argument,
+# preproc.cpp:2338-2350): MC_INVALID_VALUE, here MCGPU_ERROR_INVALID_VALUE
+# for the whole batch, as for a plain dispatch.
+# ---------------------------------------------------------------------------
+_BLOCK = 2048
+
+
+def _corners(F):
+    F = np.asarray(F, np.int64).reshape(-1, 3)
+    return F.reshape(-1), np.roll(F, -1, axis=1).reshape(-1)
+
+
+def _edge_in_one_block(F, corner_base, lo_block=0):
+    """An edge of triangle list F whose two corners (soup corners
+    corner_base + c) lie in one block at or after lo_block: (a, b, block)."""
+    a, d = _corners(F)
+    where = {}
+    for c in range(a.size):
+        where.setdefault((min(a[c], d[c]), max(a[c], d[c])), []).append(c)
+    for (u, v), cs in where.items():
+        if len(cs) == 2:
+            t0, t1 = ((corner_base + x) // _BLOCK for x in cs)
+            if t0 == t1 and t0 >= lo_block:
+                return int(u), int(v), t0
+    raise AssertionError("no edge with both corners in one block")
+
+
+def _glued_spheres():
+    """Two closed spheres whose faces follow each other, the second with two
+    vertices renamed onto an edge of the first: that edge has four corners,
+    two (one each way) in a block of the first sphere and two in a block of
+    the second."""
+    V, F, _ = mg.geodesic_sphere(12)  # 2,880 faces: 8,640 corners, over four blocks
+    a, b, ta = _edge_in_one_block(F, 0)
+    nv = V.shape[0]
+    c, d, tb = _edge_in_one_block(F, F.size, lo_block=F.size // _BLOCK + 1)
+    F2 = F.astype(np.int64) + nv
+    F2[F2 == c + nv] = a
+    F2[F2 == d + nv] = b
+    V2 = V * 0.5 + np.array([3.0, 0.0, 0.0])
+    return np.vstack([V, V2]), np.concatenate([F, F2.astype(np.uint32)]), (a, b, ta, tb)
+
+
+def _sphere_and_fin(first):
+    """A closed sphere and one more triangle on an edge of it with the
+    sphere's own direction: three corners, two in one block of the sphere and
+    the fin's in another block (the fin placed before or after the sphere)."""
+    V, F, _ = mg.geodesic_sphere(12)
+    lo = 1 if first else 0  # the fin's corners sit in block 0 when it comes first
+    a, b, t = _edge_in_one_block(F, 3 if first else 0, lo_block=lo)
+    A, B = _corners(F)
+    k = int(np.nonzero((A == a) & (B == b) | (A == b) & (B == a))[0][0])
+    u, v = int(A[k]), int(B[k])  # a corner of the edge, u -> v
+    z = V.shape[0]
+    fin = np.array([u, v, z], np.uint32)
+    V2 = np.vstack([V, [[2.0, 2.0, 2.0]]])
+    F2 = np.concatenate([fin, F]) if first else np.concatenate([F, fin])
+    fin_block = 0 if first else F.size // _BLOCK
+    assert fin_block != t
+    return V2, F2, (u, v, t, fin_block)
+
+
+def test_nonmanifold_soups_split_their_corners_over_blocks():
+    """The soups below hold what they claim: every corner of the shared edge,
+    the two blocks, and exactly one non-manifold edge."""
+    V, F, (a, b, ta, tb) = _glued_spheres()
+    A, D = _corners(F)
+    on = np.nonzero(((A == a) & (D == b)) | ((A == b) & (D == a)))[0]
+    assert on.size == 4 and ta != tb
+    assert sorted(set((on // _BLOCK).tolist())) == sorted({ta, tb})
+    keys = np.minimum(A, D) * (1 << 32) + np.maximum(A, D)
+    _, cnt = np.unique(keys, return_counts=True)
+    assert (cnt > 2).sum() == 1
+    for first in (False, True):
+        V, F, (u, v, t, ft) = _sphere_and_fin(first)
+        A, D = _corners(F)
+        on = np.nonzero(((A == u) & (D == v)) | ((A == v) & (D == u)))[0]
+        assert on.size == 3 and ((A[on] == u) & (D[on] == v)).sum() == 2
+        assert set((on // _BLOCK).tolist()) == {t, ft}
+
+
+@pytest.mark.gpu
+def test_batch_refuses_nonmanifold_soups_split_over_blocks(ctxs):
+    from mcut_amd import frontend
+    bctx, pctx = ctxs
+    box = mg.tessellated_box((0.0, 0.0, 0.0), 1.3, 4)
+    good = meshes.config1_rotated()
+    V, F, _ = _glued_spheres()
+    soups = [(V, F)] + [_sphere_and_fin(first)[:2] for first in (False, True)]
+    for k, (V, F) in enumerate(soups):
+        for side in ("src", "cut"):
+            pair = ((V, F, None), box) if side == "src" else (box, (V, F, None))
+            print(f"non-manifold soup {k} as {side}", flush=True)
+            with pytest.raises(frontend.McgpuError) as ei:
+                pctx.dispatch(*pair)
+            assert ei.value.code == -1
+            with pytest.raises(frontend.McgpuError) as ei:
+                bctx.dispatch_batch([good, pair, good])
+            assert ei.value.code == -1
+    B = bctx.dispatch_batch([good])  # the context stays usable
+    assert B.status[0] == 0
