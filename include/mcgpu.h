@@ -53,7 +53,7 @@ typedef enum {
     MCGPU_ERROR_INVALID_VALUE = -1, /* bad argument: null pointer, size 0, face size < 3, index out of range */
     MCGPU_ERROR_DEVICE = -2,        /* HIP runtime error */
     MCGPU_ERROR_OUT_OF_MEMORY = -3,
-    MCGPU_ERROR_UNSUPPORTED = -4,   /* candidate polygon with more than 255 vertices; coordinates too large to shift */
+    MCGPU_ERROR_UNSUPPORTED = -4,   /* candidate polygon with more than 65535 vertices; coordinates too large to shift */
     MCGPU_ERROR_NO_RESULT = -5      /* query before a successful dispatch */
 } mcgpu_result;
 

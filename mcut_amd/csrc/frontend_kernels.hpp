@@ -2051,10 +2051,19 @@ __global__ void __launch_bounds__(kStageBlock) k_filter_pairs(const double* __re
     }
 }
 
+// triplet-scratch units (16 bytes) of a polygon of more than five vertices:
+// its projected vertices, its triplets and the two position lists of the
+// partitions (k_big_polygons)
+__host__ __device__ inline unsigned long long big_polygon_units(unsigned long long n)
+{
+    const unsigned long long c3 = n * (n - 1) * (n - 2) / 6;
+    return n + 2 * c3; // vertices, triplets, and 2 c3 eight-byte positions
+}
+
 // orient2d of projected triplets; the first triplet of greatest |orient2d|
 // (std::sort of <= 16 triplets is an insertion sort, which keeps ties in order)
 template <typename VertexAt>
-__device__ inline bool best_triple_small(int n, VertexAt vat, const double* normal, int k, uint8_t* tri)
+__device__ inline bool best_triple_small(int n, VertexAt vat, const double* normal, int k, uint16_t* tri)
 {
     double M[6], X[2 * 5];
     projection_matrix(normal, k, M);
@@ -2072,7 +2081,7 @@ __device__ inline bool best_triple_small(int n, VertexAt vat, const double* norm
                 if (r == 0.0) continue;
                 if (!found || fabs(r) > best) {
                     best = fabs(r);
-                    tri[0] = (uint8_t)a, tri[1] = (uint8_t)b, tri[2] = (uint8_t)c;
+                    tri[0] = (uint16_t)a, tri[1] = (uint16_t)b, tri[2] = (uint16_t)c;
                     found = true;
                 }
             }
@@ -2098,13 +2107,13 @@ __global__ void k_planes(Soup S, const uint32_t* __restrict__ cand_list, Plane* 
             else atomicOr(&C->bad_src, 1u);
             pair_flag(S, g, g > S.nf[0] ? PF_BAD_CUT : PF_BAD_SRC);
         } else if (fv.n > 3 && fv.n <= 5) {
-            if (!best_triple_small((int)fv.n, fv, pl.n, k, pl.tri)) pl.tri[0] = 0xff;
+            if (!best_triple_small((int)fv.n, fv, pl.n, k, pl.tri)) pl.tri[0] = kCollinear;
         } else if (fv.n > 5 && fv.n <= kMaxPoly) {
-            // the projected vertices and every triplet, in the triplet scratch
+            // the projected vertices, every triplet and the partition lists, in the triplet scratch
             pl.big = 1;
             const unsigned long long n = fv.n;
             const unsigned long long at = atomicAdd(&C->n_big, 1ull);
-            const unsigned long long off = atomicAdd(&C->n_trip, n + n * (n - 1) * (n - 2) / 6);
+            const unsigned long long off = atomicAdd(&C->n_trip, big_polygon_units(n));
             if (at < cap_big) {
                 big_list[at] = g;
                 big_off[at] = off;
@@ -2119,55 +2128,158 @@ __global__ void k_planes(Soup S, const uint32_t* __restrict__ cand_list, Plane* 
     }
 }
 
-// Candidate polygons of more than five vertices: one thread per polygon lists
-// its C(n, 3) triplets with non-zero orient2d in a region of the triplet
-// scratch that k_planes reserved (the n projected vertices first, as 16-byte
-// records, then the triplets) and ranks them with the replay of libstdc++'s
-// std::sort (tripsort.hpp).  A region past the scratch's capacity is left
-// undone and flagged; the host grows the scratch and runs the dispatch again.
-__global__ void k_big_polygons(Soup S, const uint32_t* __restrict__ big_list, const unsigned long long* __restrict__ big_off,
-                               const Counters* __restrict__ C, uint64_t cap_big, uint64_t cap_trip,
-                               Plane* __restrict__ planes, Trip* __restrict__ scratch, Counters* Cw)
+// Candidate polygons of more than five vertices, one wave each.  The wave
+// lists the polygon's C(n, 3) triplets with non-zero orient2d in the
+// reference's (i < j < k) lexicographic order, 64 third vertices a step with
+// ballot-ranked appends, in a region of the triplet scratch that k_planes
+// reserved (big_polygon_units: the n projected vertices as 16-byte records,
+// the triplets, then the two position lists of the partitions), and takes the
+// element that the reference's std::sort puts first (trip_first in
+// tripsort.hpp: the left-most chain of the introsort's partitions), each
+// partition computed a wave at a time in the form of pairs that
+// tests/support/tripsort_check.cpp holds to std::__unguarded_partition.  A
+// region past the scratch's capacity is left undone and flagged; the host
+// grows the scratch and runs the dispatch again.
+// The unguarded partition of T[1, l) around the pivot T[0] (std::__unguarded_partition
+// after the median-of-three move), by the whole wave.  Let L be the positions in
+// [1, l) whose element is not greater than the pivot, ascending, and R those
+// in [0, l) whose element is not less than it, descending (the pivot itself
+// last).  The sequential partition swaps L[k] with R[k] for k = 0, 1, ... while
+// L[k] < R[k]; at the first k = K that fails it returns L[0] if K = 0 and
+// otherwise the smaller of L[K] (l if L has no K-th entry) and R[K - 1]
+// (a later left scan stops at the element swapped into R[K - 1] at the latest).
+// Positions of different pairs before K are distinct, so the swaps run in
+// parallel.  Lp and Rp hold the lists (at most l entries each).  Every lane
+// returns the cut.
+__device__ inline long wave_trip_partition(Trip* T, long l, unsigned long long* Lp, unsigned long long* Rp)
 {
+    const int lane = (int)(threadIdx.x & 63);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const Trip P = T[0];
+    long nL = 0, nR = 0;
+    for (long p0 = 1; p0 < l; p0 += 64) {
+        const long p = p0 + lane;
+        const bool s = p < l && !trip_less(T[p], P);
+        const unsigned long long m = __ballot(s);
+        if (s) Lp[nL + __popcll(m & below)] = (unsigned long long)p;
+        nL += __popcll(m);
+    }
+    for (long p0 = l - 1; p0 >= 0; p0 -= 64) {
+        const long p = p0 - lane;
+        const bool s = p >= 0 && !trip_less(P, T[p]);
+        const unsigned long long m = __ballot(s);
+        if (s) Rp[nR + __popcll(m & below)] = (unsigned long long)p;
+        nR += __popcll(m);
+    }
+    __syncthreads(); // the lists, written lane by lane, are read across the wave
+    const long np = nL < nR ? nL : nR;
+    long K = np;
+    for (long k0 = 0; k0 < np; k0 += 64) {
+        const long k = k0 + lane;
+        const bool fail = k < np && !(Lp[k] < Rp[k]);
+        const unsigned long long m = __ballot(fail);
+        if (m) {
+            K = k0 + __ffsll((long long)m) - 1;
+            break;
+        }
+    }
+    for (long k = lane; k < K; k += 64) trip_swap(T + Lp[k], T + Rp[k]);
+    long cut;
+    if (K == 0) {
+        cut = (long)Lp[0];
+    } else {
+        const long lK = K < nL ? (long)Lp[K] : l;
+        const long rK = (long)Rp[K - 1];
+        cut = lK < rK ? lK : rK;
+    }
+    __syncthreads(); // the swaps land before the next partition reads the range
+    return cut;
+}
+
+// trip_first over T[0, m) by the whole wave: lane 0 moves the median to the
+// front and, when the depth budget runs out, heap sorts the range; the
+// partitions are the wave's
+__device__ inline Trip wave_trip_first(Trip* T, long m, unsigned long long* Lp, unsigned long long* Rp)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    long l = m;
+    int depth = trip_depth_budget(m);
+    while (l > 16) {
+        if (depth == 0) {
+            if (lane == 0) trip_heap_sort(T, T + l);
+            __syncthreads();
+            return T[0];
+        }
+        --depth;
+        if (lane == 0) trip_median_to_first(T, T + l);
+        __syncthreads();
+        l = wave_trip_partition(T, l, Lp, Rp);
+    }
+    Trip best = T[0];
+    for (long p = 1; p < l; ++p)
+        if (trip_less(T[p], best)) best = T[p];
+    return best;
+}
+
+__global__ void __launch_bounds__(64) k_big_polygons(Soup S, const uint32_t* __restrict__ big_list,
+                                                     const unsigned long long* __restrict__ big_off,
+                                                     const Counters* __restrict__ C, uint64_t cap_big, uint64_t cap_trip,
+                                                     Plane* __restrict__ planes, Trip* __restrict__ scratch, Counters* Cw)
+{
+    const int lane = (int)threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
     const uint64_t nb = C->n_big < cap_big ? C->n_big : cap_big;
-    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nb; w += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t w = blockIdx.x; w < nb; w += gridDim.x) {
         const uint32_t g = big_list[w];
         const FaceView fv = face_view(S, g);
         const uint64_t n = fv.n;
         const uint64_t off = big_off[w];
-        if (off + n + n * (n - 1) * (n - 2) / 6 > cap_trip) {
-            atomicOr(&Cw->ov_trip, 1u);
+        if (off + big_polygon_units(n) > cap_trip) {
+            if (lane == 0) atomicOr(&Cw->ov_trip, 1u);
             continue;
         }
         Plane pl = planes[g];
         double M[6];
         projection_matrix(pl.n, pl.k, M);
         double* X = reinterpret_cast<double*>(scratch + off); // projected vertex i: X[2i], X[2i + 1]
-        for (uint32_t i = 0; i < fv.n; ++i) {
+        for (uint32_t i = lane; i < fv.n; i += 64) {
             double p[3];
             fv((int)i, p);
             project2(M, p, X + 2 * i);
         }
+        __syncthreads();
+        const uint64_t c3 = n * (n - 1) * (n - 2) / 6;
         Trip* T = scratch + off + n;
+        unsigned long long* Lp = reinterpret_cast<unsigned long long*>(scratch + off + n + c3);
+        unsigned long long* Rp = Lp + c3;
         long m = 0;
         for (int a = 0; a < (int)n; ++a)
             for (int b = a + 1; b < (int)n; ++b)
-                for (int c = b + 1; c < (int)n; ++c) {
-                    const double r = orient2d_adaptive(X + 2 * a, X + 2 * b, X + 2 * c);
-                    if (r == 0.0) continue;
-                    T[m].a = fabs(r);
-                    T[m].ijk = a | (b << 8) | (c << 16);
-                    ++m;
+                for (int c0 = b + 1; c0 < (int)n; c0 += 64) {
+                    const int c = c0 + lane;
+                    double r = 0.0;
+                    if (c < (int)n) r = orient2d_adaptive(X + 2 * a, X + 2 * b, X + 2 * c);
+                    const bool keep = r != 0.0;
+                    const unsigned long long mk = __ballot(keep);
+                    if (keep) {
+                        Trip t;
+                        t.a = fabs(r);
+                        t.ijk = (uint64_t)a | ((uint64_t)b << 16) | ((uint64_t)c << 32);
+                        T[m + __popcll(mk & below)] = t;
+                    }
+                    m += __popcll(mk);
                 }
+        __syncthreads();
         if (m == 0) {
-            pl.tri[0] = 0xff;
+            pl.tri[0] = kCollinear;
         } else {
-            trip_sort(T, T + m);
-            pl.tri[0] = (uint8_t)(T[0].ijk & 0xff);
-            pl.tri[1] = (uint8_t)((T[0].ijk >> 8) & 0xff);
-            pl.tri[2] = (uint8_t)((T[0].ijk >> 16) & 0xff);
+            const Trip best = wave_trip_first(T, m, Lp, Rp);
+            pl.tri[0] = (uint16_t)(best.ijk & 0xffff);
+            pl.tri[1] = (uint16_t)((best.ijk >> 16) & 0xffff);
+            pl.tri[2] = (uint16_t)((best.ijk >> 32) & 0xffff);
         }
-        planes[g] = pl;
+        if (lane == 0) planes[g] = pl;
+        __syncthreads();
     }
 }
 
@@ -2414,7 +2526,7 @@ __global__ void __launch_bounds__(kStageBlock) k_narrow(Soup S, const Plane* __r
             if (i >= n) continue;
             const uint32_t e = te[i], g = tg[i];
             const Plane pl = planes[g];
-            if (pl.tri[0] == 0xff || pl.k < 0) {
+            if (pl.tri[0] == kCollinear || pl.k < 0) {
                 // all polygon vertices collinear: the type is '0' (math.cpp:404-410)
                 if (TO.qs) {
                     TO.qs[i] = 0;
@@ -2515,7 +2627,7 @@ __global__ void __launch_bounds__(256) k_predicates(Soup S, const Plane* __restr
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t e = te[i], g = tg[i];
         const Plane pl = planes[g];
-        const bool tri_ok = pl.k >= 0 && pl.tri[0] != 0xff;
+        const bool tri_ok = pl.k >= 0 && pl.tri[0] != kCollinear;
         const FaceView fv = face_view(S, g);
         double a[3], b[3], c[3], x[2][3];
         fv(tri_ok ? pl.tri[0] : 0, a);
@@ -3160,7 +3272,7 @@ __global__ void k_cut_groups(const uint32_t* __restrict__ gstart, const CutCount
             // the replayed sort orders by descending key: negated projections
             // ascend, and negation keeps every comparison's outcome
             T[q].a = -pr;
-            T[q].ijk = (int32_t)k;
+            T[q].ijk = (uint64_t)k;
         }
         trip_sort(T, T + len);
         for (uint64_t q = 1; q < len; ++q) {

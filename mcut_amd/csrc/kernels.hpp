@@ -12,7 +12,8 @@ namespace mcgpu {
 
 constexpr uint64_t kEmpty64 = ~0ull;
 constexpr int kBlock = 256;
-constexpr int kMaxPoly = 255; // largest candidate polygon (its triplet is kept as three 8-bit vertex indices)
+constexpr int kMaxPoly = 65535; // largest candidate polygon (its triplet is kept as three 16-bit vertex indices)
+constexpr uint16_t kCollinear = 0xffff; // Plane::tri[0] of a polygon whose vertices are all collinear
 
 struct Params {
     double com[3];      // shift x' = ((x - com) + t) + pert
@@ -144,7 +145,7 @@ struct Plane {
     double n[3];
     double d;
     int32_t k;      // largest normal component; -1 = degenerate
-    uint8_t tri[3]; // vertex triple for orient3d; tri[0] = 0xff: all collinear
+    uint16_t tri[3]; // vertex triple for orient3d; tri[0] = kCollinear: all collinear
     uint8_t big;    // triple decided by the large-polygon pass
 };
 

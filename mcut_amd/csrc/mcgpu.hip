@@ -606,7 +606,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
                                              P_<uint32_t>(ctx->big_list), P_<unsigned long long>(ctx->big_off),
                                              ctx->cap_big, C);
     if (ctx->dsz[0] || ctx->dsz[1]) // faces of more than 5 vertices: none in two triangle meshes
-        k_big_polygons<<<grid_for(ctx->cap_big, 64, 1024), 64, 0, s>>>(
+        k_big_polygons<<<(unsigned)std::min<uint64_t>(std::max<uint64_t>(ctx->cap_big, 1), 1024), 64, 0, s>>>(
             S, P_<uint32_t>(ctx->big_list), P_<unsigned long long>(ctx->big_off), C, ctx->cap_big, ctx->cap_trip,
             P_<Plane>(ctx->planes), P_<Trip>(ctx->big_scratch), C);
     stage_event(ctx, EV_A_PLANES, s);
@@ -2399,7 +2399,7 @@ MCGPU_API mcgpu_result mcgpu_get_face_planes(mcgpu_context ctx, uint32_t* faces,
         if (faces) faces[k] = (uint32_t)f;
         for (int j = 0; j < 3; ++j) {
             if (normals) normals[3 * k + j] = pl[f].n[j];
-            if (triple) triple[3 * k + j] = pl[f].tri[0] == 0xff ? -1 : pl[f].tri[j];
+            if (triple) triple[3 * k + j] = pl[f].tri[0] == kCollinear ? -1 : pl[f].tri[j];
         }
         if (d) d[k] = pl[f].d;
         if (max_comp) max_comp[k] = pl[f].k;

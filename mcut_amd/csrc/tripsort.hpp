@@ -23,8 +23,8 @@
 namespace mcgpu {
 
 struct Trip {
-    double a; // |orient2d|
-    int32_t ijk;
+    double a;     // |orient2d|
+    uint64_t ijk; // the triplet's vertex indices, 16 bits each (i | j << 16 | k << 32)
 };
 
 __host__ __device__ inline bool trip_less(const Trip& x, const Trip& y) { return x.a > y.a; }
@@ -105,6 +105,47 @@ __host__ __device__ inline void trip_heap_sort(Trip* first, Trip* last)
     }
 }
 
+// std::__move_median_to_first(first, first + 1, mid, last - 1)
+__host__ __device__ inline void trip_median_to_first(Trip* f, Trip* l)
+{
+    Trip* mid = f + (l - f) / 2;
+    Trip *a = f + 1, *b = mid, *c = l - 1;
+    if (trip_less(*a, *b)) {
+        if (trip_less(*b, *c)) trip_swap(f, b);
+        else if (trip_less(*a, *c)) trip_swap(f, c);
+        else trip_swap(f, a);
+    } else if (trip_less(*a, *c)) {
+        trip_swap(f, a);
+    } else if (trip_less(*b, *c)) {
+        trip_swap(f, c);
+    } else {
+        trip_swap(f, b);
+    }
+}
+
+// std::__unguarded_partition(f + 1, l, f): the median-of-three move leaves an
+// element on each side that stops each scan, so neither runs off the range
+__host__ __device__ inline Trip* trip_unguarded_partition(Trip* f, Trip* l)
+{
+    Trip* lo = f + 1;
+    Trip* hi = l;
+    for (;;) {
+        while (trip_less(*lo, *f)) ++lo;
+        --hi;
+        while (trip_less(*f, *hi)) --hi;
+        if (!(lo < hi)) return lo;
+        trip_swap(lo, hi);
+        ++lo;
+    }
+}
+
+__host__ __device__ inline int trip_depth_budget(long n)
+{
+    int lg = 0;
+    while ((1L << (lg + 1)) <= n) ++lg;
+    return 2 * lg;
+}
+
 // Sorts [first, last) as libstdc++'s std::sort would.  The introsort
 // recursion is kept on an explicit stack: ranges are independent, so the
 // order in which they are finished does not change the result.  Returns the
@@ -113,8 +154,6 @@ __host__ __device__ inline int trip_sort(Trip* first, Trip* last)
 {
     const long n = last - first;
     if (n <= 0) return 0;
-    int lg = 0;
-    while ((1L << (lg + 1)) <= n) ++lg;
     // a range is split only while longer than 16 and its depth budget lasts,
     // and the budget is at most 2 log2(n) < 128 levels
     Trip* seg_first[130];
@@ -123,7 +162,7 @@ __host__ __device__ inline int trip_sort(Trip* first, Trip* last)
     int top = 0, heaps = 0;
     seg_first[top] = first;
     seg_last[top] = last;
-    seg_depth[top] = 2 * lg;
+    seg_depth[top] = trip_depth_budget(n);
     ++top;
     while (top > 0) {
         --top;
@@ -137,29 +176,8 @@ __host__ __device__ inline int trip_sort(Trip* first, Trip* last)
                 break;
             }
             --depth;
-            Trip* mid = f + (l - f) / 2;
-            Trip *a = f + 1, *b = mid, *c = l - 1;
-            if (trip_less(*a, *b)) {
-                if (trip_less(*b, *c)) trip_swap(f, b);
-                else if (trip_less(*a, *c)) trip_swap(f, c);
-                else trip_swap(f, a);
-            } else if (trip_less(*a, *c)) {
-                trip_swap(f, a);
-            } else if (trip_less(*b, *c)) {
-                trip_swap(f, c);
-            } else {
-                trip_swap(f, b);
-            }
-            Trip* lo = f + 1;
-            Trip* hi = l;
-            for (;;) {
-                while (trip_less(*lo, *f)) ++lo;
-                --hi;
-                while (trip_less(*f, *hi)) --hi;
-                if (!(lo < hi)) break;
-                trip_swap(lo, hi);
-                ++lo;
-            }
+            trip_median_to_first(f, l);
+            Trip* lo = trip_unguarded_partition(f, l);
             // the right part [lo, l) first, the left part [f, lo) later; every
             // pushed range has a smaller depth budget than its parent, so the
             // stack holds at most one range per budget level
@@ -177,6 +195,36 @@ __host__ __device__ inline int trip_sort(Trip* first, Trip* last)
         trip_insertion_sort(first, last);
     }
     return heaps;
+}
+
+// The element std::sort puts first, without the rest of the sort.  The
+// introsort recurses on the right part of each partition and loops on the
+// left one, and a right part never moves an element of the left, so only the
+// chain of left parts [first, cut) decides position 0.  That chain ends in a
+// range of at most 16 elements or in a heap sort of the range (depth budget
+// spent).  Every element right of a cut is not greater than every element
+// left of it, and the final insertion sort moves an element only past
+// strictly smaller ones, so position 0 then holds the first element of the
+// largest key within the chain's last range (or the heap sort's first).
+// Expected work is linear in n instead of n log n.  [first, last) is
+// permuted on the way.
+__host__ __device__ inline Trip trip_first(Trip* first, Trip* last)
+{
+    Trip* l = last;
+    int depth = trip_depth_budget(last - first);
+    while (l - first > 16) {
+        if (depth == 0) {
+            trip_heap_sort(first, l);
+            return *first;
+        }
+        --depth;
+        trip_median_to_first(first, l);
+        l = trip_unguarded_partition(first, l);
+    }
+    Trip best = *first;
+    for (Trip* p = first + 1; p < l; ++p)
+        if (trip_less(*p, best)) best = *p;
+    return best;
 }
 
 } // namespace mcgpu

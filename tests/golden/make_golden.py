@@ -28,11 +28,13 @@ Fixtures:
                      with collinear runs, random convex) from the reference's
                      compute_polygon_plane_coefficients and
                      determine_three_noncollinear_vertices
+  ref_huge_polygons.npz  the same for eight polygons of 256-330 vertices
+                     (beyond 8-bit vertex indices; up to 6 M triplets each)
   ref_config2.npz    the reference's broad phase on the two subdivision-7
                      icospheres (327,680 triangles each): shift parameters,
                      candidate-pair count and a SHA-256 of the sorted pairs
 
-Pass fixture names (functions, cases, edges, status, config2, bigpoly) to
+Pass fixture names (functions, cases, edges, status, config2, bigpoly, hugepoly) to
 write only those.
 """
 from __future__ import annotations
@@ -233,16 +235,16 @@ def status_fixture():
     np.savez_compressed(os.path.join(HERE, "ref_status.npz"), **d)
 
 
-def big_polygons(seed=17):
-    """Polygons of 17-40 vertices in the positive quadrant, built so that many
-    triplets tie on |orient2d|: regular polygons in a random plane, regular
-    polygons rounded to a 1/64 grid in an axis plane, rectangles with evenly
-    spaced points along their sides (collinear runs, exact ties) and random
-    convex polygons."""
+def big_polygons(seed=17, count=160, nmin=17, nmax=40):
+    """Polygons of nmin-nmax vertices in the positive quadrant, built so that
+    many triplets tie on |orient2d|: regular polygons in a random plane,
+    regular polygons rounded to a 1/64 grid in an axis plane, rectangles with
+    evenly spaced points along their sides (collinear runs, exact ties) and
+    random convex polygons."""
     rng = np.random.default_rng(seed)
     out = []
-    for it in range(160):
-        n = int(rng.integers(17, 41))
+    for it in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
         kind = it % 4
         ang = 2 * np.pi * np.arange(n) / n
         if kind == 0:
@@ -282,13 +284,14 @@ def big_polygons(seed=17):
     return out
 
 
-def bigpoly_fixture():
-    polys = big_polygons()
+def bigpoly_fixture(name="ref_big_polygons.npz", **kw):
+    polys = big_polygons(**kw)
+    width = max(P.shape[0] for P in polys) if "nmax" in kw else 40
     d = {"poly": [], "n": [], "normal": [], "k": [], "triple": [], "ok": []}
     for P in polys:
         nrm, dd, k = rp.plane(P)
         ok, ijk = rp.best_triple(P, nrm, k)
-        pad = np.zeros((40, 3))
+        pad = np.zeros((width, 3))
         pad[:P.shape[0]] = P
         d["poly"].append(pad)
         d["n"].append(P.shape[0])
@@ -296,8 +299,8 @@ def bigpoly_fixture():
         d["k"].append(k)
         d["triple"].append(ijk)
         d["ok"].append(ok)
-    np.savez_compressed(os.path.join(HERE, "ref_big_polygons.npz"), **{k: np.asarray(v) for k, v in d.items()})
-    print(f"bigpoly: {len(polys)} polygons")
+    np.savez_compressed(os.path.join(HERE, name), **{k: np.asarray(v) for k, v in d.items()})
+    print(f"{name}: {len(polys)} polygons")
 
 
 def config2_fixture():
@@ -317,9 +320,11 @@ def config2_fixture():
 def main():
     if not rp.available():
         sys.exit("build the reference first: make -C oracle ref")
-    which = set(sys.argv[1:]) or {"functions", "cases", "edges", "status", "config2", "bigpoly"}
+    which = set(sys.argv[1:]) or {"functions", "cases", "edges", "status", "config2", "bigpoly", "hugepoly"}
     if "bigpoly" in which:
         bigpoly_fixture()
+    if "hugepoly" in which:
+        bigpoly_fixture("ref_huge_polygons.npz", seed=23, count=8, nmin=256, nmax=330)
     if "status" in which:
         status_fixture()
     if "config2" in which:

@@ -5,8 +5,16 @@
 // random keys with many ties, sorted, reversed and organ-pipe keys, and
 // McIlroy's adversary ("A Killer Adversary for Quicksort", 1999) run against
 // std::sort itself, which drives it into its heap-sort fallback.  Whole
-// permutations are compared, not only the first element.  Prints
-// "cases <n> mismatches <m> heap_fallbacks <h>".
+// permutations are compared, not only the first element.  On the same inputs
+// trip_first (the element std::sort puts first, from the left-most chain of
+// partitions alone) must equal std::sort's first element, and every partition
+// of that chain must come out the same when restated as pairs: the k-th
+// element from the left not greater than the pivot swapped with the k-th from
+// the right not less than it (the pivot ending that list) while the first
+// lies left of the second, the cut then being the first failing pair's left
+// position, or the last swapped right position if that is smaller (the form
+// the GPU computes a wave at a time).  Prints
+// "cases <n> mismatches <m> heap_fallbacks <h> first_mismatches <a> pair_mismatches <b>".
 #include "tripsort.hpp"
 
 #include <algorithm>
@@ -18,17 +26,68 @@ using mcgpu::Trip;
 
 extern "C" void om_trip_sort_keys(const double* keys, long n, int* order);
 
+// the unguarded partition of [f + 1, l) around *f, restated as pairs
+static long pairs_partition(std::vector<Trip>& v, long f, long l)
+{
+    const Trip P = v[f];
+    std::vector<long> L, R;
+    for (long p = f + 1; p < l; ++p)
+        if (!mcgpu::trip_less(v[p], P)) L.push_back(p);
+    for (long p = l - 1; p >= f; --p)
+        if (!mcgpu::trip_less(P, v[p])) R.push_back(p);
+    const long np = (long)std::min(L.size(), R.size());
+    long K = np;
+    for (long k = 0; k < np; ++k)
+        if (!(L[k] < R[k])) {
+            K = k;
+            break;
+        }
+    for (long k = 0; k < K; ++k) std::swap(v[L[k]], v[R[k]]);
+    if (K == 0) return L[0];
+    const long lK = K < (long)L.size() ? L[K] : l;
+    return std::min(lK, R[K - 1]);
+}
+
+// trip_first against std::sort's first element, and every partition of the
+// left-most chain against its restatement as pairs
+static void check_first(const std::vector<double>& keys, const Trip& want, int* first_bad, int* pair_bad)
+{
+    std::vector<Trip> v(keys.size());
+    for (size_t i = 0; i < keys.size(); ++i) v[i] = Trip{keys[i], (uint64_t)i};
+    if (!v.empty()) {
+        std::vector<Trip> w = v;
+        const Trip t = mcgpu::trip_first(w.data(), w.data() + w.size());
+        if (t.ijk != want.ijk) ++*first_bad;
+    }
+    long l = (long)v.size();
+    int depth = mcgpu::trip_depth_budget(l);
+    while (l > 16 && depth > 0) {
+        --depth;
+        mcgpu::trip_median_to_first(v.data(), v.data() + l);
+        std::vector<Trip> u = v;
+        const long cut = mcgpu::trip_unguarded_partition(v.data(), v.data() + l) - v.data();
+        const long cut2 = pairs_partition(u, 0, l);
+        bool same = cut == cut2;
+        for (long i = 0; i < l && same; ++i) same = u[i].ijk == v[i].ijk;
+        if (!same) ++*pair_bad;
+        l = cut;
+    }
+}
+
+static int first_bad = 0, pair_bad = 0;
+
 static bool same_as_std(const std::vector<double>& keys, int* heaps)
 {
     std::vector<Trip> a(keys.size()), b;
-    for (size_t i = 0; i < keys.size(); ++i) a[i] = Trip{keys[i], (int32_t)i};
+    for (size_t i = 0; i < keys.size(); ++i) a[i] = Trip{keys[i], (uint64_t)i};
     b = a;
     std::sort(a.begin(), a.end(), [](const Trip& x, const Trip& y) { return mcgpu::trip_less(x, y); });
     *heaps += mcgpu::trip_sort(b.data(), b.data() + b.size());
     std::vector<int> o(keys.size() + 1);
     om_trip_sort_keys(keys.data(), (long)keys.size(), o.data());
+    if (!a.empty()) check_first(keys, a[0], &first_bad, &pair_bad);
     for (size_t i = 0; i < a.size(); ++i)
-        if (a[i].ijk != b[i].ijk || a[i].ijk != o[i]) return false;
+        if (a[i].ijk != b[i].ijk || a[i].ijk != (uint64_t)o[i]) return false;
     return true;
 }
 
@@ -80,6 +139,7 @@ int main()
         run(adversary(n));
     }
     for (int n = 17; n < 400; n += 7) run(adversary(n));
-    std::printf("cases %d mismatches %d heap_fallbacks %d\n", cases, bad, heaps);
-    return bad != 0;
+    std::printf("cases %d mismatches %d heap_fallbacks %d first_mismatches %d pair_mismatches %d\n", cases, bad, heaps,
+                first_bad, pair_bad);
+    return bad != 0 || first_bad != 0 || pair_bad != 0;
 }

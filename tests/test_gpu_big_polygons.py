@@ -1,8 +1,9 @@
 """Candidate polygons of more than sixteen vertices on the GPU: their plane
-and vertex triplet (the replay of the reference's std::sort ranking, with its
-heap-sort fallback, over a global triplet scratch) and whole dispatches on
-meshes with large polygons, against the oracle, whose triplet choice the
-reference pins for polygons of 17-40 vertices (tests/test_tripsort.py)."""
+and vertex triplet (the element the reference's std::sort ranking puts first,
+from the left-most chain of its partitions with its heap-sort fallback, over a
+global triplet scratch) and whole dispatches on meshes with large polygons,
+against the oracle, whose triplet choice the reference pins for polygons of
+17-40 and 256-330 vertices (tests/test_tripsort.py)."""
 import os
 
 import numpy as np
@@ -14,9 +15,10 @@ from mcut_amd import meshgen as mg
 pytestmark = pytest.mark.gpu
 
 G = np.load(os.path.join(meshes.GOLDEN, "ref_big_polygons.npz"))
+H = np.load(os.path.join(meshes.GOLDEN, "ref_huge_polygons.npz"))
 
 
-def polygon_pair(i):
+def polygon_pair(i, G=G):
     """Golden polygon i as a one-face source mesh whose polygon-soup vertex
     list is the polygon's own order (the soup lists a source face from its
     second vertex, so the face starts with the last one), cut by a triangle
@@ -97,6 +99,43 @@ def test_big_polygons_in_a_batch(ctx):
     plain = frontend.Context(0)
     try:
         for i, (s, c) in enumerate(pairs):
+            R = plain.dispatch(s, c)
+            assert B.status[i] == R.status, i
+            e, g, p = B.pair_registry(i)
+            e0, g0, p0 = R.registry()
+            assert np.array_equal(e, e0) and np.array_equal(g, g0), i
+            assert np.array_equal(p.view(np.uint64), p0.view(np.uint64)), i
+    finally:
+        plain.close()
+
+
+def test_polygons_past_255_vertices_match_oracle(ctx):
+    """Polygons of 256-330 vertices (vertex indices past 8 bits, up to 6 M
+    triplets each, whose triplet the reference pins for the oracle) and
+    closed prisms with 300-gon caps, alone and in a batch."""
+    from mcut_amd import frontend
+    from oracle import oracle as om
+    cases = [polygon_pair(i, H) for i in range(len(H["n"]))]
+    cases.append((prism(300, centre=(0.05, 0.02, 0.0)), mg.tessellated_box((0.6, 0.3, 0.2), 1.1, 3)))
+    big = 0
+    for k, (src, cut) in enumerate(cases):
+        print(f"polygon case {k}: {int(np.max(src[2]))} vertices", flush=True)
+        R = ctx.dispatch(src, cut)
+        O = om.frontend(src, cut)
+        assert R.status == O["status"], (k, R.status, O["status"])
+        pl = R.planes()
+        assert np.array_equal(pl["face"], O["cand_face"]), k
+        assert np.array_equal(pl["normal"].view(np.uint64), O["plane_n"].view(np.uint64)), k
+        assert np.array_equal(pl["triple"], O["plane_triple"]), k
+        big += int((pl["triple"] > 255).any())
+        e, g, p = R.registry()
+        assert np.array_equal(e, O["reg_edge"]) and np.array_equal(g, O["reg_face"]), k
+        assert np.array_equal(p.view(np.uint64), O["reg_point"].view(np.uint64)), k
+    assert big >= 4  # triplets that need more than 8 bits per index
+    B = ctx.dispatch_batch(cases[:3] + cases[-1:])
+    plain = frontend.Context(0)
+    try:
+        for i, (s, c) in enumerate(cases[:3] + cases[-1:]):
             R = plain.dispatch(s, c)
             assert B.status[i] == R.status, i
             e, g, p = B.pair_registry(i)

@@ -2,9 +2,9 @@
 GPU runs it (mcut_amd/csrc/tripsort.hpp, compiled for the host) and as the
 oracle restates it, against libstdc++'s std::sort itself on inputs that reach
 the introsort's every path, its heap-sort fallback included (McIlroy's
-adversary); and the oracle's triplet choice for polygons of 17-40 vertices
-against the reference's determine_three_noncollinear_vertices (golden
-vectors written by tests/golden/make_golden.py)."""
+adversary); and the oracle's triplet choice for polygons of 17-40 and 256-330
+vertices against the reference's determine_three_noncollinear_vertices
+(golden vectors written by tests/golden/make_golden.py)."""
 import os
 import shutil
 import subprocess
@@ -34,14 +34,18 @@ def test_sort_replay_equals_libstdcxx_std_sort(tmp_path):
     f = out.stdout.split()
     assert out.returncode == 0 and f[0] == "cases" and f[3] == "0", out.stdout
     assert int(f[1]) > 150 and int(f[5]) > 50  # the heap-sort fallback was exercised
+    assert f[6] == "first_mismatches" and f[7] == "0" and f[8] == "pair_mismatches" and f[9] == "0", out.stdout
 
 
-def test_big_polygon_triplets_against_reference(oracle_lib):
+@pytest.mark.parametrize("fixture,nmin,nmax", [("ref_big_polygons.npz", 17, 40), ("ref_huge_polygons.npz", 256, 300)])
+def test_big_polygon_triplets_against_reference(oracle_lib, fixture, nmin, nmax):
+    """Polygons of 17-40 vertices, and of 256-330 (past 8-bit vertex indices,
+    up to 6 M triplets each)."""
     from oracle import oracle as om
-    G = np.load(os.path.join(meshes.GOLDEN, "ref_big_polygons.npz"))
+    G = np.load(os.path.join(meshes.GOLDEN, fixture))
     L = oracle_lib.lib()
     n_all = G["n"]
-    assert n_all.min() >= 17 and n_all.max() >= 40
+    assert n_all.min() >= nmin and n_all.max() >= nmax
     for i, n in enumerate(n_all):
         P = np.ascontiguousarray(G["poly"][i][:n])
         nrm, d, k = om.plane(P)
