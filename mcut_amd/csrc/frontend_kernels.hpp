@@ -542,11 +542,11 @@ __global__ void k_set_perturbation(Params* P, int has_pert, double p0, double p1
 // the cut mesh's raw extent.  tol keeps a factor of 32 over every rounding
 // term.  A bound that is not finite marks the input unsupported (bit 1).
 __device__ inline bool tolerance_of(const unsigned long long (*rbox)[6], uint32_t nv0, uint32_t nv1, double gp_constant,
-                                    int absolute, double* tol)
+                                    int absolute, double* tol, bool* loose = nullptr)
 {
     const double u32 = 3.552713678800501e-15; // 2^-48 = 32 u
     const double n = (double)(nv0 > nv1 ? nv0 : nv1);
-    double cext2 = 0.0, wmax = 0.0, rmax = 0.0, R[3], W[3];
+    double cext2 = 0.0, wmax = 0.0, rmax = 0.0, emax = 0.0, R[3], W[3];
     bool finite = true;
     for (int j = 0; j < 3; ++j) {
         const double l0 = ord_value(rbox[0][j]), l1 = ord_value(rbox[1][j]);
@@ -554,6 +554,7 @@ __device__ inline bool tolerance_of(const unsigned long long (*rbox)[6], uint32_
         const double lo = l0 < l1 ? l0 : l1, hi = h0 > h1 ? h0 : h1;
         W[j] = fmax(fabs(lo), fabs(hi));
         R[j] = (hi - lo) + (n + 2.0) * 2.220446049250313e-16 * W[j];
+        emax = fmax(emax, hi - lo);
         const double ce = h1 - l1;
         cext2 += ce * ce;
         wmax = fmax(wmax, W[j]);
@@ -564,17 +565,30 @@ __device__ inline bool tolerance_of(const unsigned long long (*rbox)[6], uint32_
     // u (2W + 2|t| + E) of the raw extent
     const double diag_hi = (sqrt(cext2) + u32 * 2.0 * (wmax + 2.0 * rmax + 2.0)) * (1.0 + u32);
     const double eps_hi = (absolute ? 1.0 : diag_hi) * gp_constant * (1.0 + u32);
+    bool wide = false;
     for (int j = 0; j < 3; ++j) {
-        tol[j] = eps_hi * (1.0 + u32) + u32 * (8.0 * R[j] + W[j] + 4.0);
+        const double round = u32 * (8.0 * R[j] + W[j] + 4.0);
+        tol[j] = eps_hi * (1.0 + u32) + round;
         finite = finite && isfinite(tol[j]);
+        // the rounding of the shift, not eps, widens the boxes by more than a
+        // sixteenth of the pair's largest extent: coordinates far from the
+        // origin relative to the meshes (near 2^52 the shift itself rounds to
+        // whole units)
+        wide = wide || round * 16.0 > emax;
     }
+    if (loose) *loose = wide;
     return finite;
 }
 
-__global__ void k_tolerance(Params* P, Counters* C, uint32_t nv0, uint32_t nv1, double gp_constant, int absolute)
+// allow_loose: the dispatch can rerun with its broad phase in shifted
+// coordinates (a single pair whose source is not resident)
+__global__ void k_tolerance(Params* P, Counters* C, uint32_t nv0, uint32_t nv1, double gp_constant, int absolute,
+                            int allow_loose)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (!tolerance_of(C->rbox, nv0, nv1, gp_constant, absolute, P->tol)) C->unsupported |= 2u;
+    bool loose = false;
+    if (!tolerance_of(C->rbox, nv0, nv1, gp_constant, absolute, P->tol, &loose)) C->unsupported |= 2u;
+    else if (loose && allow_loose) C->loose = 1u;
 }
 
 // batches: one thread per pair; any pair without a finite bound rejects the batch
@@ -1318,7 +1332,7 @@ __global__ void k_morton_raw(MortonMesh M0, MortonMesh M1, uint32_t start, uint3
     // the leaf sort's group totals (radix_ghist), cleared here rather than by a launch of their own
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nzero; i += (uint64_t)gridDim.x * blockDim.x)
         zero[i] = 0u;
-    const bool aborted = dispatch_aborted(C);
+    const bool aborted = dispatch_aborted(C) || C->loose; // loose: the rerun in shifted coordinates finds the pairs
     const uint32_t F = M0.nf + M1.nf;
     for (uint64_t g = start + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < F; g += (uint64_t)gridDim.x * blockDim.x) {
         const bool cutm = g >= M0.nf;
@@ -1365,6 +1379,53 @@ __global__ void k_morton_raw(MortonMesh M0, MortonMesh M1, uint32_t start, uint3
             o[j] = __double2float_rd((dlo - t) - (fabs(dlo) + t) * 3.552713678800501e-15);
             o[3 + j] = __double2float_ru((dhi + t) + (fabs(dhi) + t) * 3.552713678800501e-15);
         }
+    }
+}
+
+// Leaves in shifted coordinates, for a pair whose raw-space broad phase is
+// loose (Counters::loose): the exact shifted face boxes of k_face_boxes (cut
+// faces already widened by eps) rounded outward to single precision, so each
+// float box contains its double box and the trees find a superset of the
+// pairs the exact filter keeps.  The Morton frame is the shifted image of the
+// raw union box (s() is monotone, so it bounds every shifted box up to
+// rounding); codes only order the leaves, and the clamp absorbs the rest.
+__global__ void k_morton_shifted(const double* __restrict__ box, uint32_t nf0, uint32_t nf1, const Params* __restrict__ P,
+                                 const Counters* __restrict__ C, uint32_t* __restrict__ code, uint32_t* __restrict__ face,
+                                 float* __restrict__ rawbox, uint32_t* __restrict__ zero, uint64_t nzero)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nzero; i += (uint64_t)gridDim.x * blockDim.x)
+        zero[i] = 0u;
+    const bool aborted = dispatch_aborted(C);
+    double lo[3], ext[3];
+    for (int j = 0; j < 3; ++j) {
+        const double a = P->bmin[0][j] < P->bmin[1][j] ? P->bmin[0][j] : P->bmin[1][j];
+        const double b = P->bmax[0][j] > P->bmax[1][j] ? P->bmax[0][j] : P->bmax[1][j];
+        lo[j] = (a - P->com[j]) + P->t[j];
+        const double hi = (b - P->com[j]) + P->t[j];
+        ext[j] = hi > lo[j] ? hi - lo[j] : 1.0;
+    }
+    const uint32_t F = nf0 + nf1;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < F; g += (uint64_t)gridDim.x * blockDim.x) {
+        const bool cutm = g >= nf0;
+        face[g] = (uint32_t)(cutm ? g - nf0 : g);
+        float* o = rawbox + 6 * g;
+        const uint32_t mesh_bit = cutm ? (1u << 21) : 0u;
+        if (aborted) {
+            code[g] = mesh_bit;
+            o[0] = o[1] = o[2] = 3.402823466e38f;
+            o[3] = o[4] = o[5] = -3.402823466e38f;
+            continue;
+        }
+        const double* b = box + 6 * g;
+        uint32_t c[3];
+        for (int j = 0; j < 3; ++j) {
+            float x = (float)((((b[j] + b[3 + j]) * 0.5) - lo[j]) / ext[j]);
+            x = fminf(fmaxf(x * 128.0f, 0.0f), 127.0f);
+            c[j] = expand_bits10((uint32_t)x);
+            o[j] = __double2float_rd(b[j]);
+            o[3 + j] = __double2float_ru(b[3 + j]);
+        }
+        code[g] = (c[0] * 4 + c[1] * 2 + c[2]) | mesh_bit;
     }
 }
 

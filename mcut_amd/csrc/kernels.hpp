@@ -49,6 +49,11 @@ struct Counters {
     unsigned int trav_q[8]; // persistent traversal: head of each work queue (blocks taken)
     unsigned int ncc[2];    // connected components of each mesh's vertex graph (all pairs of a batch)
     unsigned int cc_changed, cc_done; // component check: a hook this round; no hook in the last round
+    // the raw-space broad phase would be loose: the rounding part of its
+    // tolerance exceeds 1/16 of the pair's largest extent (k_tolerance);
+    // its leaves are left empty and the host reruns the dispatch with the
+    // broad phase in shifted coordinates
+    unsigned int loose;
     // raw bounding box of each mesh as order keys (lo xyz, hi xyz); written
     // while the input is validated and kept by the per-dispatch counter reset
     unsigned long long rbox[2][6];

@@ -73,7 +73,7 @@ const char* kStageNames[ST_COUNT] = {"coordinate_sums", "shift_params", "shift_b
                                      "traversal_kernel"};
 
 // event slots: stage boundaries on the three streams
-enum Ev { EV_A0, EV_S0, EV_A_SUMS, EV_A_PARAMS, EV_A_SHIFT, EV_A_JOIN, EV_A_FILTER, EV_A_PLANES, EV_A_JOIN2, EV_A_TESTS, EV_A_NARROW,
+enum Ev { EV_A0, EV_S0, EV_A_SUMS, EV_A_PARAMS, EV_A_SHIFT, EV_A_BOXES, EV_A_JOIN, EV_A_FILTER, EV_A_PLANES, EV_A_JOIN2, EV_A_TESTS, EV_A_NARROW,
           EV_A_END, EV_FORK, EV_B0, EV_B_SORT, EV_B_REFIT, EV_B_TRAV, EV_C0, EV_C_FILL, EV_C_TOPO, EV_SUM_ITEMS,
           EV_SUM_CHAIN, EV_TRAV_K0, EV_TRAV_K1, EV_B_CC, EV_COUNT };
 
@@ -204,6 +204,8 @@ struct mcgpu_context_t {
     uint32_t npairs = 0;
     int batch_mb = 7;                    // Morton bits per axis within a pair
     bool batch_packed = false;           // every pair's vertex ids and corner ranks fit the packed edge word
+    bool shifted_bp = false;             // this dispatch's broad phase runs in shifted coordinates (Counters::loose)
+    bool allow_loose = false;            // a single pair's own broad phase: a loose raw tolerance reruns it shifted
     bool cc_check = false;               // count each mesh's connected components (check_input_mesh)
     bool pending = false;                // an enqueued dispatch (mcgpu_dispatch_frontend_async) not yet waited for
     hipEvent_t ev_user = nullptr;        // recorded on the caller's stream before an enqueued dispatch
@@ -716,7 +718,7 @@ void enqueue_validate(mcgpu_context ctx)
     }
     const bool absolute = (ctx->flags & MCGPU_DISPATCH_ENFORCE_GENERAL_POSITION_ABSOLUTE) != 0;
     k_tolerance<<<1, 1, 0, s>>>(&P_<DevState>(ctx->state)->P, C, ctx->nv[0], ctx->nv[1], ctx->gp_constant,
-                                absolute ? 1 : 0);
+                                absolute ? 1 : 0, (ctx->allow_loose && !ctx->res) ? 1 : 0);
 }
 
 // Batches: the same reset and validation, per pair: segment ids of every
@@ -1168,12 +1170,21 @@ void enqueue_frontend(mcgpu_context ctx)
     const Soup S = make_soup(ctx);
     k_face_boxes<<<grid_for(F), kBlock, 0, s>>>(S, ctx->batch ? Pb : Pd, C, P_<double>(ctx->box), 0);
     stage_event(ctx, EV_A_SHIFT, s);
+    if (ctx->shifted_bp) ev_record(ctx, EV_A_BOXES, s);
 
-    // ---- stream B: raw-space trees and broad phase ----
-    ev_wait(ctx, s2, EV_FORK);
+    // ---- stream B: raw-space trees and broad phase (in shifted coordinates,
+    // after the face boxes, for a pair whose raw-space tolerance is loose) ----
+    ev_wait(ctx, s2, ctx->shifted_bp ? EV_A_BOXES : EV_FORK);
     stage_event(ctx, EV_B0, s2);
     float* rawbox = P_<float>(ctx->rawbox);
-    if (ctx->batch) {
+    if (ctx->shifted_bp) {
+        uint64_t nzero = 0;
+        uint32_t* zero = radix_ghist(RadixTemp{P_<uint32_t>(ctx->hist), P_<uint32_t>(ctx->scan_tmp)}, F, ctx->sort_bits,
+                                     &nzero);
+        k_morton_shifted<<<grid_for(F), kBlock, 0, s2>>>(P_<double>(ctx->box), ctx->nf[0], ctx->nf[1], Pd, C,
+                                                         P_<uint32_t>(ctx->mcode), P_<uint32_t>(ctx->mface), rawbox,
+                                                         zero, nzero);
+    } else if (ctx->batch) {
         for (int m = 0; m < 2; ++m) {
             const uint32_t nf = ctx->nf[m];
             const uint64_t o = m ? Fs : 0;
@@ -1267,6 +1278,11 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
         if ((!ctx->res && C.ncc[0] != want) || C.ncc[1] != want) return MCGPU_ERROR_INVALID_VALUE;
     }
     if (C.unsupported & 2u) return MCGPU_ERROR_UNSUPPORTED;
+    if (C.loose && !ctx->shifted_bp) { // run again with the broad phase in shifted coordinates
+        ctx->shifted_bp = true;
+        *overflow = true;
+        return MCGPU_SUCCESS;
+    }
     if (C.ov_pairs || C.ov_tests || C.ov_esc || C.ov_reg || C.ov_big || C.ov_trip) {
         *overflow = true;
         // a refusal leaves the context's capacities as they were
@@ -1404,6 +1420,7 @@ std::vector<uint64_t> graph_key(mcgpu_context ctx)
     auto put = [&k](uint64_t x) { k.push_back(x); };
     auto ptr = [&k](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
     put(ctx->batch), put(ctx->npairs), put(ctx->batch_mb), put(ctx->batch_packed), put(ctx->cc_check), put(ctx->sort_bits),
+        put(ctx->shifted_bp),
         put(ctx->gbits), put(ctx->kbits);
     for (int m = 0; m < 2; ++m) {
         put(ctx->nv[m]), put(ctx->nf[m]), put(ctx->ncorner[m]), put(ctx->depth[m]);
@@ -1818,6 +1835,8 @@ static mcgpu_result frontend_prepare(mcgpu_context ctx, const mcgpu_mesh* src, c
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = false;
+    ctx->shifted_bp = false;
+    ctx->allow_loose = true;
     ctx->cc_check = (desc->flags & MCGPU_DISPATCH_CHECK_COMPONENTS) != 0;
     ctx->sort_bits = 22;
     ctx->flags = desc->flags;
@@ -1931,6 +1950,7 @@ MCGPU_API mcgpu_result mcgpu_dispatch_batch(mcgpu_context ctx, const mcgpu_batch
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = true;
+    ctx->allow_loose = false;
     ctx->cc_check = (desc->flags & MCGPU_DISPATCH_CHECK_COMPONENTS) != 0;
     ctx->npairs = np;
     ctx->batch_mb = std::min(7, (31 - pb) / 3);
@@ -3251,6 +3271,8 @@ MCGPU_API mcgpu_result mcgpu_dispatch_narrow(mcgpu_context ctx, const mcgpu_mesh
     ctx->have = false;
     ctx->broad_only = false;
     ctx->batch = false;
+    ctx->shifted_bp = false;
+    ctx->allow_loose = false; // the caller's pairs: no broad phase runs here
     ctx->cc_check = false;
     ctx->sort_bits = 22;
     ctx->flags = desc->flags;

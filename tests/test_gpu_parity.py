@@ -223,6 +223,25 @@ def _brute_pairs(a, b):
     return np.stack([i, k], 1).astype(np.uint32)
 
 
+@pytest.mark.parametrize("case,scale", [("sphere-box", 60.0), ("sphere-box", 100.0), ("icosphere-2", 60.0)])
+def test_coordinates_far_from_the_origin_run_the_broad_phase_shifted(ctx, case, scale):
+    """Meshes of 100-250 units near 2^52, where the shift rounds to whole
+    units: the raw-space tolerance (16 units of rounding) is more than a
+    sixteenth of the pair's extent, so the dispatch reruns its broad phase on
+    the shifted face boxes.  Every intermediate equals the oracle's, these
+    pairs reach the narrow phase and succeed, and the raw-space superset is
+    the shifted leaves' (close to the exact pairs), not the loose one."""
+    from mcut_amd import meshgen as mg
+    if case == "sphere-box":
+        src, cut = mg.uv_sphere(16, 8), mg.tessellated_box((0.2, 0.1, 0.05), 1.3, 3, rotation=mg.random_rotation(4))
+    else:
+        src, cut = meshes.icosphere_pair(2, amp=0.01)
+    f = lambda m: (np.asarray(m[0], np.float64) * scale + 2.0 ** 52, m[1], m[2])
+    R, O = compare(ctx, f(src), f(cut))
+    assert R.status == om.SUCCESS and R.counts["registry"] > 0
+    assert R.counts["raw_pairs"] < 2 * R.counts["pairs"]
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_broad_phase_boxes(ctx, seed):
     """mcgpu_broad_phase against brute force on random boxes, with boxes that

@@ -201,28 +201,14 @@ def test_collapsed_mesh_raw_boxes_stay_exact():
 
 
 @pytest.mark.gpu
-def test_collapsed_mesh_is_refused_cleanly(ctx):
+def test_collapsed_mesh_runs_its_broad_phase_shifted(ctx):
     """Near 2^52 the shift rounds to whole units, so the raw-space broad
-    phase's tolerance covers the collapsed pair's whole extent and its
-    superset is every face pair (81,920^2, past the 2^32 entries a list may
-    hold): the dispatch returns MCGPU_ERROR_OUT_OF_MEMORY instead of sizing
-    lists of hundreds of GB, and the context stays usable."""
-    from mcut_amd import frontend
-    f = lambda V: V * 4 + 2.0 ** 52 + 0.5
-    src, cut = meshes.icosphere_pair(6, amp=0.01)
-    src, cut = _remap(src, f), _remap(cut, f)
-    print("sums: collapsed icosphere-6 pair", flush=True)
-    with pytest.raises(frontend.McgpuError) as ei:
-        ctx.dispatch(src, cut)
-    assert ei.value.code == -3
-    R = ctx.dispatch(*meshes.config1_rotated())
-    assert R.status == 0 and R.counts["registry"] > 0
-
-
-@pytest.mark.gpu
-@pytest.mark.skip(reason="the raw-space broad phase refuses this input (test_collapsed_mesh_is_refused_cleanly); "
-                         "a broad phase in shifted coordinates for it is not built")
-def test_centre_bits_on_a_collapsed_mesh(ctx):
+    phase's tolerance covers the collapsed pair's whole extent (its superset
+    would be every face pair, 81,920^2).  The dispatch sees that its raw
+    tolerance is loose, leaves the raw leaves empty and runs again with the
+    broad phase on the shifted face boxes: MCUT's 29.7 M candidate pairs and
+    its INVALID_CUT_MESH, with the centre's bits, as the oracle has them.  A
+    normal dispatch on the same context then runs in raw space again."""
     from oracle import oracle as om
     f = lambda V: V * 4 + 2.0 ** 52 + 0.5
     src, cut = meshes.icosphere_pair(6, amp=0.01)
@@ -231,4 +217,11 @@ def test_centre_bits_on_a_collapsed_mesh(ctx):
     R = ctx.dispatch(src, cut)
     O = om.frontend(src, cut, detail=False)
     assert np.array_equal(R.com.view(np.uint64), np.asarray(O["com"]).view(np.uint64))
-    assert R.status == O["status"] and R.counts["pairs"] == O["counts"]["pairs"]
+    assert np.array_equal(R.t.view(np.uint64), np.asarray(O["t"]).view(np.uint64))
+    assert R.status == O["status"] == om.INVALID_CUT_MESH
+    assert R.counts["pairs"] == O["counts"]["pairs"] == 29_690_882
+    assert R.counts["raw_pairs"] < 2 * R.counts["pairs"]  # shifted leaves, not the raw superset
+    a = R.pairs()
+    assert np.array_equal(a, O["pairs"])
+    R2 = ctx.dispatch(*meshes.config1_rotated())
+    assert R2.status == 0 and R2.counts["registry"] > 0
