@@ -518,7 +518,8 @@ def bench_single(args, torch, frontend, mg):
                                f"({n['F'] // 2} tris each, +-2% radial noise, offset +0.5x)",
                    "parallelism": "single GPU"},
         "frontend_device_ms": device_ms,
-        "latency_ms": {"p50": lat_sorted[len(lat) // 2], "p99": lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))]},
+        "latency_ms": {"p10": float(np.percentile(lat, 10)), "p50": lat_sorted[len(lat) // 2],
+                       "p90": float(np.percentile(lat, 90)), "p99": lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))]},
         "counts": counts,
         "stage_ms": stages,
         "traversal_shape": {k: shape_used[k] for k in ("wave_leaves", "tree_stride", "queues", "chunk")},
