@@ -156,6 +156,10 @@ struct mcgpu_context_t {
 
     Buf in_v[2], in_idx[2], in_sz[2];
     Buf off[2], state, ps_v, box; // state: DevState (counters and shift parameters)
+    // pinned host copy of the state, enqueued right behind a dispatch's last
+    // kernel so that finish() waits once instead of copying after the wait
+    DevState* hstate = nullptr;
+    bool state_copied = false;
     Buf hkeys[2], hfirst[2], eflag[2], eid[2], cedge[2], cface[2], etotal;
     Buf ev0, ev1, ef0, ef1;
     Buf mcode, mface, mcode2, mface2; // Morton keys and leaf order of both meshes
@@ -683,6 +687,7 @@ void enqueue_narrow(mcgpu_context ctx, const Soup& S, bool join_topology)
 // rejection after the dispatch.
 void enqueue_validate(mcgpu_context ctx)
 {
+    ctx->state_copied = false; // a state copy belongs to the dispatch it follows
     ctx->last_timed = ctx->stage_timing || ctx->kernel_timing;
     ctx->last_kernel_only = !ctx->stage_timing && ctx->kernel_timing;
     hipStream_t s = ctx->s;
@@ -727,6 +732,7 @@ void enqueue_validate(mcgpu_context ctx)
 // mesh), each pair's broad-phase tolerance, and the cell layout.
 void enqueue_validate_batch(mcgpu_context ctx)
 {
+    ctx->state_copied = false;
     ctx->last_timed = ctx->stage_timing || ctx->kernel_timing;
     ctx->last_kernel_only = !ctx->stage_timing && ctx->kernel_timing;
     hipStream_t s = ctx->s;
@@ -1241,6 +1247,17 @@ void enqueue_frontend(mcgpu_context ctx)
     enqueue_narrow(ctx, S, true);
 }
 
+// The state's copy to pinned host memory, on stream A behind the dispatch
+// (stream A joins the other three before its last kernel).
+mcgpu_result enqueue_state_copy(mcgpu_context ctx)
+{
+    ctx->state_copied = false;
+    if (!ctx->hstate) CK(hipHostMalloc((void**)&ctx->hstate, sizeof(DevState), hipHostMallocDefault));
+    CK(hipMemcpyAsync(ctx->hstate, ctx->state.p, sizeof(DevState), hipMemcpyDeviceToHost, ctx->s));
+    ctx->state_copied = true;
+    return MCGPU_SUCCESS;
+}
+
 mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow)
 {
     ctx->cp_have = false; // cut-path edges belong to the dispatch they were made from
@@ -1248,8 +1265,13 @@ mcgpu_result finish(mcgpu_context ctx, mcgpu_frontend_info* info, bool* overflow
     CK(hipStreamSynchronize(ctx->s));
     CK(hipGetLastError());
     DevState hs;
-    CK(hipMemcpyAsync(&hs, ctx->state.p, sizeof(DevState), hipMemcpyDeviceToHost, ctx->s));
-    CK(hipStreamSynchronize(ctx->s));
+    if (ctx->state_copied) { // enqueued with the dispatch: complete with the stream
+        std::memcpy(&hs, ctx->hstate, sizeof(DevState));
+        ctx->state_copied = false;
+    } else {
+        CK(hipMemcpyAsync(&hs, ctx->state.p, sizeof(DevState), hipMemcpyDeviceToHost, ctx->s));
+        CK(hipStreamSynchronize(ctx->s));
+    }
     ctx->C = hs.C;
     ctx->P = hs.P;
     const Counters& C = ctx->C;
@@ -1450,11 +1472,12 @@ void graph_drop(mcgpu_context ctx)
 // dispatch's shape differs from the one it was recorded for.
 mcgpu_result enqueue_dispatch(mcgpu_context ctx)
 {
+    ctx->state_copied = false;
     if (!ctx->graphs || ctx->stage_timing || ctx->kernel_timing || !ctx->own_streams) {
         if (ctx->batch) enqueue_validate_batch(ctx);
         else enqueue_validate(ctx);
         enqueue_frontend(ctx);
-        return MCGPU_SUCCESS;
+        return enqueue_state_copy(ctx);
     }
     std::vector<uint64_t> key = graph_key(ctx);
     if (key != ctx->prog_key || ctx->prog.empty()) {
@@ -1491,7 +1514,7 @@ mcgpu_result enqueue_dispatch(mcgpu_context ctx)
         }
     }
     ++ctx->replays;
-    return MCGPU_SUCCESS;
+    return enqueue_state_copy(ctx);
 }
 
 mcgpu_result upload_mesh(mcgpu_context ctx, int m, const mcgpu_mesh* mesh, bool on_device)
@@ -1810,6 +1833,7 @@ MCGPU_API mcgpu_result mcgpu_release_context(mcgpu_context ctx)
         if (op.exec) (void)hipGraphExecDestroy(op.exec);
     for (Buf* b : context_bufs(ctx))
         if (b->p) (void)hipFree(b->p);
+    if (ctx->hstate) (void)hipHostFree(ctx->hstate);
     for (int i = 0; i < EV_COUNT; ++i) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->ev_user) (void)hipEventDestroy(ctx->ev_user);
     if (ctx->own_streams) {
@@ -2206,6 +2230,7 @@ MCGPU_API mcgpu_result mcgpu_redispatch_perturbed(mcgpu_context ctx, const doubl
         ctx->last_timed = ctx->stage_timing || ctx->kernel_timing;
     ctx->last_kernel_only = !ctx->stage_timing && ctx->kernel_timing;
         (void)hipEventRecord(ctx->ev[EV_A0], s);
+        ctx->state_copied = false; // finish() copies the state after its wait
         k_reset_narrow<<<1, 1, 0, s>>>(C);
         k_set_perturbation<<<1, 1, 0, s>>>(&P_<DevState>(ctx->state)->P, 1, perturbation[0], perturbation[1], perturbation[2]);
         stage_event(ctx, EV_A_PARAMS, s);
@@ -2805,8 +2830,10 @@ mcgpu_result slot_bind(mcgpu_server srv, mcgpu_context ctx, const mcgpu_mesh* cu
 void slot_enqueue(mcgpu_context ctx, hipEvent_t done)
 {
     ctx->stage_mode = 0;
+    ctx->state_copied = false;
     enqueue_validate(ctx);
     enqueue_frontend(ctx);
+    (void)enqueue_state_copy(ctx); // on failure finish() copies after its wait
     (void)hipEventRecord(done, ctx->s);
 }
 
