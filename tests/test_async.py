@@ -31,6 +31,16 @@ def crowded_pair():
     return fan(), (V, np.asarray(F, np.uint32), S)
 
 
+def far_pair():
+    """A sphere and a box of about 130 units near 2^52: the raw-space
+    tolerance is loose there, so the dispatch runs its broad phase a second
+    time in shifted coordinates (inside the wait, for the enqueued form)."""
+    f = lambda m: (np.asarray(m[0], np.float64) * 60.0 + 2.0 ** 52, m[1], m[2])
+    src = mg.uv_sphere(16, 8)
+    cut = mg.tessellated_box((0.2, 0.1, 0.05), 1.3, 3, rotation=mg.random_rotation(4))
+    return f(src), f(cut)
+
+
 def snapshot(R):
     e, g, p = R.registry()
     return dict(status=R.status, counts=R.counts, pairs=R.pairs(), e=e, g=g, p=p.view(np.uint64))
@@ -42,11 +52,11 @@ def same(a, b):
         assert np.array_equal(a[k], b[k]), k
 
 
-@pytest.mark.parametrize("name", ["config1_rotated", "icosphere5", "crowded"])
+@pytest.mark.parametrize("name", ["config1_rotated", "icosphere5", "crowded", "far"])
 def test_async_equals_blocking(name):
     from mcut_amd import frontend
     pair = {"config1_rotated": meshes.config1_rotated, "icosphere5": lambda: meshes.icosphere_pair(5, amp=0.005),
-            "crowded": crowded_pair}[name]()
+            "crowded": crowded_pair, "far": far_pair}[name]()
     print(f"async case {name}", flush=True)
     a, b = frontend.Context(0), frontend.Context(0)
     try:

@@ -117,3 +117,24 @@ def test_stage_timing_runs_eagerly():
         assert c.graph_stats()["replays"] == 0
     finally:
         c.close()
+
+
+def test_far_from_origin_pair_replays(ctxs):
+    """A pair far from the origin (a sphere and a box of about 130 units near
+    2^52) runs its broad phase a second time in shifted coordinates: every
+    dispatch is two programs, the raw one and the shifted one, and every
+    replay equals the eager dispatch.  A context keeps one program, so the
+    two evict each other and each dispatch records both."""
+    eager, graph = ctxs
+    f = lambda m: (np.asarray(m[0], np.float64) * 60.0 + 2.0 ** 52, m[1], m[2])
+    pair = (f(mg.uv_sphere(16, 8)), f(mg.tessellated_box((0.2, 0.1, 0.05), 1.3, 3, rotation=mg.random_rotation(4))))
+    print("graphs: far-from-origin pair", flush=True)
+    ref = snapshot(eager.dispatch(*pair))
+    assert ref["status"] == 0 and ref["counts"]["registry"] > 0
+    before = graph.graph_stats()
+    for i in range(3):
+        same(ref, snapshot(graph.dispatch(*pair)))
+        same(ref, snapshot(eager.dispatch(*pair)))
+    st = graph.graph_stats()
+    assert st["recordings"] == before["recordings"] + 6
+    assert st["replays"] == before["replays"] + 6
