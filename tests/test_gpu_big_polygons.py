@@ -112,12 +112,15 @@ def test_big_polygons_in_a_batch(ctx):
 def test_polygons_past_255_vertices_match_oracle(ctx):
     """Polygons of 256-330 vertices (vertex indices past 8 bits, up to 6 M
     triplets each, whose triplet the reference pins for the oracle) and
-    closed prisms with 300-gon caps, alone and in a batch."""
+    closed prisms with 300-gon and 600-gon caps, alone and (up to the
+    300-gon prism) in a batch."""
     from mcut_amd import frontend
     from oracle import oracle as om
     cases = [polygon_pair(i, H) for i in range(len(H["n"]))]
     cases.append((prism(300, centre=(0.05, 0.02, 0.0)), mg.tessellated_box((0.6, 0.3, 0.2), 1.1, 3)))
     big = 0
+    # and caps of 600 vertices: 35.8 M triplets a cap, many more partitions deep
+    cases = cases + [(prism(600, centre=(0.05, 0.02, 0.0)), mg.tessellated_box((0.6, 0.3, 0.2), 1.1, 3))]
     for k, (src, cut) in enumerate(cases):
         print(f"polygon case {k}: {int(np.max(src[2]))} vertices", flush=True)
         R = ctx.dispatch(src, cut)
@@ -132,10 +135,11 @@ def test_polygons_past_255_vertices_match_oracle(ctx):
         assert np.array_equal(e, O["reg_edge"]) and np.array_equal(g, O["reg_face"]), k
         assert np.array_equal(p.view(np.uint64), O["reg_point"].view(np.uint64)), k
     assert big >= 4  # triplets that need more than 8 bits per index
-    B = ctx.dispatch_batch(cases[:3] + cases[-1:])
+    batch = cases[:3] + cases[-2:-1]
+    B = ctx.dispatch_batch(batch)
     plain = frontend.Context(0)
     try:
-        for i, (s, c) in enumerate(cases[:3] + cases[-1:]):
+        for i, (s, c) in enumerate(batch):
             R = plain.dispatch(s, c)
             assert B.status[i] == R.status, i
             e, g, p = B.pair_registry(i)
