@@ -189,6 +189,22 @@ def test_empty_intersection(ctx):
     assert R.counts["pairs"] == 0 and R.counts["registry"] == 0 and R.status == 0
 
 
+@pytest.mark.parametrize("src_name,cut_name", [("bunny", "bunnyCuttingPlanePartial"),
+                                               ("cube-flattened", "cube-flattened-with-holes")])
+def test_reference_test_meshes(ctx, src_name, cut_name):
+    """The reference's own test meshes: the bunny cut partly by a plane
+    (tests/source/dispatchFilterFlags.cpp:155-204, which expects success) and
+    a flattened cube against one with holes (tests/source/polygonWithHoles.cpp:
+    95-140, which expects MC_INVALID_OPERATION, the front end's
+    general-position violation)."""
+    import os
+    from mcut_amd import meshgen as mg
+    d = os.path.join(meshes.GOLDEN, "meshes")
+    src, cut = mg.read_off(os.path.join(d, f"{src_name}.off")), mg.read_off(os.path.join(d, f"{cut_name}.off"))
+    R, O = compare(ctx, src, cut)
+    assert R.status == (om.SUCCESS if src_name == "bunny" else om.GENERAL_POSITION_VIOLATION)
+
+
 def test_invalid_input_rejected(ctx):
     from mcut_amd import frontend, meshgen as mg
     V, F, S = mg.uv_sphere(8, 4)

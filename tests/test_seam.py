@@ -159,3 +159,23 @@ def test_whole_front_end_with_helper_threads(gpu_libs, name, pair):
     a = ref.dispatch(pair[0], pair[1], flags=fl, helpers=7)
     b = gpu.dispatch(pair[0], pair[1], flags=fl, helpers=7)
     same(a, b)
+
+
+def test_whole_front_end_on_the_reference_test_meshes(gpu_libs):
+    """Two of the reference's own test cases, with the answers its tests
+    expect, from MCUT unchanged and from MCUT with its front end on the GPU:
+    the bunny cut partly by a plane with the undefined-location and
+    inside-sealing filters gives 4 components
+    (tests/source/dispatchFilterFlags.cpp:155-204), identical in both builds;
+    a flattened cube against one with holes fails with MC_INVALID_OPERATION
+    (tests/source/polygonWithHoles.cpp:95-140)."""
+    from mcut_amd import meshgen as mg
+    ref, gpu = gpu_libs
+    d = os.path.join(meshes.GOLDEN, "meshes")
+    read = lambda n: mg.read_off(os.path.join(d, f"{n}.off"))
+    fl = (1 << 7) | (1 << 8)  # MC_DISPATCH_FILTER_FRAGMENT_LOCATION_UNDEFINED | _SEALING_INSIDE
+    a, b = run(ref, read("bunny"), read("bunnyCuttingPlanePartial"), fl), run(gpu, read("bunny"), read("bunnyCuttingPlanePartial"), fl)
+    assert not isinstance(a, int) and len(a) == 4
+    same(a, b)
+    a, b = run(ref, read("cube-flattened"), read("cube-flattened-with-holes"), 0), run(gpu, read("cube-flattened"), read("cube-flattened-with-holes"), 0)
+    assert a == b == -2  # MC_INVALID_OPERATION
