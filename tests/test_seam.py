@@ -179,3 +179,24 @@ def test_whole_front_end_on_the_reference_test_meshes(gpu_libs):
     same(a, b)
     a, b = run(ref, read("cube-flattened"), read("cube-flattened-with-holes"), 0), run(gpu, read("cube-flattened"), read("cube-flattened-with-holes"), 0)
     assert a == b == -2  # MC_INVALID_OPERATION
+
+
+@pytest.mark.parametrize("i", range(61))
+def test_reference_benchmark_with_enforced_general_position(gpu_libs, i):
+    """The reference's own benchmark test (tests/source/benchmark.cpp:104-240):
+    benchmark pairs 000-060 with MC_DISPATCH_ENFORCE_GENERAL_POSITION must
+    dispatch without error and give well-formed components (at least one
+    triangle of vertices, face indices in range, faces of three or more
+    vertices).  MCUT with its front end on the GPU does, with components
+    identical to MCUT unchanged."""
+    from mcut_amd.seam import mcut_api
+    ref, gpu = gpu_libs
+    src, cut = meshes.bench_pair(i)
+    fl = mcut_api.MC_DISPATCH_ENFORCE_GENERAL_POSITION
+    a, b = run(ref, src, cut, fl), run(gpu, src, cut, fl)
+    assert not isinstance(b, int), (i, b)
+    for cc in b:
+        nv = cc["vertices"].size // 3
+        assert nv >= 3 and cc["face_sizes"].size > 0 and (cc["face_sizes"] >= 3).all()
+        assert cc["faces"].size == int(cc["face_sizes"].sum()) and (cc["faces"] < nv).all()
+    same(a, b)
