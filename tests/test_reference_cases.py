@@ -12,6 +12,8 @@ are identical to the unchanged build's, array for array.
   give 3 components each.  The reference passes float arrays; the values are
   exact in double, and MCUT widens float input to double before anything
   else, so the double arrays here are the same input.
+* tests/source/computeSeams.cpp:104-352: the seam components of pairs 013
+  and 014.
 * tests/source/degenerateInput.cpp:58-112: two triangles meeting edge on
   edge, and a cut triangle whose vertex lies on the source triangle, fail
   with MC_INVALID_OPERATION.
@@ -103,3 +105,30 @@ def test_reference_known_answers(libs, name, pair, flags, expect):
         assert np.array_equal(x["faces"], y["faces"]) and np.array_equal(x["face_sizes"], y["face_sizes"])
         assert np.array_equal(x["seam"], y["seam"])
     assert "libmcgpu.so" in open("/proc/self/maps").read()
+
+
+SEAM = 1 << 3  # MC_CONNECTED_COMPONENT_TYPE_SEAM
+THROUGH = 1 << 2  # MC_DISPATCH_REQUIRE_THROUGH_CUTS
+
+
+@pytest.mark.parametrize("pair_id,flags,seams,seam_vertices", [(13, 0, 1, 4), (14, 0, 2, None), (14, THROUGH, 2, None)],
+                         ids=["pair013", "pair014", "pair014_through"])
+def test_reference_seam_components(libs, pair_id, flags, seams, seam_vertices):
+    """tests/source/computeSeams.cpp:104-352: pair 013 gives one seam component
+    (the cut mesh's, the source's cut being partial) with four seam vertices;
+    pair 014 gives two, also with MC_DISPATCH_REQUIRE_THROUGH_CUTS.  Both
+    builds agree array for array."""
+    ref, gpu = libs
+    pair = meshes.bench_pair(pair_id)
+    a, b = run(ref, pair, flags), run(gpu, pair, flags)
+    for ccs in (a, b):
+        assert not isinstance(ccs, int), ccs
+        sc = [cc for cc in ccs if cc["type"] == SEAM]
+        assert len(sc) == seams
+        if seam_vertices is not None:
+            assert all(cc["seam"].size == seam_vertices for cc in sc)
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        assert x["type"] == y["type"]
+        assert np.array_equal(x["vertices"].view(np.uint64), y["vertices"].view(np.uint64))
+        assert np.array_equal(x["faces"], y["faces"]) and np.array_equal(x["seam"], y["seam"])
