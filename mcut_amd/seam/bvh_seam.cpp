@@ -5,7 +5,7 @@
 // build_oibvh keeps computing the per-face AABBs the rest of MCUT reads
 // (kernel.cpp uses them for the edge-face cull), with the reference's own
 // bounding_box_t::expand/enlarge, and hands them to intersectOIBVHs in place
-// of a tree.  intersectOIBVHs sends the two sets of face boxes to
+// of a tree, behind the tree's root box (their union).  intersectOIBVHs sends the two sets of face boxes to
 // mcgpu_broad_phase and fills ps_face_to_potentially_intersecting_others the
 // way bvh.cpp:709-716 does: each pair in both directions, cut faces offset by
 // the source face count.  The order inside each list differs from the CPU
@@ -39,13 +39,15 @@ mcgpu_context seam_context()
     return ctx;
 }
 
-void flatten(const std::vector<bounding_box_t<vec3_<double>>>& boxes, std::vector<double>& out)
+// boxes[first..] as six doubles each (minimum, then maximum)
+void flatten(const std::vector<bounding_box_t<vec3_<double>>>& boxes, std::vector<double>& out, size_t first)
 {
-    out.resize(6 * boxes.size());
-    for (size_t i = 0; i < boxes.size(); ++i) {
+    const size_t n = boxes.size() > first ? boxes.size() - first : 0;
+    out.resize(6 * n);
+    for (size_t i = 0; i < n; ++i) {
         for (int j = 0; j < 3; ++j) {
-            out[6 * i + j] = boxes[i].minimum()[j];
-            out[6 * i + 3 + j] = boxes[i].maximum()[j];
+            out[6 * i + j] = boxes[first + i].minimum()[j];
+            out[6 * i + 3 + j] = boxes[first + i].maximum()[j];
         }
     }
 }
@@ -67,8 +69,13 @@ void build_oibvh(
         for (const vd_t& v : vertices_on_face) face_bboxes[fi].expand(mesh.vertex(v));
         if (slightEnlargmentEps > 0.0) face_bboxes[fi].enlarge(slightEnlargmentEps);
     }
-    // the GPU broad phase needs only the face boxes
-    bvhAABBs = face_bboxes;
+    // Entry 0 is the root box, the union of the face boxes (what bvh.cpp's
+    // internal levels leave there and preproc.cpp:2896 and :3721 read as the
+    // mesh's box for the intersection type); the face boxes follow, which is
+    // all the GPU broad phase needs.
+    bvhAABBs.assign(1, bounding_box_t<vec3_<double>>());
+    for (const auto& b : face_bboxes) bvhAABBs[0].expand(b);
+    bvhAABBs.insert(bvhAABBs.end(), face_bboxes.begin(), face_bboxes.end());
     bvhLeafNodeFaces.resize(nf);
     for (int i = 0; i < nf; ++i) bvhLeafNodeFaces[i] = fd_t(i);
 }
@@ -82,8 +89,8 @@ void intersectOIBVHs(std::map<fd_t, std::vector<fd_t>>& ps_face_to_potentially_i
     const uint32_t ns = static_cast<uint32_t>(srcMeshBvhLeafNodeFaces.size());
     const uint32_t nc = static_cast<uint32_t>(cutMeshBvhLeafNodeFaces.size());
     std::vector<double> a, b;
-    flatten(srcMeshBvhAABBs, a);
-    flatten(cutMeshBvhAABBs, b);
+    flatten(srcMeshBvhAABBs, a, 1); // the face boxes, past the root box
+    flatten(cutMeshBvhAABBs, b, 1);
     std::lock_guard<std::mutex> lock(seam_mutex);
     mcgpu_context ctx = seam_context();
     uint64_t n = 0;

@@ -18,6 +18,8 @@ MC_DISPATCH_VERTEX_ARRAY_DOUBLE = 1 << 1
 MC_DISPATCH_REQUIRE_THROUGH_CUTS = 1 << 2
 MC_DISPATCH_FILTER_ALL = sum(1 << b for b in range(5, 15))
 MC_DISPATCH_ENFORCE_GENERAL_POSITION = 1 << 15
+MC_DISPATCH_INCLUDE_INTERSECTION_TYPE = 1 << 17
+MC_CONTEXT_DISPATCH_INTERSECTION_TYPE = 1 << 13
 MC_CONNECTED_COMPONENT_TYPE_ALL = 0xFFFFFFFF
 MC_CONNECTED_COMPONENT_TYPE_INPUT = 1 << 4
 MC_CONNECTED_COMPONENT_DATA_VERTEX_DOUBLE = 1 << 2
@@ -48,10 +50,14 @@ class Mcut:
         L.mcGetConnectedComponentData.argtypes = [vp, vp, u32, C.c_size_t, vp, C.POINTER(C.c_size_t)]
         L.mcReleaseConnectedComponents.argtypes = [vp, u32, vp]
         L.mcReleaseContext.argtypes = [vp]
+        L.mcGetInfo.argtypes = [vp, u32, C.c_size_t, vp, C.POINTER(C.c_size_t)]
         for f in ("mcCreateContextWithHelpers", "mcCreateContext", "mcDispatch", "mcGetConnectedComponents",
-                  "mcGetConnectedComponentData", "mcReleaseConnectedComponents", "mcReleaseContext"):
+                  "mcGetConnectedComponentData", "mcReleaseConnectedComponents", "mcReleaseContext", "mcGetInfo"):
             getattr(L, f).restype = C.c_int
         self.L = L
+        # McDispatchIntersectionType of the last dispatch that asked for it
+        # (MC_DISPATCH_INCLUDE_INTERSECTION_TYPE), else None
+        self.intersection_type = None
 
     def cutpath_record(self):
         """The cut-path edges of this library's last dispatch that made them
@@ -152,6 +158,12 @@ class Mcut:
                 self._ok(L.mcDispatch(ctx, flags | MC_DISPATCH_VERTEX_ARRAY_DOUBLE, sv, sf, ss, snv, snf, cv, cf, cs,
                                       cnv, cnf), "mcDispatch")
                 os.chdir(cwd)
+            self.intersection_type = None
+            if flags & MC_DISPATCH_INCLUDE_INTERSECTION_TYPE:
+                t = C.c_uint32(0)
+                self._ok(L.mcGetInfo(ctx, MC_CONTEXT_DISPATCH_INTERSECTION_TYPE, C.sizeof(t), C.byref(t), None),
+                         "mcGetInfo")
+                self.intersection_type = t.value
             n = C.c_uint32(0)
             self._ok(L.mcGetConnectedComponents(ctx, MC_CONNECTED_COMPONENT_TYPE_ALL, 0, None, C.byref(n)),
                      "mcGetConnectedComponents")

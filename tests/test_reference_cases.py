@@ -17,6 +17,9 @@ are identical to the unchanged build's, array for array.
 * tests/source/degenerateInput.cpp:58-112: two triangles meeting edge on
   edge, and a cut triangle whose vertex lies on the source triangle, fail
   with MC_INVALID_OPERATION.
+* tests/source/intersectionType.cpp:91-760: the intersection type that
+  mcGetInfo(MC_CONTEXT_DISPATCH_INTERSECTION_TYPE) reports for cubes and
+  open quads nested, separated and crossing (13 cases).
 """
 import os
 
@@ -127,6 +130,65 @@ def test_reference_seam_components(libs, pair_id, flags, seams, seam_vertices):
         assert len(sc) == seams
         if seam_vertices is not None:
             assert all(cc["seam"].size == seam_vertices for cc in sc)
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        assert x["type"] == y["type"]
+        assert np.array_equal(x["vertices"].view(np.uint64), y["vertices"].view(np.uint64))
+        assert np.array_equal(x["faces"], y["faces"]) and np.array_equal(x["seam"], y["seam"])
+
+
+INCLUDE_TYPE = 1 << 17  # MC_DISPATCH_INCLUDE_INTERSECTION_TYPE
+STANDARD, INSIDE_CUT, INSIDE_SRC, NONE = 0, 1 << 1, 1 << 2, 1 << 3  # McDispatchIntersectionType
+
+
+def box(h, t=(0.0, 0.0, 0.0)):
+    """Quad-faced cube of half extent h about t, outward faces
+    (tests/source/intersectionType.cpp:91-200)."""
+    s = np.array([[-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1],
+                  [-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1]], np.float64)
+    F = np.array([0, 1, 2, 3, 7, 6, 5, 4, 4, 0, 3, 7, 1, 5, 6, 2, 3, 2, 6, 7, 0, 4, 5, 1], np.uint32)
+    return s * h + np.array(t, np.float64), F, np.full(6, 4, np.uint32)
+
+
+def quad(h, t, plane):
+    """One open quad of half extent h about t, in the xz plane (normal +y) or
+    the xy plane (tests/source/intersectionType.cpp:203-312)."""
+    if plane == "xz":
+        s = np.array([[-1, 0, 1], [1, 0, 1], [1, 0, -1], [-1, 0, -1]], np.float64)
+    else:
+        s = np.array([[-1, 1, 0], [1, 1, 0], [1, -1, 0], [-1, -1, 0]], np.float64)
+    return s * h + np.array(t, np.float64), np.arange(4, dtype=np.uint32), np.array([4], np.uint32)
+
+
+TYPE_CASES = [
+    ("cut_cube_inside_source_cube", box(2), box(1), 0, INSIDE_SRC),
+    ("source_cube_inside_cut_cube", box(1), box(2), 0, INSIDE_CUT),
+    ("separated_cubes", box(1, (-2, 0, 0)), box(1, (2, 0, 0)), 0, NONE),
+    ("crossing_cubes", box(1), box(1, (0.5, 0.5, 0.5)), 0, STANDARD),
+    ("coincident_cubes_enforced", box(1), box(1), ENFORCE, STANDARD),
+    ("quad_above_cube", box(1), quad(2, (0, 2, 0), "xz"), 0, NONE),
+    ("quad_inside_source_cube", box(10), quad(2, (0, 2, 0), "xz"), 0, INSIDE_SRC),
+    ("quad_cuts_source_cube", box(1), quad(1, (0.5, 0, 0.5), "xz"), 0, STANDARD),
+    ("cube_cuts_source_quad", quad(1, (0.5, 0, 0.5), "xz"), box(1), 0, STANDARD),
+    ("source_quad_inside_cut_cube", quad(1, (0, 0, 0), "xz"), box(10), 0, INSIDE_CUT),
+    ("source_quad_apart_from_cube", quad(1, (10, 0, 0), "xz"), box(1, (-10, 0, 0)), 0, NONE),
+    ("separated_quads", quad(1, (10, 0, 0), "xz"), quad(1, (-10, 0, 0), "xz"), 0, NONE),
+    ("crossing_quads", quad(1, (0, 0, 0), "xz"), quad(2, (0, 0, 0), "xy"), 0, STANDARD),
+]
+
+
+@pytest.mark.parametrize("name,src,cut,flags,expect", TYPE_CASES, ids=[c[0] for c in TYPE_CASES])
+def test_reference_intersection_types(libs, name, src, cut, flags, expect):
+    """Both builds report the intersection type the reference's test asserts,
+    and give the same components array for array."""
+    print(f"intersection type case {name}", flush=True)
+    out = []
+    for lib in libs:
+        ccs = run(lib, (src, cut), flags | INCLUDE_TYPE)
+        assert not isinstance(ccs, int), ccs
+        assert lib.intersection_type == expect, (lib.intersection_type, expect)
+        out.append(ccs)
+    a, b = out
     assert len(a) == len(b)
     for x, y in zip(a, b):
         assert x["type"] == y["type"]
